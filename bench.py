@@ -1,0 +1,201 @@
+"""Benchmark: facet.trace evaluations/s of the coherent facet kernel on one MI355X.
+
+Workload: the surface + bed refraction scene (512 x 512 facets per interface
+at 25 m, 256 traces at 20 km, 4096 samples at 48 MHz, exponential sub-facet
+roughness + grazing fix, cos^2 pattern, 60 MHz / 15 MHz chirp; see
+radar_sounder_simulator_amd/workloads.py config2), synthetic seeded data,
+resident in HBM before the timed region.  One step = one full pass of the
+hot path: surface kernel + refracted bed kernel + range compression of both
+layers.  The headline value is the mixed-precision tier; the plain float64
+tier is timed beside it on the same scene, and the mixed result's error
+against it (max dB / max mrad over bins within 30 dB of each trace's peak)
+is reported.  The float64 tier reproduces the float64 oracle to rel L2
+<= 1e-9 (tests/test_gpu_parity.py), so it is the stand-in for the oracle at
+this size.
+
+Output: one JSON line (rank 0).  ``roofline`` covers the dominant kernel
+(the refracted bed kernel) with its duration measured by HIP events on the
+launch stream; ``cpu_baseline`` times this project's float64 C restatement
+(oracle/csrc/oracle.c, OpenMP over traces) on a bounded sample of traces.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def errors_db_mrad(mixed, ref, floor_db=30.0):
+    """Max power error (dB) and phase error (mrad) over bins within
+    ``floor_db`` of each (layer, trace)'s peak."""
+    a = mixed[..., 0] + 1j * mixed[..., 1]
+    b = ref[..., 0] + 1j * ref[..., 1]
+    pk = np.abs(b).max(axis=-1, keepdims=True)
+    m = np.abs(b) > pk * 10 ** (-floor_db / 20.0)
+    db = np.abs(20 * np.log10(np.abs(a[m]) / np.abs(b[m]))).max()
+    mrad = np.abs(np.angle(a[m] / b[m])).max() * 1e3
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    return float(db), float(mrad), float(rel)
+
+
+def cpu_baseline(wl, n_traces):
+    """Time the C float64 oracle on ``n_traces`` traces x every facet of both
+    layers, threads = available host cores (capped at the trace count)."""
+    from oracle import cbuild, cref
+    if not os.path.exists(cbuild.OUT):
+        cbuild.build()
+    cores = len(os.sched_getaffinity(0))
+    nth = max(1, min(cores, n_traces))
+    tr = wl.track
+    idx = np.linspace(0, len(tr.positions) - 1, n_traces).round().astype(int)
+    pos = tr.positions[idx]
+    pat = None
+    if wl.pattern is not None:
+        k, pv, pa, pb = wl.pattern
+        pat = (k, pv[idx], pa, pb)
+    s, b = wl.layers
+    t0 = time.perf_counter()
+    f = s.facets
+    cref.coherent_cluttergram(pos, tr.u_ct[idx], f.centers, f.normals, f.areas, f.e1, f.e2,
+                              k=wl.k0, gamma=s.gamma, t0=wl.t0, dt=wl.dt,
+                              n_samples=wl.n_samples, c=wl.c, pattern=pat,
+                              roughness=s.roughness, taper_s=wl.taper_s,
+                              d_phi_area=wl.d_phi_area, nthreads=nth)
+    cref.refracted_cluttergram(pos, tr.u_ct[idx], b.facets, b.crossed, b.eps_leg, b.att_leg,
+                               t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c,
+                               gamma=b.gamma, k0=wl.k0, pattern=pat, roughness=b.roughness,
+                               taper_s=wl.taper_s, d_phi_area=wl.d_phi_area, nthreads=nth)
+    el = time.perf_counter() - t0
+    pairs = n_traces * (len(s.facets) + len(b.facets))
+    cpu = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(": ")
+    return {"value": pairs / el, "unit": "facet*trace/s", "cores": nth, "kind": "port",
+            "sample": f"{n_traces} of {len(tr.positions)} traces x all "
+                      f"{len(s.facets)}+{len(b.facets)} facets (surface+bed), float64 C "
+                      f"restatement, OpenMP {nth} threads on {cpu}; {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512, help="facets per side per interface")
+    ap.add_argument("--traces", type=int, default=256)
+    ap.add_argument("--samples", type=int, default=4096)
+    ap.add_argument("--cpu-traces", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the coherent facet kernel shards by "
+              "traces with no exchange step, and this build measures one MI355X "
+              "(--gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from radar_sounder_simulator_amd import roofline, workloads
+    from radar_sounder_simulator_amd.pipeline import ResidentScene
+
+    log(f"bench: building workload n={args.n} traces={args.traces} samples={args.samples}")
+    wl = workloads.config2(n=args.n, n_traces=args.traces, n_samples=args.samples)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rs = ResidentScene(wl, device=dev)
+    pairs_l = rs.pairs_per_layer()
+    pairs = sum(pairs_l)
+    st = torch.cuda.current_stream()
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup):
+            rs.run(precision)
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record(st)
+        for s in range(steps):
+            e = evs[s]
+            e[0].record(st)
+            rs.run_layer(0, precision)
+            e[1].record(st)
+            rs.run_layer(1, precision)
+            e[2].record(st)
+            rs.compress()
+            e[3].record(st)
+        t1.record(st)
+        torch.cuda.synchronize()
+        total = t0.elapsed_time(t1) / steps
+        k_surf = np.mean([e[0].elapsed_time(e[1]) for e in evs])
+        k_bed = np.mean([e[1].elapsed_time(e[2]) for e in evs])
+        k_cmp = np.mean([e[2].elapsed_time(e[3]) for e in evs])
+        return total, k_surf, k_bed, k_cmp
+
+    log("bench: mixed precision")
+    ms, ms_s, ms_b, ms_c = timed("mixed", args.steps, args.warmup)
+    out_mixed = rs.out.cpu().numpy()
+    log(f"bench: mixed {ms:.2f} ms/step (surface {ms_s:.2f}, bed {ms_b:.2f}, "
+        f"compress {ms_c:.2f})")
+    extra = {}
+    if not args.no_fp64:
+        log("bench: float64")
+        md, md_s, md_b, md_c = timed("fp64", args.fp64_steps, 1)
+        out_fp64 = rs.out.cpu().numpy()
+        db, mrad, rel = errors_db_mrad(out_mixed, out_fp64)
+        per_layer = [errors_db_mrad(out_mixed[j:j + 1], out_fp64[j:j + 1])
+                     for j in range(len(pairs_l))]
+        log(f"bench: fp64 {md:.2f} ms/step (surface {md_s:.2f}, bed {md_b:.2f}); "
+            f"mixed vs fp64 max {db:.2e} dB {mrad:.3f} mrad rel {rel:.2e}")
+        extra = {
+            "fp64_value": pairs / (md * 1e-3), "fp64_ms_per_step": md,
+            "speedup_vs_fp64": md / ms,
+            "fp64_kernel_ms": {"surface": md_s, "bed": md_b, "compress": md_c},
+            "mixed_vs_fp64": {"max_db": db, "max_mrad": mrad, "rel_l2": rel,
+                              "domain": "bins within 30 dB of each trace's peak",
+                              "per_layer": {n: {"max_db": e[0], "max_mrad": e[1]}
+                                            for n, e in zip(["surface", "bed"], per_layer)}},
+        }
+    s, b = wl.layers
+    f_bed = roofline.refracted_pair_flops(1, "tabulated", True, b.roughness[3])
+    f_surf = roofline.surface_pair_flops("tabulated", True, s.roughness[3])
+    achieved = f_bed * pairs_l[1] / (ms_b * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": roofline.PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / roofline.PEAK_FP32_TFLOPS,
+            "traffic": None, "kernel": "refracted_kernel<mixed>",
+            "flops_per_pair": f_bed, "kernel_ms": ms_b,
+            "surface_kernel": {"flops_per_pair": f_surf, "kernel_ms": ms_s,
+                               "achieved": f_surf * pairs_l[0] / (ms_s * 1e-3) / 1e12}}
+    cpu = None
+    if not args.no_cpu:
+        log(f"bench: cpu baseline ({args.cpu_traces} traces)")
+        cpu = cpu_baseline(wl, args.cpu_traces)
+    line = {
+        "metric": "facet*trace evals/s (mixed precision, surface+bed refraction scene)",
+        "value": pairs / (ms * 1e-3), "unit": "facet*trace/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": "surface+bed refraction, 512x512 facets/interface at 25 m, "
+                               "256 traces at 20 km, 4096 samples, 60/15 MHz",
+                   "facets_per_interface": args.n * args.n, "traces": args.traces,
+                   "samples": args.samples, "pairs_per_step": pairs},
+        "kernel_ms": {"surface": ms_s, "bed": ms_b, "compress": ms_c},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

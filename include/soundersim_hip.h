@@ -1,0 +1,214 @@
+/*
+ * soundersim_hip.h -- C ABI of the MI355X (gfx950) coherent facet kernel.
+ *
+ * Plain extern "C" functions over device pointers and sizes: no C++ types,
+ * no torch types, no exceptions cross this boundary.  Every entry point
+ * returns SS_OK (0) or a negative SS_E* code; ss_last_error() then gives a
+ * message (thread-local).  All work is enqueued on the caller's HIP stream
+ * (``stream`` is a hipStream_t, NULL = default stream); the caller owns every
+ * buffer and the library never frees caller memory.
+ *
+ * The reference (soundersim, englacial/radar-sounder-simulator) has no
+ * native layer: its path sits behind plain Python functions.  Each entry
+ * point below replaces one of them and is bound from Python by ctypes
+ * (radar_sounder_simulator_amd/_lib.py) behind a shim with the reference's
+ * signature.
+ *
+ * Data layout (device memory, float64 structure-of-arrays):
+ *   facets  [SS_FROW_COUNT][facet_stride]  rows SS_FROW_* below; the
+ *           optional rows are read only when the matching flag is set.
+ *           Facets are expected in a spatially compact order (the Python
+ *           shim Morton-sorts them); the kernels take 256 consecutive
+ *           facets as one tile.
+ *   traces  [SS_TROW_COUNT][trace_stride]  rows SS_TROW_* below.
+ *   field   (n_traces, n_seg) complex128 interleaved (re, im), n_seg =
+ *           n_samples (x2 with SS_SPLIT_SIDES: [left | right] halves);
+ *           incoherent refracted output uses the same layout with the
+ *           power in the real part and a zero imaginary part.
+ *   dropped (n_traces,) float64.
+ *   pair_bins (optional, may be NULL): (n_traces, n_facets) int32, the
+ *           floor bin of every pair before the side offset, or
+ *           SS_BIN_INVALID for a refracted pair whose path is invalid.
+ * Outputs are zeroed by the call (no memset node: a fill kernel).
+ */
+#ifndef SOUNDERSIM_HIP_H
+#define SOUNDERSIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_OK 0
+#define SS_EINVAL (-1)
+#define SS_EHIP (-2)
+#define SS_ERANGE (-3)
+
+/* precision */
+#define SS_PREC_FP64 0  /* plain float64 everywhere (follows the oracle)    */
+#define SS_PREC_MIXED 1 /* float64 range/bin/phase cycles + Snell polish,
+                           float32 amplitudes, sinc, antenna, roughness    */
+
+/* flags */
+#define SS_SPLIT_SIDES 0x001
+#define SS_INTERP_BINS 0x002
+#define SS_TAPER 0x004
+#define SS_D_PHI_AREA 0x008
+#define SS_ROUGH 0x010
+#define SS_GAMMA_FACET 0x020
+#define SS_COHERENT 0x040     /* refracted: coherent (field) vs incoherent  */
+#define SS_CROSS_ROUGH 0x080  /* refracted: crossing roughness sig_c        */
+#define SS_DIFFUSE 0x100      /* refracted: diffuse channel                 */
+
+/* antenna pattern kinds (antenna.py gain_fn) */
+#define SS_PAT_ISOTROPIC 0
+#define SS_PAT_DIPOLE 1
+#define SS_PAT_ARRAY 2
+#define SS_PAT_ARRAY_TAPERED 3
+#define SS_PAT_FINITE_DIPOLE 4
+#define SS_PAT_TABULATED 5
+
+/* roughness correlation functions */
+#define SS_ACF_GAUSSIAN 0
+#define SS_ACF_EXPONENTIAL 1
+
+/* facet rows */
+#define SS_FROW_CX 0
+#define SS_FROW_NX 3
+#define SS_FROW_AREA 6
+#define SS_FROW_E1X 7
+#define SS_FROW_E2X 10
+#define SS_FROW_GAMMA 13
+#define SS_FROW_PH_RE 14
+#define SS_FROW_PH_IM 15
+#define SS_FROW_DIF_AMP 16
+#define SS_FROW_DIF_RE 17
+#define SS_FROW_DIF_IM 18
+#define SS_FROW_COUNT 19
+
+/* trace rows */
+#define SS_TROW_PX 0
+#define SS_TROW_UX 3      /* cross-track unit vector (split sides)        */
+#define SS_TROW_RREF 6    /* surface: |p - mean(centres)| (phase fold-back) */
+#define SS_TROW_PV0 7     /* pattern vector (axis / boresight)           */
+#define SS_TROW_PV1 10    /* second pattern vector (array boresight)     */
+#define SS_TROW_COUNT 13
+
+/* crossed-interface descriptor rows (refracted), one row of 16 per iface */
+#define SS_IDESC_NY 0
+#define SS_IDESC_NX 1
+#define SS_IDESC_COEF 2   /* (3, 2) row-major: coef[0..5]                */
+#define SS_IDESC_MP 8
+#define SS_IDESC_MN 11
+#define SS_IDESC_SIGC 14
+#define SS_IDESC_TABOFF 15 /* first row of this interface in ``tables``   */
+#define SS_IDESC_COUNT 16
+
+#define SS_BIN_INVALID (-2147483647 - 1)
+#define SS_MAX_CROSSED 64
+#define SS_MAX_TERMS 300
+#define SS_MAX_PATTERN_TABLE 4096
+
+/* Kernel-wide constants table (device, float64), see ss_fill_constants. */
+#define SS_CONST_WEIDEMAN_L 0
+#define SS_CONST_WEIDEMAN_A 1   /* 32 coefficients, highest power first    */
+#define SS_CONST_LGAMMA 33      /* lgamma(m + 1), m = 0 .. SS_MAX_TERMS     */
+#define SS_CONST_COUNT (33 + SS_MAX_TERMS + 1)
+
+typedef struct ss_surface_params {
+  double k;         /* carrier wavenumber 2 pi / lambda (rad/m)             */
+  double gamma;     /* scalar field reflection coefficient                 */
+  double t0;        /* window start (s)                                    */
+  double dt;        /* sample spacing (s)                                  */
+  double c;         /* propagation speed (m/s)                             */
+  double taper_s;   /* off-specular taper s_eff (SS_TAPER)                 */
+  double sigma;     /* roughness rms height (m) (SS_ROUGH)                 */
+  double corr_len;  /* roughness correlation length (m)                    */
+  double pat_a;     /* array: n_elements; finite dipole: length_lam        */
+  double pat_b[4];  /* array: spacing_lam, field scale, q, element enabled */
+  int32_t n_samples;
+  int32_t flags;
+  int32_t n_terms;  /* D_Phi series length (SS_ROUGH)                      */
+  int32_t acf;      /* SS_ACF_*                                            */
+  int32_t pattern;  /* SS_PAT_*                                            */
+  int32_t pat_n;    /* tabulated samples / array_tapered elements          */
+} ss_surface_params;
+
+typedef struct ss_refracted_params {
+  double k0;        /* vacuum wavenumber (coherent)                        */
+  double gamma;     /* scalar target reflection coefficient               */
+  double t0, dt, c;
+  double taper_s;
+  double sigma;     /* target roughness rms height                         */
+  double corr_len;
+  double n_exp;     /* diffuse cos^n exponent                              */
+  double pat_a;
+  double pat_b[4];
+  int32_t n_samples;
+  int32_t flags;
+  int32_t n_crossed;  /* interfaces above the target (1..SS_MAX_CROSSED)   */
+  int32_t n_terms;
+  int32_t acf;
+  int32_t pattern;
+  int32_t pat_n;
+  int32_t reserved;
+} ss_refracted_params;
+
+/* Library version string. */
+const char* ss_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* ss_last_error(void);
+
+/* Number of HIP devices visible (0 when none). */
+int ss_device_count(void);
+
+/* Fill the kernel constants table (SS_CONST_COUNT float64, host memory):
+ * Weideman (1994) N = 32 rational-approximation constants for the Faddeeva
+ * function and lgamma(m + 1).  Replaces roughness.py:55-64 (_weideman_coeffs)
+ * and the gammaln call of roughness.py:172.  Host-only, no device work. */
+int ss_fill_constants(double* host_out);
+
+/* Surface-layer coherent field.  Replaces kernels/coherent.py:247-362
+ * (coherent_cluttergram) and its device program _coherent_fn (:171-244).
+ * pattern_table: tabulated [theta (pat_n) | gain (pat_n)] or array_tapered
+ * [tx (pat_n) | rx (pat_n)] float64 device array, else NULL.
+ * constants: SS_CONST_COUNT float64 device array (required with SS_ROUGH). */
+int ss_surface_field(int precision, const ss_surface_params* params,
+                     const double* facets, int64_t facet_stride,
+                     int64_t n_facets, const double* traces,
+                     int64_t trace_stride, int64_t n_traces,
+                     const double* pattern_table, const double* constants,
+                     double* field, double* dropped, int32_t* pair_bins,
+                     void* stream);
+
+/* Refracted-path field of one target interface under n_crossed interfaces
+ * (sequential two-point chain).  Replaces kernels/multilayer.py:594-770
+ * (refracted_cluttergram, refraction="sequential") and _refracted_fn
+ * (:333-591).  iface_desc: (n_crossed, SS_IDESC_COUNT) float64 device;
+ * tables: concatenated (sum ny*nx, 6) float64 [centre, normal] device;
+ * legs: (3, n_crossed + 1) float64 device rows [n, eps, att_db_per_km]. */
+int ss_refracted_field(int precision, const ss_refracted_params* params,
+                       const double* facets, int64_t facet_stride,
+                       int64_t n_facets, const double* traces,
+                       int64_t trace_stride, int64_t n_traces,
+                       const double* iface_desc, const double* tables,
+                       const double* legs, const double* pattern_table,
+                       const double* constants, double* out, double* dropped,
+                       int32_t* pair_bins, void* stream);
+
+/* Fast-time 'same' convolution of n_rows rows of n_samples complex128
+ * samples with a (2m+1)-tap complex128 pulse whose zero lag is tap m,
+ * direct summation in float64.  Replaces waveform.py:126-138
+ * (convolve_fast_time) as used by apply_waveform (:141-160).  in/out may
+ * not alias. */
+int ss_range_compress(const double* in, int64_t n_rows, int64_t n_samples,
+                      const double* pulse, int64_t m, double* out,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOUNDERSIM_HIP_H */

@@ -1,0 +1,114 @@
+"""ctypes binding of the C ABI declared in include/soundersim_hip.h.
+
+The shared library ``libsoundersim_hip.so`` is built in-tree by
+``radar_sounder_simulator_amd.build.build_library()``.  There is no CPU
+fallback: if the library is missing, every entry point raises.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsoundersim_hip.so")
+
+SS_OK = 0
+SS_PREC_FP64 = 0
+SS_PREC_MIXED = 1
+SS_SPLIT_SIDES = 0x001
+SS_INTERP_BINS = 0x002
+SS_TAPER = 0x004
+SS_D_PHI_AREA = 0x008
+SS_ROUGH = 0x010
+SS_GAMMA_FACET = 0x020
+SS_COHERENT = 0x040
+SS_CROSS_ROUGH = 0x080
+SS_DIFFUSE = 0x100
+
+PATTERNS = {"isotropic": 0, "dipole": 1, "array": 2, "array_tapered": 3,
+            "finite_dipole": 4, "tabulated": 5}
+ACFS = {"gaussian": 0, "exponential": 1}
+
+FROW = dict(CX=0, NX=3, AREA=6, E1X=7, E2X=10, GAMMA=13, PH_RE=14, PH_IM=15,
+            DIF_AMP=16, DIF_RE=17, DIF_IM=18, COUNT=19)
+TROW = dict(PX=0, UX=3, RREF=6, PV0=7, PV1=10, COUNT=13)
+IDESC = dict(NY=0, NX=1, COEF=2, MP=8, MN=11, SIGC=14, TABOFF=15, COUNT=16)
+BIN_INVALID = -2147483648
+MAX_CROSSED = 64
+MAX_TERMS = 300
+CONST_COUNT = 33 + MAX_TERMS + 1
+
+EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
+            "ss_fill_constants", "ss_surface_field", "ss_refracted_field",
+            "ss_range_compress")
+
+
+class SurfaceParams(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("t0", ctypes.c_double), ("dt", ctypes.c_double),
+                ("c", ctypes.c_double), ("taper_s", ctypes.c_double),
+                ("sigma", ctypes.c_double), ("corr_len", ctypes.c_double),
+                ("pat_a", ctypes.c_double), ("pat_b", ctypes.c_double * 4),
+                ("n_samples", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("n_terms", ctypes.c_int32), ("acf", ctypes.c_int32),
+                ("pattern", ctypes.c_int32), ("pat_n", ctypes.c_int32)]
+
+
+class RefractedParams(ctypes.Structure):
+    _fields_ = [("k0", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("t0", ctypes.c_double), ("dt", ctypes.c_double),
+                ("c", ctypes.c_double), ("taper_s", ctypes.c_double),
+                ("sigma", ctypes.c_double), ("corr_len", ctypes.c_double),
+                ("n_exp", ctypes.c_double), ("pat_a", ctypes.c_double),
+                ("pat_b", ctypes.c_double * 4),
+                ("n_samples", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("n_crossed", ctypes.c_int32), ("n_terms", ctypes.c_int32),
+                ("acf", ctypes.c_int32), ("pattern", ctypes.c_int32),
+                ("pat_n", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def load():
+    """Load (once) and return the ctypes library; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"HIP library not built: {LIB_PATH} is missing (run "
+            "radar_sounder_simulator_amd.build.build_library())")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32, dp = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
+    lib.ss_version.restype = ctypes.c_char_p
+    lib.ss_last_error.restype = ctypes.c_char_p
+    lib.ss_device_count.restype = ctypes.c_int
+    lib.ss_fill_constants.argtypes = [dp]
+    lib.ss_fill_constants.restype = ctypes.c_int
+    lib.ss_surface_field.argtypes = [i32, ctypes.POINTER(SurfaceParams), dp, i64,
+                                     i64, dp, i64, i64, dp, dp, dp, dp, dp, vp]
+    lib.ss_surface_field.restype = ctypes.c_int
+    lib.ss_refracted_field.argtypes = [i32, ctypes.POINTER(RefractedParams), dp,
+                                       i64, i64, dp, i64, i64, dp, dp, dp, dp,
+                                       dp, dp, dp, dp, vp]
+    lib.ss_refracted_field.restype = ctypes.c_int
+    lib.ss_range_compress.argtypes = [dp, i64, i64, dp, i64, dp, vp]
+    lib.ss_range_compress.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != SS_OK:
+        msg = load().ss_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def constants_host():
+    """The kernel constants table (Weideman coefficients, lgamma) as numpy."""
+    import numpy as np
+    out = np.zeros(CONST_COUNT, np.float64)
+    check(load().ss_fill_constants(out.ctypes.data), "ss_fill_constants")
+    return out

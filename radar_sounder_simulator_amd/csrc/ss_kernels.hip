@@ -1,0 +1,645 @@
+// ss_kernels.hip -- gfx950 kernels and the C ABI of soundersim_hip.h.
+//
+// Work decomposition (surface and refracted kernels alike): one workgroup of
+// 256 lanes = one facet tile (256 consecutive facets of a Morton-ordered
+// scene, i.e. a compact 2-D patch) x one trace tile (TT traces).  Each lane
+// keeps its facet in registers; the trace tile is staged in LDS and walked
+// trace by trace.  For each trace the workgroup
+//   1. evaluates its 256 pairs,
+//   2. finds the tile's bin range with a wavefront min/max reduction
+//      (cross-lane shuffles) and one LDS exchange between the 4 waves,
+//   3. accumulates the pairs into an LDS bin window (ds_add_f64 / ds_add_f32)
+//      -- a compact patch spans a few to a few tens of bins per trace --
+//   4. flushes the occupied window slots to HBM with one float64 atomic add
+//      per slot (the flush of trace k overlaps the accumulation of trace
+//      k+1 through double-buffered windows, one barrier per trace).
+// Contributions beyond the window go straight to a global atomic; out-of-
+// window and invalid-path power accumulates per trace in LDS (dropped).
+// Block ids are remapped so each XCD gets a contiguous range of
+// (trace tile, facet tile) work, keeping a trace tile's output rows in one
+// XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "ss_refract.h"
+
+namespace ss {
+
+constexpr int kBlock = 256;
+constexpr int kWin = 512;      // LDS slots per side per trace
+constexpr int kTTSurf = 16;    // traces per workgroup, surface
+constexpr int kTTRefr = 8;     // traces per workgroup, refracted
+
+thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+__device__ inline int wave_min_i(int v) {
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline int wave_max_i(int v) {
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+struct KCommon {
+  const double* facets;
+  long fs;
+  long nf;
+  const double* traces;
+  long ts;
+  long nt;
+  double* field;
+  double* dropped;
+  int* pair_bins;
+  int n_ftiles, n_ttiles;
+  long per_xcd;
+  int n_seg;
+};
+
+// Work id of this block (XCD-aware remap; -1 when idle).
+__device__ inline long work_id(const KCommon& k) {
+  long bid = blockIdx.x;
+  long w = (bid & 7) * k.per_xcd + (bid >> 3);
+  long total = (long)k.n_ftiles * k.n_ttiles;
+  return w < total ? w : -1;
+}
+
+template <typename ACC, int TT>
+struct Shared {
+  TraceRec tr[TT];
+  ACC win[2][2 * kWin][2];
+  int wmin[2][4];
+  int wmax[2][4];
+  double drop[TT];
+};
+
+template <typename ACC, int TT>
+__device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int buf,
+                                    long t, int base, int span, int n_samples,
+                                    bool split) {
+  if (span <= 0) return;
+  int w = span < kWin ? span : kWin;
+  int nside = split ? 2 : 1;
+  for (int s = threadIdx.x; s < nside * w; s += kBlock) {
+    int side = s / w, slot = s - side * w;
+    ACC* v = sh.win[buf][side * kWin + slot];
+    ACC re = v[0], im = v[1];
+    if (re != (ACC)0 || im != (ACC)0) {
+      long idx = (t * k.n_seg + (long)side * n_samples + base + slot) * 2;
+      if (re != (ACC)0) atomicAdd(&k.field[idx], (double)re);
+      if (im != (ACC)0) atomicAdd(&k.field[idx + 1], (double)im);
+      v[0] = (ACC)0;
+      v[1] = (ACC)0;
+    }
+  }
+}
+
+// Trace loop shared by both kernel families.  PAIR(t, o) evaluates this
+// lane's facet against trace record t into o.
+template <typename ACC, int TT, typename PairFn>
+__device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
+                                  bool live, long f, int n_samples, bool split,
+                                  PairFn pair) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long tt_idx = w / k.n_ftiles;
+  const long t_base = tt_idx * TT;
+  const int nT = (int)((k.nt - t_base) < TT ? (k.nt - t_base) : TT);
+  for (int i = tid; i < nT * SS_TROW_COUNT; i += kBlock) {
+    int tt = i / SS_TROW_COUNT, r = i - tt * SS_TROW_COUNT;
+    reinterpret_cast<double*>(&sh.tr[tt])[r] = k.traces[(long)r * k.ts + t_base + tt];
+  }
+  for (int i = tid; i < 2 * 2 * kWin; i += kBlock) {
+    ACC* v = sh.win[i / (2 * kWin)][i % (2 * kWin)];
+    v[0] = (ACC)0;
+    v[1] = (ACC)0;
+  }
+  if (tid < TT) sh.drop[tid] = 0.0;
+  __syncthreads();
+  int prev_base = 0, prev_span = 0;
+  for (int it = 0; it < nT; ++it) {
+    const int buf = it & 1;
+    PairOut o;
+    o.nparts = 0;
+    o.valid = false;
+    if (live) pair(sh.tr[it], o);
+    int lmin = INT_MAX, lmax = INT_MIN;
+    if (live && o.valid) {
+      for (int p = 0; p < o.nparts; ++p) {
+        int b = o.bin[p];
+        if (b >= 0 && b < n_samples) {
+          lmin = min(lmin, b);
+          lmax = max(lmax, b);
+        }
+      }
+    }
+    lmin = wave_min_i(lmin);
+    lmax = wave_max_i(lmax);
+    if (lane == 0) {
+      sh.wmin[buf][wave] = lmin;
+      sh.wmax[buf][wave] = lmax;
+    }
+    __syncthreads();
+    if (it > 0)
+      flush_window<ACC, TT>(sh, k, buf ^ 1, t_base + it - 1, prev_base, prev_span,
+                            n_samples, split);
+    int base = min(min(sh.wmin[buf][0], sh.wmin[buf][1]),
+                   min(sh.wmin[buf][2], sh.wmin[buf][3]));
+    int top = max(max(sh.wmax[buf][0], sh.wmax[buf][1]),
+                  max(sh.wmax[buf][2], sh.wmax[buf][3]));
+    int span = base <= top ? top - base + 1 : 0;
+    const long t = t_base + it;
+    if (live) {
+      if (!o.valid) {
+        if (o.pwr_invalid != 0.0) atomicAdd(&sh.drop[it], o.pwr_invalid);
+      } else {
+        for (int p = 0; p < o.nparts; ++p) {
+          int b = o.bin[p];
+          double re = o.re[p], im = o.im[p];
+          if (b >= 0 && b < n_samples) {
+            int off = b - base;
+            int side = split ? o.right : 0;
+            if (off < kWin) {
+              atomicAdd(&sh.win[buf][side * kWin + off][0], (ACC)re);
+              atomicAdd(&sh.win[buf][side * kWin + off][1], (ACC)im);
+            } else {
+              long idx = (t * k.n_seg + (long)side * n_samples + b) * 2;
+              if (re != 0.0) atomicAdd(&k.field[idx], re);
+              if (im != 0.0) atomicAdd(&k.field[idx + 1], im);
+            }
+          } else {
+            double pw = re * re + im * im;
+            if (pw != 0.0) atomicAdd(&sh.drop[it], pw);
+          }
+        }
+      }
+      if (k.pair_bins) k.pair_bins[t * k.nf + f] = o.raw_bin;
+    }
+    prev_base = base;
+    prev_span = span;
+  }
+  __syncthreads();
+  if (nT > 0)
+    flush_window<ACC, TT>(sh, k, (nT - 1) & 1, t_base + nT - 1, prev_base, prev_span,
+                          n_samples, split);
+  __syncthreads();
+  if (tid < nT && sh.drop[tid] != 0.0) atomicAdd(&k.dropped[t_base + tid], sh.drop[tid]);
+}
+
+// ----------------------------------------------------------------- surface
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst sc) {
+  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  __shared__ Shared<ACC, kTTSurf> sh;
+  const long w = work_id(k);
+  if (w < 0) return;
+  const long ft = w % k.n_ftiles;
+  const long f = ft * kBlock + threadIdx.x;
+  const bool live = f < k.nf;
+  const double* F = k.facets;
+  const long fs = k.fs;
+  const long fi = live ? f : 0;
+  SurfFacet64 f64;
+  for (int j = 0; j < 3; ++j) {
+    f64.c[j] = F[(SS_FROW_CX + j) * fs + fi];
+    f64.n[j] = F[(SS_FROW_NX + j) * fs + fi];
+    f64.e1[j] = F[(SS_FROW_E1X + j) * fs + fi];
+    f64.e2[j] = F[(SS_FROW_E2X + j) * fs + fi];
+  }
+  f64.A = F[SS_FROW_AREA * fs + fi];
+  f64.gam = (sc.flags & SS_GAMMA_FACET) ? F[SS_FROW_GAMMA * fs + fi] : 0.0;
+  f64.ph_re = f64.ph_im = 0.0;
+  f64.l1 = f64.l2 = 1.0;
+  if (sc.flags & SS_ROUGH) {
+    f64.ph_re = F[SS_FROW_PH_RE * fs + fi];
+    f64.ph_im = F[SS_FROW_PH_IM * fs + fi];
+    double l1 = sqrt((f64.e1[0] * f64.e1[0] + f64.e1[1] * f64.e1[1]) + f64.e1[2] * f64.e1[2]);
+    double l2 = sqrt((f64.e2[0] * f64.e2[0] + f64.e2[1] * f64.e2[1]) + f64.e2[2] * f64.e2[2]);
+    f64.l1 = l1 > 0.0 ? l1 : 1.0;
+    f64.l2 = l2 > 0.0 ? l2 : 1.0;
+  }
+  const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
+  if constexpr (PREC == SS_PREC_FP64) {
+    trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
+                             [&](const TraceRec& t, PairOut& o) {
+                               surface_pair_fp64(f64, t, sc, o);
+                             });
+  } else {
+    SurfFacet32 f32;
+    for (int j = 0; j < 3; ++j) {
+      f32.c[j] = f64.c[j];
+      f32.n[j] = (float)f64.n[j];
+      f32.e1[j] = (float)f64.e1[j];
+      f32.e2[j] = (float)f64.e2[j];
+    }
+    f32.A = (float)f64.A;
+    f32.gam = (float)f64.gam;
+    f32.ph_re = (float)f64.ph_re;
+    f32.ph_im = (float)f64.ph_im;
+    f32.l1 = (float)f64.l1;
+    f32.l2 = (float)f64.l2;
+    trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
+                             [&](const TraceRec& t, PairOut& o) {
+                               surface_pair_mixed(f32, nullptr, t, sc, o);
+                             });
+  }
+}
+
+// Per-trace reference-phase fold-back exp(-j ((2 k r_ref) mod 2 pi)), float64
+// surface path (kernels/coherent.py:311-313, 359).
+__global__ void fold_kernel(double* field, const double* traces, long ts, long nt,
+                            int n_seg, double k) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt * n_seg) return;
+  long t = i / n_seg;
+  double rref = traces[(long)SS_TROW_RREF * ts + t];
+  double th;
+  {
+#pragma clang fp contract(off)
+    th = fmod(2.0 * k * rref, kTwoPi);
+  }
+  double c = cos(-th), s = sin(-th);
+  double a = field[2 * i], b = field[2 * i + 1];
+  field[2 * i] = a * c - b * s;
+  field[2 * i + 1] = a * s + b * c;
+}
+
+// --------------------------------------------------------------- refracted
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc) {
+  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  __shared__ Shared<ACC, kTTRefr> sh;
+  const long w = work_id(k);
+  if (w < 0) return;
+  const long ft = w % k.n_ftiles;
+  const long f = ft * kBlock + threadIdx.x;
+  const bool live = f < k.nf;
+  const double* F = k.facets;
+  const long fs = k.fs;
+  const long fi = live ? f : 0;
+  RefrFacet fr;
+  for (int j = 0; j < 3; ++j) {
+    fr.c[j] = F[(SS_FROW_CX + j) * fs + fi];
+    fr.n[j] = F[(SS_FROW_NX + j) * fs + fi];
+    fr.e1[j] = F[(SS_FROW_E1X + j) * fs + fi];
+    fr.e2[j] = F[(SS_FROW_E2X + j) * fs + fi];
+  }
+  fr.A = F[SS_FROW_AREA * fs + fi];
+  fr.gam = (rc.flags & SS_GAMMA_FACET) ? F[SS_FROW_GAMMA * fs + fi] : 0.0;
+  fr.ph_re = fr.ph_im = 0.0;
+  fr.l1 = fr.l2 = 1.0;
+  if (rc.flags & SS_ROUGH) {
+    fr.ph_re = F[SS_FROW_PH_RE * fs + fi];
+    fr.ph_im = F[SS_FROW_PH_IM * fs + fi];
+    double l1 = sqrt((fr.e1[0] * fr.e1[0] + fr.e1[1] * fr.e1[1]) + fr.e1[2] * fr.e1[2]);
+    double l2 = sqrt((fr.e2[0] * fr.e2[0] + fr.e2[1] * fr.e2[1]) + fr.e2[2] * fr.e2[2]);
+    fr.l1 = l1 > 0.0 ? l1 : 1.0;
+    fr.l2 = l2 > 0.0 ? l2 : 1.0;
+  }
+  fr.dif = fr.dph_re = fr.dph_im = 0.0;
+  if (rc.flags & SS_DIFFUSE) {
+    fr.dif = F[SS_FROW_DIF_AMP * fs + fi];
+    fr.dph_re = F[SS_FROW_DIF_RE * fs + fi];
+    fr.dph_im = F[SS_FROW_DIF_IM * fs + fi];
+  }
+  const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
+  if constexpr (PREC == SS_PREC_FP64) {
+    trace_loop<ACC, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
+                             [&](const TraceRec& t, PairOut& o) {
+                               refracted_pair_fp64(fr, t, rc, o);
+                             });
+  } else {
+    trace_loop<ACC, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
+                             [&](const TraceRec& t, PairOut& o) {
+                               if (!refracted_pair_mixed(fr, t, rc, o))
+                                 refracted_pair_fp64(fr, t, rc, o);
+                             });
+  }
+}
+
+// ------------------------------------------------------- range compression
+constexpr int kTapChunk = 512;
+
+__global__ __launch_bounds__(kBlock) void compress_kernel(const double2* __restrict__ in,
+                                                          long n, const double2* __restrict__ pulse,
+                                                          long m, double2* __restrict__ out) {
+  __shared__ double2 sx[kBlock + kTapChunk];
+  __shared__ double2 sp[kTapChunk];
+  const long row = blockIdx.y;
+  const long i0 = (long)blockIdx.x * kBlock;
+  const int tid = threadIdx.x;
+  const long ntaps = 2 * m + 1;
+  double ar = 0.0, ai = 0.0;
+  const double2* x = in + row * n;
+  for (long j0 = 0; j0 < ntaps; j0 += kTapChunk) {
+    int cnt = (int)((ntaps - j0) < kTapChunk ? (ntaps - j0) : kTapChunk);
+    long lo = i0 + m - j0 - cnt + 1;
+    for (int s = tid; s < kBlock + cnt - 1; s += kBlock) {
+      long xi = lo + s;
+      sx[s] = (xi >= 0 && xi < n) ? x[xi] : make_double2(0.0, 0.0);
+    }
+    for (int s = tid; s < cnt; s += kBlock) sp[s] = pulse[j0 + s];
+    __syncthreads();
+    const int base = tid + cnt - 1;
+    for (int jj = 0; jj < cnt; ++jj) {
+      double2 pj = sp[jj];
+      double2 xv = sx[base - jj];
+      ar = fma(pj.x, xv.x, ar);
+      ar = fma(-pj.y, xv.y, ar);
+      ai = fma(pj.x, xv.y, ai);
+      ai = fma(pj.y, xv.x, ai);
+    }
+    __syncthreads();
+  }
+  if (i0 + tid < n) out[row * n + i0 + tid] = make_double2(ar, ai);
+}
+
+__global__ void fill_kernel(double* p, long n, double v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+static int fill(double* p, long n, hipStream_t st) {
+  if (n <= 0) return SS_OK;
+  long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, 0.0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("fill: ") + hipGetErrorString(e));
+  return SS_OK;
+}
+
+static PatternArgs make_pattern(int kind, int n, double a, const double b[4],
+                                const double* table) {
+  PatternArgs p;
+  p.kind = kind;
+  p.n = n;
+  p.a = a;
+  for (int i = 0; i < 4; ++i) p.b[i] = b[i];
+  p.table = table;
+  return p;
+}
+
+static int check_pattern(int kind, int n, const double* table) {
+  if (kind < SS_PAT_ISOTROPIC || kind > SS_PAT_TABULATED)
+    return fail(SS_EINVAL, "unknown antenna pattern kind");
+  if (kind == SS_PAT_TABULATED || kind == SS_PAT_ARRAY_TAPERED) {
+    if (!table) return fail(SS_EINVAL, "pattern table required for this pattern kind");
+    if (n < 2 || n > SS_MAX_PATTERN_TABLE)
+      return fail(SS_ERANGE, "pattern table length out of range [2, 4096]");
+  }
+  return SS_OK;
+}
+
+static KCommon make_common(const double* facets, int64_t fs, int64_t nf,
+                           const double* traces, int64_t ts, int64_t nt, double* field,
+                           double* dropped, int32_t* pair_bins, int tt, int n_seg) {
+  KCommon k;
+  k.facets = facets;
+  k.fs = fs;
+  k.nf = nf;
+  k.traces = traces;
+  k.ts = ts;
+  k.nt = nt;
+  k.field = field;
+  k.dropped = dropped;
+  k.pair_bins = pair_bins;
+  k.n_ftiles = (int)((nf + kBlock - 1) / kBlock);
+  k.n_ttiles = (int)((nt + tt - 1) / tt);
+  long total = (long)k.n_ftiles * k.n_ttiles;
+  k.per_xcd = (total + 7) / 8;
+  k.n_seg = n_seg;
+  return k;
+}
+
+}  // namespace ss
+
+using namespace ss;
+
+extern "C" {
+
+const char* ss_version(void) { return "soundersim-hip 0.1 (gfx950)"; }
+
+const char* ss_last_error(void) { return g_last_error.c_str(); }
+
+int ss_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int ss_fill_constants(double* out) {
+  if (!out) return fail(SS_EINVAL, "null output");
+  // Weideman (1994): L = sqrt(N/sqrt(2)), samples of exp(-t^2)(L^2 + t^2) at
+  // t = L tan(k pi / 2M), M = 2N, FFT -> polynomial coefficients.
+  const int N = 32, M = 2 * N;
+  const double L = sqrt(N / sqrt(2.0));
+  double f[2 * M];
+  f[0] = 0.0;
+  for (int i = 1; i < 2 * M; ++i) {
+    int kk = -M + i;  // k = -M+1 .. M-1
+    double t = L * tan(kPi * kk / (2.0 * M));
+    f[i] = exp(-t * t) * (L * L + t * t);
+  }
+  // fftshift then real part of DFT / (2M)
+  double g[2 * M];
+  for (int i = 0; i < 2 * M; ++i) g[i] = f[(i + M) % (2 * M)];
+  double a[2 * M];
+  for (int j = 0; j < 2 * M; ++j) {
+    double acc = 0.0;
+    for (int i = 0; i < 2 * M; ++i) acc += g[i] * cos(-2.0 * kPi * (double)i * j / (2.0 * M));
+    a[j] = acc / (2.0 * M);
+  }
+  out[SS_CONST_WEIDEMAN_L] = L;
+  for (int i = 0; i < N; ++i) out[SS_CONST_WEIDEMAN_A + i] = a[N - i];  // a[1..N] reversed
+  for (int m = 0; m <= SS_MAX_TERMS; ++m) out[SS_CONST_LGAMMA + m] = lgamma((double)m + 1.0);
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_surface_field(int precision, const ss_surface_params* prm, const double* facets,
+                     int64_t facet_stride, int64_t n_facets, const double* traces,
+                     int64_t trace_stride, int64_t n_traces, const double* pattern_table,
+                     const double* constants, double* field, double* dropped,
+                     int32_t* pair_bins, void* stream) {
+  if (!prm) return fail(SS_EINVAL, "null params");
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED)
+    return fail(SS_EINVAL, "precision must be SS_PREC_FP64 or SS_PREC_MIXED");
+  if (n_facets < 0 || n_traces < 0) return fail(SS_EINVAL, "negative sizes");
+  if (prm->n_samples <= 0) return fail(SS_EINVAL, "n_samples must be > 0");
+  if (!(prm->dt > 0.0) || !(prm->c > 0.0)) return fail(SS_EINVAL, "dt and c must be > 0");
+  if (n_traces > 0 && (!field || !dropped || !traces))
+    return fail(SS_EINVAL, "null trace/output pointer");
+  if (n_facets > 0 && !facets) return fail(SS_EINVAL, "null facets");
+  if (facet_stride < n_facets || trace_stride < n_traces)
+    return fail(SS_EINVAL, "stride smaller than count");
+  if ((prm->flags & SS_ROUGH)) {
+    if (!constants) return fail(SS_EINVAL, "roughness needs the constants table");
+    if (prm->n_terms < 0 || prm->n_terms > SS_MAX_TERMS)
+      return fail(SS_ERANGE, "n_terms out of range [0, 300]");
+    if (prm->acf == SS_ACF_EXPONENTIAL && !(prm->flags & SS_D_PHI_AREA))
+      return fail(SS_EINVAL, "roughness acf='exponential' requires the area-only D_Phi");
+  }
+  int rcp = check_pattern(prm->pattern, prm->pat_n, pattern_table);
+  if (rcp) return rcp;
+  const bool split = (prm->flags & SS_SPLIT_SIDES) != 0;
+  const int n_seg = (split ? 2 : 1) * prm->n_samples;
+  if ((long)n_seg * 2 > (long)INT_MAX) return fail(SS_ERANGE, "n_samples too large");
+  hipStream_t st = (hipStream_t)stream;
+  int r = fill(field, (long)n_traces * n_seg * 2, st);
+  if (r) return r;
+  r = fill(dropped, n_traces, st);
+  if (r) return r;
+  if (n_traces == 0 || n_facets == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  KCommon k = make_common(facets, facet_stride, n_facets, traces, trace_stride, n_traces,
+                          field, dropped, pair_bins, kTTSurf, n_seg);
+  SurfaceConst sc;
+  sc.k = prm->k;
+  sc.gamma = prm->gamma;
+  sc.t0 = prm->t0;
+  sc.dt = prm->dt;
+  sc.c = prm->c;
+  sc.taper_s = prm->taper_s;
+  sc.sigma = prm->sigma;
+  sc.lc = prm->corr_len;
+  sc.n_samples = prm->n_samples;
+  sc.flags = prm->flags;
+  sc.n_terms = prm->n_terms;
+  sc.acf = prm->acf;
+  sc.pat = make_pattern(prm->pattern, prm->pat_n, prm->pat_a, prm->pat_b, pattern_table);
+  sc.cst = constants;
+  sc.k_over_pi = prm->k / kPi;
+  sc.kfac = prm->k / kTwoPi;
+  sc.k_bin = 2.0 / (prm->c * prm->dt);
+  sc.t0_dt = prm->t0 / prm->dt;
+  sc.cyc_per_m = prm->k / kPi;
+  dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
+  if (precision == SS_PREC_FP64)
+    hipLaunchKernelGGL(surface_kernel<SS_PREC_FP64>, grid, block, 0, st, k, sc);
+  else
+    hipLaunchKernelGGL(surface_kernel<SS_PREC_MIXED>, grid, block, 0, st, k, sc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("surface launch: ") + hipGetErrorString(e));
+  if (precision == SS_PREC_FP64) {
+    long n = (long)n_traces * n_seg;
+    hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, field,
+                       traces, (long)trace_stride, (long)n_traces, n_seg, prm->k);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(SS_EHIP, std::string("fold launch: ") + hipGetErrorString(e));
+  }
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_refracted_field(int precision, const ss_refracted_params* prm, const double* facets,
+                       int64_t facet_stride, int64_t n_facets, const double* traces,
+                       int64_t trace_stride, int64_t n_traces, const double* iface_desc,
+                       const double* tables, const double* legs, const double* pattern_table,
+                       const double* constants, double* out, double* dropped,
+                       int32_t* pair_bins, void* stream) {
+  if (!prm) return fail(SS_EINVAL, "null params");
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED)
+    return fail(SS_EINVAL, "precision must be SS_PREC_FP64 or SS_PREC_MIXED");
+  if (n_facets < 0 || n_traces < 0) return fail(SS_EINVAL, "negative sizes");
+  if (prm->n_samples <= 0) return fail(SS_EINVAL, "n_samples must be > 0");
+  if (!(prm->dt > 0.0) || !(prm->c > 0.0)) return fail(SS_EINVAL, "dt and c must be > 0");
+  if (prm->n_crossed < 1 || prm->n_crossed > SS_MAX_CROSSED)
+    return fail(SS_ERANGE, "n_crossed out of range [1, 64]");
+  if (!iface_desc || !tables || !legs) return fail(SS_EINVAL, "null interface data");
+  if (n_traces > 0 && (!out || !dropped || !traces))
+    return fail(SS_EINVAL, "null trace/output pointer");
+  if (n_facets > 0 && !facets) return fail(SS_EINVAL, "null facets");
+  if (facet_stride < n_facets || trace_stride < n_traces)
+    return fail(SS_EINVAL, "stride smaller than count");
+  const bool coherent = (prm->flags & SS_COHERENT) != 0;
+  const int feat = SS_ROUGH | SS_CROSS_ROUGH | SS_TAPER | SS_D_PHI_AREA | SS_GAMMA_FACET | SS_DIFFUSE;
+  if (!coherent && (prm->flags & feat))
+    return fail(SS_EINVAL, "roughness, grazing fix, per-facet gamma and diffuse require coherent mode");
+  if (prm->flags & SS_ROUGH) {
+    if (!constants) return fail(SS_EINVAL, "roughness needs the constants table");
+    if (prm->n_terms < 0 || prm->n_terms > SS_MAX_TERMS)
+      return fail(SS_ERANGE, "n_terms out of range [0, 300]");
+    if (prm->acf == SS_ACF_EXPONENTIAL && !(prm->flags & SS_D_PHI_AREA))
+      return fail(SS_EINVAL, "roughness acf='exponential' requires the area-only D_Phi");
+  }
+  int rcp = check_pattern(prm->pattern, prm->pat_n, pattern_table);
+  if (rcp) return rcp;
+  const bool split = (prm->flags & SS_SPLIT_SIDES) != 0;
+  const int n_seg = (split ? 2 : 1) * prm->n_samples;
+  hipStream_t st = (hipStream_t)stream;
+  const long out_len = (long)n_traces * n_seg * 2;
+  int r = fill(out, out_len, st);
+  if (r) return r;
+  r = fill(dropped, n_traces, st);
+  if (r) return r;
+  if (n_traces == 0 || n_facets == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  KCommon k = make_common(facets, facet_stride, n_facets, traces, trace_stride, n_traces, out,
+                          dropped, pair_bins, kTTRefr, n_seg);
+  RefrConst rc;
+  rc.k0 = prm->k0;
+  rc.gamma = prm->gamma;
+  rc.t0 = prm->t0;
+  rc.dt = prm->dt;
+  rc.c = prm->c;
+  rc.taper_s = prm->taper_s;
+  rc.sigma = prm->sigma;
+  rc.lc = prm->corr_len;
+  rc.n_exp = prm->n_exp;
+  rc.n_samples = prm->n_samples;
+  rc.flags = prm->flags;
+  rc.n_crossed = prm->n_crossed;
+  rc.n_terms = prm->n_terms;
+  rc.acf = prm->acf;
+  rc.pat = make_pattern(prm->pattern, prm->pat_n, prm->pat_a, prm->pat_b, pattern_table);
+  rc.cst = constants;
+  rc.desc = iface_desc;
+  rc.tables = tables;
+  rc.legs = legs;
+  rc.k_bin = 2.0 / (prm->c * prm->dt);
+  rc.t0_dt = prm->t0 / prm->dt;
+  rc.cyc_per_m = prm->k0 / kPi;
+  dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
+  if (precision == SS_PREC_FP64)
+    hipLaunchKernelGGL(refracted_kernel<SS_PREC_FP64>, grid, block, 0, st, k, rc);
+  else
+    hipLaunchKernelGGL(refracted_kernel<SS_PREC_MIXED>, grid, block, 0, st, k, rc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("refracted launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_range_compress(const double* in, int64_t n_rows, int64_t n_samples, const double* pulse,
+                      int64_t m, double* out, void* stream) {
+  if (n_rows < 0 || n_samples < 0 || m < 0) return fail(SS_EINVAL, "negative sizes");
+  if (n_rows == 0 || n_samples == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  if (!in || !out || !pulse) return fail(SS_EINVAL, "null pointer");
+  if (in == out) return fail(SS_EINVAL, "in and out must not alias");
+  if (n_rows > 65535) return fail(SS_ERANGE, "n_rows > 65535: split the call");
+  hipStream_t st = (hipStream_t)stream;
+  dim3 grid((unsigned)((n_samples + kBlock - 1) / kBlock), (unsigned)n_rows);
+  hipLaunchKernelGGL(compress_kernel, grid, dim3(kBlock), 0, st, (const double2*)in,
+                     (long)n_samples, (const double2*)pulse, (long)m, (double2*)out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("compress launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,563 @@
+// ss_math.h -- per-pair math of the coherent facet kernel, shared by the
+// gfx950 kernels and by the CPU harness in tests/native (which compiles the
+// very same functions with g++ to check them against the oracle).
+//
+// Two precision tiers:
+//   *_fp64   float64 throughout, evaluation order written out and FMA
+//            contraction disabled, so that bins, validity and crossing
+//            geometry reproduce the FP64 oracle bit for bit.
+//   *_mixed  float64 only where the error budget needs it (facet-platform
+//            range, bin position, carrier-phase cycles, Snell crossing
+//            polish); float32 for amplitudes, sinc, antenna, roughness.  A
+//            pair whose bin position or lookup cell lies within a guard band
+//            of a rounding boundary is recomputed on the float64 path, so
+//            bins and masks stay identical to the float64 result.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define SS_HD __host__ __device__ inline
+#else
+#define SS_HD inline
+#endif
+
+#include "../../include/soundersim_hip.h"
+
+namespace ss {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 6.28318530717958647692;
+constexpr double kInvSqrtPi = 0.56418958354775628695;
+constexpr double kSqrtPi = 1.77245385090551602730;
+constexpr double kTiny = 1e-30;
+constexpr double kEps64 = 2.220446049250313e-16;
+constexpr double kCMin = 1e-9;
+constexpr float kLog2E = 1.4426950408889634f;
+constexpr float kLn2 = 0.6931471805599453f;
+
+// ---------------------------------------------------------------------------
+// float32 hardware primitives (gfx950: v_sin/v_cos take revolutions,
+// v_rcp/v_rsq/v_sqrt/v_exp/v_log are the ~1 ulp transcendental units).
+// The host harness uses libm so the same code runs on the CPU.
+// ---------------------------------------------------------------------------
+SS_HD float f_sin_rev(float x) {  // sin(2 pi x), |x| <= 0.5
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sinf(x);
+#else
+  return sinf((float)kTwoPi * x);
+#endif
+}
+SS_HD float f_cos_rev(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_cosf(x);
+#else
+  return cosf((float)kTwoPi * x);
+#endif
+}
+SS_HD float f_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+SS_HD float f_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+SS_HD float f_rsq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
+SS_HD float f_exp2(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_exp2f(x);
+#else
+  return exp2f(x);
+#endif
+}
+SS_HD float f_log2(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_logf(x);
+#else
+  return log2f(x);
+#endif
+}
+SS_HD float f_exp(float x) { return f_exp2(x * kLog2E); }
+SS_HD float f_log(float x) { return f_log2(x) * kLn2; }
+
+// sin(2 pi x) for any float x: exact reduction by the nearest integer.
+SS_HD float f_sin_rev_any(float x) { return f_sin_rev(x - rintf(x)); }
+SS_HD float f_cos_rev_any(float x) { return f_cos_rev(x - rintf(x)); }
+
+// normalised sinc sin(pi x)/(pi x) in float32.
+SS_HD float f_sinc(float x) {
+  float ax = fabsf(x);
+  if (ax < 1e-4f) {
+    float y = (float)kPi * x;
+    return 1.0f - y * y * (1.0f / 6.0f);
+  }
+  float h = 0.5f * x;
+  float s = f_sin_rev(h - rintf(h));  // sin(pi x)
+  return s * f_rcp((float)kPi * x);
+}
+
+// np.sinc in float64: y = pi * (x == 0 ? 1e-20 : x); sin(y)/y.
+SS_HD double d_sinc(double x) {
+#pragma clang fp contract(off)
+  double y = kPi * (x == 0.0 ? 1.0e-20 : x);
+  return sin(y) / y;
+}
+
+// ---------------------------------------------------------------------------
+// Faddeeva w(z), Im z >= 0 (Weideman 1994, N = 32 coefficients a[] highest
+// power first, constants table layout SS_CONST_*), and D_Phi (Gerekos 2023
+// Eq 21) in either precision.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct cplx {
+  T re, im;
+};
+template <typename T>
+SS_HD cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+SS_HD cplx<T> cdiv(cplx<T> a, cplx<T> b) {
+  T den = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+}
+
+template <typename T>
+SS_HD cplx<T> faddeeva(T x, T y, const double* cst) {
+  const T L = (T)cst[SS_CONST_WEIDEMAN_L];
+  // iz = i z = -y + i x; d = L - iz; zz = (L + iz)/d
+  cplx<T> d = {L + y, -x};
+  cplx<T> num = {L - y, x};
+  cplx<T> zz = cdiv(num, d);
+  cplx<T> p = {(T)0, (T)0};
+  for (int i = 0; i < 32; ++i) {
+    p = cmul(p, zz);
+    p.re += (T)cst[SS_CONST_WEIDEMAN_A + i];
+  }
+  cplx<T> dd = cmul(d, d);
+  cplx<T> t1 = cdiv(cplx<T>{(T)2 * p.re, (T)2 * p.im}, dd);
+  cplx<T> t2 = cdiv(cplx<T>{(T)kInvSqrtPi, (T)0}, d);
+  return {t1.re + t2.re, t1.im + t2.im};
+}
+
+template <typename T>
+SS_HD T t_exp(T x) { return exp(x); }
+template <>
+SS_HD float t_exp<float>(float x) { return f_exp(x); }
+template <typename T>
+SS_HD T t_sqrt(T x) { return sqrt(x); }
+template <>
+SS_HD float t_sqrt<float>(float x) { return f_sqrt(x); }
+template <typename T>
+SS_HD T t_log(T x) { return log(x); }
+template <>
+SS_HD float t_log<float>(float x) { return f_log(x); }
+template <typename T>
+SS_HD T t_cos(T x) { return cos(x); }
+template <>
+SS_HD float t_cos<float>(float x) { return f_cos_rev_any(x * (float)(1.0 / kTwoPi)); }
+template <typename T>
+SS_HD T t_sin(T x) { return sin(x); }
+template <>
+SS_HD float t_sin<float>(float x) { return f_sin_rev_any(x * (float)(1.0 / kTwoPi)); }
+
+// One F factor of Eqs 22-24 through the scaled Faddeeva combination.
+template <typename T>
+SS_HD T f_factor(T m, T a0, T edge, T l, const double* cst) {
+  T sq = t_sqrt<T>(m);
+  T x = a0 * l / ((T)2 * sq);
+  T y = edge * sq / l;
+  T ex2 = t_exp<T>(-x * x);
+  T ey2 = t_exp<T>(-y * y);
+  cplx<T> wz = faddeeva<T>(x, y, cst);
+  cplx<T> wx = faddeeva<T>(x, (T)0, cst);
+  T ang = a0 * edge;
+  cplx<T> rot = {t_cos<T>(ang), t_sin<T>(ang)};
+  // real(1j * (x + 1j y) * rot * w_z)
+  cplx<T> u = {-y, x};
+  cplx<T> v = cmul(cmul(u, rot), wz);
+  T t1 = -y * ex2 - ey2 * v.re;
+  T t2 = x * wx.im;
+  return (T)1 - ey2 * t_cos<T>(ang) + (T)kSqrtPi * (t1 - t2);
+}
+
+// D_Phi, clamped at 0 (exactly 0 when (sigma K)^2 == 0).
+template <typename T>
+SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
+              bool area_only, int acf, const double* cst) {
+  T sk = sigma * K;
+  T x = sk * sk;
+  if (!(x > (T)0)) return (T)0;
+  T logx = t_log<T>(x);
+  T acc = (T)0;
+  T ab2 = A0 * A0 + B0 * B0;
+  for (int m = 1; m <= n_terms; ++m) {
+    T mf = (T)m;
+    T logp = mf * logx - (T)cst[SS_CONST_LGAMMA + m] - x;
+    T term;
+    if (area_only && acf == SS_ACF_EXPONENTIAL) {
+      T kb2 = ab2;
+      T q = (l / mf) * (l / mf);
+      T base = (T)1 + kb2 * q;
+      // base^-1.5
+      T rb = (T)1 / t_sqrt<T>(base);
+      term = t_exp<T>(logp) * ((T)kTwoPi * q * (rb * rb * rb)) * Lx * Ly;
+    } else if (area_only) {
+      term = t_exp<T>(logp - ab2 * (l * l) / ((T)4 * mf)) *
+             ((T)kPi * (l * l) / mf) * Lx * Ly;
+    } else {
+      T l4 = (l * l) * (l * l);
+      term = t_exp<T>(logp) * (l4 / (mf * mf)) * f_factor<T>(mf, A0, Lx, l, cst) *
+             f_factor<T>(mf, B0, Ly, l, cst);
+    }
+    acc += term;
+  }
+  return acc > (T)0 ? acc : (T)0;
+}
+
+// ---------------------------------------------------------------------------
+// Antenna: one-way FIELD gain at the departure direction (antenna.py:204-258)
+// ---------------------------------------------------------------------------
+struct PatternArgs {
+  int kind;
+  int n;               // tabulated samples / tapered elements
+  double a;            // n_elements / length_lam
+  double b[4];         // spacing_lam, field_scale, q, enabled
+  const double* table; // tabulated: theta[n] | gain[n]; tapered: tx[n] | rx[n]
+};
+
+template <typename T>
+SS_HD T t_pow(T x, T y) { return pow(x, y); }
+template <>
+SS_HD float t_pow<float>(float x, float y) {
+  if (y == 0.0f) return 1.0f;
+  if (x <= 0.0f) return 0.0f;
+  return f_exp2(y * f_log2(x));
+}
+template <typename T>
+SS_HD T t_acos(T x) { return acos(x); }
+
+template <typename T>
+SS_HD T element_field(T c, T q, bool enabled) {
+  if (!enabled) return (T)1;
+  if (!(c >= (T)0)) return (T)0;
+  T cl = c > (T)1 ? (T)1 : c;
+  return t_pow<T>(cl, q / (T)2);
+}
+
+// dh: departure unit vector; pv0/pv1: trace pattern vectors.
+template <typename T>
+SS_HD T antenna_gain(const PatternArgs& pa, const T dh[3], const T pv0[3],
+                     const T pv1[3]) {
+  switch (pa.kind) {
+    case SS_PAT_DIPOLE: {
+      T ca = (dh[0] * pv0[0] + dh[1] * pv0[1]) + dh[2] * pv0[2];
+      T s2 = (T)1 - ca * ca;
+      T s = t_sqrt<T>(s2 > (T)1e-12 ? s2 : (T)1e-12);
+      return t_cos<T>((T)(kPi / 2.0) * ca) / s;
+    }
+    case SS_PAT_FINITE_DIPOLE: {
+      T kh = (T)kPi * (T)pa.a;
+      T ca = (dh[0] * pv0[0] + dh[1] * pv0[1]) + dh[2] * pv0[2];
+      T s2 = (T)1 - ca * ca;
+      T s = t_sqrt<T>(s2 > (T)1e-12 ? s2 : (T)1e-12);
+      T ckh = t_cos<T>(kh);
+      return fabs(t_cos<T>(kh * ca) - ckh) / (s * ((T)1 - ckh));
+    }
+    case SS_PAT_ARRAY: {
+      T u = (dh[0] * pv0[0] + dh[1] * pv0[1]) + dh[2] * pv0[2];
+      T ct = (dh[0] * pv1[0] + dh[1] * pv1[1]) + dh[2] * pv1[2];
+      T na = (T)pa.a;
+      T x = (T)kPi * (T)pa.b[0] * u;
+      T sx = t_sin<T>(x);
+      T af;
+      if (fabs(sx) < (T)1e-5)
+        af = t_cos<T>(na * x) / t_cos<T>(x);
+      else
+        af = t_sin<T>(na * x) / (na * sx);
+      T ef = element_field<T>(ct, (T)pa.b[2], pa.b[3] != 0.0);
+      return (T)pa.b[1] * ef * fabs(af);
+    }
+    case SS_PAT_ARRAY_TAPERED: {
+      T u = (dh[0] * pv0[0] + dh[1] * pv0[1]) + dh[2] * pv0[2];
+      T ct = (dh[0] * pv1[0] + dh[1] * pv1[1]) + dh[2] * pv1[2];
+      int n = pa.n;
+      T ctx = 0, stx = 0, crx = 0, srx = 0, wt = 0, wr = 0;
+      T half = (T)(n - 1) / (T)2;
+      for (int i = 0; i < n; ++i) {
+        T mi = (T)i - half;
+        T ph = ((T)kTwoPi * (T)pa.b[0]) * u * mi;
+        T c = t_cos<T>(ph), s = t_sin<T>(ph);
+        T w1 = (T)pa.table[i], w2 = (T)pa.table[n + i];
+        ctx += c * w1;
+        stx += s * w1;
+        crx += c * w2;
+        srx += s * w2;
+        wt += w1;
+        wr += w2;
+      }
+      T aft = t_sqrt<T>(ctx * ctx + stx * stx) / wt;
+      T afr = t_sqrt<T>(crx * crx + srx * srx) / wr;
+      T ef = element_field<T>(ct, (T)pa.b[2], pa.b[3] != 0.0);
+      return (T)pa.b[1] * ef * t_sqrt<T>(aft * afr);
+    }
+    case SS_PAT_TABULATED: {
+      T ct = (dh[0] * pv0[0] + dh[1] * pv0[1]) + dh[2] * pv0[2];
+      ct = ct < (T)-1 ? (T)-1 : (ct > (T)1 ? (T)1 : ct);
+      T th = t_acos<T>(ct);
+      const double* xp = pa.table;
+      const double* fp = pa.table + pa.n;
+      int n = pa.n;
+      if (!(th > (T)xp[0])) return (T)fp[0];
+      if (!(th < (T)xp[n - 1])) return (T)fp[n - 1];
+      int lo = 0, hi = n - 1;
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if ((T)xp[mid] <= th)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      T x0 = (T)xp[lo], x1 = (T)xp[hi];
+      T y0 = (T)fp[lo], y1 = (T)fp[hi];
+      T slope = (y1 - y0) / (x1 - x0);
+      return slope * (th - x0) + y0;
+    }
+    default:
+      return (T)1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-trace and per-facet records
+// ---------------------------------------------------------------------------
+struct TraceRec {
+  double p[3];
+  double u[3];
+  double rref;
+  double pv0[3];
+  double pv1[3];
+};
+
+struct SurfaceConst {
+  double k, gamma, t0, dt, c, taper_s, sigma, lc;
+  int n_samples, flags, n_terms, acf;
+  PatternArgs pat;
+  const double* cst;
+  // derived (host-filled)
+  double k_over_pi, kfac;        // k/pi, k/(2 pi)
+  double k_bin, t0_dt;           // 2/(c dt), t0/dt   (mixed fast bin)
+  double cyc_per_m;              // 2/lambda = k/pi   (mixed phase cycles)
+};
+
+struct SurfFacet64 {
+  double c[3], n[3], A, e1[3], e2[3], gam, ph_re, ph_im, l1, l2;
+};
+
+struct SurfFacet32 {
+  double c[3];  // float64 centre (range and bin need it)
+  float n[3], A, e1[3], e2[3], gam, ph_re, ph_im, l1, l2;
+};
+
+// Result of one pair: up to two binned parts (interp bins) + side.
+struct PairOut {
+  double re[2], im[2];
+  int bin[2];      // floor bins (part 1 = bin + 1 with interp)
+  int nparts;
+  int right;       // split sides: 1 when the facet is right of track
+  int raw_bin;     // diagnostic: floor bin of the pair (or SS_BIN_INVALID)
+  bool valid;      // path validity (refracted); always true for the surface
+  double pwr_invalid;  // |contrib|^2 when the path is invalid
+};
+
+SS_HD double exact_bin_pos(double r_or_opl, double c, double t0, double dt) {
+#pragma clang fp contract(off)
+  double twtt = 2.0 * r_or_opl / c;
+  return (twtt - t0) / dt;
+}
+
+// ---------------------------------------------------------------------------
+// Surface pair, float64 (kernels/coherent.py:107-168, 216-236)
+// ---------------------------------------------------------------------------
+SS_HD void surface_pair_fp64(const SurfFacet64& f, const TraceRec& t,
+                             const SurfaceConst& sc, PairOut& o) {
+#pragma clang fp contract(off)
+  const int flags = sc.flags;
+  double dx = t.p[0] - f.c[0], dy = t.p[1] - f.c[1], dz = t.p[2] - f.c[2];
+  double r = sqrt((dx * dx + dy * dy) + dz * dz);
+  double cs = ((dx * f.n[0] + dy * f.n[1]) + dz * f.n[2]) / r;
+  double rx = dx / r, ry = dy / r, rz = dz / r;
+  double d1 = (rx * f.e1[0] + ry * f.e1[1]) + rz * f.e1[2];
+  double d2 = (rx * f.e2[0] + ry * f.e2[1]) + rz * f.e2[2];
+  double s1 = d_sinc(d1 * (sc.k / kPi));
+  double s2 = d_sinc(d2 * (sc.k / kPi));
+  double gam = (flags & SS_GAMMA_FACET) ? f.gam : sc.gamma;
+  double amp = (sc.k / kTwoPi) * gam * cs * f.A * s1 * s2 / (r * r);
+  if (flags & SS_TAPER) {
+    double cc = cs < 1e-9 ? 1e-9 : (cs > 1.0 ? 1.0 : cs);
+    double c2 = cc * cc;
+    amp = amp * exp(-(1.0 - c2) / (c2 * (2.0 * sc.taper_s * sc.taper_s)));
+  }
+  double phase = (-2.0 * sc.k) * (r - t.rref);
+  double cre = amp, cim = 0.0;
+  if (flags & SS_ROUGH) {
+    double kk = 2.0 * sc.k * cs;
+    double dp = 0.0;
+    if (f.A > 0.0)
+      dp = d_phi<double>(sc.sigma, sc.lc, kk, 2.0 * sc.k * d1 / f.l1,
+                         2.0 * sc.k * d2 / f.l2, f.l1, f.l2, sc.n_terms,
+                         (flags & SS_D_PHI_AREA) != 0, sc.acf, sc.cst);
+    double amp_i = f.A > 0.0 ? (sc.k / kTwoPi) * gam * cs / (r * r) * sqrt(dp) : 0.0;
+    double sk = sc.sigma * kk;
+    double matt = exp(-0.5 * (sk * sk));
+    cre = amp * matt + amp_i * f.ph_re;
+    cim = amp_i * f.ph_im;
+  }
+  double sp = sin(phase), cp = cos(phase);
+  // j (cre + j cim) (cp + j sp)
+  double re = -(cre * sp + cim * cp);
+  double im = cre * cp - cim * sp;
+  if (sc.pat.kind != SS_PAT_ISOTROPIC) {
+    double dh[3] = {-dx / r, -dy / r, -dz / r};
+    double g = antenna_gain<double>(sc.pat, dh, t.pv0, t.pv1);
+    double g2 = g * g;
+    re *= g2;
+    im *= g2;
+  }
+  double twtt = 2.0 * r / sc.c;
+  double pos = (twtt - sc.t0) / sc.dt;
+  double fb = floor(pos);
+  int b = (fb < -2147483647.0) ? (-2147483647) : (fb > 2147483646.0 ? 2147483646 : (int)fb);
+  o.raw_bin = b;
+  o.valid = true;
+  o.pwr_invalid = 0.0;
+  if (flags & SS_INTERP_BINS) {
+    double w = pos - fb;
+    o.nparts = 2;
+    o.re[0] = re * (1.0 - w);
+    o.im[0] = im * (1.0 - w);
+    o.bin[0] = b;
+    o.re[1] = re * w;
+    o.im[1] = im * w;
+    o.bin[1] = b + 1;
+  } else {
+    o.nparts = 1;
+    o.re[0] = re;
+    o.im[0] = im;
+    o.bin[0] = b;
+  }
+  o.right = (((-dx) * t.u[0] + (-dy) * t.u[1]) + (-dz) * t.u[2]) > 0.0 ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Surface pair, mixed precision
+// ---------------------------------------------------------------------------
+constexpr double kBinGuard = 1e-7;   // bin units; fast-path error is ~1e-10
+
+SS_HD void surface_pair_mixed(const SurfFacet32& f, const SurfFacet64* f64,
+                              const TraceRec& t, const SurfaceConst& sc,
+                              PairOut& o) {
+  const int flags = sc.flags;
+  double dx = t.p[0] - f.c[0], dy = t.p[1] - f.c[1], dz = t.p[2] - f.c[2];
+  double r2 = dx * dx + dy * dy + dz * dz;
+  float r0 = f_sqrt((float)r2);
+  float ir0 = f_rcp(r0);
+  double res = fma(-(double)r0, (double)r0, r2);
+  float corr = (float)res * (0.5f * ir0);
+  double r = (double)r0 + (double)corr;
+  // bin position (fast) with exact fallback near a bin edge
+  double pos = fma(r, sc.k_bin, -sc.t0_dt);
+  double fb = floor(pos);
+  double fr = pos - fb;
+  if (fr < kBinGuard || fr > 1.0 - kBinGuard) {
+    pos = exact_bin_pos(sqrt((dx * dx + dy * dy) + dz * dz), sc.c, sc.t0, sc.dt);
+    fb = floor(pos);
+  }
+  // carrier phase in cycles: -2 k r = -2 pi (r * 2/lambda)
+  double cyc = r * sc.cyc_per_m;
+  float ph = (float)(cyc - rint(cyc));
+  float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+  float cs = (dxf * f.n[0] + dyf * f.n[1] + dzf * f.n[2]) * ir0;
+  float rx = dxf * ir0, ry = dyf * ir0, rz = dzf * ir0;
+  float d1 = rx * f.e1[0] + ry * f.e1[1] + rz * f.e1[2];
+  float d2 = rx * f.e2[0] + ry * f.e2[1] + rz * f.e2[2];
+  float kop = (float)sc.k_over_pi;
+  float s1 = f_sinc(d1 * kop);
+  float s2 = f_sinc(d2 * kop);
+  float gam = (flags & SS_GAMMA_FACET) ? f.gam : (float)sc.gamma;
+  float kg = (float)sc.kfac * gam;
+  float ir2 = ir0 * ir0;
+  float amp = kg * cs * f.A * s1 * s2 * ir2;
+  if (flags & SS_TAPER) {
+    float cc = cs < 1e-9f ? 1e-9f : (cs > 1.0f ? 1.0f : cs);
+    float c2 = cc * cc;
+    float ts = (float)sc.taper_s;
+    amp *= f_exp(-(1.0f - c2) * f_rcp(c2 * (2.0f * ts * ts)));
+  }
+  float cre = amp, cim = 0.0f;
+  if (flags & SS_ROUGH) {
+    float k2 = 2.0f * (float)sc.k;
+    float kk = k2 * cs;
+    float dp = 0.0f;
+    if (f.A > 0.0f)
+      dp = d_phi<float>((float)sc.sigma, (float)sc.lc, kk, k2 * d1 * f_rcp(f.l1),
+                        k2 * d2 * f_rcp(f.l2), f.l1, f.l2, sc.n_terms,
+                        (flags & SS_D_PHI_AREA) != 0, sc.acf, sc.cst);
+    float amp_i = f.A > 0.0f ? kg * cs * ir2 * f_sqrt(dp) : 0.0f;
+    float sk = (float)sc.sigma * kk;
+    float matt = f_exp(-0.5f * sk * sk);
+    cre = amp * matt + amp_i * f.ph_re;
+    cim = amp_i * f.ph_im;
+  }
+  // e^{j phase}, phase = -2 pi ph: cos = cos(2 pi ph), sin = -sin(2 pi ph)
+  float sp = -f_sin_rev(ph), cp = f_cos_rev(ph);
+  float re = -(cre * sp + cim * cp);
+  float im = cre * cp - cim * sp;
+  if (sc.pat.kind != SS_PAT_ISOTROPIC) {
+    float dh[3] = {-rx, -ry, -rz};
+    float pv0[3] = {(float)t.pv0[0], (float)t.pv0[1], (float)t.pv0[2]};
+    float pv1[3] = {(float)t.pv1[0], (float)t.pv1[1], (float)t.pv1[2]};
+    float g = antenna_gain<float>(sc.pat, dh, pv0, pv1);
+    float g2 = g * g;
+    re *= g2;
+    im *= g2;
+  }
+  int b = (fb < -2147483647.0) ? (-2147483647) : (fb > 2147483646.0 ? 2147483646 : (int)fb);
+  o.raw_bin = b;
+  o.valid = true;
+  o.pwr_invalid = 0.0;
+  if (flags & SS_INTERP_BINS) {
+    float w = (float)(pos - fb);
+    o.nparts = 2;
+    o.re[0] = re * (1.0f - w);
+    o.im[0] = im * (1.0f - w);
+    o.bin[0] = b;
+    o.re[1] = re * w;
+    o.im[1] = im * w;
+    o.bin[1] = b + 1;
+  } else {
+    o.nparts = 1;
+    o.re[0] = re;
+    o.im[0] = im;
+    o.bin[0] = b;
+  }
+  o.right = (-(dxf * (float)t.u[0] + dyf * (float)t.u[1] + dzf * (float)t.u[2])) > 0.0f ? 1 : 0;
+  (void)f64;
+}
+
+}  // namespace ss

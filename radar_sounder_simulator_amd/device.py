@@ -1,0 +1,360 @@
+"""Device-resident scenes and kernel launches (torch tensors as HBM buffers).
+
+A ``DeviceFacets`` holds one interface's facets in HBM as the float64
+structure-of-arrays the kernels read (rows of include/soundersim_hip.h
+SS_FROW_*), in Morton (Z-curve) order of the horizontal centre coordinates,
+so that 256 consecutive facets -- one workgroup tile -- form a compact 2-D
+patch whose pairs fall in a narrow band of fast-time bins for any trace.
+The facet sum is order independent up to rounding (the reference itself
+reorders facets along track, ``kernels/geometry.py:48-60``).
+
+A ``DeviceTraces`` holds the per-trace records (positions, cross-track
+vectors, surface reference range, antenna pattern vectors).
+
+``surface_field`` / ``refracted_field`` / ``range_compress`` enqueue the
+kernels on a stream and return device tensors; nothing is copied to the
+host.  These are the allocation-aware building blocks under the reference-
+signature shims in ``kernels.py``, the batch runner and the serving loop.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import FROW, IDESC, TROW
+
+
+def _dev(device):
+    """torch device for HBM buffers ("cpu" is accepted for host-side
+    inspection of the buffer layout; the kernels need a GPU tensor)."""
+    if device is None:
+        return torch.device("cuda", torch.cuda.current_device())
+    return torch.device(device)
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def morton_order(xy):
+    """Stable argsort of 2-D points along a Z-order curve (16 bits/axis)."""
+    xy = np.asarray(xy, np.float64)
+    if len(xy) == 0:
+        return np.zeros(0, np.int64)
+    lo = xy.min(0)
+    span = np.maximum(xy.max(0) - lo, 1e-12)
+    q = np.minimum(((xy - lo) / span * 65535.0).astype(np.uint64), 65535)
+
+    def spread(v):
+        v = v & np.uint64(0xFFFF)
+        v = (v | (v << np.uint64(8))) & np.uint64(0x00FF00FF)
+        v = (v | (v << np.uint64(4))) & np.uint64(0x0F0F0F0F)
+        v = (v | (v << np.uint64(2))) & np.uint64(0x33333333)
+        v = (v | (v << np.uint64(1))) & np.uint64(0x55555555)
+        return v
+
+    code = spread(q[:, 0]) | (spread(q[:, 1]) << np.uint64(1))
+    return np.argsort(code, kind="stable")
+
+
+_CONST_CACHE = {}
+
+
+def constants(device=None):
+    """Kernel constants table on ``device`` (cached)."""
+    dev = _dev(device)
+    key = str(dev)
+    if key not in _CONST_CACHE:
+        _CONST_CACHE[key] = torch.as_tensor(_lib.constants_host(), device=dev)
+    return _CONST_CACHE[key]
+
+
+class DeviceFacets:
+    """One interface's facets resident in HBM (float64 SoA, Morton order)."""
+
+    def __init__(self, centers, normals, areas, e1, e2, *, gamma=None,
+                 phasors=None, diffuse_amp=None, diffuse_phasors=None,
+                 device=None, order=None):
+        centers = np.asarray(centers, np.float64)
+        n = centers.shape[0]
+        self.n = n
+        self.order = morton_order(centers[:, :2]) if order is None else np.asarray(order)
+        o = self.order
+        buf = np.zeros((FROW["COUNT"], max(n, 1)), np.float64)
+        buf[0:3, :n] = centers[o].T
+        buf[3:6, :n] = np.asarray(normals, np.float64)[o].T
+        buf[6, :n] = np.asarray(areas, np.float64)[o]
+        buf[7:10, :n] = np.asarray(e1, np.float64)[o].T
+        buf[10:13, :n] = np.asarray(e2, np.float64)[o].T
+        self.has_gamma = gamma is not None
+        if gamma is not None:
+            buf[13, :n] = np.asarray(gamma, np.float64)[o]
+        self.has_phasors = phasors is not None
+        if phasors is not None:
+            ph = np.asarray(phasors)[o]
+            buf[14, :n] = ph.real
+            buf[15, :n] = ph.imag
+        self.has_diffuse = diffuse_amp is not None
+        if diffuse_amp is not None:
+            buf[16, :n] = np.asarray(diffuse_amp, np.float64)[o]
+            dp = np.asarray(diffuse_phasors)[o]
+            buf[17, :n] = dp.real
+            buf[18, :n] = dp.imag
+        self.device = _dev(device)
+        self.buf = torch.as_tensor(buf, device=self.device)
+        self.stride = buf.shape[1]
+
+    @classmethod
+    def from_facets(cls, f, **kw):
+        return cls(f.centers, f.normals, f.areas, f.e1, f.e2, **kw)
+
+    def nbytes(self):
+        return self.buf.numel() * 8
+
+    def unpermute_pairs(self, pairs):
+        """(T, F) per-pair array in device order -> caller's facet order."""
+        out = np.empty_like(pairs)
+        out[:, self.order] = pairs
+        return out
+
+
+class DeviceTraces:
+    """Per-trace records resident in HBM (rows SS_TROW_*)."""
+
+    def __init__(self, positions, u_ct, r_ref=None, pv=None, device=None):
+        pos = np.asarray(positions, np.float64)
+        n = pos.shape[0]
+        self.n = n
+        buf = np.zeros((TROW["COUNT"], max(n, 1)), np.float64)
+        buf[0:3, :n] = pos.T
+        buf[3:6, :n] = np.asarray(u_ct, np.float64).reshape(n, 3).T
+        if r_ref is not None:
+            buf[6, :n] = r_ref
+        if pv is not None:
+            pv = np.asarray(pv, np.float64)
+            if pv.ndim == 3:          # arrays: (T, 2, 3)
+                buf[7:10, :n] = pv[:, 0].T
+                buf[10:13, :n] = pv[:, 1].T
+            else:
+                buf[7:10, :n] = pv.reshape(n, 3).T
+        self.device = _dev(device)
+        self.buf = torch.as_tensor(buf, device=self.device)
+        self.stride = buf.shape[1]
+
+
+def pattern_setup(pattern, device):
+    """(kind id, n, a, b[4], table tensor or None) from pattern_args output."""
+    if pattern is None or pattern[0] == "isotropic":
+        return 0, 0, 0.0, (0.0, 0.0, 0.0, 0.0), None
+    kind, pv, pa, pb = pattern
+    kid = _lib.PATTERNS[kind]
+    b = [0.0, 0.0, 0.0, 0.0]
+    table = None
+    n = 0
+    a = 0.0
+    if kind == "array":
+        a = float(pa)
+        b = [float(v) for v in np.asarray(pb, np.float64)[:4]]
+    elif kind == "array_tapered":
+        w = np.asarray(pa, np.float64)
+        n = w.shape[1]
+        table = torch.as_tensor(np.concatenate([w[0], w[1]]), device=device)
+        b = [float(v) for v in np.asarray(pb, np.float64)[:4]]
+    elif kind == "finite_dipole":
+        a = float(pa)
+    elif kind == "tabulated":
+        xp = np.asarray(pa, np.float64)
+        fp = np.asarray(pb, np.float64)
+        n = len(xp)
+        table = torch.as_tensor(np.concatenate([xp, fp]), device=device)
+    return kid, n, a, tuple(b), table
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def surface_field(facets, traces, *, k, gamma, t0, dt, n_samples, c,
+                  precision="mixed", split_sides=False, interp_bins=False,
+                  pattern=None, roughness=None, taper_s=None, d_phi_area=False,
+                  out=None, dropped=None, pair_bins=False, stream=None):
+    """Enqueue the surface kernel.  Returns (field (T, n_seg, 2) float64,
+    dropped (T,) float64[, pair_bins (T, F) int32]) device tensors.
+
+    ``roughness``: None or (sigma, corr_len, n_terms, acf); the phasors live
+    in ``facets``.  ``pattern``: None or a ``pattern_args`` tuple whose pv
+    rows are already in ``traces``."""
+    dev = facets.device
+    flags = 0
+    flags |= _lib.SS_SPLIT_SIDES if split_sides else 0
+    flags |= _lib.SS_INTERP_BINS if interp_bins else 0
+    flags |= _lib.SS_TAPER if taper_s is not None else 0
+    flags |= _lib.SS_D_PHI_AREA if d_phi_area else 0
+    flags |= _lib.SS_GAMMA_FACET if facets.has_gamma else 0
+    p = _lib.SurfaceParams()
+    p.k, p.gamma, p.t0, p.dt, p.c = float(k), float(gamma), float(t0), float(dt), float(c)
+    p.taper_s = 0.0 if taper_s is None else float(taper_s)
+    p.n_samples = int(n_samples)
+    if roughness is not None:
+        sigma, lc, n_terms, acf = roughness
+        if not facets.has_phasors:
+            raise ValueError("roughness needs facets uploaded with phasors")
+        flags |= _lib.SS_ROUGH
+        p.sigma, p.corr_len, p.n_terms = float(sigma), float(lc), int(n_terms)
+        p.acf = _lib.ACFS[acf]
+    kid, pn, pa, pb, table = pattern_setup(pattern, dev)
+    p.pattern, p.pat_n, p.pat_a = kid, pn, pa
+    for i in range(4):
+        p.pat_b[i] = pb[i]
+    p.flags = flags
+    n_seg = (2 if split_sides else 1) * int(n_samples)
+    T = traces.n
+    if out is None:
+        out = torch.empty((T, n_seg, 2), dtype=torch.float64, device=dev)
+    if dropped is None:
+        dropped = torch.empty((T,), dtype=torch.float64, device=dev)
+    pb_t = (torch.empty((T, facets.n), dtype=torch.int32, device=dev)
+            if pair_bins else None)
+    cst = constants(dev)
+    prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
+    rc = _lib.load().ss_surface_field(
+        prec, ctypes.byref(p), _ptr(facets.buf), facets.stride, facets.n,
+        _ptr(traces.buf), traces.stride, T, _ptr(table), _ptr(cst), _ptr(out),
+        _ptr(dropped), _ptr(pb_t), _stream_handle(stream))
+    _lib.check(rc, "ss_surface_field")
+    if pair_bins:
+        return out, dropped, pb_t, table
+    return out, dropped, table
+
+
+class DeviceInterfaces:
+    """Crossed-interface lookup data for the refracted kernel: per interface
+    an (ny*nx, 6) float64 [centre, normal] table in grid order, the affine
+    (x, y, 1) -> (row, col) fit and the area-weighted mean plane
+    (``kernels/multilayer.py:269-283``, float64)."""
+
+    def __init__(self, crossed, crossed_sigma=None, device=None):
+        self.device = _dev(device)
+        n = len(crossed)
+        if n < 1 or n > _lib.MAX_CROSSED:
+            raise ValueError(f"need 1..{_lib.MAX_CROSSED} crossed interfaces, got {n}")
+        desc = np.zeros((n, IDESC["COUNT"]), np.float64)
+        tabs = []
+        off = 0
+        for i, f in enumerate(crossed):
+            ny, nx = f.grid_shape
+            tab = np.concatenate([f.centers.reshape(ny * nx, 3),
+                                  f.normals.reshape(ny * nx, 3)], axis=1)
+            a = np.column_stack([f.centers[:, 0], f.centers[:, 1],
+                                 np.ones(len(f.centers))])
+            coef, *_ = np.linalg.lstsq(a, f.cell.astype(np.float64), rcond=None)
+            w = (f.areas / f.areas.sum())[:, None]
+            mp = (f.centers * w).sum(0)
+            mn = (f.normals * w).sum(0)
+            mn = mn / np.linalg.norm(mn)
+            desc[i, IDESC["NY"]] = ny
+            desc[i, IDESC["NX"]] = nx
+            desc[i, IDESC["COEF"]:IDESC["COEF"] + 6] = coef.ravel()
+            desc[i, IDESC["MP"]:IDESC["MP"] + 3] = mp
+            desc[i, IDESC["MN"]:IDESC["MN"] + 3] = mn
+            desc[i, IDESC["SIGC"]] = 0.0 if crossed_sigma is None else crossed_sigma[i]
+            desc[i, IDESC["TABOFF"]] = off
+            off += ny * nx
+            tabs.append(tab)
+        self.n = n
+        self.desc = torch.as_tensor(desc, device=self.device)
+        self.tables = torch.as_tensor(np.ascontiguousarray(np.concatenate(tabs)),
+                                      device=self.device)
+        self.has_sigma = crossed_sigma is not None
+
+    def nbytes(self):
+        return (self.desc.numel() + self.tables.numel()) * 8
+
+
+def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
+                    n_samples, c, gamma=0.0, k0=None, precision="mixed",
+                    split_sides=False, pattern=None, roughness=None,
+                    taper_s=None, d_phi_area=False, n_exp=None, out=None,
+                    dropped=None, pair_bins=False, stream=None):
+    """Enqueue the refracted kernel (sequential chain).  Returns (out
+    (T, n_seg, 2) float64, dropped (T,)[, pair_bins]) device tensors."""
+    dev = target.device
+    coherent = mode == "coherent"
+    flags = _lib.SS_COHERENT if coherent else 0
+    flags |= _lib.SS_SPLIT_SIDES if split_sides else 0
+    flags |= _lib.SS_TAPER if taper_s is not None else 0
+    flags |= _lib.SS_D_PHI_AREA if d_phi_area else 0
+    flags |= _lib.SS_GAMMA_FACET if target.has_gamma else 0
+    flags |= _lib.SS_CROSS_ROUGH if ifaces.has_sigma else 0
+    flags |= _lib.SS_DIFFUSE if target.has_diffuse else 0
+    p = _lib.RefractedParams()
+    p.k0 = 0.0 if k0 is None else float(k0)
+    p.gamma, p.t0, p.dt, p.c = float(gamma), float(t0), float(dt), float(c)
+    p.taper_s = 0.0 if taper_s is None else float(taper_s)
+    p.n_exp = 0.0 if n_exp is None else float(n_exp)
+    p.n_samples = int(n_samples)
+    p.n_crossed = ifaces.n
+    if roughness is not None:
+        sigma, lc, n_terms, acf = roughness
+        flags |= _lib.SS_ROUGH
+        p.sigma, p.corr_len, p.n_terms = float(sigma), float(lc), int(n_terms)
+        p.acf = _lib.ACFS[acf]
+    kid, pn, pa, pb, table = pattern_setup(pattern, dev)
+    p.pattern, p.pat_n, p.pat_a = kid, pn, pa
+    for i in range(4):
+        p.pat_b[i] = pb[i]
+    p.flags = flags
+    eps = np.asarray(eps_leg, np.float64)
+    legs = np.stack([np.sqrt(eps), eps, np.asarray(att_leg, np.float64)])
+    if legs.shape[1] != ifaces.n + 1:
+        raise ValueError(f"need {ifaces.n + 1} legs, got {legs.shape[1]}")
+    legs_t = torch.as_tensor(np.ascontiguousarray(legs), device=dev)
+    n_seg = (2 if split_sides else 1) * int(n_samples)
+    T = traces.n
+    if out is None:
+        out = torch.empty((T, n_seg, 2), dtype=torch.float64, device=dev)
+    if dropped is None:
+        dropped = torch.empty((T,), dtype=torch.float64, device=dev)
+    pb_t = (torch.empty((T, target.n), dtype=torch.int32, device=dev)
+            if pair_bins else None)
+    cst = constants(dev)
+    prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
+    rc = _lib.load().ss_refracted_field(
+        prec, ctypes.byref(p), _ptr(target.buf), target.stride, target.n,
+        _ptr(traces.buf), traces.stride, T, _ptr(ifaces.desc), _ptr(ifaces.tables),
+        _ptr(legs_t), _ptr(table), _ptr(cst), _ptr(out), _ptr(dropped), _ptr(pb_t),
+        _stream_handle(stream))
+    _lib.check(rc, "ss_refracted_field")
+    keep = (legs_t, table)
+    if pair_bins:
+        return out, dropped, pb_t, keep
+    return out, dropped, keep
+
+
+def range_compress(field, pulse, m, out=None, stream=None):
+    """'Same' convolution of every row of a (..., n, 2) float64 device tensor
+    (complex interleaved) with a complex pulse of 2m+1 taps."""
+    n = field.shape[-2]
+    rows = field.numel() // (2 * n)
+    p = np.asarray(pulse)
+    pt = torch.as_tensor(np.stack([p.real, p.imag], -1).astype(np.float64)
+                         if np.iscomplexobj(p) else
+                         np.stack([p, np.zeros_like(p)], -1).astype(np.float64),
+                         device=field.device).contiguous()
+    src = field.contiguous()
+    if out is None:
+        out = torch.empty_like(src)
+    lib = _lib.load()
+    for r0 in range(0, rows, 65535):
+        nr = min(65535, rows - r0)
+        off = r0 * n * 2
+        rc = lib.ss_range_compress(ctypes.c_void_p(src.data_ptr() + off * 8), nr, n,
+                                   _ptr(pt), int(m),
+                                   ctypes.c_void_p(out.data_ptr() + off * 8),
+                                   _stream_handle(stream))
+        _lib.check(rc, "ss_range_compress")
+    return out, pt

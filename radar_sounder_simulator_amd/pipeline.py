@@ -1,0 +1,101 @@
+"""Resident scene: one workload's facets, interfaces and traces kept in HBM,
+evaluated end to end (every target layer + range compression) on a stream.
+
+This is the simulate() hot path of the reference (``simulate.py:263-395``:
+surface through the coherent kernel, each deeper interface through the
+refracted kernel, stacked per layer, then ``apply_waveform``) without the
+host-side scene construction, with all buffers allocated once.
+"""
+
+import numpy as np
+import torch
+
+from . import device as dv
+from .kernels import _reference_range
+from .waveform import compressed_pulse
+
+
+class ResidentScene:
+    """Workload uploaded to ``device``; ``run(precision)`` enqueues one pass."""
+
+    def __init__(self, wl, device=None):
+        self.wl = wl
+        self.device = dv._dev(device)
+        tr = wl.track
+        pv = None if wl.pattern is None else wl.pattern[1]
+        surf = wl.layers[0].facets
+        self.traces = dv.DeviceTraces(tr.positions, tr.u_ct,
+                                      _reference_range(tr.positions, surf.centers),
+                                      pv, device=self.device)
+        self.layers = []
+        for L in wl.layers:
+            ph = None if L.roughness is None else L.roughness[2]
+            g = np.asarray(L.gamma, np.float64)
+            fac = dv.DeviceFacets.from_facets(L.facets, gamma=g if g.ndim else None,
+                                              phasors=ph, device=self.device)
+            ifc = (dv.DeviceInterfaces(L.crossed, device=self.device)
+                   if L.crossed else None)
+            rough = (None if L.roughness is None else
+                     (L.roughness[0], L.roughness[1], L.roughness[3],
+                      L.roughness[4] if len(L.roughness) > 4 else "gaussian"))
+            self.layers.append((L, fac, ifc, rough))
+        T = tr.positions.shape[0]
+        n_seg = (2 if wl.split_sides else 1) * wl.n_samples
+        nl = len(wl.layers)
+        self.raw = torch.zeros((nl, T, n_seg, 2), dtype=torch.float64, device=self.device)
+        self.out = torch.zeros_like(self.raw)
+        self.dropped = torch.zeros((nl, T), dtype=torch.float64, device=self.device)
+        self.pulse = None
+        if wl.waveform is not None and wl.waveform.kind != "delta":
+            p, m = compressed_pulse(wl.waveform.bandwidth, wl.waveform.pulse_length,
+                                    wl.dt, wl.waveform.window, wl.waveform.construction,
+                                    wl.waveform.tukey_alpha)
+            self.pulse = (p, m)
+        self._keep = []
+
+    def pairs_per_layer(self):
+        T = self.traces.n
+        return [T * fac.n for _, fac, _, _ in self.layers]
+
+    def nbytes(self):
+        b = self.traces.buf.numel() * 8 + self.raw.numel() * 16 + self.dropped.numel() * 8
+        for _, fac, ifc, _ in self.layers:
+            b += fac.nbytes() + (ifc.nbytes() if ifc is not None else 0)
+        return b
+
+    def run_layer(self, j, precision, stream=None):
+        wl = self.wl
+        L, fac, ifc, rough = self.layers[j]
+        if ifc is None:
+            res = dv.surface_field(
+                fac, self.traces, k=wl.k0, gamma=0.0 if fac.has_gamma else L.gamma,
+                t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c,
+                precision=precision, split_sides=wl.split_sides, pattern=wl.pattern,
+                roughness=rough, taper_s=wl.taper_s, d_phi_area=wl.d_phi_area,
+                out=self.raw[j], dropped=self.dropped[j], stream=stream)
+        else:
+            res = dv.refracted_field(
+                fac, self.traces, ifc, L.eps_leg, L.att_leg, mode="coherent",
+                t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c,
+                gamma=0.0 if fac.has_gamma else L.gamma, k0=wl.k0,
+                precision=precision, split_sides=wl.split_sides, pattern=wl.pattern,
+                roughness=rough, taper_s=wl.taper_s, d_phi_area=wl.d_phi_area,
+                out=self.raw[j], dropped=self.dropped[j], stream=stream)
+        self._keep.append(res[-1])
+
+    def compress(self, stream=None):
+        if self.pulse is None:
+            self.out.copy_(self.raw)
+            return
+        p, m = self.pulse
+        n = self.wl.n_samples
+        _, pt = dv.range_compress(self.raw.view(-1, n, 2), p, m,
+                                  out=self.out.view(-1, n, 2), stream=stream)
+        self._keep.append(pt)
+
+    def run(self, precision, stream=None):
+        self._keep = []
+        for j in range(len(self.layers)):
+            self.run_layer(j, precision, stream)
+        self.compress(stream)
+        return self.out

@@ -1,0 +1,138 @@
+"""Scene data model on the host: facets, tracks and seeded synthetic surfaces.
+
+Facets follow the reference's record (``scene.py:80-101``): per rectangular
+mean-plane facet a centre, a unit +z-oriented normal, the parallelogram area
+|e1 x e2| and the two mean edge vectors, plus the source grid cell and grid
+shape (the refracted kernel's crossing lookup uses both).  Coordinates are a
+planar local frame in metres (east, north, up); the reference's WGS84/ENU
+geodesy (pyproj) is not part of this library -- callers that have geodetic
+scenes convert them to the local frame first.
+
+Corner math restates ``scene.py:158-176`` / ``compare/brute_force_layered.py
+:44-71``: from the (ny+1, nx+1) vertex lattice V, e1 = ((v01 + v11) - (v00 +
+v10)) / 2, e2 = ((v10 + v11) - (v00 + v01)) / 2, centre = mean of the four
+corners, normal = (e1 x e2)/|e1 x e2| flipped to +z.
+
+The Gaussian-ACF height field restates ``compare/haynes.py:94-112`` (circular
+FFT convolution of white noise with exp(-rho^2/a^2), a = l/sqrt(2),
+normalised by sqrt(sum g^2) so the periodic field has unit variance).
+"""
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Facets:
+    """Rectangular facets (float64, (N, 3) arrays as in the reference)."""
+
+    centers: np.ndarray
+    normals: np.ndarray
+    areas: np.ndarray
+    e1: np.ndarray
+    e2: np.ndarray
+    cell: np.ndarray = None          # (N, 2) int (row, col) source cell
+    grid_shape: tuple = None         # (ny, nx) cells
+    phase_keys: np.ndarray = None    # (N, 2) stable lattice keys
+
+    def __len__(self):
+        return int(self.centers.shape[0])
+
+    def subset(self, idx):
+        """Facets at integer/boolean index ``idx`` (grid metadata dropped)."""
+        return Facets(self.centers[idx], self.normals[idx], self.areas[idx],
+                      self.e1[idx], self.e2[idx])
+
+
+@dataclass
+class NavTrack:
+    """Per-trace platform geometry (``nav.py:9-30``)."""
+
+    positions: np.ndarray   # (T, 3)
+    u_at: np.ndarray        # (T, 3) unit along-track
+    u_ct: np.ndarray        # (T, 3) unit cross-track, right of travel
+    roll: np.ndarray = None  # (T,) radians or None
+
+
+def facets_from_vertices(V):
+    """Facets of a (ny+1, nx+1, 3) vertex lattice (corner math above)."""
+    V = np.asarray(V, dtype=np.float64)
+    v00, v01 = V[:-1, :-1], V[:-1, 1:]
+    v10, v11 = V[1:, :-1], V[1:, 1:]
+    e1 = ((v01 + v11) - (v00 + v10)) / 2.0
+    e2 = ((v10 + v11) - (v00 + v01)) / 2.0
+    centers = ((v00 + v01 + v10 + v11) / 4.0).reshape(-1, 3)
+    raw = np.cross(e1, e2).reshape(-1, 3)
+    mag = np.linalg.norm(raw, axis=1)
+    normals = raw / mag[:, None]
+    normals *= np.where(normals[:, 2] < 0.0, -1.0, 1.0)[:, None]
+    ny, nx = V.shape[0] - 1, V.shape[1] - 1
+    ci, cj = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    cell = np.stack([ci.ravel(), cj.ravel()], axis=1)
+    return Facets(centers, normals, mag, e1.reshape(-1, 3), e2.reshape(-1, 3),
+                  cell, (ny, nx), cell.astype(np.int64).copy())
+
+
+def grid_facets(heights, spacing, origin=(0.0, 0.0)):
+    """Facets of a height lattice ``heights`` (ny+1, nx+1) at ``spacing`` m,
+    centred on ``origin`` (x = columns/east, y = rows/north)."""
+    h = np.asarray(heights, dtype=np.float64)
+    ny1, nx1 = h.shape
+    xs = (np.arange(nx1) - (nx1 - 1) / 2.0) * spacing + origin[0]
+    ys = (np.arange(ny1) - (ny1 - 1) / 2.0) * spacing + origin[1]
+    X, Y = np.meshgrid(xs, ys)
+    return facets_from_vertices(np.stack([X, Y, h], axis=-1))
+
+
+def gaussian_surface(n, dx, corr_len, rng, m=None):
+    """(n, m) zero-mean unit-variance Gaussian-ACF field, C = exp(-rho^2/l^2)
+    (``compare/haynes.py:94-112``, extended to rectangular grids)."""
+    m = n if m is None else m
+    cy = np.arange(n, dtype=np.float64)
+    cy = np.minimum(cy, n - cy) * dx
+    cx = np.arange(m, dtype=np.float64)
+    cx = np.minimum(cx, m - cx) * dx
+    r2 = cy[:, None] ** 2 + cx[None, :] ** 2
+    a = corr_len / np.sqrt(2.0)
+    g = np.exp(-r2 / (a * a))
+    w = rng.standard_normal((n, m))
+    z = np.fft.irfft2(np.fft.rfft2(w) * np.fft.rfft2(g), s=(n, m))
+    return z / np.sqrt((g * g).sum())
+
+
+def straight_track(n_traces, spacing, altitude, start=None, heading=(1.0, 0.0),
+                   cross_offset=0.0):
+    """Straight level track centred on the origin: (T, 3) positions, u_at
+    from central differences and u_ct = u_at x up (``nav.py:21-30``)."""
+    hd = np.asarray(heading, np.float64)
+    hd = hd / np.linalg.norm(hd)
+    s = (np.arange(n_traces) - (n_traces - 1) / 2.0) * spacing
+    if start is not None:
+        s = s - s[0] + start
+    perp = np.array([hd[1], -hd[0]])
+    xy = s[:, None] * hd[None, :] + cross_offset * perp[None, :]
+    pos = np.column_stack([xy, np.full(n_traces, float(altitude))])
+    if n_traces > 1:
+        g = np.gradient(pos, axis=0)
+    else:
+        g = np.array([[hd[0], hd[1], 0.0]])
+    u_at = g / np.linalg.norm(g, axis=1, keepdims=True)
+    u_ct = np.cross(u_at, np.array([0.0, 0.0, 1.0]))
+    u_ct = u_ct / np.linalg.norm(u_ct, axis=1, keepdims=True)
+    return NavTrack(pos, u_at, u_ct)
+
+
+def eps_kovacs(rho_kgm3):
+    """Kovacs (1993) permittivity (1 + 0.845 rho[g/cc])^2 (``firn.py:19-21``)."""
+    return (1.0 + 0.845 * np.asarray(rho_kgm3, np.float64) / 1000.0) ** 2
+
+
+@dataclass
+class LayeredScene:
+    """Interfaces top-down plus media (one more medium than interfaces)."""
+
+    interfaces: list
+    eps: list
+    att_db_per_km: list
+    names: list = field(default_factory=list)

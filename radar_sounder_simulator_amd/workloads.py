@@ -1,0 +1,124 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+Every workload is a planar scene (metres, x east / y north / z up) with a
+straight level track along +x centred over the scene, built from seeded
+Gaussian-ACF height lattices (``scene.gaussian_surface``).  Frozen choices
+the configuration text leaves open are recorded here:
+
+* sampling: 48 MHz (dt = 1/48 MHz) wherever the rate is not given; window
+  start t0 = 2 (h - 100 m)/c so the surface nadir return lands ~32 bins in;
+* pulse length 20 us (the reference's 60 MHz instrument preset) with the
+  analytic Hann compressed pulse unless stated (config 3: Tukey(0.2));
+* "cos^2 antenna pattern": one-way POWER cos^2(theta) off the nadir
+  boresight, i.e. field gain cos(theta), as a ``tabulated`` pattern
+  (181 samples over 0..90 deg, zero beyond);
+* "-10 dB bed reflectivity": field coefficient -10^(-10/20);
+* trace spacing 25 m for the 256-trace surface+bed scene;
+* exponential sub-facet roughness implies the grazing fix (taper
+  s_eff = 0.05 + area-only D_Phi), as the reference's config validator
+  requires (``config.py:360-365``).
+"""
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import antenna as _ant
+from .physics import C, fresnel_normal
+from .roughness import n_terms_for, speckle_phasors
+from .scene import gaussian_surface, grid_facets, straight_track
+from .waveform import Waveform
+
+
+@dataclass
+class Layer:
+    """One target interface of a workload."""
+
+    name: str
+    facets: object
+    gamma: float
+    roughness: tuple = None       # (sigma, l, phasors, n_terms, acf)
+    crossed: list = field(default_factory=list)
+    eps_leg: list = None
+    att_leg: list = None
+
+
+@dataclass
+class Workload:
+    name: str
+    track: object
+    layers: list
+    k0: float
+    t0: float
+    dt: float
+    n_samples: int
+    c: float
+    pattern: tuple = None
+    taper_s: float = None
+    d_phi_area: bool = False
+    waveform: Waveform = None
+    split_sides: bool = False
+    meta: dict = field(default_factory=dict)
+
+    def pairs(self):
+        return sum(len(L.facets) for L in self.layers) * len(self.track.positions)
+
+
+def config1(n=128, n_traces=64, seed=1, n_samples=1024):
+    """128 x 128 facets at 20 m, sigma 2 m / l 200 m Gaussian heights, ice
+    eps 3.15 (Fresnel only), 20 km altitude, 64 traces at 10 m, isotropic,
+    60 MHz / 15 MHz chirp, 1024 samples at 48 MHz."""
+    rng = np.random.default_rng(seed)
+    dx = 20.0
+    h = 2.0 * gaussian_surface(n + 1, dx, 200.0, rng)
+    surf = grid_facets(h, dx)
+    alt = 20000.0
+    track = straight_track(n_traces, 10.0, alt)
+    f0 = 60e6
+    k0 = 2.0 * np.pi * f0 / C
+    dt = 1.0 / 48e6
+    return Workload(
+        name="cfg1_surface_fp64", track=track,
+        layers=[Layer("surface", surf, fresnel_normal(1.0, 3.15))],
+        k0=k0, t0=2.0 * (alt - 100.0) / C, dt=dt, n_samples=n_samples, c=C,
+        waveform=Waveform(kind="chirp", bandwidth=15e6, pulse_length=20e-6),
+        meta=dict(f0=f0, facets=f"{n}x{n}@{dx}m", seed=seed))
+
+
+def config2(n=512, n_traces=256, seed=2, n_samples=4096, trace_spacing=25.0):
+    """Surface + bed with refraction: 512^2 facets at 25 m (surface sigma 3 m
+    / l 300 m; bed 2000 m deep, sigma 300 m / l 2 km, independent seed),
+    bed -10 dB, ice eps 3.15 with 10 dB/km, exponential sub-facet roughness
+    (surface 0.2 m, bed 0.5 m, l 20 m) + grazing fix, 256 traces at 20 km,
+    cos^2 pattern, 60 MHz / 15 MHz chirp, 4096 samples at 48 MHz."""
+    rng_s = np.random.default_rng([seed, 0])
+    rng_b = np.random.default_rng([seed, 1])
+    dx = 25.0
+    hs = 3.0 * gaussian_surface(n + 1, dx, 300.0, rng_s)
+    hb = -2000.0 + 300.0 * gaussian_surface(n + 1, dx, 2000.0, rng_b)
+    surf = grid_facets(hs, dx)
+    bed = grid_facets(hb, dx)
+    alt = 20000.0
+    track = straight_track(n_traces, trace_spacing, alt)
+    f0 = 60e6
+    k0 = 2.0 * np.pi * f0 / C
+    eps_ice = 3.15
+    k_ice = k0 * np.sqrt(eps_ice)
+    nt_s = n_terms_for((2.0 * k0 * 0.2) ** 2)
+    nt_b = n_terms_for((2.0 * k_ice * 0.5) ** 2)
+    rough_s = (0.2, 20.0, speckle_phasors(len(surf), (seed, 0)), nt_s, "exponential")
+    rough_b = (0.5, 20.0, speckle_phasors(len(bed), (seed, 1)), nt_b, "exponential")
+    ant = _ant.cos_power_table(2.0)
+    pattern = _ant.pattern_args(ant, track.u_at, track.u_ct)
+    layers = [
+        Layer("surface", surf, fresnel_normal(1.0, eps_ice), rough_s),
+        Layer("bed", bed, -10.0 ** (-10.0 / 20.0), rough_b, crossed=[surf],
+              eps_leg=[1.0, eps_ice], att_leg=[0.0, 10.0]),
+    ]
+    return Workload(
+        name="cfg2_surface_bed", track=track, layers=layers, k0=k0,
+        t0=2.0 * (alt - 100.0) / C, dt=1.0 / 48e6, n_samples=n_samples, c=C,
+        pattern=pattern, taper_s=0.05, d_phi_area=True,
+        waveform=Waveform(kind="chirp", bandwidth=15e6, pulse_length=20e-6),
+        meta=dict(f0=f0, facets=f"{n}x{n}@{dx}m x2", seed=seed,
+                  n_terms=(nt_s, nt_b)))

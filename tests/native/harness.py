@@ -1,0 +1,165 @@
+"""ctypes wrapper of tests/native/pair_host.cpp (CPU build of the device pair
+math).  Builds the harness on first use into tests/native/_build/."""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from radar_sounder_simulator_amd import _lib
+from radar_sounder_simulator_amd import device as dv
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "_build", "pair_host.so")
+
+
+def build():
+    os.makedirs(os.path.dirname(SO), exist_ok=True)
+    src = os.path.join(HERE, "pair_host.cpp")
+    deps = [src] + [os.path.join(HERE, "..", "..", "radar_sounder_simulator_amd", "csrc", h)
+                    for h in ("ss_math.h", "ss_refract.h")]
+    if os.path.exists(SO) and all(os.path.getmtime(SO) >= os.path.getmtime(d) for d in deps):
+        return SO
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared",
+                    "-Wno-unknown-pragmas", "-o", SO, src], check=True)
+    return SO
+
+
+_h = None
+
+
+def lib():
+    global _h
+    if _h is None:
+        _h = ctypes.CDLL(build())
+    return _h
+
+
+def _p(a):
+    return ctypes.c_void_p(0 if a is None else a.ctypes.data)
+
+
+def _const():
+    out = np.zeros(_lib.CONST_COUNT)
+    # the table is host-computed; reproduce the library's own fill via numpy
+    from oracle.roughness import weideman_coeffs
+    import math
+    L, a = weideman_coeffs(32)
+    out[0] = L
+    out[1:33] = a
+    for m in range(_lib.MAX_TERMS + 1):
+        out[33 + m] = math.lgamma(m + 1.0)
+    return out
+
+
+def surface(prec, positions, u_ct, centers, normals, areas, e1, e2, *, k, gamma, t0, dt,
+            n_samples, c, split_sides=False, interp_bins=False, pattern=None, roughness=None,
+            taper_s=None, d_phi_area=False):
+    g = np.asarray(gamma, np.float64)
+    phasors = None if roughness is None else np.asarray(roughness[2])
+    fac = dv.DeviceFacets(centers, normals, areas, e1, e2, gamma=g if g.ndim else None,
+                          phasors=phasors, device="cpu")
+    from radar_sounder_simulator_amd.kernels import _reference_range
+    tr = dv.DeviceTraces(positions, u_ct, _reference_range(positions, centers),
+                         None if pattern is None else pattern[1], device="cpu")
+    p = _lib.SurfaceParams()
+    p.k, p.gamma, p.t0, p.dt, p.c = k, (0.0 if g.ndim else float(gamma)), t0, dt, c
+    p.taper_s = 0.0 if taper_s is None else taper_s
+    p.n_samples = n_samples
+    fl = 0
+    fl |= _lib.SS_SPLIT_SIDES if split_sides else 0
+    fl |= _lib.SS_INTERP_BINS if interp_bins else 0
+    fl |= _lib.SS_TAPER if taper_s is not None else 0
+    fl |= _lib.SS_D_PHI_AREA if d_phi_area else 0
+    fl |= _lib.SS_GAMMA_FACET if g.ndim else 0
+    if roughness is not None:
+        sig, lc, _, nt, *rest = roughness
+        fl |= _lib.SS_ROUGH
+        p.sigma, p.corr_len, p.n_terms = sig, lc, nt
+        p.acf = _lib.ACFS[rest[0] if rest else "gaussian"]
+    kid, pn, pa, pb, table = dv.pattern_setup(pattern, "cpu")
+    p.pattern, p.pat_n, p.pat_a = kid, pn, pa
+    for i in range(4):
+        p.pat_b[i] = pb[i]
+    p.flags = fl
+    T, F = len(positions), fac.n
+    n_seg = (2 if split_sides else 1) * n_samples
+    field = np.zeros((T, n_seg, 2))
+    drop = np.zeros(T)
+    bins = np.zeros((T, F), np.int32)
+    fb = fac.buf.numpy()
+    tb = tr.buf.numpy()
+    tab = None if table is None else table.numpy()
+    cst = _lib.constants_host()
+    lib().host_surface(prec, ctypes.byref(p), _p(fb), ctypes.c_long(fac.stride), ctypes.c_long(F),
+                       _p(tb), ctypes.c_long(tr.stride), ctypes.c_long(T), _p(tab), _p(cst),
+                       _p(field), _p(drop), _p(bins))
+    out = field[..., 0] + 1j * field[..., 1]
+    if split_sides:
+        out = out.reshape(T, 2, n_samples).transpose(0, 2, 1)
+    return out, drop, fac.unpermute_pairs(bins)
+
+
+def refracted(prec, positions, u_ct, target, crossed, eps_leg, att_leg, *, mode, t0, dt,
+              n_samples, c, gamma=0.0, k0=None, split_sides=False, pattern=None,
+              roughness=None, crossed_sigma=None, diffuse=None, taper_s=None,
+              d_phi_area=False):
+    g = np.asarray(gamma, np.float64)
+    phasors = None if roughness is None else np.asarray(roughness[2])
+    dif_amp = dif_ph = None
+    n_exp = 0.0
+    if diffuse is not None:
+        dif_amp, dif_ph, n_exp = diffuse
+    fac = dv.DeviceFacets(target.centers, target.normals, target.areas, target.e1, target.e2,
+                          gamma=g if g.ndim else None, phasors=phasors, diffuse_amp=dif_amp,
+                          diffuse_phasors=dif_ph, device="cpu")
+    ifc = dv.DeviceInterfaces(crossed, crossed_sigma, device="cpu")
+    tr = dv.DeviceTraces(positions, u_ct, None, None if pattern is None else pattern[1],
+                         device="cpu")
+    p = _lib.RefractedParams()
+    p.k0 = 0.0 if k0 is None else k0
+    p.gamma, p.t0, p.dt, p.c = (0.0 if g.ndim else float(gamma)), t0, dt, c
+    p.taper_s = 0.0 if taper_s is None else taper_s
+    p.n_exp = n_exp
+    p.n_samples = n_samples
+    p.n_crossed = len(crossed)
+    fl = _lib.SS_COHERENT if mode == "coherent" else 0
+    fl |= _lib.SS_SPLIT_SIDES if split_sides else 0
+    fl |= _lib.SS_TAPER if taper_s is not None else 0
+    fl |= _lib.SS_D_PHI_AREA if d_phi_area else 0
+    fl |= _lib.SS_GAMMA_FACET if g.ndim else 0
+    fl |= _lib.SS_CROSS_ROUGH if crossed_sigma is not None else 0
+    fl |= _lib.SS_DIFFUSE if diffuse is not None else 0
+    if roughness is not None:
+        sig, lc, _, nt, *rest = roughness
+        fl |= _lib.SS_ROUGH
+        p.sigma, p.corr_len, p.n_terms = sig, lc, nt
+        p.acf = _lib.ACFS[rest[0] if rest else "gaussian"]
+    kid, pn, pa, pb, table = dv.pattern_setup(pattern, "cpu")
+    p.pattern, p.pat_n, p.pat_a = kid, pn, pa
+    for i in range(4):
+        p.pat_b[i] = pb[i]
+    p.flags = fl
+    eps = np.asarray(eps_leg, np.float64)
+    legs = np.ascontiguousarray(np.stack([np.sqrt(eps), eps, np.asarray(att_leg, np.float64)]))
+    T, F = len(positions), fac.n
+    n_seg = (2 if split_sides else 1) * n_samples
+    out = np.zeros((T, n_seg, 2))
+    drop = np.zeros(T)
+    bins = np.zeros((T, F), np.int32)
+    nfb = ctypes.c_long(0)
+    fb = fac.buf.numpy()
+    tb = tr.buf.numpy()
+    tab = None if table is None else table.numpy()
+    desc = ifc.desc.numpy()
+    tabs = ifc.tables.numpy()
+    cst = _lib.constants_host()
+    lib().host_refracted(prec, ctypes.byref(p), _p(fb), ctypes.c_long(fac.stride),
+                         ctypes.c_long(F), _p(tb), ctypes.c_long(tr.stride), ctypes.c_long(T),
+                         _p(desc), _p(tabs), _p(legs), _p(tab), _p(cst), _p(out), _p(drop),
+                         _p(bins), ctypes.byref(nfb))
+    res = out[..., 0] + 1j * out[..., 1] if mode == "coherent" else out[..., 0].copy()
+    if split_sides:
+        res = res.reshape(T, 2, n_samples).transpose(0, 2, 1)
+    return res, drop, fac.unpermute_pairs(bins), nfb.value

@@ -1,0 +1,67 @@
+"""The C-ABI library: it loads without a GPU, exports every function that
+include/soundersim_hip.h declares, and rejects bad arguments with SS_EINVAL /
+SS_ERANGE before any device work (host-side validation only; no kernel is
+launched here)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from radar_sounder_simulator_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(__file__), "..", "include", "soundersim_hip.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(ss_\w+)\s*\(", src, re.M)))
+
+
+def test_header_declares_the_bound_symbols():
+    assert declared() == sorted(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in declared():
+        assert hasattr(lib, name), name
+    assert b"gfx950" in lib.ss_version()
+
+
+def test_struct_layouts_match_header():
+    # doubles first then int32s: no padding; sizes pinned against the header
+    assert ctypes.sizeof(_lib.SurfaceParams) == 13 * 8 + 6 * 4
+    assert ctypes.sizeof(_lib.RefractedParams) == 14 * 8 + 8 * 4
+
+
+def test_invalid_arguments_rejected_on_host():
+    lib = _lib.load()
+    p = _lib.SurfaceParams()
+    p.n_samples, p.dt, p.c = 64, 1e-8, 3e8
+    args = [ctypes.byref(p), None, 0, 0, None, 0, 0, None, None, None, None, None, None]
+    assert lib.ss_surface_field(7, *args) == -1
+    assert b"precision" in lib.ss_last_error()
+    p.n_samples = 0
+    assert lib.ss_surface_field(0, *args) == -1
+    p.n_samples = 64
+    p.flags = _lib.SS_ROUGH
+    assert lib.ss_surface_field(0, *args) == -1           # no constants table
+    p.flags = 0
+    p.pattern = 5
+    assert lib.ss_surface_field(0, *args) == -1           # tabulated without table
+    r = _lib.RefractedParams()
+    r.n_samples, r.dt, r.c, r.n_crossed = 64, 1e-8, 3e8, 0
+    rargs = [ctypes.byref(r), None, 0, 0, None, 0, 0, None, None, None, None, None, None,
+             None, None, None]
+    assert lib.ss_refracted_field(0, *rargs) == -3          # n_crossed out of range
+    r.n_crossed, r.flags = 1, _lib.SS_ROUGH                  # incoherent + roughness
+    assert lib.ss_refracted_field(0, *rargs) == -1
+    assert lib.ss_range_compress(None, 1, 8, None, 1, None, None) == -1
+
+
+def test_fill_constants():
+    c = _lib.constants_host()
+    assert c.shape == (_lib.CONST_COUNT,) and np.isfinite(c).all()

@@ -1,0 +1,182 @@
+"""GPU parity: the HIP kernels (through the C ABI) vs the FP64 oracle.
+
+Float64 tier: bins and validity masks identical to the oracle pair by pair,
+field relative L2 <= 1e-9.  Mixed tier: bins and masks identical, max power
+error <= 0.05 dB and max phase error <= 2 mrad over bins within 30 dB of
+each trace's peak.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import coherent as oc
+from oracle import multilayer as om
+from radar_sounder_simulator_amd import antenna as A
+from radar_sounder_simulator_amd import kernels as K
+from radar_sounder_simulator_amd import workloads as W
+from radar_sounder_simulator_amd.roughness import speckle_phasors
+from radar_sounder_simulator_amd.scene import gaussian_surface, grid_facets, straight_track
+
+pytestmark = pytest.mark.gpu
+
+REL_FP64 = 1e-9
+MAX_DB = 0.05
+MAX_MRAD = 2.0
+
+
+def _err(out, ref, floor_db=30.0):
+    pk = np.abs(ref).max(axis=-1, keepdims=True) if ref.ndim == 2 else np.abs(ref).max()
+    m = np.abs(ref) > pk * 10 ** (-floor_db / 20)
+    db = np.abs(20 * np.log10(np.abs(out[m]) / np.abs(ref[m]))).max()
+    mrad = np.abs(np.angle(out[m] / ref[m])).max() * 1e3
+    return db, mrad
+
+
+def _surface_case(name):
+    wl = W.config1(n=32, n_traces=6, n_samples=512, seed=11)
+    f = wl.layers[0].facets
+    tr = wl.track
+    kw = dict(k=wl.k0, gamma=wl.layers[0].gamma, t0=wl.t0, dt=wl.dt,
+              n_samples=wl.n_samples, c=wl.c)
+    ph = speckle_phasors(len(f), 7)
+    if name == "rough_exp":
+        kw.update(roughness=(0.3, 10.0, ph, 14, "exponential"), d_phi_area=True, taper_s=0.05)
+    elif name == "rough_gauss":
+        kw.update(roughness=(0.3, 10.0, ph, 14))
+    elif name == "split":
+        kw.update(split_sides=True)
+    elif name == "interp":
+        kw.update(interp_bins=True)
+    elif name == "gamma_facet":
+        kw.update(gamma=np.linspace(-0.3, -0.2, len(f)))
+    elif name == "tabulated":
+        kw.update(pattern=A.pattern_args(A.cos_power_table(2.0), tr.u_at, tr.u_ct))
+    elif name == "array":
+        kw.update(pattern=A.pattern_args(A.Antenna(kind="array", n_elements=4,
+                                                   element_directivity_db=6.0),
+                                         tr.u_at, tr.u_ct))
+    elif name == "window_edge":
+        kw.update(n_samples=40)   # cut the window: most facets dropped
+    return wl, f, tr, kw
+
+
+SURFACE_CASES = ["smooth", "rough_exp", "rough_gauss", "split", "interp", "gamma_facet",
+                 "tabulated", "array", "window_edge"]
+
+
+@pytest.mark.parametrize("name", SURFACE_CASES)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_surface_vs_oracle(name, precision):
+    wl, f, tr, kw = _surface_case(name)
+    print(f"case surface/{name}/{precision}", flush=True)
+    ref, dref, bref, _ = oc.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals,
+                                                 f.areas, f.e1, f.e2, return_bins=True, **kw)
+    out, drop, bins = K.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals,
+                                             f.areas, f.e1, f.e2, precision=precision,
+                                             return_bins=True, **kw)
+    np.testing.assert_array_equal(bins, bref)
+    if precision == "fp64":
+        assert out.dtype == np.complex128
+        rel = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+        assert rel <= REL_FP64, rel
+        np.testing.assert_allclose(drop, dref, rtol=1e-9, atol=1e-30)
+    else:
+        assert out.dtype == np.complex64
+        db, mrad = _err(out.reshape(len(tr.positions), -1), ref.reshape(len(tr.positions), -1))
+        assert db <= MAX_DB and mrad <= MAX_MRAD, (db, mrad)
+        np.testing.assert_allclose(drop, dref, rtol=1e-4, atol=1e-30)
+
+
+def _bed_scene(seed=5):
+    rng = np.random.default_rng(seed)
+    n, dx = 20, 25.0
+    surf = grid_facets(3.0 * gaussian_surface(n + 1, dx, 100.0, rng), dx)
+    bed = grid_facets(-300.0 + 30.0 * gaussian_surface(n + 1, dx, 200.0, rng), dx)
+    mid = grid_facets(-100.0 + 5.0 * gaussian_surface(n + 1, dx, 200.0, rng), dx)
+    tr = straight_track(5, 40.0, 3000.0)
+    return surf, bed, mid, tr
+
+
+REFR_CASES = ["smooth", "incoherent", "rough_exp", "rough_gauss", "cross_sigma", "diffuse",
+              "split", "tabulated", "gamma_facet", "two_crossings", "shadow"]
+
+
+@pytest.mark.parametrize("name", REFR_CASES)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_refracted_vs_oracle(name, precision):
+    surf, bed, mid, tr = _bed_scene()
+    C = 299792458.0
+    k0 = 2 * np.pi * 60e6 / C
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=k0)
+    target, crossed, eps, att = bed, [surf], [1.0, 3.15], [0.0, 10.0]
+    ph = speckle_phasors(len(bed), 3)
+    if name == "incoherent":
+        kw.update(mode="incoherent", gamma=0.0, k0=None)
+    elif name == "rough_exp":
+        kw.update(roughness=(0.5, 20.0, ph, 24, "exponential"), d_phi_area=True, taper_s=0.05)
+    elif name == "rough_gauss":
+        kw.update(roughness=(0.5, 20.0, ph, 24))
+    elif name == "cross_sigma":
+        kw.update(crossed_sigma=[0.2])
+    elif name == "diffuse":
+        kw.update(diffuse=(np.full(len(bed), 0.1), speckle_phasors(len(bed), 9), 1.0))
+    elif name == "split":
+        kw.update(split_sides=True)
+    elif name == "tabulated":
+        kw.update(pattern=A.pattern_args(A.cos_power_table(2.0), tr.u_at, tr.u_ct))
+    elif name == "gamma_facet":
+        kw.update(gamma=np.linspace(-0.3, 0.3, len(bed)))
+    elif name == "two_crossings":
+        crossed, eps, att = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0]
+    elif name == "shadow":
+        n = 20
+        h = np.where(np.arange(n + 1)[None, :] > 10, 20.0, -50.0) * np.ones((n + 1, 1))
+        target = grid_facets(h, 25.0)
+    print(f"case refracted/{name}/{precision}", flush=True)
+    ref, dref, bref, vref = om.refracted_cluttergram(tr.positions, tr.u_ct, target, crossed,
+                                                     eps, att, return_bins=True, **kw)
+    bref = np.where(vref, bref, K.BIN_INVALID)
+    out, drop, bins = K.refracted_cluttergram(tr.positions, tr.u_ct, target, crossed, eps,
+                                              att, precision=precision, return_bins=True, **kw)
+    np.testing.assert_array_equal(bins, bref)
+    if precision == "fp64":
+        rel = np.linalg.norm(out - ref) / max(np.linalg.norm(ref), 1e-300)
+        assert rel <= REL_FP64, rel
+        np.testing.assert_allclose(drop, dref, rtol=1e-9, atol=1e-30)
+    else:
+        if np.iscomplexobj(ref):
+            db, mrad = _err(out.reshape(len(tr.positions), -1), ref.reshape(len(tr.positions), -1))
+            assert db <= MAX_DB and mrad <= MAX_MRAD, (db, mrad)
+        else:
+            m = ref > ref.max() * 1e-3
+            assert np.abs(10 * np.log10(out[m] / ref[m])).max() <= MAX_DB
+        np.testing.assert_allclose(drop, dref, rtol=1e-3, atol=1e-30)
+
+
+def test_empty_inputs():
+    f = grid_facets(np.zeros((3, 3)), 10.0)
+    tr = straight_track(2, 10.0, 1000.0)
+    kw = dict(k=1.0, gamma=-0.3, t0=0.0, dt=1e-8, n_samples=64, c=299792458.0)
+    out, drop = K.coherent_cluttergram(tr.positions[:0], tr.u_ct[:0], f.centers, f.normals,
+                                       f.areas, f.e1, f.e2, **kw)
+    assert out.shape == (0, 64) and drop.shape == (0,)
+    out, drop = K.coherent_cluttergram(tr.positions, tr.u_ct, f.centers[:0], f.normals[:0],
+                                       f.areas[:0], f.e1[:0], f.e2[:0], **kw)
+    assert out.shape == (2, 64) and not np.any(out) and not np.any(drop)
+
+
+def test_range_compress_vs_fft():
+    from oracle import waveform as ow
+    from radar_sounder_simulator_amd import device as dv
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((7, 1024)) + 1j * rng.standard_normal((7, 1024))
+    for construction in ("analytic", "chirp"):
+        p, m = ow.compressed_pulse(15e6, 20e-6, 1 / 48e6, "hann", construction)
+        ref = ow.convolve_fast_time(x, p, m)
+        t = torch.as_tensor(np.stack([x.real, x.imag], -1), device="cuda")
+        y, _ = dv.range_compress(t, p, m)
+        y = y.cpu().numpy()
+        y = y[..., 0] + 1j * y[..., 1]
+        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-12

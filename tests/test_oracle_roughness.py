@@ -1,0 +1,125 @@
+"""Pins of the roughness oracle (oracle/roughness.py) -- CPU.
+
+Mirrors tests/test_roughness.py:40-183 and tests/test_roughness_exponential
+.py:48-91 of the reference: Faddeeva vs scipy.special.wofz, D_Phi vs an
+independent 50-digit mpmath evaluation of Eqs 21-24 written with erfi (not
+through the Faddeeva identity the oracle uses), the ACF spectra W_m vs
+numerical Hankel transforms of rho(r)^m, sigma = 0 exact zero, the area-only
+term as the large-facet limit of the finite-facet term, series lengths and
+the speckle phasor generators.
+"""
+
+import math
+
+import mpmath
+import numpy as np
+import pytest
+from scipy.integrate import quad
+from scipy.special import j0, wofz
+
+from oracle import roughness as orr
+from radar_sounder_simulator_amd import roughness as prod
+
+
+def test_faddeeva_vs_wofz():
+    x = np.linspace(-60, 60, 241)
+    y = np.concatenate([[0.0], np.logspace(-3, 2, 60)])
+    Z = x[:, None] + 1j * y[None, :]
+    w = orr.faddeeva(Z)
+    ref = wofz(Z)
+    assert np.max(np.abs(w - ref) / np.abs(ref)) < 1e-12
+
+
+def _dphi_mp(sigma, l, K, A0, B0, Lx, Ly, n_terms):
+    """Eqs 21-24 with erfi in 50-digit arithmetic."""
+    mpmath.mp.dps = 50
+    x = mpmath.mpf(sigma * K) ** 2
+
+    def F(m, a0, edge):
+        xm = mpmath.mpf(a0) * l / (2 * mpmath.sqrt(m))
+        ym = mpmath.mpf(edge) * mpmath.sqrt(m) / l
+        Am = xm + 1j * ym
+        t = mpmath.re(Am * mpmath.erfi(Am)) - xm * mpmath.erfi(xm)
+        return (1 - mpmath.exp(-ym ** 2) * mpmath.cos(a0 * edge)
+                + mpmath.sqrt(mpmath.pi) * mpmath.exp(-xm ** 2) * t)
+
+    acc = mpmath.mpf(0)
+    for m in range(1, n_terms + 1):
+        w = mpmath.exp(-x) * x ** m / mpmath.factorial(m)
+        acc += w * (mpmath.mpf(l) ** 4 / m ** 2) * F(m, A0, Lx) * F(m, B0, Ly)
+    return float(acc)
+
+
+@pytest.mark.parametrize("case", [(0.05, 1.0, 2.5, 0.3, -0.7, 4.0, 3.0, 12),
+                                  (0.2, 2.0, 3.0, 1.5, 0.2, 6.0, 6.0, 20),
+                                  (0.5, 3.0, 4.4, 0.0, 2.0, 10.0, 5.0, 30)])
+def test_d_phi_vs_mpmath(case):
+    sigma, l, K, A0, B0, Lx, Ly, nt = case
+    got = orr.d_phi(sigma, l, K, A0, B0, Lx, Ly, n_terms=nt)
+    ref = _dphi_mp(*case)
+    assert float(got) == pytest.approx(ref, rel=1e-9)
+
+
+def test_sigma_zero_exact_and_attenuation():
+    assert orr.d_phi(0.0, 2.0, 3.0, 0.1, 0.2, 5.0, 5.0, n_terms=10) == 0.0
+    assert orr.mean_attenuation(0.0, 3.0) == 1.0
+    assert orr.mean_attenuation(0.5, 2.0) == pytest.approx(np.exp(-0.5))
+
+
+def test_area_only_is_large_facet_limit():
+    """For facets much larger than the correlation length the finite-facet
+    term tends to its area-scaling part."""
+    full = orr.d_phi(0.3, 1.0, 3.0, 0.4, 0.3, 60.0, 60.0, n_terms=20)
+    area = orr.d_phi(0.3, 1.0, 3.0, 0.4, 0.3, 60.0, 60.0, n_terms=20, area_only=True)
+    assert float(full) == pytest.approx(float(area), rel=0.05)
+
+
+@pytest.mark.parametrize("acf", ["gaussian", "exponential"])
+@pytest.mark.parametrize("m", [1, 3, 7])
+def test_acf_spectrum_is_hankel_transform(acf, m):
+    l, kb = 2.0, 1.3
+    rho = (lambda r: np.exp(-r * r / l / l)) if acf == "gaussian" else (lambda r: np.exp(-r / l))
+    val, _ = quad(lambda r: rho(r) ** m * j0(kb * r) * r, 0, 80 * l, limit=800)
+    assert orr.acf_spectrum(m, kb, l, acf) == pytest.approx(2 * np.pi * val, rel=1e-7)
+
+
+def test_exponential_requires_area_only():
+    with pytest.raises(ValueError):
+        orr.d_phi(0.1, 1.0, 1.0, 0.1, 0.1, 1.0, 1.0, n_terms=10, acf="exponential")
+
+
+def test_n_terms_for():
+    assert orr.n_terms_for(0.0) == 10
+    assert orr.n_terms_for(4.98) == 24
+    assert orr.n_terms_for(1e6) == 300
+    for x in (0.0, 0.25, 4.98, 37.0, 1e4):
+        assert prod.n_terms_for(x) == orr.n_terms_for(x)
+
+
+def test_speckle_phasors_product_matches_oracle():
+    a = prod.speckle_phasors(1000, (3, 1))
+    b = orr.speckle_phasors(1000, (3, 1))
+    np.testing.assert_array_equal(a, b.astype(np.complex64))
+    assert abs(np.mean(np.abs(b) ** 2) - 1) < 0.1
+
+
+def test_keyed_phasors_vs_scalar_splitmix():
+    rng = np.random.default_rng(0)
+    keys = rng.integers(-5000, 5000, size=(300, 2))
+    a = prod.speckle_phasors_for_keys(keys, (4, 2))
+    b = orr.speckle_phasors_for_keys(keys, (4, 2))
+    np.testing.assert_array_equal(a, b.astype(np.complex64))
+    big = prod.speckle_phasors_for_keys(rng.integers(0, 10 ** 6, size=(20000, 2)), 9)
+    assert abs(np.mean(np.abs(big.astype(complex)) ** 2) - 1) < 0.05
+    with pytest.raises(ValueError):
+        prod.speckle_phasors_for_keys(np.zeros((3, 3)), 1)
+
+
+def test_lgamma_table_in_library_constants():
+    from radar_sounder_simulator_amd import _lib
+    c = _lib.constants_host()
+    L, a = orr.weideman_coeffs(32)
+    assert c[0] == pytest.approx(L, rel=1e-15)
+    np.testing.assert_allclose(c[1:33], a, rtol=0, atol=1e-13)
+    for m in (0, 1, 5, 100, 300):
+        assert c[33 + m] == pytest.approx(math.lgamma(m + 1.0), rel=1e-14)

@@ -1,0 +1,126 @@
+"""The kernels' per-pair arithmetic (csrc/ss_math.h, csrc/ss_refract.h),
+compiled for the CPU by tests/native/pair_host.cpp and summed sequentially,
+against the FP64 oracle -- CPU, no GPU needed.
+
+Float64 tier: bins/masks identical pair by pair, relative L2 <= 1e-12.
+Mixed tier: bins/masks identical, <= 0.05 dB and <= 2 mrad over bins within
+30 dB of each trace's peak.
+"""
+
+import numpy as np
+import pytest
+
+import harness as H
+from oracle import coherent as oc
+from oracle import multilayer as om
+from radar_sounder_simulator_amd import antenna as A
+from radar_sounder_simulator_amd import workloads as W
+from radar_sounder_simulator_amd.roughness import speckle_phasors
+from radar_sounder_simulator_amd.scene import gaussian_surface, grid_facets, straight_track
+
+C = 299792458.0
+
+
+def _err(out, ref):
+    out = out.reshape(out.shape[0], -1)
+    ref = ref.reshape(ref.shape[0], -1)
+    pk = np.abs(ref).max(axis=1, keepdims=True)
+    m = np.abs(ref) > pk * 10 ** (-30 / 20)
+    return (np.abs(20 * np.log10(np.abs(out[m]) / np.abs(ref[m]))).max(),
+            np.abs(np.angle(out[m] / ref[m])).max() * 1e3)
+
+
+SURF = ["smooth", "taper", "rough_gauss", "rough_area", "rough_exp", "split", "interp",
+        "gamma_facet", "tabulated", "dipole", "finite_dipole", "array", "array_tapered"]
+
+
+@pytest.mark.parametrize("name", SURF)
+def test_surface_pair_math(name):
+    wl = W.config1(n=20, n_traces=5, n_samples=256, seed=3)
+    f, tr = wl.layers[0].facets, wl.track
+    kw = dict(k=wl.k0, gamma=wl.layers[0].gamma, t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples,
+              c=wl.c)
+    ph = speckle_phasors(len(f), 3)
+    ants = dict(dipole=A.Antenna(kind="dipole"),
+                finite_dipole=A.Antenna(kind="finite_dipole", length_lam=0.7),
+                array=A.Antenna(kind="array", n_elements=6, element_directivity_db=6.0),
+                array_tapered=A.Antenna(kind="array_tapered", tx_weights=[1, 2, 2, 1],
+                                        rx_weights=[1, 1, 1, 1]),
+                tabulated=A.cos_power_table(2.0))
+    extra = dict(taper=dict(taper_s=0.05),
+                 rough_gauss=dict(roughness=(0.3, 5.0, ph, 12)),
+                 rough_area=dict(roughness=(0.3, 5.0, ph, 12), d_phi_area=True),
+                 rough_exp=dict(roughness=(0.3, 5.0, ph, 12, "exponential"), d_phi_area=True,
+                                taper_s=0.05),
+                 split=dict(split_sides=True), interp=dict(interp_bins=True),
+                 gamma_facet=dict(gamma=np.linspace(-0.3, -0.2, len(f))))
+    if name in ants:
+        kw["pattern"] = A.pattern_args(ants[name], tr.u_at, tr.u_ct)
+    kw.update(extra.get(name, {}))
+    ref, dref, bref, _ = oc.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals,
+                                                 f.areas, f.e1, f.e2, return_bins=True, **kw)
+    for prec in (0, 1):
+        out, drop, bins = H.surface(prec, tr.positions, tr.u_ct, f.centers, f.normals,
+                                    f.areas, f.e1, f.e2, **kw)
+        np.testing.assert_array_equal(bins, bref)
+        if prec == 0:
+            assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-12
+        else:
+            db, mrad = _err(out, ref)
+            assert db < 0.05 and mrad < 2.0
+
+
+REFR = ["smooth", "incoherent", "rough_exp", "rough_gauss", "cross_sigma", "diffuse",
+        "split", "tabulated", "gamma_facet", "two_crossings", "shadow"]
+
+
+@pytest.mark.parametrize("name", REFR)
+def test_refracted_pair_math(name):
+    rng = np.random.default_rng(5)
+    n, dx = 16, 25.0
+    surf = grid_facets(3.0 * gaussian_surface(n + 1, dx, 100.0, rng), dx)
+    bed = grid_facets(-300.0 + 30.0 * gaussian_surface(n + 1, dx, 200.0, rng), dx)
+    mid = grid_facets(-100.0 + 5.0 * gaussian_surface(n + 1, dx, 200.0, rng), dx)
+    tr = straight_track(4, 40.0, 3000.0)
+    k0 = 2 * np.pi * 60e6 / C
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=k0)
+    target, crossed, eps, att = bed, [surf], [1.0, 3.15], [0.0, 10.0]
+    ph = speckle_phasors(len(bed), 3)
+    if name == "incoherent":
+        kw.update(mode="incoherent", gamma=0.0, k0=None)
+    elif name == "rough_exp":
+        kw.update(roughness=(0.5, 20.0, ph, 24, "exponential"), d_phi_area=True, taper_s=0.05)
+    elif name == "rough_gauss":
+        kw.update(roughness=(0.5, 20.0, ph, 24))
+    elif name == "cross_sigma":
+        kw.update(crossed_sigma=[0.2])
+    elif name == "diffuse":
+        kw.update(diffuse=(np.full(len(bed), 0.1), speckle_phasors(len(bed), 9), 1.0))
+    elif name == "split":
+        kw.update(split_sides=True)
+    elif name == "tabulated":
+        kw.update(pattern=A.pattern_args(A.cos_power_table(2.0), tr.u_at, tr.u_ct))
+    elif name == "gamma_facet":
+        kw.update(gamma=np.linspace(-0.3, 0.3, len(bed)))
+    elif name == "two_crossings":
+        crossed, eps, att = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0]
+    elif name == "shadow":
+        h = np.where(np.arange(n + 1)[None, :] > 8, 20.0, -50.0) * np.ones((n + 1, 1))
+        target = grid_facets(h, dx)
+    ref, dref, bref, vref = om.refracted_cluttergram(tr.positions, tr.u_ct, target, crossed,
+                                                     eps, att, return_bins=True, **kw)
+    bref = np.where(vref, bref, -2 ** 31)
+    for prec in (0, 1):
+        out, drop, bins, nfb = H.refracted(prec, tr.positions, tr.u_ct, target, crossed, eps,
+                                           att, **kw)
+        np.testing.assert_array_equal(bins, bref)
+        if prec == 0:
+            assert np.linalg.norm(out - ref) / max(np.linalg.norm(ref), 1e-300) < 1e-12
+            np.testing.assert_allclose(drop, dref, rtol=1e-12, atol=1e-30)
+        elif np.iscomplexobj(ref):
+            db, mrad = _err(out, ref)
+            assert db < 0.05 and mrad < 2.0
+        else:
+            m = ref > ref.max() * 1e-3
+            assert np.abs(10 * np.log10(out[m] / ref[m])).max() < 0.05
