@@ -107,6 +107,7 @@ extern "C" {
 
 #define SS_BIN_INVALID (-2147483647 - 1)
 #define SS_MAX_CROSSED 64
+#define SS_MAX_JOINT 32
 #define SS_MAX_TERMS 300
 #define SS_MAX_PATTERN_TABLE 4096
 
@@ -152,6 +153,10 @@ typedef struct ss_refracted_params {
   int32_t acf;
   int32_t pattern;
   int32_t pat_n;
+  int32_t refraction;       /* 0 sequential chain, 1 joint solve          */
+  int32_t k_pad;            /* joint: index-matched pads at the top       */
+  int32_t joint_newton;     /* joint: damped Newton steps (reference 6)   */
+  int32_t joint_backtrack;  /* joint: step halvings (reference 4)         */
   int32_t reserved;
 } ss_refracted_params;
 
@@ -183,10 +188,11 @@ int ss_surface_field(int precision, const ss_surface_params* params,
                      double* field, double* dropped, int32_t* pair_bins,
                      void* stream);
 
-/* Refracted-path field of one target interface under n_crossed interfaces
- * (sequential two-point chain).  Replaces kernels/multilayer.py:594-770
- * (refracted_cluttergram, refraction="sequential") and _refracted_fn
- * (:333-591).  iface_desc: (n_crossed, SS_IDESC_COUNT) float64 device;
+/* Refracted-path field of one target interface under n_crossed interfaces,
+ * sequential two-point chain or joint N-interface solve (refraction = 1,
+ * stack incl. pads at most SS_MAX_JOINT deep).  Replaces
+ * kernels/multilayer.py:594-770 (refracted_cluttergram) and _refracted_fn
+ * (:333-591), refraction_joint.py:249-322 (joint_crossings).  iface_desc: (n_crossed, SS_IDESC_COUNT) float64 device;
  * tables: concatenated (sum ny*nx, 6) float64 [centre, normal] device;
  * legs: (3, n_crossed + 1) float64 device rows [n, eps, att_db_per_km]. */
 int ss_refracted_field(int precision, const ss_refracted_params* params,

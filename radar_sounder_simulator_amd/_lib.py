@@ -34,6 +34,7 @@ TROW = dict(PX=0, UX=3, RREF=6, PV0=7, PV1=10, COUNT=13)
 IDESC = dict(NY=0, NX=1, COEF=2, MP=8, MN=11, SIGC=14, TABOFF=15, COUNT=16)
 BIN_INVALID = -2147483648
 MAX_CROSSED = 64
+MAX_JOINT = 32
 MAX_TERMS = 300
 CONST_COUNT = 33 + MAX_TERMS + 1
 
@@ -63,7 +64,9 @@ class RefractedParams(ctypes.Structure):
                 ("n_samples", ctypes.c_int32), ("flags", ctypes.c_int32),
                 ("n_crossed", ctypes.c_int32), ("n_terms", ctypes.c_int32),
                 ("acf", ctypes.c_int32), ("pattern", ctypes.c_int32),
-                ("pat_n", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("pat_n", ctypes.c_int32), ("refraction", ctypes.c_int32),
+                ("k_pad", ctypes.c_int32), ("joint_newton", ctypes.c_int32),
+                ("joint_backtrack", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 _lib = None

@@ -25,7 +25,7 @@
 #include <cstring>
 #include <string>
 
-#include "ss_refract.h"
+#include "ss_joint.h"
 
 namespace ss {
 
@@ -273,7 +273,8 @@ __global__ void fold_kernel(double* field, const double* traces, long ts, long n
 }
 
 // --------------------------------------------------------------- refracted
-template <int PREC>
+// NMAX = 0: sequential chain only; NMAX > 0: joint solve for stacks <= NMAX.
+template <int PREC, int NMAX>
 __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc) {
   using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
   __shared__ Shared<ACC, kTTRefr> sh;
@@ -314,13 +315,13 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
   if constexpr (PREC == SS_PREC_FP64) {
     trace_loop<ACC, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o) {
-                               refracted_pair_fp64(fr, t, rc, o);
+                               refracted_pair_fp64<NMAX>(fr, t, rc, o);
                              });
   } else {
     trace_loop<ACC, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o) {
-                               if (!refracted_pair_mixed(fr, t, rc, o))
-                                 refracted_pair_fp64(fr, t, rc, o);
+                               if (!refracted_pair_mixed<NMAX>(fr, t, rc, o))
+                                 refracted_pair_fp64<NMAX>(fr, t, rc, o);
                              });
   }
 }
@@ -556,6 +557,17 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   if (prm->n_crossed < 1 || prm->n_crossed > SS_MAX_CROSSED)
     return fail(SS_ERANGE, "n_crossed out of range [1, 64]");
   if (!iface_desc || !tables || !legs) return fail(SS_EINVAL, "null interface data");
+  if (prm->refraction != 0 && prm->refraction != 1)
+    return fail(SS_EINVAL, "refraction must be 0 (sequential) or 1 (joint)");
+  if (prm->refraction == 1) {
+    if (prm->n_crossed > SS_MAX_JOINT)
+      return fail(SS_ERANGE, "joint stack deeper than SS_MAX_JOINT (32)");
+    if (prm->k_pad < 0 || prm->k_pad >= prm->n_crossed)
+      return fail(SS_EINVAL, "k_pad must be in [0, n_crossed)");
+    if (prm->joint_newton < 0 || prm->joint_backtrack < 1 || prm->joint_newton > 64 ||
+        prm->joint_backtrack > 32)
+      return fail(SS_ERANGE, "joint budgets out of range (newton 0..64, backtrack 1..32)");
+  }
   if (n_traces > 0 && (!out || !dropped || !traces))
     return fail(SS_EINVAL, "null trace/output pointer");
   if (n_facets > 0 && !facets) return fail(SS_EINVAL, "null facets");
@@ -608,14 +620,33 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   rc.desc = iface_desc;
   rc.tables = tables;
   rc.legs = legs;
+  rc.joint = prm->refraction == 1;
+  rc.k_pad = prm->k_pad;
+  rc.n_newton = prm->joint_newton;
+  rc.n_backtrack = prm->joint_backtrack;
   rc.k_bin = 2.0 / (prm->c * prm->dt);
   rc.t0_dt = prm->t0 / prm->dt;
   rc.cyc_per_m = prm->k0 / kPi;
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
-  if (precision == SS_PREC_FP64)
-    hipLaunchKernelGGL(refracted_kernel<SS_PREC_FP64>, grid, block, 0, st, k, rc);
-  else
-    hipLaunchKernelGGL(refracted_kernel<SS_PREC_MIXED>, grid, block, 0, st, k, rc);
+  const bool fp64 = precision == SS_PREC_FP64;
+  if (!rc.joint) {
+    if (fp64)
+      hipLaunchKernelGGL((refracted_kernel<SS_PREC_FP64, 0>), grid, block, 0, st, k, rc);
+    else
+      hipLaunchKernelGGL((refracted_kernel<SS_PREC_MIXED, 0>), grid, block, 0, st, k, rc);
+  } else if (prm->n_crossed <= 8) {
+    if (fp64)
+      hipLaunchKernelGGL((refracted_kernel<SS_PREC_FP64, 8>), grid, block, 0, st, k, rc);
+    else
+      hipLaunchKernelGGL((refracted_kernel<SS_PREC_MIXED, 8>), grid, block, 0, st, k, rc);
+  } else {
+    if (fp64)
+      hipLaunchKernelGGL((refracted_kernel<SS_PREC_FP64, SS_MAX_JOINT>), grid, block, 0, st,
+                         k, rc);
+    else
+      hipLaunchKernelGGL((refracted_kernel<SS_PREC_MIXED, SS_MAX_JOINT>), grid, block, 0, st,
+                         k, rc);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("refracted launch: ") + hipGetErrorString(e));
   g_last_error.clear();

@@ -250,8 +250,17 @@ struct RefrConst {
   const double* desc;    // (n_crossed, SS_IDESC_COUNT)
   const double* tables;  // (rows, 6)
   const double* legs;    // (3, n_crossed + 1): n, eps, att
+  int joint, k_pad, n_newton, n_backtrack;  // joint solver (pads first)
   // derived
   double k_bin, t0_dt, cyc_per_m;
+};
+
+// Path sums handed from a crossing solver to the radiometry stage.
+struct PathSums {
+  double cur[3];       // last crossing (start of the final leg)
+  double x_first[3];   // first crossing (air-leg departure)
+  bool valid;
+  double opl, loss_db, sum_par, sum_perp, tau2, c_first, c_last;
 };
 
 struct RefrFacet {
@@ -282,9 +291,9 @@ SS_HD int clamp_round(double v, int hi) {
   return (int)r;
 }
 
-// Float64 tier, sequential chain + radiometry (kernels/multilayer.py:352-583).
-SS_HD void refracted_pair_fp64(const RefrFacet& f, const TraceRec& t,
-                               const RefrConst& rc, PairOut& o) {
+// Float64 tier, sequential chain (kernels/multilayer.py:352-406).
+SS_HD void seq_path_fp64(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
+                         PathSums& ps) {
 #pragma clang fp contract(off)
   const int flags = rc.flags;
   const double* nl = rc.legs;
@@ -341,6 +350,32 @@ SS_HD void refracted_pair_fp64(const RefrFacet& f, const TraceRec& t,
     cur[1] = s.x[1];
     cur[2] = s.x[2];
   }
+  for (int c = 0; c < 3; ++c) {
+    ps.cur[c] = cur[c];
+    ps.x_first[c] = x_first[c];
+  }
+  ps.valid = valid;
+  ps.opl = opl;
+  ps.loss_db = loss_db;
+  ps.sum_par = sum_par;
+  ps.sum_perp = sum_perp;
+  ps.tau2 = tau2;
+  ps.c_first = c_first;
+  ps.c_last = c_last;
+}
+
+// Float64 tier, final leg + radiometry + bin (kernels/multilayer.py:500-583).
+SS_HD void radiometry_fp64(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
+                           const PathSums& ps, PairOut& o) {
+#pragma clang fp contract(off)
+  const int flags = rc.flags;
+  const double* nl = rc.legs;
+  const double* attl = rc.legs + 2 * (rc.n_crossed + 1);
+  const double* cur = ps.cur;
+  const double* x_first = ps.x_first;
+  const bool valid = ps.valid;
+  double opl = ps.opl, loss_db = ps.loss_db, sum_par = ps.sum_par, sum_perp = ps.sum_perp;
+  const double tau2 = ps.tau2, c_first = ps.c_first, c_last = ps.c_last;
   const int J = rc.n_crossed;
   double dd[3] = {cur[0] - f.c[0], cur[1] - f.c[1], cur[2] - f.c[2]};
   double s_j = sqrt((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
@@ -428,9 +463,10 @@ SS_HD void refracted_pair_fp64(const RefrFacet& f, const TraceRec& t,
                 : 0;
 }
 
-// Mixed tier.  Returns false when the pair must be redone on the float64 tier.
-SS_HD bool refracted_pair_mixed(const RefrFacet& f, const TraceRec& t,
-                                const RefrConst& rc, PairOut& o) {
+// Mixed tier, sequential chain.  Returns false when the pair must be redone
+// on the float64 tier (ambiguous Snell residual).
+SS_HD bool seq_path_mixed(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
+                          PathSums& ps) {
   const int flags = rc.flags;
   const double* nl = rc.legs;
   const double* epsl = rc.legs + (rc.n_crossed + 1);
@@ -508,6 +544,36 @@ SS_HD bool refracted_pair_mixed(const RefrFacet& f, const TraceRec& t,
     cur[1] = s.x[1];
     cur[2] = s.x[2];
   }
+  for (int c = 0; c < 3; ++c) {
+    ps.cur[c] = cur[c];
+    ps.x_first[c] = x_first[c];
+  }
+  ps.valid = valid;
+  ps.opl = opl;
+  ps.loss_db = loss_db;
+  ps.sum_par = sum_par;
+  ps.sum_perp = sum_perp;
+  ps.tau2 = tau2;
+  ps.c_first = c_first;
+  ps.c_last = c_last;
+  return true;
+}
+
+// Mixed tier, final leg + radiometry + bin.  Returns false when the bin
+// position lies within kBinGuard of an edge (the caller redoes the pair on
+// the float64 tier).
+SS_HD bool radiometry_mixed(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
+                            const PathSums& ps, PairOut& o) {
+  const int flags = rc.flags;
+  const double* nl = rc.legs;
+  const double* attl = rc.legs + 2 * (rc.n_crossed + 1);
+  const double* cur = ps.cur;
+  const double* x_first = ps.x_first;
+  const bool valid = ps.valid;
+  double opl = ps.opl;
+  float loss_db = (float)ps.loss_db, sum_par = (float)ps.sum_par,
+        sum_perp = (float)ps.sum_perp, tau2 = (float)ps.tau2;
+  const float c_first = (float)ps.c_first, c_last = (float)ps.c_last;
   const int J = rc.n_crossed;
   double dd[3] = {cur[0] - f.c[0], cur[1] - f.c[1], cur[2] - f.c[2]};
   double s_j = sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);

@@ -237,15 +237,39 @@ class DeviceInterfaces:
     (x, y, 1) -> (row, col) fit and the area-weighted mean plane
     (``kernels/multilayer.py:269-283``, float64)."""
 
-    def __init__(self, crossed, crossed_sigma=None, device=None):
+    def __init__(self, crossed, crossed_sigma=None, device=None, pad_to=None,
+                 z_platform=None):
         self.device = _dev(device)
-        n = len(crossed)
-        if n < 1 or n > _lib.MAX_CROSSED:
-            raise ValueError(f"need 1..{_lib.MAX_CROSSED} crossed interfaces, got {n}")
+        n_real = len(crossed)
+        if n_real < 1 or n_real > _lib.MAX_CROSSED:
+            raise ValueError(f"need 1..{_lib.MAX_CROSSED} crossed interfaces, got {n_real}")
+        k = 0 if pad_to is None else int(pad_to) - n_real
+        if k < 0:
+            raise ValueError(f"pad_to={pad_to} < {n_real} crossed interfaces")
+        # joint solver: index-matched no-op planes spread through the air gap
+        # above the surface (kernels/multilayer.py:286-330), top-down first
+        pads = []
+        if k:
+            z_surf = float(crossed[0].centers[:, 2].max())
+            gap = float(z_platform) - z_surf
+            if gap <= 0:
+                raise ValueError("joint refraction padding needs the platform above the "
+                                 f"surface (gap {gap:.1f} m)")
+            pads = list(z_surf + gap * (np.arange(k, 0, -1) / (k + 1.0)))
+        self.k_pad = k
+        n = n_real + k
         desc = np.zeros((n, IDESC["COUNT"]), np.float64)
         tabs = []
         off = 0
-        for i, f in enumerate(crossed):
+        for i, z in enumerate(pads):
+            desc[i, IDESC["NY"]] = desc[i, IDESC["NX"]] = 1
+            desc[i, IDESC["MP"] + 2] = z
+            desc[i, IDESC["MN"] + 2] = 1.0
+            desc[i, IDESC["TABOFF"]] = off
+            tabs.append(np.array([[0.0, 0.0, z, 0.0, 0.0, 1.0]]))
+            off += 1
+        for i0, f in enumerate(crossed):
+            i = i0 + k
             ny, nx = f.grid_shape
             tab = np.concatenate([f.centers.reshape(ny * nx, 3),
                                   f.normals.reshape(ny * nx, 3)], axis=1)
@@ -261,7 +285,7 @@ class DeviceInterfaces:
             desc[i, IDESC["COEF"]:IDESC["COEF"] + 6] = coef.ravel()
             desc[i, IDESC["MP"]:IDESC["MP"] + 3] = mp
             desc[i, IDESC["MN"]:IDESC["MN"] + 3] = mn
-            desc[i, IDESC["SIGC"]] = 0.0 if crossed_sigma is None else crossed_sigma[i]
+            desc[i, IDESC["SIGC"]] = 0.0 if crossed_sigma is None else crossed_sigma[i0]
             desc[i, IDESC["TABOFF"]] = off
             off += ny * nx
             tabs.append(tab)
@@ -278,10 +302,13 @@ class DeviceInterfaces:
 def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
                     n_samples, c, gamma=0.0, k0=None, precision="mixed",
                     split_sides=False, pattern=None, roughness=None,
-                    taper_s=None, d_phi_area=False, n_exp=None, out=None,
+                    taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
+                    joint_newton=6, joint_backtrack=4, out=None,
                     dropped=None, pair_bins=False, stream=None):
-    """Enqueue the refracted kernel (sequential chain).  Returns (out
-    (T, n_seg, 2) float64, dropped (T,)[, pair_bins]) device tensors."""
+    """Enqueue the refracted kernel (sequential chain or joint solve over the
+    possibly padded stack of ``ifaces``; ``eps_leg``/``att_leg`` are the
+    unpadded per-leg media).  Returns (out (T, n_seg, 2) float64, dropped
+    (T,)[, pair_bins]) device tensors."""
     dev = target.device
     coherent = mode == "coherent"
     flags = _lib.SS_COHERENT if coherent else 0
@@ -298,6 +325,9 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
     p.n_exp = 0.0 if n_exp is None else float(n_exp)
     p.n_samples = int(n_samples)
     p.n_crossed = ifaces.n
+    p.refraction = 1 if refraction == "joint" else 0
+    p.k_pad = ifaces.k_pad
+    p.joint_newton, p.joint_backtrack = int(joint_newton), int(joint_backtrack)
     if roughness is not None:
         sigma, lc, n_terms, acf = roughness
         flags |= _lib.SS_ROUGH
@@ -309,7 +339,11 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
         p.pat_b[i] = pb[i]
     p.flags = flags
     eps = np.asarray(eps_leg, np.float64)
-    legs = np.stack([np.sqrt(eps), eps, np.asarray(att_leg, np.float64)])
+    att = np.asarray(att_leg, np.float64)
+    if ifaces.k_pad:
+        eps = np.concatenate([np.full(ifaces.k_pad, eps[0]), eps])
+        att = np.concatenate([np.full(ifaces.k_pad, att[0]), att])
+    legs = np.stack([np.sqrt(eps), eps, att])
     if legs.shape[1] != ifaces.n + 1:
         raise ValueError(f"need {ifaces.n + 1} legs, got {legs.shape[1]}")
     legs_t = torch.as_tensor(np.ascontiguousarray(legs), device=dev)

@@ -140,18 +140,11 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
             raise ValueError(f"diffuse amp shape {dif_amp.shape} != ({n},)")
     if len(eps_leg) != len(crossed) + 1 or len(att_leg) != len(crossed) + 1:
         raise ValueError("eps_leg/att_leg need len(crossed) + 1 entries")
-    if refraction == "joint":
-        from .joint import refracted_joint
-        return refracted_joint(
-            positions, u_ct, target, crossed, eps_leg, att_leg, mode=mode, t0=t0,
-            dt=dt, n_samples=n_samples, c=c, gamma=gamma, k0=k0,
-            split_sides=split_sides, pattern=pattern, roughness=roughness,
-            crossed_sigma=crossed_sigma, diffuse=diffuse, taper_s=taper_s,
-            d_phi_area=d_phi_area, pad_to=pad_to, joint_newton=joint_newton,
-            joint_backtrack=joint_backtrack, precision=precision, device=device,
-            return_bins=return_bins)
-    if refraction != "sequential":
+    if refraction not in ("sequential", "joint"):
         raise ValueError(f"unknown refraction solver {refraction!r}")
+    joint = refraction == "joint"
+    if joint and (pad_to or len(crossed)) > 32:
+        raise ValueError("joint refraction supports stacks of at most 32 interfaces")
     if precision not in ("mixed", "fp64"):
         raise ValueError("precision must be 'mixed' or 'fp64'")
     _check_gpu()
@@ -161,7 +154,9 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
                            gamma=g if gamma_facet else None, phasors=phasors,
                            diffuse_amp=dif_amp, diffuse_phasors=dif_ph,
                            device=device)
-    ifaces = _dv.DeviceInterfaces(crossed, crossed_sigma, device=device)
+    ifaces = _dv.DeviceInterfaces(crossed, crossed_sigma, device=device,
+                                  pad_to=(pad_to or len(crossed)) if joint else None,
+                                  z_platform=pos[:, 2].min())
     pv = None if pattern is None else pattern[1]
     traces = _dv.DeviceTraces(pos, u_ct, None, pv, device=device)
     res = _dv.refracted_field(
@@ -169,6 +164,7 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
         n_samples=n_samples, c=c, gamma=0.0 if gamma_facet else float(gamma),
         k0=k0, precision=precision, split_sides=split_sides, pattern=pattern,
         roughness=rough, taper_s=taper_s, d_phi_area=d_phi_area, n_exp=n_exp,
+        refraction=refraction, joint_newton=joint_newton, joint_backtrack=joint_backtrack,
         pair_bins=return_bins)
     torch.cuda.synchronize(tgt.device)
     raw = res[0].cpu().numpy()

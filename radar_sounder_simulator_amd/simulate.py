@@ -1,0 +1,195 @@
+"""simulate(): the front door of the hot path on a planar scene.
+
+Follows the reference orchestrator (``simulate.py:172-412``) step for step,
+minus geodesy (scenes here are already in a planar local frame):
+
+* single interface: the surface coherent kernel, gamma = fresnel_normal of
+  the first two media, roughness arguments with the wavenumber of the medium
+  above, grazing fix, antenna pattern, waveform (``:172-226``);
+* several interfaces: interface 0 through the surface kernel, interface j >= 1
+  through the refracted kernel -- sequential for j = 1, ``cfg.refraction``
+  for j > 1 with joint calls padded to power-of-two interface buckets --
+  per-layer gamma (or a per-facet gamma map), roughness, crossing roughness,
+  diffuse channel, then one waveform pass over the layer axis (linear) and
+  per-layer nadir delays (``:263-412``).
+
+The output is a ``SimResult`` (``output.py``); interface kinds dem / flat /
+offset resolve as in ``layered.py:56-118`` (offsets shift along +z here).
+"""
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import antenna as _ant
+from .config import DemInterface, FlatInterface, OffsetInterface, SimConfig
+from .kernels import coherent_cluttergram, refracted_cluttergram
+from .output import SimResult
+from .physics import fresnel_normal
+from .roughness import n_terms_for, speckle_phasors, speckle_phasors_for_keys
+from .scene import Facets, grid_facets
+from .waveform import apply_waveform
+
+
+@dataclass
+class PlanarScene:
+    """Heights of one or more (ny+1, nx+1) vertex lattices at ``spacing`` m
+    (``dems[0]`` is the surface; a bare DemInterface at position i takes
+    ``dems[i]``), a track in the same frame, optional per-interface maps of
+    per-facet gamma / diffuse amplitude (name -> (ny, nx) array) and roll."""
+
+    dems: list
+    spacing: float
+    track: object
+    gamma_maps: dict = field(default_factory=dict)
+    diffuse_maps: dict = field(default_factory=dict)
+    nav_roll: np.ndarray = None
+    name: str = "planar"
+
+
+def _pattern(rc, scene, track):
+    a = rc.antenna
+    if a.kind == "isotropic":
+        return None
+    ant = _ant.Antenna(**a.model_dump())
+    roll = None
+    if a.roll_source == "nav":
+        roll = scene.nav_roll if scene.nav_roll is not None else np.zeros(len(track.positions))
+    return _ant.pattern_args(ant, track.u_at, track.u_ct, roll)
+
+
+def _roughness_args(iface, facets, k_local, seed):
+    rc = iface.roughness
+    if rc is None or rc.sigma_m == 0.0:
+        return None
+    n_terms = n_terms_for((2.0 * k_local * rc.sigma_m) ** 2)
+    ph = (speckle_phasors_for_keys(facets.phase_keys, seed)
+          if facets.phase_keys is not None else speckle_phasors(len(facets), seed))
+    if rc.acf != "gaussian":
+        return (rc.sigma_m, rc.corr_length_m, ph, n_terms, rc.acf)
+    return (rc.sigma_m, rc.corr_length_m, ph, n_terms)
+
+
+def joint_pad_to(j, n_max):
+    """Power-of-two padding bucket (``simulate.py:253-260``)."""
+    b = 1
+    while b < j:
+        b *= 2
+    return min(b, n_max)
+
+
+def build_interfaces(scene, cfg):
+    """Facets per configured interface, top-down."""
+    out, names = [], []
+    for i, ic in enumerate(cfg.interfaces):
+        names.append(ic.name or f"interface_{i}")
+        if isinstance(ic, DemInterface):
+            if ic.path is not None:
+                raise ValueError("DEM files are not read by this library: give the "
+                                 "interface heights in PlanarScene.dems")
+            if i >= len(scene.dems):
+                raise ValueError(f"interface {i} ({ic.name or 'unnamed'}) needs a DEM: the "
+                                 "scene provides none")
+            out.append(grid_facets(scene.dems[i], scene.spacing))
+        elif isinstance(ic, FlatInterface):
+            ref = np.asarray(scene.dems[0], np.float64)
+            out.append(grid_facets(np.full_like(ref, ic.elevation), scene.spacing))
+        else:
+            out.append(None)
+    for i, ic in enumerate(cfg.interfaces):
+        if isinstance(ic, OffsetInterface):
+            r = ic.reference if isinstance(ic.reference, int) else names.index(ic.reference)
+            src = out[r]
+            if src is None:
+                raise ValueError(f"offset interface {i} references an unresolved offset")
+            c = src.centers.copy()
+            c[:, 2] += ic.offset
+            out[i] = Facets(c, src.normals.copy(), src.areas.copy(), src.e1.copy(),
+                            src.e2.copy(), src.cell.copy(), src.grid_shape,
+                            None if src.phase_keys is None else src.phase_keys.copy())
+    return out, names
+
+
+def nadir_twtt_layers(pos, interfaces, eps, c):
+    """Per-layer nadir two-way time via the horizontally nearest facet centre
+    per interface (``simulate.py:398-412``), O(T F) per interface."""
+    nadir = np.empty((pos.shape[0], len(interfaces)))
+    prev = pos[:, 2].copy()
+    acc = np.zeros(pos.shape[0])
+    for j, f in enumerate(interfaces):
+        d2 = ((f.centers[None, :, :2] - pos[:, None, :2]) ** 2).sum(-1)
+        z = f.centers[d2.argmin(axis=1), 2]
+        acc = acc + np.sqrt(eps[j]) * (prev - z)
+        nadir[:, j] = 2.0 * acc / c
+        prev = z
+    return nadir
+
+
+def simulate(scene: PlanarScene, sim_config: SimConfig, device=None):
+    """Run the coherent hot path on ``scene``; returns a ``SimResult``."""
+    cfg = sim_config
+    if cfg.mode != "coherent":
+        raise NotImplementedError("incoherent surface mode is not part of this library "
+                                  "(coherent facet kernel only)")
+    rc = cfg.radar
+    if rc.f0 is None:
+        raise ValueError("coherent mode requires radar.f0 to be set")
+    if len(cfg.interfaces) > 1 and rc.waveform.interp_bins:
+        raise ValueError("waveform.interp_bins is not yet supported for multilayer runs")
+    track = scene.track
+    ifaces, names = build_interfaces(scene, cfg)
+    eps = [m.eps_r for m in cfg.media]
+    att = [m.attenuation_db_per_km for m in cfg.media]
+    k0 = 2.0 * np.pi / rc.wavelength
+    window = dict(t0=rc.t0, dt=rc.dt, n_samples=rc.n_samples, c=rc.c,
+                  split_sides=cfg.split_sides, pattern=_pattern(rc, scene, track),
+                  precision=cfg.precision, device=device)
+    gfx = cfg.grazing_fix
+    gfx_kw = dict(taper_s=gfx.s_eff if gfx else None, d_phi_area=gfx is not None)
+    sig_all = np.array([(ic.roughness.sigma_m if ic.roughness else 0.0)
+                        for ic in cfg.interfaces])
+    unknown = (set(scene.gamma_maps) | set(scene.diffuse_maps)) - set(names)
+    if unknown:
+        raise ValueError(f"maps for unknown interfaces: {unknown}")
+    if names[0] in scene.diffuse_maps:
+        raise ValueError("diffuse_maps on the top interface is not wired (the surface "
+                         "uses the surface kernel)")
+    outs, drops = [], []
+    for j, target in enumerate(ifaces):
+        gamma_j = fresnel_normal(eps[j], eps[j + 1])
+        if names[j] in scene.gamma_maps:
+            gamma_j = np.asarray(scene.gamma_maps[names[j]], np.float64).ravel()
+        rough = _roughness_args(cfg.interfaces[j], target, k0 * np.sqrt(eps[j]),
+                                (cfg.roughness_seed, j))
+        if j == 0:
+            out, drop = coherent_cluttergram(
+                track.positions, track.u_ct, target.centers, target.normals,
+                target.areas, target.e1, target.e2, k=k0, gamma=gamma_j,
+                interp_bins=rc.waveform.interp_bins, roughness=rough, **gfx_kw, **window)
+        else:
+            refr = cfg.refraction if j > 1 else "sequential"
+            n_max = len(ifaces) - 1
+            crossed_sig = sig_all[:j] if sig_all[:j].any() else None
+            dif = None
+            if names[j] in scene.diffuse_maps:
+                ph = (speckle_phasors_for_keys(target.phase_keys,
+                                               (cfg.roughness_seed, 1000 + j))
+                      if target.phase_keys is not None else
+                      speckle_phasors(len(target), (cfg.roughness_seed, 1000 + j)))
+                dif = (np.asarray(scene.diffuse_maps[names[j]], np.float64).ravel(), ph,
+                       float(cfg.diffuse_exponent))
+            out, drop = refracted_cluttergram(
+                track.positions, track.u_ct, target, ifaces[:j], eps[:j + 1], att[:j + 1],
+                mode="coherent", gamma=gamma_j, k0=k0, refraction=refr,
+                pad_to=joint_pad_to(j, n_max) if refr == "joint" else None,
+                roughness=rough, crossed_sigma=crossed_sig, diffuse=dif, **gfx_kw, **window)
+        outs.append(out)
+        drops.append(drop)
+    single = len(ifaces) == 1
+    field = outs[0] if single else np.stack(outs, axis=-1)
+    field = apply_waveform(field, rc, "coherent")
+    dropped = drops[0] if single else np.stack(drops, axis=-1)
+    nadir = nadir_twtt_layers(track.positions, ifaces, eps, rc.c)
+    return SimResult.build(field=field, dropped=dropped, nadir_twtt=nadir,
+                           t0=rc.t0, dt=rc.dt, positions=track.positions,
+                           layers=None if single else names, config=cfg)

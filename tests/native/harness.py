@@ -104,7 +104,8 @@ def surface(prec, positions, u_ct, centers, normals, areas, e1, e2, *, k, gamma,
 def refracted(prec, positions, u_ct, target, crossed, eps_leg, att_leg, *, mode, t0, dt,
               n_samples, c, gamma=0.0, k0=None, split_sides=False, pattern=None,
               roughness=None, crossed_sigma=None, diffuse=None, taper_s=None,
-              d_phi_area=False):
+              d_phi_area=False, refraction="sequential", pad_to=None, joint_newton=6,
+              joint_backtrack=4):
     g = np.asarray(gamma, np.float64)
     phasors = None if roughness is None else np.asarray(roughness[2])
     dif_amp = dif_ph = None
@@ -114,7 +115,10 @@ def refracted(prec, positions, u_ct, target, crossed, eps_leg, att_leg, *, mode,
     fac = dv.DeviceFacets(target.centers, target.normals, target.areas, target.e1, target.e2,
                           gamma=g if g.ndim else None, phasors=phasors, diffuse_amp=dif_amp,
                           diffuse_phasors=dif_ph, device="cpu")
-    ifc = dv.DeviceInterfaces(crossed, crossed_sigma, device="cpu")
+    joint = refraction == "joint"
+    ifc = dv.DeviceInterfaces(crossed, crossed_sigma, device="cpu",
+                              pad_to=(pad_to or len(crossed)) if joint else None,
+                              z_platform=np.asarray(positions)[:, 2].min())
     tr = dv.DeviceTraces(positions, u_ct, None, None if pattern is None else pattern[1],
                          device="cpu")
     p = _lib.RefractedParams()
@@ -123,7 +127,10 @@ def refracted(prec, positions, u_ct, target, crossed, eps_leg, att_leg, *, mode,
     p.taper_s = 0.0 if taper_s is None else taper_s
     p.n_exp = n_exp
     p.n_samples = n_samples
-    p.n_crossed = len(crossed)
+    p.n_crossed = ifc.n
+    p.refraction = 1 if joint else 0
+    p.k_pad = ifc.k_pad
+    p.joint_newton, p.joint_backtrack = joint_newton, joint_backtrack
     fl = _lib.SS_COHERENT if mode == "coherent" else 0
     fl |= _lib.SS_SPLIT_SIDES if split_sides else 0
     fl |= _lib.SS_TAPER if taper_s is not None else 0
@@ -142,7 +149,11 @@ def refracted(prec, positions, u_ct, target, crossed, eps_leg, att_leg, *, mode,
         p.pat_b[i] = pb[i]
     p.flags = fl
     eps = np.asarray(eps_leg, np.float64)
-    legs = np.ascontiguousarray(np.stack([np.sqrt(eps), eps, np.asarray(att_leg, np.float64)]))
+    att = np.asarray(att_leg, np.float64)
+    if ifc.k_pad:
+        eps = np.concatenate([np.full(ifc.k_pad, eps[0]), eps])
+        att = np.concatenate([np.full(ifc.k_pad, att[0]), att])
+    legs = np.ascontiguousarray(np.stack([np.sqrt(eps), eps, att]))
     T, F = len(positions), fac.n
     n_seg = (2 if split_sides else 1) * n_samples
     out = np.zeros((T, n_seg, 2))

@@ -6,7 +6,7 @@
 #include <cstring>
 #include <vector>
 
-#include "../../radar_sounder_simulator_amd/csrc/ss_refract.h"
+#include "../../radar_sounder_simulator_amd/csrc/ss_joint.h"
 
 using namespace ss;
 
@@ -108,6 +108,8 @@ extern "C" int host_refracted(int prec, const ss_refracted_params* prm, const do
   for (int i = 0; i < 4; ++i) rc.pat.b[i] = prm->pat_b[i];
   rc.pat.table = table; rc.cst = cst; rc.desc = desc; rc.tables = tables; rc.legs = legs;
   rc.k_bin = 2.0 / (prm->c * prm->dt); rc.t0_dt = prm->t0 / prm->dt; rc.cyc_per_m = prm->k0 / kPi;
+  rc.joint = prm->refraction == 1; rc.k_pad = prm->k_pad;
+  rc.n_newton = prm->joint_newton; rc.n_backtrack = prm->joint_backtrack;
   bool split = prm->flags & SS_SPLIT_SIDES;
   int n = prm->n_samples, n_seg = (split ? 2 : 1) * n;
   std::memset(out, 0, sizeof(double) * nt * n_seg * 2);
@@ -132,8 +134,11 @@ extern "C" int host_refracted(int prec, const ss_refracted_params* prm, const do
     for (long t = 0; t < nt; ++t) {
       TraceRec r = load_trace(tr, ts, t);
       PairOut o;
-      if (prec == SS_PREC_FP64) refracted_pair_fp64(a, r, rc, o);
-      else if (!refracted_pair_mixed(a, r, rc, o)) { ++fb; refracted_pair_fp64(a, r, rc, o); }
+      if (prec == SS_PREC_FP64) refracted_pair_fp64<SS_MAX_JOINT>(a, r, rc, o);
+      else if (!refracted_pair_mixed<SS_MAX_JOINT>(a, r, rc, o)) {
+        ++fb;
+        refracted_pair_fp64<SS_MAX_JOINT>(a, r, rc, o);
+      }
       accumulate(o, t, n, n_seg, split, out, dropped);
       if (bins) bins[t * nf + f] = o.raw_bin;
     }

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     # doubles first then int32s: no padding; sizes pinned against the header
     assert ctypes.sizeof(_lib.SurfaceParams) == 13 * 8 + 6 * 4
-    assert ctypes.sizeof(_lib.RefractedParams) == 14 * 8 + 8 * 4
+    assert ctypes.sizeof(_lib.RefractedParams) == 14 * 8 + 12 * 4
 
 
 def test_invalid_arguments_rejected_on_host():

@@ -99,7 +99,8 @@ def _bed_scene(seed=5):
 
 
 REFR_CASES = ["smooth", "incoherent", "rough_exp", "rough_gauss", "cross_sigma", "diffuse",
-              "split", "tabulated", "gamma_facet", "two_crossings", "shadow"]
+              "split", "tabulated", "gamma_facet", "two_crossings", "shadow", "joint2",
+        "joint_pad", "joint3_sigma"]
 
 
 @pytest.mark.parametrize("name", REFR_CASES)
@@ -130,6 +131,18 @@ def test_refracted_vs_oracle(name, precision):
         kw.update(gamma=np.linspace(-0.3, 0.3, len(bed)))
     elif name == "two_crossings":
         crossed, eps, att = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0]
+    elif name == "joint2":
+        crossed, eps, att = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0]
+        kw.update(refraction="joint")
+    elif name == "joint_pad":
+        crossed, eps, att = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0]
+        kw.update(refraction="joint", pad_to=4)
+    elif name == "joint3_sigma":
+        mid2 = grid_facets(mid.centers[:, 2].reshape(mid.grid_shape).mean() - 60.0
+                           + 0.0 * np.zeros((mid.grid_shape[0] + 1, mid.grid_shape[1] + 1)),
+                           mid.e1[0, 0])
+        crossed, eps, att = [surf, mid, mid2], [1.0, 1.6, 2.0, 3.15], [0.0, 2.0, 5.0, 10.0]
+        kw.update(refraction="joint", crossed_sigma=[0.1, 0.05, 0.0])
     elif name == "shadow":
         n = 20
         h = np.where(np.arange(n + 1)[None, :] > 10, 20.0, -50.0) * np.ones((n + 1, 1))

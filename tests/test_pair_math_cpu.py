@@ -71,7 +71,8 @@ def test_surface_pair_math(name):
 
 
 REFR = ["smooth", "incoherent", "rough_exp", "rough_gauss", "cross_sigma", "diffuse",
-        "split", "tabulated", "gamma_facet", "two_crossings", "shadow"]
+        "split", "tabulated", "gamma_facet", "two_crossings", "shadow", "joint2",
+        "joint_pad", "joint3_sigma"]
 
 
 @pytest.mark.parametrize("name", REFR)
@@ -105,6 +106,18 @@ def test_refracted_pair_math(name):
         kw.update(gamma=np.linspace(-0.3, 0.3, len(bed)))
     elif name == "two_crossings":
         crossed, eps, att = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0]
+    elif name == "joint2":
+        crossed, eps, att = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0]
+        kw.update(refraction="joint")
+    elif name == "joint_pad":
+        crossed, eps, att = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0]
+        kw.update(refraction="joint", pad_to=4)
+    elif name == "joint3_sigma":
+        mid2 = grid_facets(mid.centers[:, 2].reshape(mid.grid_shape).mean() - 60.0
+                           + 0.0 * np.zeros((mid.grid_shape[0] + 1, mid.grid_shape[1] + 1)),
+                           mid.e1[0, 0])
+        crossed, eps, att = [surf, mid, mid2], [1.0, 1.6, 2.0, 3.15], [0.0, 2.0, 5.0, 10.0]
+        kw.update(refraction="joint", crossed_sigma=[0.1, 0.05, 0.0])
     elif name == "shadow":
         h = np.where(np.arange(n + 1)[None, :] > 8, 20.0, -50.0) * np.ones((n + 1, 1))
         target = grid_facets(h, dx)
