@@ -93,7 +93,7 @@ def main():
     ap.add_argument("--n", type=int, default=512, help="facets per side per interface")
     ap.add_argument("--traces", type=int, default=256)
     ap.add_argument("--samples", type=int, default=4096)
-    ap.add_argument("--cpu-traces", type=int, default=32)
+    ap.add_argument("--cpu-traces", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()

@@ -160,6 +160,28 @@ typedef struct ss_refracted_params {
   int32_t reserved;
 } ss_refracted_params;
 
+/* One layer of one request in a batched launch (ss_batch_field).  kind 0:
+ * surface layer (uses ``surface``), kind 1: refracted layer, sequential
+ * chain (uses ``refracted`` and the interface pointers).  All pointers are
+ * device pointers with the layouts above; outputs are zeroed by the call. */
+typedef struct ss_job {
+  int32_t kind;
+  int32_t precision_unused;   /* the batch precision is the call's          */
+  int64_t facet_stride, n_facets;
+  int64_t trace_stride, n_traces;
+  const double* facets;
+  const double* traces;
+  double* field;
+  double* dropped;
+  const double* iface_desc;
+  const double* tables;
+  const double* legs;
+  const double* pattern_table;
+  const double* constants;
+  ss_surface_params surface;
+  ss_refracted_params refracted;
+} ss_job;
+
 /* Library version string. */
 const char* ss_version(void);
 
@@ -203,6 +225,23 @@ int ss_refracted_field(int precision, const ss_refracted_params* params,
                        const double* legs, const double* pattern_table,
                        const double* constants, double* out, double* dropped,
                        int32_t* pair_bins, void* stream);
+
+/* Batched persistent launch over uneven jobs: every (job, trace tile, facet
+ * tile) work item of ``n_jobs`` jobs is drained from one device work counter
+ * by ``n_blocks`` persistent workgroups (0 = a default of 4 per CU).  The
+ * job table is copied from host memory into ``jobs_device`` (caller-owned
+ * device buffer of at least n_jobs * sizeof(ss_job) bytes) and
+ * ``counter_device`` (one int64 of device memory) is reset on the stream, so
+ * the call allocates nothing.  trace_tile: traces per workgroup (4, 8 or
+ * 16; chosen by the autotuner).  No counterpart in the reference (its
+ * requests run one jit call after another); serves config 4 and the request
+ * loop. */
+int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* jobs_device,
+                   int64_t* counter_device, int trace_tile, int n_blocks, void* stream);
+
+/* sizeof(ss_job) as compiled into the library (for sizing jobs_device and
+ * checking a binding's struct mirror).  No counterpart in the reference. */
+int64_t ss_sizeof_job(void);
 
 /* Fast-time 'same' convolution of n_rows rows of n_samples complex128
  * samples with a (2m+1)-tap complex128 pulse whose zero lag is tap m,

@@ -40,7 +40,7 @@ CONST_COUNT = 33 + MAX_TERMS + 1
 
 EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_fill_constants", "ss_surface_field", "ss_refracted_field",
-            "ss_range_compress")
+            "ss_batch_field", "ss_sizeof_job", "ss_range_compress")
 
 
 class SurfaceParams(ctypes.Structure):
@@ -69,6 +69,19 @@ class RefractedParams(ctypes.Structure):
                 ("joint_backtrack", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class Job(ctypes.Structure):
+    """Mirror of ``ss_job`` (one layer of one request in a batched launch)."""
+    _fields_ = [("kind", ctypes.c_int32), ("precision_unused", ctypes.c_int32),
+                ("facet_stride", ctypes.c_int64), ("n_facets", ctypes.c_int64),
+                ("trace_stride", ctypes.c_int64), ("n_traces", ctypes.c_int64),
+                ("facets", ctypes.c_void_p), ("traces", ctypes.c_void_p),
+                ("field", ctypes.c_void_p), ("dropped", ctypes.c_void_p),
+                ("iface_desc", ctypes.c_void_p), ("tables", ctypes.c_void_p),
+                ("legs", ctypes.c_void_p), ("pattern_table", ctypes.c_void_p),
+                ("constants", ctypes.c_void_p), ("surface", SurfaceParams),
+                ("refracted", RefractedParams)]
+
+
 _lib = None
 
 
@@ -95,6 +108,9 @@ def load():
                                        i64, i64, dp, i64, i64, dp, dp, dp, dp,
                                        dp, dp, dp, dp, vp]
     lib.ss_refracted_field.restype = ctypes.c_int
+    lib.ss_batch_field.argtypes = [i32, ctypes.POINTER(Job), i32, dp, dp, i32, i32, vp]
+    lib.ss_batch_field.restype = ctypes.c_int
+    lib.ss_sizeof_job.restype = ctypes.c_int64
     lib.ss_range_compress.argtypes = [dp, i64, i64, dp, i64, dp, vp]
     lib.ss_range_compress.restype = ctypes.c_int
     _lib = lib

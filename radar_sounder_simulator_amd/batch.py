@@ -1,0 +1,109 @@
+"""Batched persistent launches over uneven requests (``ss_batch_field``).
+
+A ``BatchRunner`` owns the two small device buffers the C entry point needs
+(the job table and the work counter), so a launch allocates nothing.  Every
+layer of every scene becomes one ``ss_job``; one persistent launch drains all
+(job, trace tile, facet tile) work items; range compression follows per scene.
+
+``uneven_workloads`` draws the 64-scene mix: facets log-uniform in
+[1e4, 2e6] per interface, traces uniform in [50, 2000], 1-4 layers, centre
+frequency 30-330 MHz with 10-30 MHz bandwidth, per-scene seeds; subsurface
+layers use the sequential refraction chain (the kernel-level default of the
+reference, ``kernels/multilayer.py:599``).  ``scale`` shrinks facet and
+trace counts for quick runs (1.0 = the stated distribution).
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as dv
+from .physics import C, fresnel_normal
+from .scene import gaussian_surface, grid_facets, straight_track
+from .waveform import Waveform
+from .workloads import Layer, Workload
+
+
+class BatchRunner:
+    def __init__(self, device=None, max_jobs=1024):
+        self.device = dv._dev(device)
+        self.size = int(_lib.load().ss_sizeof_job())
+        self.jobs_dev = torch.empty(max_jobs * self.size, dtype=torch.uint8, device=self.device)
+        self.counter = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.max_jobs = max_jobs
+        self._host = None
+
+    def launch(self, jobs, precision="mixed", trace_tile=8, n_blocks=0, stream=None):
+        """Enqueue one persistent launch over ``jobs`` (list of _lib.Job)."""
+        n = len(jobs)
+        if n > self.max_jobs:
+            raise ValueError(f"{n} jobs > {self.max_jobs}")
+        arr = (_lib.Job * max(n, 1))(*jobs)
+        self._host = arr            # the async copy reads it; keep it alive
+        prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
+        rc = _lib.load().ss_batch_field(prec, arr, n, ctypes.c_void_p(self.jobs_dev.data_ptr()),
+                                        ctypes.c_void_p(self.counter.data_ptr()),
+                                        int(trace_tile), int(n_blocks),
+                                        dv._stream_handle(stream))
+        _lib.check(rc, "ss_batch_field")
+
+    def run_scenes(self, scenes, precision="mixed", trace_tile=8, n_blocks=0, stream=None,
+                   compress=True):
+        jobs, keep = [], []
+        for rs in scenes:
+            j, k = rs.jobs(precision)
+            jobs += j
+            keep += k
+        self.launch(jobs, precision, trace_tile, n_blocks, stream)
+        if compress:
+            for rs in scenes:
+                rs.compress(stream)
+        return keep
+
+
+def uneven_workloads(n_scenes=64, seed=4, scale=1.0, n_samples=None, max_facets=None):
+    """The uneven 64-scene mix (module docstring) as a list of Workloads."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_scenes):
+        F = math.exp(rng.uniform(math.log(1e4), math.log(2e6))) * scale
+        if max_facets:
+            F = min(F, max_facets)
+        T = int(max(4, round(rng.uniform(50, 2000) * scale)))
+        L = int(rng.integers(1, 5))
+        f0 = rng.uniform(30e6, 330e6)
+        B = rng.uniform(10e6, 30e6)
+        side = max(8, int(round(math.sqrt(F))))
+        dx = 25.0
+        srng = np.random.default_rng([seed, s])
+        alt = 20000.0
+        surf = grid_facets(2.0 * gaussian_surface(side + 1, dx, 200.0, srng), dx)
+        eps_ice = 3.15
+        layers = [Layer("surface", surf, fresnel_normal(1.0, eps_ice))]
+        crossed = [surf]
+        eps = [1.0, eps_ice]
+        att = [0.0, 10.0]
+        depth = 0.0
+        for j in range(1, L):
+            depth += rng.uniform(150.0, 800.0)
+            h = -depth + 20.0 * gaussian_surface(side + 1, dx, 1000.0, srng)
+            f = grid_facets(h, dx)
+            eps_below = eps_ice if j < L - 1 else 8.0
+            layers.append(Layer(f"layer{j}", f, fresnel_normal(eps[-1], eps_below)
+                                if j == L - 1 else -10 ** (-40 / 20), crossed=list(crossed),
+                                eps_leg=list(eps), att_leg=list(att)))
+            crossed.append(f)
+            eps = eps + [eps_ice]
+            att = att + [10.0]
+        dt = 1.0 / (2.5 * B)
+        n = n_samples or int(2 ** math.ceil(math.log2(2.0 * (depth * 1.78 + 2000.0) / C / dt)))
+        track = straight_track(T, 25.0, alt)
+        out.append(Workload(
+            name=f"scene{s}", track=track, layers=layers, k0=2 * math.pi * f0 / C,
+            t0=2.0 * (alt - 100.0) / C, dt=dt, n_samples=n, c=C,
+            waveform=Waveform(kind="chirp", bandwidth=B, pulse_length=10e-6),
+            meta=dict(f0=f0, B=B, F=side * side, T=T, L=L)))
+    return out

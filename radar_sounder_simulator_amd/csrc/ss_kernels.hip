@@ -194,20 +194,82 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
   if (tid < nT && sh.drop[tid] != 0.0) atomicAdd(&k.dropped[t_base + tid], sh.drop[tid]);
 }
 
-// ----------------------------------------------------------------- surface
-template <int PREC>
-__global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst sc) {
-  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
-  __shared__ Shared<ACC, kTTSurf> sh;
-  const long w = work_id(k);
-  if (w < 0) return;
-  const long ft = w % k.n_ftiles;
-  const long f = ft * kBlock + threadIdx.x;
-  const bool live = f < k.nf;
+// Kernel-side constants from the ABI parameter structs.
+__host__ __device__ inline PatternArgs make_pattern(int kind, int n, double a, const double b[4],
+                                                    const double* table) {
+  PatternArgs p;
+  p.kind = kind;
+  p.n = n;
+  p.a = a;
+  for (int i = 0; i < 4; ++i) p.b[i] = b[i];
+  p.table = table;
+  return p;
+}
+
+__host__ __device__ inline SurfaceConst make_surface_const(const ss_surface_params& prm,
+                                                           const double* table,
+                                                           const double* cst) {
+  SurfaceConst sc;
+  sc.k = prm.k;
+  sc.gamma = prm.gamma;
+  sc.t0 = prm.t0;
+  sc.dt = prm.dt;
+  sc.c = prm.c;
+  sc.taper_s = prm.taper_s;
+  sc.sigma = prm.sigma;
+  sc.lc = prm.corr_len;
+  sc.n_samples = prm.n_samples;
+  sc.flags = prm.flags;
+  sc.n_terms = prm.n_terms;
+  sc.acf = prm.acf;
+  sc.pat = make_pattern(prm.pattern, prm.pat_n, prm.pat_a, prm.pat_b, table);
+  sc.cst = cst;
+  sc.k_over_pi = prm.k / kPi;
+  sc.kfac = prm.k / kTwoPi;
+  sc.k_bin = 2.0 / (prm.c * prm.dt);
+  sc.t0_dt = prm.t0 / prm.dt;
+  sc.cyc_per_m = prm.k / kPi;
+  return sc;
+}
+
+__host__ __device__ inline RefrConst make_refr_const(const ss_refracted_params& prm,
+                                                     const double* desc, const double* tables,
+                                                     const double* legs, const double* table,
+                                                     const double* cst) {
+  RefrConst rc;
+  rc.k0 = prm.k0;
+  rc.gamma = prm.gamma;
+  rc.t0 = prm.t0;
+  rc.dt = prm.dt;
+  rc.c = prm.c;
+  rc.taper_s = prm.taper_s;
+  rc.sigma = prm.sigma;
+  rc.lc = prm.corr_len;
+  rc.n_exp = prm.n_exp;
+  rc.n_samples = prm.n_samples;
+  rc.flags = prm.flags;
+  rc.n_crossed = prm.n_crossed;
+  rc.n_terms = prm.n_terms;
+  rc.acf = prm.acf;
+  rc.pat = make_pattern(prm.pattern, prm.pat_n, prm.pat_a, prm.pat_b, table);
+  rc.cst = cst;
+  rc.desc = desc;
+  rc.tables = tables;
+  rc.legs = legs;
+  rc.joint = prm.refraction == 1;
+  rc.k_pad = prm.k_pad;
+  rc.n_newton = prm.joint_newton;
+  rc.n_backtrack = prm.joint_backtrack;
+  rc.k_bin = 2.0 / (prm.c * prm.dt);
+  rc.t0_dt = prm.t0 / prm.dt;
+  rc.cyc_per_m = prm.k0 / kPi;
+  return rc;
+}
+
+__device__ inline void load_surface_facet(const KCommon& k, long fi, int flags,
+                                          SurfFacet64& f64) {
   const double* F = k.facets;
   const long fs = k.fs;
-  const long fi = live ? f : 0;
-  SurfFacet64 f64;
   for (int j = 0; j < 3; ++j) {
     f64.c[j] = F[(SS_FROW_CX + j) * fs + fi];
     f64.n[j] = F[(SS_FROW_NX + j) * fs + fi];
@@ -215,10 +277,10 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
     f64.e2[j] = F[(SS_FROW_E2X + j) * fs + fi];
   }
   f64.A = F[SS_FROW_AREA * fs + fi];
-  f64.gam = (sc.flags & SS_GAMMA_FACET) ? F[SS_FROW_GAMMA * fs + fi] : 0.0;
+  f64.gam = (flags & SS_GAMMA_FACET) ? F[SS_FROW_GAMMA * fs + fi] : 0.0;
   f64.ph_re = f64.ph_im = 0.0;
   f64.l1 = f64.l2 = 1.0;
-  if (sc.flags & SS_ROUGH) {
+  if (flags & SS_ROUGH) {
     f64.ph_re = F[SS_FROW_PH_RE * fs + fi];
     f64.ph_im = F[SS_FROW_PH_IM * fs + fi];
     double l1 = sqrt((f64.e1[0] * f64.e1[0] + f64.e1[1] * f64.e1[1]) + f64.e1[2] * f64.e1[2]);
@@ -226,6 +288,63 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
     f64.l1 = l1 > 0.0 ? l1 : 1.0;
     f64.l2 = l2 > 0.0 ? l2 : 1.0;
   }
+}
+
+__device__ inline void to_f32(const SurfFacet64& a, SurfFacet32& b) {
+  for (int j = 0; j < 3; ++j) {
+    b.c[j] = a.c[j];
+    b.n[j] = (float)a.n[j];
+    b.e1[j] = (float)a.e1[j];
+    b.e2[j] = (float)a.e2[j];
+  }
+  b.A = (float)a.A;
+  b.gam = (float)a.gam;
+  b.ph_re = (float)a.ph_re;
+  b.ph_im = (float)a.ph_im;
+  b.l1 = (float)a.l1;
+  b.l2 = (float)a.l2;
+}
+
+__device__ inline void load_refr_facet(const KCommon& k, long fi, int flags, RefrFacet& fr) {
+  const double* F = k.facets;
+  const long fs = k.fs;
+  for (int j = 0; j < 3; ++j) {
+    fr.c[j] = F[(SS_FROW_CX + j) * fs + fi];
+    fr.n[j] = F[(SS_FROW_NX + j) * fs + fi];
+    fr.e1[j] = F[(SS_FROW_E1X + j) * fs + fi];
+    fr.e2[j] = F[(SS_FROW_E2X + j) * fs + fi];
+  }
+  fr.A = F[SS_FROW_AREA * fs + fi];
+  fr.gam = (flags & SS_GAMMA_FACET) ? F[SS_FROW_GAMMA * fs + fi] : 0.0;
+  fr.ph_re = fr.ph_im = 0.0;
+  fr.l1 = fr.l2 = 1.0;
+  if (flags & SS_ROUGH) {
+    fr.ph_re = F[SS_FROW_PH_RE * fs + fi];
+    fr.ph_im = F[SS_FROW_PH_IM * fs + fi];
+    double l1 = sqrt((fr.e1[0] * fr.e1[0] + fr.e1[1] * fr.e1[1]) + fr.e1[2] * fr.e1[2]);
+    double l2 = sqrt((fr.e2[0] * fr.e2[0] + fr.e2[1] * fr.e2[1]) + fr.e2[2] * fr.e2[2]);
+    fr.l1 = l1 > 0.0 ? l1 : 1.0;
+    fr.l2 = l2 > 0.0 ? l2 : 1.0;
+  }
+  fr.dif = fr.dph_re = fr.dph_im = 0.0;
+  if (flags & SS_DIFFUSE) {
+    fr.dif = F[SS_FROW_DIF_AMP * fs + fi];
+    fr.dph_re = F[SS_FROW_DIF_RE * fs + fi];
+    fr.dph_im = F[SS_FROW_DIF_IM * fs + fi];
+  }
+}
+
+// ----------------------------------------------------------------- surface
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst sc) {
+  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  __shared__ Shared<ACC, kTTSurf> sh;
+  const long w = work_id(k);
+  if (w < 0) return;
+  const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
+  const bool live = f < k.nf;
+  SurfFacet64 f64;
+  load_surface_facet(k, live ? f : 0, sc.flags, f64);
   const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
   if constexpr (PREC == SS_PREC_FP64) {
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
@@ -234,18 +353,7 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
                              });
   } else {
     SurfFacet32 f32;
-    for (int j = 0; j < 3; ++j) {
-      f32.c[j] = f64.c[j];
-      f32.n[j] = (float)f64.n[j];
-      f32.e1[j] = (float)f64.e1[j];
-      f32.e2[j] = (float)f64.e2[j];
-    }
-    f32.A = (float)f64.A;
-    f32.gam = (float)f64.gam;
-    f32.ph_re = (float)f64.ph_re;
-    f32.ph_im = (float)f64.ph_im;
-    f32.l1 = (float)f64.l1;
-    f32.l2 = (float)f64.l2;
+    to_f32(f64, f32);
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o) {
                                surface_pair_mixed(f32, nullptr, t, sc, o);
@@ -280,37 +388,10 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
   __shared__ Shared<ACC, kTTRefr> sh;
   const long w = work_id(k);
   if (w < 0) return;
-  const long ft = w % k.n_ftiles;
-  const long f = ft * kBlock + threadIdx.x;
+  const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
   const bool live = f < k.nf;
-  const double* F = k.facets;
-  const long fs = k.fs;
-  const long fi = live ? f : 0;
   RefrFacet fr;
-  for (int j = 0; j < 3; ++j) {
-    fr.c[j] = F[(SS_FROW_CX + j) * fs + fi];
-    fr.n[j] = F[(SS_FROW_NX + j) * fs + fi];
-    fr.e1[j] = F[(SS_FROW_E1X + j) * fs + fi];
-    fr.e2[j] = F[(SS_FROW_E2X + j) * fs + fi];
-  }
-  fr.A = F[SS_FROW_AREA * fs + fi];
-  fr.gam = (rc.flags & SS_GAMMA_FACET) ? F[SS_FROW_GAMMA * fs + fi] : 0.0;
-  fr.ph_re = fr.ph_im = 0.0;
-  fr.l1 = fr.l2 = 1.0;
-  if (rc.flags & SS_ROUGH) {
-    fr.ph_re = F[SS_FROW_PH_RE * fs + fi];
-    fr.ph_im = F[SS_FROW_PH_IM * fs + fi];
-    double l1 = sqrt((fr.e1[0] * fr.e1[0] + fr.e1[1] * fr.e1[1]) + fr.e1[2] * fr.e1[2]);
-    double l2 = sqrt((fr.e2[0] * fr.e2[0] + fr.e2[1] * fr.e2[1]) + fr.e2[2] * fr.e2[2]);
-    fr.l1 = l1 > 0.0 ? l1 : 1.0;
-    fr.l2 = l2 > 0.0 ? l2 : 1.0;
-  }
-  fr.dif = fr.dph_re = fr.dph_im = 0.0;
-  if (rc.flags & SS_DIFFUSE) {
-    fr.dif = F[SS_FROW_DIF_AMP * fs + fi];
-    fr.dph_re = F[SS_FROW_DIF_RE * fs + fi];
-    fr.dph_im = F[SS_FROW_DIF_IM * fs + fi];
-  }
+  load_refr_facet(k, live ? f : 0, rc.flags, fr);
   const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
   if constexpr (PREC == SS_PREC_FP64) {
     trace_loop<ACC, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
@@ -323,6 +404,107 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
                                if (!refracted_pair_mixed<NMAX>(fr, t, rc, o))
                                  refracted_pair_fp64<NMAX>(fr, t, rc, o);
                              });
+  }
+}
+
+// ------------------------------------------------------------ batched launch
+constexpr int kMaxJobs = 1024;
+
+// Work items of one job under trace tile TT.
+__host__ __device__ inline long job_work(const ss_job& j, int tt) {
+  long nft = (j.n_facets + kBlock - 1) / kBlock;
+  long ntt = (j.n_traces + tt - 1) / tt;
+  return nft * ntt;
+}
+
+// Persistent workgroups drain (job, trace tile, facet tile) items from one
+// counter; every workgroup exits once the counter passes the total.
+template <int PREC, int TT>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict__ jobs,
+                                                      int n_jobs, long total,
+                                                      unsigned long long* counter) {
+  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  __shared__ Shared<ACC, TT> sh;
+  __shared__ long begins[kMaxJobs + 1];
+  __shared__ long s_w;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n_jobs; i += kBlock) begins[i + 1] = job_work(jobs[i], TT);
+  __syncthreads();
+  if (tid == 0) {
+    begins[0] = 0;
+    for (int i = 0; i < n_jobs; ++i) begins[i + 1] += begins[i];
+  }
+  __syncthreads();
+  for (;;) {
+    if (tid == 0) s_w = (long)atomicAdd(counter, 1ULL);
+    __syncthreads();
+    const long gw = s_w;
+    __syncthreads();
+    if (gw >= total) break;
+    int lo = 0, hi = n_jobs;  // begins[lo] <= gw < begins[hi]
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (begins[mid] <= gw)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const ss_job& jb = jobs[lo];
+    const long w = gw - begins[lo];
+    KCommon k;
+    k.facets = jb.facets;
+    k.fs = jb.facet_stride;
+    k.nf = jb.n_facets;
+    k.traces = jb.traces;
+    k.ts = jb.trace_stride;
+    k.nt = jb.n_traces;
+    k.field = jb.field;
+    k.dropped = jb.dropped;
+    k.pair_bins = nullptr;
+    k.n_ftiles = (int)((jb.n_facets + kBlock - 1) / kBlock);
+    k.n_ttiles = (int)((jb.n_traces + TT - 1) / TT);
+    k.per_xcd = 0;
+    const long f = (w % k.n_ftiles) * kBlock + tid;
+    const bool live = f < k.nf;
+    if (jb.kind == 0) {
+      const SurfaceConst sc = make_surface_const(jb.surface, jb.pattern_table, jb.constants);
+      k.n_seg = ((sc.flags & SS_SPLIT_SIDES) ? 2 : 1) * sc.n_samples;
+      const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
+      SurfFacet64 f64;
+      load_surface_facet(k, live ? f : 0, sc.flags, f64);
+      if constexpr (PREC == SS_PREC_FP64) {
+        trace_loop<ACC, TT>(sh, k, w, live, f, sc.n_samples, split,
+                            [&](const TraceRec& t, PairOut& o) {
+                              surface_pair_fp64(f64, t, sc, o);
+                            });
+      } else {
+        SurfFacet32 f32;
+        to_f32(f64, f32);
+        trace_loop<ACC, TT>(sh, k, w, live, f, sc.n_samples, split,
+                            [&](const TraceRec& t, PairOut& o) {
+                              surface_pair_mixed(f32, nullptr, t, sc, o);
+                            });
+      }
+    } else {
+      const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
+                                           jb.pattern_table, jb.constants);
+      k.n_seg = ((rc.flags & SS_SPLIT_SIDES) ? 2 : 1) * rc.n_samples;
+      const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
+      RefrFacet fr;
+      load_refr_facet(k, live ? f : 0, rc.flags, fr);
+      if constexpr (PREC == SS_PREC_FP64) {
+        trace_loop<ACC, TT>(sh, k, w, live, f, rc.n_samples, split,
+                            [&](const TraceRec& t, PairOut& o) {
+                              refracted_pair_fp64<0>(fr, t, rc, o);
+                            });
+      } else {
+        trace_loop<ACC, TT>(sh, k, w, live, f, rc.n_samples, split,
+                            [&](const TraceRec& t, PairOut& o) {
+                              if (!refracted_pair_mixed<0>(fr, t, rc, o))
+                                refracted_pair_fp64<0>(fr, t, rc, o);
+                            });
+      }
+    }
   }
 }
 
@@ -375,17 +557,6 @@ static int fill(double* p, long n, hipStream_t st) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("fill: ") + hipGetErrorString(e));
   return SS_OK;
-}
-
-static PatternArgs make_pattern(int kind, int n, double a, const double b[4],
-                                const double* table) {
-  PatternArgs p;
-  p.kind = kind;
-  p.n = n;
-  p.a = a;
-  for (int i = 0; i < 4; ++i) p.b[i] = b[i];
-  p.table = table;
-  return p;
 }
 
 static int check_pattern(int kind, int n, const double* table) {
@@ -504,26 +675,7 @@ int ss_surface_field(int precision, const ss_surface_params* prm, const double* 
   }
   KCommon k = make_common(facets, facet_stride, n_facets, traces, trace_stride, n_traces,
                           field, dropped, pair_bins, kTTSurf, n_seg);
-  SurfaceConst sc;
-  sc.k = prm->k;
-  sc.gamma = prm->gamma;
-  sc.t0 = prm->t0;
-  sc.dt = prm->dt;
-  sc.c = prm->c;
-  sc.taper_s = prm->taper_s;
-  sc.sigma = prm->sigma;
-  sc.lc = prm->corr_len;
-  sc.n_samples = prm->n_samples;
-  sc.flags = prm->flags;
-  sc.n_terms = prm->n_terms;
-  sc.acf = prm->acf;
-  sc.pat = make_pattern(prm->pattern, prm->pat_n, prm->pat_a, prm->pat_b, pattern_table);
-  sc.cst = constants;
-  sc.k_over_pi = prm->k / kPi;
-  sc.kfac = prm->k / kTwoPi;
-  sc.k_bin = 2.0 / (prm->c * prm->dt);
-  sc.t0_dt = prm->t0 / prm->dt;
-  sc.cyc_per_m = prm->k / kPi;
+  const SurfaceConst sc = make_surface_const(*prm, pattern_table, constants);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
   if (precision == SS_PREC_FP64)
     hipLaunchKernelGGL(surface_kernel<SS_PREC_FP64>, grid, block, 0, st, k, sc);
@@ -600,33 +752,8 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   }
   KCommon k = make_common(facets, facet_stride, n_facets, traces, trace_stride, n_traces, out,
                           dropped, pair_bins, kTTRefr, n_seg);
-  RefrConst rc;
-  rc.k0 = prm->k0;
-  rc.gamma = prm->gamma;
-  rc.t0 = prm->t0;
-  rc.dt = prm->dt;
-  rc.c = prm->c;
-  rc.taper_s = prm->taper_s;
-  rc.sigma = prm->sigma;
-  rc.lc = prm->corr_len;
-  rc.n_exp = prm->n_exp;
-  rc.n_samples = prm->n_samples;
-  rc.flags = prm->flags;
-  rc.n_crossed = prm->n_crossed;
-  rc.n_terms = prm->n_terms;
-  rc.acf = prm->acf;
-  rc.pat = make_pattern(prm->pattern, prm->pat_n, prm->pat_a, prm->pat_b, pattern_table);
-  rc.cst = constants;
-  rc.desc = iface_desc;
-  rc.tables = tables;
-  rc.legs = legs;
-  rc.joint = prm->refraction == 1;
-  rc.k_pad = prm->k_pad;
-  rc.n_newton = prm->joint_newton;
-  rc.n_backtrack = prm->joint_backtrack;
-  rc.k_bin = 2.0 / (prm->c * prm->dt);
-  rc.t0_dt = prm->t0 / prm->dt;
-  rc.cyc_per_m = prm->k0 / kPi;
+  const RefrConst rc = make_refr_const(*prm, iface_desc, tables, legs, pattern_table,
+                                       constants);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
   const bool fp64 = precision == SS_PREC_FP64;
   if (!rc.joint) {
@@ -649,6 +776,122 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("refracted launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+__global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ULL; }
+
+int64_t ss_sizeof_job(void) { return (int64_t)sizeof(ss_job); }
+
+static int validate_job(const ss_job& j, int idx) {
+  std::string pre = "job " + std::to_string(idx) + ": ";
+  if (j.kind != 0 && j.kind != 1) return fail(SS_EINVAL, pre + "kind must be 0 or 1");
+  if (j.n_facets < 0 || j.n_traces < 0) return fail(SS_EINVAL, pre + "negative sizes");
+  if (j.facet_stride < j.n_facets || j.trace_stride < j.n_traces)
+    return fail(SS_EINVAL, pre + "stride smaller than count");
+  if (j.n_traces > 0 && (!j.traces || !j.field || !j.dropped))
+    return fail(SS_EINVAL, pre + "null trace/output pointer");
+  if (j.n_facets > 0 && !j.facets) return fail(SS_EINVAL, pre + "null facets");
+  int n_samples, flags, pattern, pat_n, n_terms, acf;
+  double dt, c;
+  if (j.kind == 0) {
+    n_samples = j.surface.n_samples;
+    flags = j.surface.flags;
+    pattern = j.surface.pattern;
+    pat_n = j.surface.pat_n;
+    n_terms = j.surface.n_terms;
+    acf = j.surface.acf;
+    dt = j.surface.dt;
+    c = j.surface.c;
+  } else {
+    const ss_refracted_params& r = j.refracted;
+    n_samples = r.n_samples;
+    flags = r.flags;
+    pattern = r.pattern;
+    pat_n = r.pat_n;
+    n_terms = r.n_terms;
+    acf = r.acf;
+    dt = r.dt;
+    c = r.c;
+    if (r.refraction != 0)
+      return fail(SS_EINVAL, pre + "batched refracted jobs use the sequential chain");
+    if (r.n_crossed < 1 || r.n_crossed > SS_MAX_CROSSED)
+      return fail(SS_ERANGE, pre + "n_crossed out of range [1, 64]");
+    if (!j.iface_desc || !j.tables || !j.legs)
+      return fail(SS_EINVAL, pre + "null interface data");
+  }
+  if (n_samples <= 0) return fail(SS_EINVAL, pre + "n_samples must be > 0");
+  if (!(dt > 0.0) || !(c > 0.0)) return fail(SS_EINVAL, pre + "dt and c must be > 0");
+  if (flags & SS_ROUGH) {
+    if (!j.constants) return fail(SS_EINVAL, pre + "roughness needs the constants table");
+    if (n_terms < 0 || n_terms > SS_MAX_TERMS) return fail(SS_ERANGE, pre + "n_terms range");
+    if (acf == SS_ACF_EXPONENTIAL && !(flags & SS_D_PHI_AREA))
+      return fail(SS_EINVAL, pre + "exponential ACF requires the area-only D_Phi");
+  }
+  return check_pattern(pattern, pat_n, j.pattern_table);
+}
+
+int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* jobs_device,
+                   int64_t* counter_device, int trace_tile, int n_blocks, void* stream) {
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED)
+    return fail(SS_EINVAL, "precision must be SS_PREC_FP64 or SS_PREC_MIXED");
+  if (n_jobs < 0 || n_jobs > kMaxJobs) return fail(SS_ERANGE, "n_jobs out of range [0, 1024]");
+  if (trace_tile != 4 && trace_tile != 8 && trace_tile != 16)
+    return fail(SS_EINVAL, "trace_tile must be 4, 8 or 16");
+  if (n_jobs == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  if (!jobs_host || !jobs_device || !counter_device) return fail(SS_EINVAL, "null pointer");
+  long total = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    int r = validate_job(jobs_host[i], i);
+    if (r) return r;
+    total += job_work(jobs_host[i], trace_tile);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  for (int i = 0; i < n_jobs; ++i) {
+    const ss_job& j = jobs_host[i];
+    int n_seg_i = j.kind == 0
+                      ? ((j.surface.flags & SS_SPLIT_SIDES) ? 2 : 1) * j.surface.n_samples
+                      : ((j.refracted.flags & SS_SPLIT_SIDES) ? 2 : 1) * j.refracted.n_samples;
+    int r = fill(j.field, j.n_traces * (long)n_seg_i * 2, st);
+    if (r) return r;
+    r = fill(j.dropped, j.n_traces, st);
+    if (r) return r;
+  }
+  hipError_t e = hipMemcpyAsync(jobs_device, jobs_host, sizeof(ss_job) * (size_t)n_jobs,
+                                hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("job upload: ") + hipGetErrorString(e));
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(counter_device);
+  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, st, ctr);
+  if (total == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  if (n_blocks <= 0) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    n_blocks = 4 * cus;
+  }
+  if ((long)n_blocks > total) n_blocks = (int)total;
+  const ss_job* jd = (const ss_job*)jobs_device;
+  dim3 grid((unsigned)n_blocks), block(kBlock);
+  const bool fp64 = precision == SS_PREC_FP64;
+#define SS_LAUNCH_BATCH(P, T) \
+  hipLaunchKernelGGL((batch_kernel<P, T>), grid, block, 0, st, jd, n_jobs, total, ctr)
+  if (trace_tile == 4) {
+    if (fp64) SS_LAUNCH_BATCH(SS_PREC_FP64, 4); else SS_LAUNCH_BATCH(SS_PREC_MIXED, 4);
+  } else if (trace_tile == 8) {
+    if (fp64) SS_LAUNCH_BATCH(SS_PREC_FP64, 8); else SS_LAUNCH_BATCH(SS_PREC_MIXED, 8);
+  } else {
+    if (fp64) SS_LAUNCH_BATCH(SS_PREC_FP64, 16); else SS_LAUNCH_BATCH(SS_PREC_MIXED, 16);
+  }
+#undef SS_LAUNCH_BATCH
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
 }

@@ -178,17 +178,14 @@ def _ptr(t):
     return ctypes.c_void_p(0 if t is None else t.data_ptr())
 
 
-def surface_field(facets, traces, *, k, gamma, t0, dt, n_samples, c,
-                  precision="mixed", split_sides=False, interp_bins=False,
-                  pattern=None, roughness=None, taper_s=None, d_phi_area=False,
-                  out=None, dropped=None, pair_bins=False, stream=None):
-    """Enqueue the surface kernel.  Returns (field (T, n_seg, 2) float64,
-    dropped (T,) float64[, pair_bins (T, F) int32]) device tensors.
+def surface_params(facets, *, k, gamma, t0, dt, n_samples, c, split_sides=False,
+                   interp_bins=False, pattern=None, roughness=None, taper_s=None,
+                   d_phi_area=False):
+    """(SurfaceParams, pattern table tensor or None) for the surface kernel.
 
     ``roughness``: None or (sigma, corr_len, n_terms, acf); the phasors live
     in ``facets``.  ``pattern``: None or a ``pattern_args`` tuple whose pv
-    rows are already in ``traces``."""
-    dev = facets.device
+    rows are in the traces buffer."""
     flags = 0
     flags |= _lib.SS_SPLIT_SIDES if split_sides else 0
     flags |= _lib.SS_INTERP_BINS if interp_bins else 0
@@ -206,11 +203,25 @@ def surface_field(facets, traces, *, k, gamma, t0, dt, n_samples, c,
         flags |= _lib.SS_ROUGH
         p.sigma, p.corr_len, p.n_terms = float(sigma), float(lc), int(n_terms)
         p.acf = _lib.ACFS[acf]
-    kid, pn, pa, pb, table = pattern_setup(pattern, dev)
+    kid, pn, pa, pb, table = pattern_setup(pattern, facets.device)
     p.pattern, p.pat_n, p.pat_a = kid, pn, pa
     for i in range(4):
         p.pat_b[i] = pb[i]
     p.flags = flags
+    return p, table
+
+
+def surface_field(facets, traces, *, k, gamma, t0, dt, n_samples, c,
+                  precision="mixed", split_sides=False, interp_bins=False,
+                  pattern=None, roughness=None, taper_s=None, d_phi_area=False,
+                  out=None, dropped=None, pair_bins=False, stream=None):
+    """Enqueue the surface kernel.  Returns (field (T, n_seg, 2) float64,
+    dropped (T,) float64[, pair_bins (T, F) int32], keep-alive) device tensors."""
+    dev = facets.device
+    p, table = surface_params(facets, k=k, gamma=gamma, t0=t0, dt=dt, n_samples=n_samples,
+                              c=c, split_sides=split_sides, interp_bins=interp_bins,
+                              pattern=pattern, roughness=roughness, taper_s=taper_s,
+                              d_phi_area=d_phi_area)
     n_seg = (2 if split_sides else 1) * int(n_samples)
     T = traces.n
     if out is None:
@@ -299,17 +310,13 @@ class DeviceInterfaces:
         return (self.desc.numel() + self.tables.numel()) * 8
 
 
-def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
-                    n_samples, c, gamma=0.0, k0=None, precision="mixed",
-                    split_sides=False, pattern=None, roughness=None,
-                    taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
-                    joint_newton=6, joint_backtrack=4, out=None,
-                    dropped=None, pair_bins=False, stream=None):
-    """Enqueue the refracted kernel (sequential chain or joint solve over the
-    possibly padded stack of ``ifaces``; ``eps_leg``/``att_leg`` are the
-    unpadded per-leg media).  Returns (out (T, n_seg, 2) float64, dropped
-    (T,)[, pair_bins]) device tensors."""
-    dev = target.device
+def refracted_params(target, ifaces, eps_leg, att_leg, *, mode, t0, dt, n_samples, c,
+                     gamma=0.0, k0=None, split_sides=False, pattern=None, roughness=None,
+                     taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
+                     joint_newton=6, joint_backtrack=4):
+    """(RefractedParams, legs tensor, pattern table or None) for the
+    refracted kernel over the (possibly padded) stack ``ifaces``;
+    ``eps_leg``/``att_leg`` are the unpadded per-leg media."""
     coherent = mode == "coherent"
     flags = _lib.SS_COHERENT if coherent else 0
     flags |= _lib.SS_SPLIT_SIDES if split_sides else 0
@@ -333,7 +340,7 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
         flags |= _lib.SS_ROUGH
         p.sigma, p.corr_len, p.n_terms = float(sigma), float(lc), int(n_terms)
         p.acf = _lib.ACFS[acf]
-    kid, pn, pa, pb, table = pattern_setup(pattern, dev)
+    kid, pn, pa, pb, table = pattern_setup(pattern, target.device)
     p.pattern, p.pat_n, p.pat_a = kid, pn, pa
     for i in range(4):
         p.pat_b[i] = pb[i]
@@ -345,8 +352,26 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
         att = np.concatenate([np.full(ifaces.k_pad, att[0]), att])
     legs = np.stack([np.sqrt(eps), eps, att])
     if legs.shape[1] != ifaces.n + 1:
-        raise ValueError(f"need {ifaces.n + 1} legs, got {legs.shape[1]}")
-    legs_t = torch.as_tensor(np.ascontiguousarray(legs), device=dev)
+        raise ValueError(f"need {ifaces.n + 1 - ifaces.k_pad} legs, got {len(eps_leg)}")
+    legs_t = torch.as_tensor(np.ascontiguousarray(legs), device=target.device)
+    return p, legs_t, table
+
+
+def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
+                    n_samples, c, gamma=0.0, k0=None, precision="mixed",
+                    split_sides=False, pattern=None, roughness=None,
+                    taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
+                    joint_newton=6, joint_backtrack=4, out=None,
+                    dropped=None, pair_bins=False, stream=None):
+    """Enqueue the refracted kernel (sequential chain or joint solve).
+    Returns (out (T, n_seg, 2) float64, dropped (T,)[, pair_bins],
+    keep-alive) device tensors."""
+    dev = target.device
+    p, legs_t, table = refracted_params(
+        target, ifaces, eps_leg, att_leg, mode=mode, t0=t0, dt=dt, n_samples=n_samples,
+        c=c, gamma=gamma, k0=k0, split_sides=split_sides, pattern=pattern,
+        roughness=roughness, taper_s=taper_s, d_phi_area=d_phi_area, n_exp=n_exp,
+        refraction=refraction, joint_newton=joint_newton, joint_backtrack=joint_backtrack)
     n_seg = (2 if split_sides else 1) * int(n_samples)
     T = traces.n
     if out is None:

@@ -99,3 +99,44 @@ class ResidentScene:
             self.run_layer(j, precision, stream)
         self.compress(stream)
         return self.out
+
+    def jobs(self, precision, out=None, dropped=None):
+        """``ss_job`` records (one per layer; sequential refraction) writing
+        into this scene's ``raw``/``dropped`` buffers (or the given ones), and
+        the tensors that must outlive the launch."""
+        from . import _lib
+        wl = self.wl
+        out = self.raw if out is None else out
+        dropped = self.dropped if dropped is None else dropped
+        jobs, keep = [], []
+        cst = dv.constants(self.device)
+        for j, (L, fac, ifc, rough) in enumerate(self.layers):
+            jb = _lib.Job()
+            jb.facet_stride, jb.n_facets = fac.stride, fac.n
+            jb.trace_stride, jb.n_traces = self.traces.stride, self.traces.n
+            jb.facets, jb.traces = fac.buf.data_ptr(), self.traces.buf.data_ptr()
+            jb.field, jb.dropped = out[j].data_ptr(), dropped[j].data_ptr()
+            jb.constants = cst.data_ptr()
+            g = 0.0 if fac.has_gamma else L.gamma
+            if ifc is None:
+                p, table = dv.surface_params(
+                    fac, k=wl.k0, gamma=g, t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples,
+                    c=wl.c, split_sides=wl.split_sides, pattern=wl.pattern, roughness=rough,
+                    taper_s=wl.taper_s, d_phi_area=wl.d_phi_area)
+                jb.kind = 0
+                jb.surface = p
+            else:
+                p, legs, table = dv.refracted_params(
+                    fac, ifc, L.eps_leg, L.att_leg, mode="coherent", t0=wl.t0, dt=wl.dt,
+                    n_samples=wl.n_samples, c=wl.c, gamma=g, k0=wl.k0,
+                    split_sides=wl.split_sides, pattern=wl.pattern, roughness=rough,
+                    taper_s=wl.taper_s, d_phi_area=wl.d_phi_area)
+                jb.kind = 1
+                jb.refracted = p
+                jb.iface_desc, jb.tables = ifc.desc.data_ptr(), ifc.tables.data_ptr()
+                jb.legs = legs.data_ptr()
+                keep.append(legs)
+            jb.pattern_table = 0 if table is None else table.data_ptr()
+            keep.append(table)
+            jobs.append(jb)
+        return jobs, keep

@@ -35,6 +35,7 @@ def test_struct_layouts_match_header():
     # doubles first then int32s: no padding; sizes pinned against the header
     assert ctypes.sizeof(_lib.SurfaceParams) == 13 * 8 + 6 * 4
     assert ctypes.sizeof(_lib.RefractedParams) == 14 * 8 + 12 * 4
+    assert ctypes.sizeof(_lib.Job) == _lib.load().ss_sizeof_job() == 400
 
 
 def test_invalid_arguments_rejected_on_host():
@@ -60,6 +61,12 @@ def test_invalid_arguments_rejected_on_host():
     r.n_crossed, r.flags = 1, _lib.SS_ROUGH                  # incoherent + roughness
     assert lib.ss_refracted_field(0, *rargs) == -1
     assert lib.ss_range_compress(None, 1, 8, None, 1, None, None) == -1
+    assert lib.ss_batch_field(0, None, 0, None, None, 8, 0, None) == 0   # empty batch
+    assert lib.ss_batch_field(0, None, 1, None, None, 3, 0, None) == -1  # bad tile
+    j = _lib.Job()
+    j.kind = 7
+    assert lib.ss_batch_field(0, ctypes.byref(j), 1, 1, 1, 8, 0, None) == -1
+    assert b"kind" in lib.ss_last_error()
 
 
 def test_fill_constants():
