@@ -60,6 +60,8 @@ extern "C" {
 #define SS_COHERENT 0x040     /* refracted: coherent (field) vs incoherent  */
 #define SS_CROSS_ROUGH 0x080  /* refracted: crossing roughness sig_c        */
 #define SS_DIFFUSE 0x100      /* refracted: diffuse channel                 */
+#define SS_REDO_ALL 0x200     /* refracted, testing: every mixed-tier pair  */
+                              /* takes the float64 redo launch              */
 
 /* antenna pattern kinds (antenna.py gain_fn) */
 #define SS_PAT_ISOTROPIC 0
@@ -216,7 +218,12 @@ int ss_surface_field(int precision, const ss_surface_params* params,
  * kernels/multilayer.py:594-770 (refracted_cluttergram) and _refracted_fn
  * (:333-591), refraction_joint.py:249-322 (joint_crossings).  iface_desc: (n_crossed, SS_IDESC_COUNT) float64 device;
  * tables: concatenated (sum ny*nx, 6) float64 [centre, normal] device;
- * legs: (3, n_crossed + 1) float64 device rows [n, eps, att_db_per_km]. */
+ * legs: (3, n_crossed + 1) float64 device rows [n, eps, att_db_per_km].
+ * workspace: caller-owned device memory, 256-byte aligned, of at least
+ * ss_refracted_workspace_bytes(precision, n_facets, n_traces) bytes (may be
+ * NULL when that is 0, i.e. for SS_PREC_FP64).  It holds the queue of pairs
+ * the mixed tier hands to its float64 redo launch; the call resets it on the
+ * stream, so concurrent calls need distinct workspaces. */
 int ss_refracted_field(int precision, const ss_refracted_params* params,
                        const double* facets, int64_t facet_stride,
                        int64_t n_facets, const double* traces,
@@ -224,20 +231,37 @@ int ss_refracted_field(int precision, const ss_refracted_params* params,
                        const double* iface_desc, const double* tables,
                        const double* legs, const double* pattern_table,
                        const double* constants, double* out, double* dropped,
-                       int32_t* pair_bins, void* stream);
+                       int32_t* pair_bins, void* workspace, int64_t workspace_bytes,
+                       void* stream);
+
+/* Workspace bytes ss_refracted_field needs: 0 for SS_PREC_FP64 or an empty
+ * call; for SS_PREC_MIXED 256 + 264 bytes per (8-trace, 256-facet) work
+ * item (the float64 redo queue and its lane masks).  No counterpart in the
+ * reference (its kernels run one precision). */
+int64_t ss_refracted_workspace_bytes(int precision, int64_t n_facets, int64_t n_traces);
 
 /* Batched persistent launch over uneven jobs: every (job, trace tile, facet
  * tile) work item of ``n_jobs`` jobs is drained from one device work counter
  * by ``n_blocks`` persistent workgroups (0 = a default of 4 per CU).  The
  * job table is copied from host memory into ``jobs_device`` (caller-owned
- * device buffer of at least n_jobs * sizeof(ss_job) bytes) and
- * ``counter_device`` (one int64 of device memory) is reset on the stream, so
- * the call allocates nothing.  trace_tile: traces per workgroup (4, 8 or
- * 16; chosen by the autotuner).  No counterpart in the reference (its
- * requests run one jit call after another); serves config 4 and the request
- * loop. */
+ * device buffer of at least n_jobs * sizeof(ss_job) bytes).  ``workspace``
+ * (caller-owned device memory, 256-byte aligned, at least
+ * ss_batch_workspace_bytes() bytes) holds the work counter and, for
+ * SS_PREC_MIXED, the float64 redo queue; it is reset on the stream, so the
+ * call allocates nothing.  trace_tile: traces per workgroup (4, 8 or 16;
+ * chosen by the autotuner).  No counterpart in the reference (its requests
+ * run one jit call after another); serves batched and continuous-batching
+ * callers. */
 int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* jobs_device,
-                   int64_t* counter_device, int trace_tile, int n_blocks, void* stream);
+                   void* workspace, int64_t workspace_bytes, int trace_tile, int n_blocks,
+                   void* stream);
+
+/* Workspace bytes of ss_batch_field for these jobs (host table) and trace
+ * tile: 256 for SS_PREC_FP64, plus 8 + 32 * trace_tile bytes per work item
+ * for SS_PREC_MIXED; -1 for an invalid trace_tile.  No counterpart in the
+ * reference. */
+int64_t ss_batch_workspace_bytes(int precision, const ss_job* jobs_host, int n_jobs,
+                                 int trace_tile);
 
 /* sizeof(ss_job) as compiled into the library (for sizing jobs_device and
  * checking a binding's struct mirror).  No counterpart in the reference. */

@@ -23,6 +23,7 @@ SS_GAMMA_FACET = 0x020
 SS_COHERENT = 0x040
 SS_CROSS_ROUGH = 0x080
 SS_DIFFUSE = 0x100
+SS_REDO_ALL = 0x200
 
 PATTERNS = {"isotropic": 0, "dipole": 1, "array": 2, "array_tapered": 3,
             "finite_dipole": 4, "tabulated": 5}
@@ -40,7 +41,8 @@ CONST_COUNT = 33 + MAX_TERMS + 1
 
 EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_fill_constants", "ss_surface_field", "ss_refracted_field",
-            "ss_batch_field", "ss_sizeof_job", "ss_range_compress")
+            "ss_refracted_workspace_bytes", "ss_batch_field", "ss_batch_workspace_bytes",
+            "ss_sizeof_job", "ss_range_compress")
 
 
 class SurfaceParams(ctypes.Structure):
@@ -106,10 +108,14 @@ def load():
     lib.ss_surface_field.restype = ctypes.c_int
     lib.ss_refracted_field.argtypes = [i32, ctypes.POINTER(RefractedParams), dp,
                                        i64, i64, dp, i64, i64, dp, dp, dp, dp,
-                                       dp, dp, dp, dp, vp]
+                                       dp, dp, dp, dp, dp, i64, vp]
     lib.ss_refracted_field.restype = ctypes.c_int
-    lib.ss_batch_field.argtypes = [i32, ctypes.POINTER(Job), i32, dp, dp, i32, i32, vp]
+    lib.ss_refracted_workspace_bytes.argtypes = [i32, i64, i64]
+    lib.ss_refracted_workspace_bytes.restype = ctypes.c_int64
+    lib.ss_batch_field.argtypes = [i32, ctypes.POINTER(Job), i32, dp, dp, i64, i32, i32, vp]
     lib.ss_batch_field.restype = ctypes.c_int
+    lib.ss_batch_workspace_bytes.argtypes = [i32, ctypes.POINTER(Job), i32, i32]
+    lib.ss_batch_workspace_bytes.restype = ctypes.c_int64
     lib.ss_sizeof_job.restype = ctypes.c_int64
     lib.ss_range_compress.argtypes = [dp, i64, i64, dp, i64, dp, vp]
     lib.ss_range_compress.restype = ctypes.c_int

@@ -1,7 +1,8 @@
 """Batched persistent launches over uneven requests (``ss_batch_field``).
 
-A ``BatchRunner`` owns the two small device buffers the C entry point needs
-(the job table and the work counter), so a launch allocates nothing.  Every
+A ``BatchRunner`` owns the device buffers the C entry point needs (the job
+table and the workspace: work counter + float64 redo queue), grown only when
+a larger batch arrives, so a steady launch allocates nothing.  Every
 layer of every scene becomes one ``ss_job``; one persistent launch drains all
 (job, trace tile, facet tile) work items; range compression follows per scene.
 
@@ -32,7 +33,7 @@ class BatchRunner:
         self.device = dv._dev(device)
         self.size = int(_lib.load().ss_sizeof_job())
         self.jobs_dev = torch.empty(max_jobs * self.size, dtype=torch.uint8, device=self.device)
-        self.counter = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.workspace = torch.empty(256, dtype=torch.uint8, device=self.device)
         self.max_jobs = max_jobs
         self._host = None
 
@@ -44,10 +45,16 @@ class BatchRunner:
         arr = (_lib.Job * max(n, 1))(*jobs)
         self._host = arr            # the async copy reads it; keep it alive
         prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
-        rc = _lib.load().ss_batch_field(prec, arr, n, ctypes.c_void_p(self.jobs_dev.data_ptr()),
-                                        ctypes.c_void_p(self.counter.data_ptr()),
-                                        int(trace_tile), int(n_blocks),
-                                        dv._stream_handle(stream))
+        lib = _lib.load()
+        need = int(lib.ss_batch_workspace_bytes(prec, arr, n, int(trace_tile)))
+        if need < 0:
+            raise ValueError(f"trace_tile must be 4, 8 or 16, got {trace_tile}")
+        if need > self.workspace.numel():
+            self.workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
+        rc = lib.ss_batch_field(prec, arr, n, ctypes.c_void_p(self.jobs_dev.data_ptr()),
+                                ctypes.c_void_p(self.workspace.data_ptr()),
+                                self.workspace.numel(), int(trace_tile), int(n_blocks),
+                                dv._stream_handle(stream))
         _lib.check(rc, "ss_batch_field")
 
     def run_scenes(self, scenes, precision="mixed", trace_tile=8, n_blocks=0, stream=None,

@@ -378,15 +378,15 @@ SS_HD void joint_path(const double p[3], const double q[3], const RefrConst& rc,
 
 // Top-level refracted pair: crossing solver (sequential chain, or joint
 // solve when rc.joint with a stack at most NMAX deep) then radiometry.
-template <int NMAX>
-SS_HD void joint_sums(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
+template <int NMAX, typename FT>
+SS_HD void joint_sums(const FT& f, const TraceRec& t, const RefrConst& rc,
                       PathSums& ps) {
   joint_path<NMAX>(t.p, f.c, rc, rc.k_pad, rc.n_newton, rc.n_backtrack, ps.cur, ps.valid,
                    ps.opl, ps.loss_db, ps.sum_par, ps.sum_perp, ps.tau2, ps.c_first,
                    ps.c_last, ps.x_first);
 }
 
-template <int NMAX>
+template <int NMAX, int RM = -1, int PAT = -1>
 SS_HD void refracted_pair_fp64(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
                                PairOut& o) {
   PathSums ps;
@@ -394,20 +394,20 @@ SS_HD void refracted_pair_fp64(const RefrFacet& f, const TraceRec& t, const Refr
     joint_sums<(NMAX > 0 ? NMAX : 1)>(f, t, rc, ps);
   else
     seq_path_fp64(f, t, rc, ps);
-  radiometry_fp64(f, t, rc, ps, o);
+  radiometry_fp64<RM, PAT>(f, t, rc, ps, o);
 }
 
 // Mixed tier: the joint solve itself stays float64 (its damped Newton needs
 // the float64 merit comparisons); radiometry runs in the mixed tier.
-template <int NMAX>
-SS_HD bool refracted_pair_mixed(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
+template <int NMAX, int RM = -1, int PAT = -1, typename FT = RefrFacet>
+SS_HD bool refracted_pair_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
                                 PairOut& o) {
   PathSums ps;
   if (NMAX > 0 && rc.joint)
     joint_sums<(NMAX > 0 ? NMAX : 1)>(f, t, rc, ps);
   else if (!seq_path_mixed(f, t, rc, ps))
     return false;
-  return radiometry_mixed(f, t, rc, ps, o);
+  return radiometry_mixed<RM, PAT, FT>(f, t, rc, ps, o);
 }
 
 }  // namespace ss

@@ -18,6 +18,17 @@
 // Block ids are remapped so each XCD gets a contiguous range of
 // (trace tile, facet tile) work, keeping a trace tile's output rows in one
 // XCD's L2.
+//
+// Deferred float64 redo (mixed refracted kernels).  A pair whose reduced-
+// precision path cannot be resolved to the float64 bin/cell decisions (a
+// residual in the ambiguous band, a bin position within the guard of an
+// edge) contributes nothing in the hot kernel; each wave records the lanes
+// concerned per trace as a 64-bit ballot in LDS, and a work item with any
+// such lane appends its id and masks to a queue in the caller's workspace.
+// A second launch (redo_kernel) evaluates exactly those pairs on the float64
+// tier and adds them with global atomics.  The queue holds one slot per work
+// item, so it cannot overflow, and the hot kernel never carries the float64
+// path's registers (about 130 vs 280 VGPRs).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -80,7 +91,32 @@ struct Shared {
   int wmin[2][4];
   int wmax[2][4];
   double drop[TT];
+  unsigned long long redo[TT][4];  // per trace, per wave: lanes left to the redo
 };
+
+// Workspace header of the entry points that take a workspace.
+constexpr long kWsHeader = 256;  // [0] work counter, [1] redo queue length
+
+// Redo queue of one launch: ``queue`` holds work-item ids, ``masks`` holds
+// TT x 4 lane masks per work item (indexed by the item id).
+struct Redo {
+  unsigned long long* count;
+  long long* queue;
+  unsigned long long* masks;
+};
+
+__host__ __device__ inline Redo redo_view(void* ws, long n_items) {
+  Redo r;
+  char* b = static_cast<char*>(ws);
+  r.count = reinterpret_cast<unsigned long long*>(b) + 1;
+  r.queue = reinterpret_cast<long long*>(b + kWsHeader);
+  r.masks = reinterpret_cast<unsigned long long*>(b + kWsHeader + 8 * n_items);
+  return r;
+}
+
+__host__ __device__ inline long redo_bytes(long n_items, int tt) {
+  return kWsHeader + 8 * n_items + 8L * 4 * tt * n_items;
+}
 
 template <typename ACC, int TT>
 __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int buf,
@@ -104,11 +140,12 @@ __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int b
 }
 
 // Trace loop shared by both kernel families.  PAIR(t, o) evaluates this
-// lane's facet against trace record t into o.
-template <typename ACC, int TT, typename PairFn>
+// lane's facet against trace record t into o.  REDO: pairs with o.redo set
+// are queued under work item ``item`` of ``rd`` instead of accumulated.
+template <typename ACC, int TT, bool REDO = false, typename PairFn>
 __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
                                   bool live, long f, int n_samples, bool split,
-                                  PairFn pair) {
+                                  PairFn pair, const Redo& rd = Redo{}, long item = 0) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long tt_idx = w / k.n_ftiles;
   const long t_base = tt_idx * TT;
@@ -130,10 +167,17 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     PairOut o;
     o.nparts = 0;
     o.valid = false;
+    o.redo = false;
     if (live) pair(sh.tr[it], o);
+    if constexpr (REDO) {
+      const unsigned long long m = __ballot(live && o.redo);
+      if (lane == 0) sh.redo[it][wave] = m;
+    }
     int lmin = INT_MAX, lmax = INT_MIN;
     if (live && o.valid) {
-      for (int p = 0; p < o.nparts; ++p) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {  // fixed trip count keeps o in registers
+        if (p >= o.nparts) break;
         int b = o.bin[p];
         if (b >= 0 && b < n_samples) {
           lmin = min(lmin, b);
@@ -161,7 +205,9 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
       if (!o.valid) {
         if (o.pwr_invalid != 0.0) atomicAdd(&sh.drop[it], o.pwr_invalid);
       } else {
-        for (int p = 0; p < o.nparts; ++p) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          if (p >= o.nparts) break;
           int b = o.bin[p];
           double re = o.re[p], im = o.im[p];
           if (b >= 0 && b < n_samples) {
@@ -181,7 +227,7 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
           }
         }
       }
-      if (k.pair_bins) k.pair_bins[t * k.nf + f] = o.raw_bin;
+      if (k.pair_bins && !o.redo) k.pair_bins[t * k.nf + f] = o.raw_bin;
     }
     prev_base = base;
     prev_span = span;
@@ -192,6 +238,48 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
                           n_samples, split);
   __syncthreads();
   if (tid < nT && sh.drop[tid] != 0.0) atomicAdd(&k.dropped[t_base + tid], sh.drop[tid]);
+  if constexpr (REDO) {
+    // every wave's ballot of every trace is visible after the barriers above
+    bool any = false;
+    for (int i = 0; i < nT; ++i)
+      any |= ((sh.redo[i][0] | sh.redo[i][1]) | (sh.redo[i][2] | sh.redo[i][3])) != 0ULL;
+    if (any) {
+      if (tid < TT * 4)
+        rd.masks[item * (TT * 4) + tid] = (tid >> 2) < nT ? sh.redo[tid >> 2][tid & 3] : 0ULL;
+      if (tid == 0) rd.queue[atomicAdd(rd.count, 1ULL)] = item;
+    }
+  }
+}
+
+__device__ inline TraceRec load_trace_rec(const KCommon& k, long t) {
+  TraceRec r;
+  double* d = reinterpret_cast<double*>(&r);
+  for (int i = 0; i < SS_TROW_COUNT; ++i) d[i] = k.traces[(long)i * k.ts + t];
+  return r;
+}
+
+// Adds one pair to the field / dropped power with global atomics (the redo
+// path; same bookkeeping as the LDS path of trace_loop).
+__device__ inline void accumulate_global(const KCommon& k, long t, const PairOut& o,
+                                         int n_samples, bool split) {
+  if (!o.valid) {
+    if (o.pwr_invalid != 0.0) atomicAdd(&k.dropped[t], o.pwr_invalid);
+    return;
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    if (p >= o.nparts) break;
+    const int b = o.bin[p];
+    const double re = o.re[p], im = o.im[p];
+    if (b >= 0 && b < n_samples) {
+      const long idx = (t * k.n_seg + (long)(split ? o.right : 0) * n_samples + b) * 2;
+      if (re != 0.0) atomicAdd(&k.field[idx], re);
+      if (im != 0.0) atomicAdd(&k.field[idx + 1], im);
+    } else {
+      const double pw = re * re + im * im;
+      if (pw != 0.0) atomicAdd(&k.dropped[t], pw);
+    }
+  }
 }
 
 // Kernel-side constants from the ABI parameter structs.
@@ -335,7 +423,10 @@ __device__ inline void load_refr_facet(const KCommon& k, long fi, int flags, Ref
 }
 
 // ----------------------------------------------------------------- surface
-template <int PREC>
+// RM: roughness mode (0 none, 1 area-only D_Phi, 2 finite-facet Gaussian,
+// -1 any); PAT: antenna kind (0 isotropic, 5 tabulated, -1 any).  The host
+// picks the narrowest instantiation so unused features cost no registers.
+template <int PREC, int RM, int PAT>
 __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst sc) {
   using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
   __shared__ Shared<ACC, kTTSurf> sh;
@@ -349,14 +440,14 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
   if constexpr (PREC == SS_PREC_FP64) {
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o) {
-                               surface_pair_fp64(f64, t, sc, o);
+                               surface_pair_fp64<RM, PAT>(f64, t, sc, o);
                              });
   } else {
     SurfFacet32 f32;
     to_f32(f64, f32);
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o) {
-                               surface_pair_mixed(f32, nullptr, t, sc, o);
+                               surface_pair_mixed<RM, PAT>(f32, nullptr, t, sc, o);
                              });
   }
 }
@@ -382,29 +473,88 @@ __global__ void fold_kernel(double* field, const double* traces, long ts, long n
 
 // --------------------------------------------------------------- refracted
 // NMAX = 0: sequential chain only; NMAX > 0: joint solve for stacks <= NMAX.
-template <int PREC, int NMAX>
-__global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc) {
+// The mixed instantiations leave unresolved pairs to redo_kernel (queue rd).
+template <int PREC, int NMAX, int RM, int PAT>
+__global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc, Redo rd) {
   using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
   __shared__ Shared<ACC, kTTRefr> sh;
   const long w = work_id(k);
   if (w < 0) return;
   const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
   const bool live = f < k.nf;
-  RefrFacet fr;
-  load_refr_facet(k, live ? f : 0, rc.flags, fr);
+  const long fi = live ? f : 0;
   const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
   if constexpr (PREC == SS_PREC_FP64) {
+    RefrFacet fr;
+    load_refr_facet(k, fi, rc.flags, fr);
     trace_loop<ACC, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o) {
-                               refracted_pair_fp64<NMAX>(fr, t, rc, o);
+                               refracted_pair_fp64<NMAX, RM, PAT>(fr, t, rc, o);
                              });
   } else {
-    trace_loop<ACC, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
-                             [&](const TraceRec& t, PairOut& o) {
-                               if (!refracted_pair_mixed<NMAX>(fr, t, rc, o))
-                                 refracted_pair_fp64<NMAX>(fr, t, rc, o);
-                             });
+    RefrFacet32 f32;
+    {
+      RefrFacet fr;
+      load_refr_facet(k, fi, rc.flags, fr);
+      to_f32(fr, f32);
+    }
+    trace_loop<ACC, kTTRefr, true>(sh, k, w, live, f, rc.n_samples, split,
+                                   [&](const TraceRec& t, PairOut& o) {
+                                     if ((rc.flags & SS_REDO_ALL) ||
+                                         !refracted_pair_mixed<NMAX, RM, PAT>(f32, t, rc, o)) {
+                                       o.valid = false;
+                                       o.pwr_invalid = 0.0;
+                                       o.redo = true;
+                                     }
+                                   },
+                                   rd, w);
   }
+}
+
+// Float64 evaluation of the queued pairs of work item w (TT traces x 256
+// facets; mask word [trace][wave], bit = lane), accumulated with global
+// atomics.  Every lane of the block runs this; only flagged lanes work.
+template <int NMAX, int RM, int PAT, int TT>
+__device__ inline void redo_item(const KCommon& k, const RefrConst& rc, long w,
+                                 const unsigned long long* m) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long f = (w % k.n_ftiles) * kBlock + tid;
+  const long t_base = (w / k.n_ftiles) * TT;
+  unsigned bits = 0;
+  for (int it = 0; it < TT; ++it) bits |= (unsigned)((m[it * 4 + wave] >> lane) & 1ULL) << it;
+  if (bits == 0 || f >= k.nf) return;
+  const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
+  RefrFacet fr;
+  load_refr_facet(k, f, rc.flags, fr);
+  for (int it = 0; it < TT; ++it) {
+    if (!((bits >> it) & 1u)) continue;
+    const long t = t_base + it;
+    if (t >= k.nt) break;
+    const TraceRec tr = load_trace_rec(k, t);
+    PairOut o;
+    o.nparts = 0;
+    o.valid = false;
+    o.redo = false;
+    refracted_pair_fp64<NMAX, RM, PAT>(fr, tr, rc, o);
+    accumulate_global(k, t, o, rc.n_samples, split);
+    if (k.pair_bins) k.pair_bins[t * k.nf + f] = o.raw_bin;
+  }
+}
+
+// Drains the redo queue of one refracted_kernel launch.  The queue length is
+// read on the device, so the launch needs no host synchronisation; with an
+// empty queue every block exits at once.
+template <int NMAX, int RM, int PAT>
+__global__ __launch_bounds__(kBlock) void redo_kernel(KCommon k, RefrConst rc, Redo rd) {
+  const long n = (long)*rd.count;
+  for (long i = blockIdx.x; i < n; i += gridDim.x) {
+    const long w = rd.queue[i];
+    redo_item<NMAX, RM, PAT, kTTRefr>(k, rc, w, rd.masks + w * (kTTRefr * 4));
+  }
+}
+
+__global__ void reset_workspace_kernel(unsigned long long* hdr) {
+  if (threadIdx.x < 2) hdr[threadIdx.x] = 0ULL;
 }
 
 // ------------------------------------------------------------ batched launch
@@ -417,16 +567,9 @@ __host__ __device__ inline long job_work(const ss_job& j, int tt) {
   return nft * ntt;
 }
 
-// Persistent workgroups drain (job, trace tile, facet tile) items from one
-// counter; every workgroup exits once the counter passes the total.
-template <int PREC, int TT>
-__global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict__ jobs,
-                                                      int n_jobs, long total,
-                                                      unsigned long long* counter) {
-  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
-  __shared__ Shared<ACC, TT> sh;
-  __shared__ long begins[kMaxJobs + 1];
-  __shared__ long s_w;
+// LDS prefix of per-job work counts: begins[i] = first work id of job i.
+template <int TT>
+__device__ inline void job_prefix(const ss_job* jobs, int n_jobs, long* begins) {
   const int tid = threadIdx.x;
   for (int i = tid; i < n_jobs; i += kBlock) begins[i + 1] = job_work(jobs[i], TT);
   __syncthreads();
@@ -435,40 +578,69 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
     for (int i = 0; i < n_jobs; ++i) begins[i + 1] += begins[i];
   }
   __syncthreads();
+}
+
+// Job owning global work id gw (begins[lo] <= gw < begins[lo + 1]).
+__device__ inline int find_job(const long* begins, int n_jobs, long gw) {
+  int lo = 0, hi = n_jobs;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (begins[mid] <= gw)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int TT>
+__device__ inline KCommon job_common(const ss_job& jb) {
+  KCommon k;
+  k.facets = jb.facets;
+  k.fs = jb.facet_stride;
+  k.nf = jb.n_facets;
+  k.traces = jb.traces;
+  k.ts = jb.trace_stride;
+  k.nt = jb.n_traces;
+  k.field = jb.field;
+  k.dropped = jb.dropped;
+  k.pair_bins = nullptr;
+  k.n_ftiles = (int)((jb.n_facets + kBlock - 1) / kBlock);
+  k.n_ttiles = (int)((jb.n_traces + TT - 1) / TT);
+  k.per_xcd = 0;
+  const int n = jb.kind == 0 ? jb.surface.n_samples : jb.refracted.n_samples;
+  const int fl = jb.kind == 0 ? jb.surface.flags : jb.refracted.flags;
+  k.n_seg = ((fl & SS_SPLIT_SIDES) ? 2 : 1) * n;
+  return k;
+}
+
+// Persistent workgroups drain (job, trace tile, facet tile) items from one
+// counter; every workgroup exits once the counter passes the total.  Mixed
+// refracted items queue their unresolved pairs under their global work id.
+template <int PREC, int TT>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict__ jobs,
+                                                      int n_jobs, long total,
+                                                      unsigned long long* counter, Redo rd) {
+  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  __shared__ Shared<ACC, TT> sh;
+  __shared__ long begins[kMaxJobs + 1];
+  __shared__ long s_w;
+  const int tid = threadIdx.x;
+  job_prefix<TT>(jobs, n_jobs, begins);
   for (;;) {
     if (tid == 0) s_w = (long)atomicAdd(counter, 1ULL);
     __syncthreads();
     const long gw = s_w;
     __syncthreads();
     if (gw >= total) break;
-    int lo = 0, hi = n_jobs;  // begins[lo] <= gw < begins[hi]
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (begins[mid] <= gw)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const ss_job& jb = jobs[lo];
-    const long w = gw - begins[lo];
-    KCommon k;
-    k.facets = jb.facets;
-    k.fs = jb.facet_stride;
-    k.nf = jb.n_facets;
-    k.traces = jb.traces;
-    k.ts = jb.trace_stride;
-    k.nt = jb.n_traces;
-    k.field = jb.field;
-    k.dropped = jb.dropped;
-    k.pair_bins = nullptr;
-    k.n_ftiles = (int)((jb.n_facets + kBlock - 1) / kBlock);
-    k.n_ttiles = (int)((jb.n_traces + TT - 1) / TT);
-    k.per_xcd = 0;
+    const int j = find_job(begins, n_jobs, gw);
+    const ss_job& jb = jobs[j];
+    const long w = gw - begins[j];
+    const KCommon k = job_common<TT>(jb);
     const long f = (w % k.n_ftiles) * kBlock + tid;
     const bool live = f < k.nf;
     if (jb.kind == 0) {
       const SurfaceConst sc = make_surface_const(jb.surface, jb.pattern_table, jb.constants);
-      k.n_seg = ((sc.flags & SS_SPLIT_SIDES) ? 2 : 1) * sc.n_samples;
       const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
       SurfFacet64 f64;
       load_surface_facet(k, live ? f : 0, sc.flags, f64);
@@ -488,23 +660,53 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
     } else {
       const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
                                            jb.pattern_table, jb.constants);
-      k.n_seg = ((rc.flags & SS_SPLIT_SIDES) ? 2 : 1) * rc.n_samples;
       const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
-      RefrFacet fr;
-      load_refr_facet(k, live ? f : 0, rc.flags, fr);
+      const long fi = live ? f : 0;
       if constexpr (PREC == SS_PREC_FP64) {
+        RefrFacet fr;
+        load_refr_facet(k, fi, rc.flags, fr);
         trace_loop<ACC, TT>(sh, k, w, live, f, rc.n_samples, split,
                             [&](const TraceRec& t, PairOut& o) {
                               refracted_pair_fp64<0>(fr, t, rc, o);
                             });
       } else {
-        trace_loop<ACC, TT>(sh, k, w, live, f, rc.n_samples, split,
-                            [&](const TraceRec& t, PairOut& o) {
-                              if (!refracted_pair_mixed<0>(fr, t, rc, o))
-                                refracted_pair_fp64<0>(fr, t, rc, o);
-                            });
+        RefrFacet32 f32;
+        {
+          RefrFacet fr;
+          load_refr_facet(k, fi, rc.flags, fr);
+          to_f32(fr, f32);
+        }
+        trace_loop<ACC, TT, true>(sh, k, w, live, f, rc.n_samples, split,
+                                  [&](const TraceRec& t, PairOut& o) {
+                                    if ((rc.flags & SS_REDO_ALL) ||
+                                        !refracted_pair_mixed<0>(f32, t, rc, o)) {
+                                      o.valid = false;
+                                      o.pwr_invalid = 0.0;
+                                      o.redo = true;
+                                    }
+                                  },
+                                  rd, gw);
       }
     }
+  }
+}
+
+// Drains the redo queue of one mixed batch_kernel launch (global work ids).
+template <int TT>
+__global__ __launch_bounds__(kBlock) void batch_redo_kernel(const ss_job* __restrict__ jobs,
+                                                           int n_jobs, Redo rd) {
+  __shared__ long begins[kMaxJobs + 1];
+  const long n = (long)*rd.count;
+  if ((long)blockIdx.x >= n) return;  // uniform per block: before any barrier
+  job_prefix<TT>(jobs, n_jobs, begins);
+  for (long i = blockIdx.x; i < n; i += gridDim.x) {
+    const long gw = rd.queue[i];
+    const int j = find_job(begins, n_jobs, gw);
+    const ss_job& jb = jobs[j];
+    const KCommon k = job_common<TT>(jb);
+    const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
+                                         jb.pattern_table, jb.constants);
+    redo_item<0, -1, -1, TT>(k, rc, gw - begins[j], rd.masks + gw * (TT * 4));
   }
 }
 
@@ -557,6 +759,17 @@ static int fill(double* p, long n, hipStream_t st) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("fill: ") + hipGetErrorString(e));
   return SS_OK;
+}
+
+static int rough_mode(int flags) {
+  if (!(flags & SS_ROUGH)) return 0;
+  return (flags & SS_D_PHI_AREA) ? 1 : 2;
+}
+
+static int pattern_mode(int kind) {
+  if (kind == SS_PAT_ISOTROPIC) return 0;
+  if (kind == SS_PAT_TABULATED) return 5;
+  return -1;
 }
 
 static int check_pattern(int kind, int n, const double* table) {
@@ -677,10 +890,22 @@ int ss_surface_field(int precision, const ss_surface_params* prm, const double* 
                           field, dropped, pair_bins, kTTSurf, n_seg);
   const SurfaceConst sc = make_surface_const(*prm, pattern_table, constants);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
-  if (precision == SS_PREC_FP64)
-    hipLaunchKernelGGL(surface_kernel<SS_PREC_FP64>, grid, block, 0, st, k, sc);
-  else
-    hipLaunchKernelGGL(surface_kernel<SS_PREC_MIXED>, grid, block, 0, st, k, sc);
+  const int rm = rough_mode(prm->flags), pm = pattern_mode(prm->pattern);
+#define SS_SURF(P, R, A) \
+  hipLaunchKernelGGL((surface_kernel<P, R, A>), grid, block, 0, st, k, sc)
+#define SS_SURF_PA(P, R) \
+  if (pm == 0) SS_SURF(P, R, 0); else if (pm == 5) SS_SURF(P, R, 5); else SS_SURF(P, R, -1)
+#define SS_SURF_P(P) \
+  if (rm == 0) { SS_SURF_PA(P, 0); } else if (rm == 1) { SS_SURF_PA(P, 1); } \
+  else { SS_SURF_PA(P, 2); }
+  if (precision == SS_PREC_FP64) {
+    SS_SURF_P(SS_PREC_FP64);
+  } else {
+    SS_SURF_P(SS_PREC_MIXED);
+  }
+#undef SS_SURF_P
+#undef SS_SURF_PA
+#undef SS_SURF
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("surface launch: ") + hipGetErrorString(e));
   if (precision == SS_PREC_FP64) {
@@ -699,7 +924,8 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
                        int64_t trace_stride, int64_t n_traces, const double* iface_desc,
                        const double* tables, const double* legs, const double* pattern_table,
                        const double* constants, double* out, double* dropped,
-                       int32_t* pair_bins, void* stream) {
+                       int32_t* pair_bins, void* workspace, int64_t workspace_bytes,
+                       void* stream) {
   if (!prm) return fail(SS_EINVAL, "null params");
   if (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED)
     return fail(SS_EINVAL, "precision must be SS_PREC_FP64 or SS_PREC_MIXED");
@@ -738,6 +964,11 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   }
   int rcp = check_pattern(prm->pattern, prm->pat_n, pattern_table);
   if (rcp) return rcp;
+  const int64_t need = ss_refracted_workspace_bytes(precision, n_facets, n_traces);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return fail(SS_EINVAL, "workspace smaller than ss_refracted_workspace_bytes()");
+  if (need > 0 && (reinterpret_cast<uintptr_t>(workspace) & 255) != 0)
+    return fail(SS_EINVAL, "workspace must be 256-byte aligned");
   const bool split = (prm->flags & SS_SPLIT_SIDES) != 0;
   const int n_seg = (split ? 2 : 1) * prm->n_samples;
   hipStream_t st = (hipStream_t)stream;
@@ -756,31 +987,54 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
                                        constants);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
   const bool fp64 = precision == SS_PREC_FP64;
-  if (!rc.joint) {
-    if (fp64)
-      hipLaunchKernelGGL((refracted_kernel<SS_PREC_FP64, 0>), grid, block, 0, st, k, rc);
-    else
-      hipLaunchKernelGGL((refracted_kernel<SS_PREC_MIXED, 0>), grid, block, 0, st, k, rc);
-  } else if (prm->n_crossed <= 8) {
-    if (fp64)
-      hipLaunchKernelGGL((refracted_kernel<SS_PREC_FP64, 8>), grid, block, 0, st, k, rc);
-    else
-      hipLaunchKernelGGL((refracted_kernel<SS_PREC_MIXED, 8>), grid, block, 0, st, k, rc);
-  } else {
-    if (fp64)
-      hipLaunchKernelGGL((refracted_kernel<SS_PREC_FP64, SS_MAX_JOINT>), grid, block, 0, st,
-                         k, rc);
-    else
-      hipLaunchKernelGGL((refracted_kernel<SS_PREC_MIXED, SS_MAX_JOINT>), grid, block, 0, st,
-                         k, rc);
+  const int rm = rough_mode(prm->flags), pm = pattern_mode(prm->pattern);
+  const long n_items = (long)k.n_ftiles * k.n_ttiles;
+  Redo rd{};
+  if (!fp64) {
+    rd = redo_view(workspace, n_items);
+    hipLaunchKernelGGL(reset_workspace_kernel, dim3(1), dim3(64), 0, st,
+                       static_cast<unsigned long long*>(workspace));
   }
+  // Redo grid: enough blocks for the rare queued items, never idle-heavy.
+  dim3 rgrid((unsigned)(n_items < 256 ? n_items : 256));
+#define SS_REFR(P, N, R, A) \
+  hipLaunchKernelGGL((refracted_kernel<P, N, R, A>), grid, block, 0, st, k, rc, rd)
+#define SS_REDO(N, R, A) \
+  hipLaunchKernelGGL((redo_kernel<N, R, A>), rgrid, block, 0, st, k, rc, rd)
+#define SS_BOTH(N, R, A) \
+  if (fp64) { SS_REFR(SS_PREC_FP64, N, R, A); } \
+  else { SS_REFR(SS_PREC_MIXED, N, R, A); SS_REDO(N, R, A); }
+#define SS_REFR_PA(R) \
+  if (pm == 0) { SS_BOTH(0, R, 0); } else if (pm == 5) { SS_BOTH(0, R, 5); } \
+  else { SS_BOTH(0, R, -1); }
+  if (!rc.joint) {
+    if (rm == 0) {
+      SS_REFR_PA(0);
+    } else if (rm == 1) {
+      SS_REFR_PA(1);
+    } else {
+      SS_REFR_PA(2);
+    }
+  } else if (prm->n_crossed <= 8) {
+    SS_BOTH(8, -1, -1);
+  } else {
+    SS_BOTH(SS_MAX_JOINT, -1, -1);
+  }
+#undef SS_REFR_PA
+#undef SS_BOTH
+#undef SS_REDO
+#undef SS_REFR
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("refracted launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
 }
 
-__global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ULL; }
+int64_t ss_refracted_workspace_bytes(int precision, int64_t n_facets, int64_t n_traces) {
+  if (precision != SS_PREC_MIXED || n_facets <= 0 || n_traces <= 0) return 0;
+  const long n_items = ((n_facets + kBlock - 1) / kBlock) * ((n_traces + kTTRefr - 1) / kTTRefr);
+  return redo_bytes(n_items, kTTRefr);
+}
 
 int64_t ss_sizeof_job(void) { return (int64_t)sizeof(ss_job); }
 
@@ -832,8 +1086,23 @@ static int validate_job(const ss_job& j, int idx) {
   return check_pattern(pattern, pat_n, j.pattern_table);
 }
 
+static int64_t batch_items(const ss_job* jobs_host, int n_jobs, int trace_tile) {
+  int64_t total = 0;
+  for (int i = 0; i < n_jobs; ++i) total += job_work(jobs_host[i], trace_tile);
+  return total;
+}
+
+int64_t ss_batch_workspace_bytes(int precision, const ss_job* jobs_host, int n_jobs,
+                                 int trace_tile) {
+  if (n_jobs <= 0 || !jobs_host) return kWsHeader;
+  if (trace_tile != 4 && trace_tile != 8 && trace_tile != 16) return -1;
+  if (precision != SS_PREC_MIXED) return kWsHeader;
+  return redo_bytes(batch_items(jobs_host, n_jobs, trace_tile), trace_tile);
+}
+
 int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* jobs_device,
-                   int64_t* counter_device, int trace_tile, int n_blocks, void* stream) {
+                   void* workspace, int64_t workspace_bytes, int trace_tile, int n_blocks,
+                   void* stream) {
   if (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED)
     return fail(SS_EINVAL, "precision must be SS_PREC_FP64 or SS_PREC_MIXED");
   if (n_jobs < 0 || n_jobs > kMaxJobs) return fail(SS_ERANGE, "n_jobs out of range [0, 1024]");
@@ -843,13 +1112,16 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
     g_last_error.clear();
     return SS_OK;
   }
-  if (!jobs_host || !jobs_device || !counter_device) return fail(SS_EINVAL, "null pointer");
-  long total = 0;
+  if (!jobs_host || !jobs_device || !workspace) return fail(SS_EINVAL, "null pointer");
+  if ((reinterpret_cast<uintptr_t>(workspace) & 255) != 0)
+    return fail(SS_EINVAL, "workspace must be 256-byte aligned");
   for (int i = 0; i < n_jobs; ++i) {
     int r = validate_job(jobs_host[i], i);
     if (r) return r;
-    total += job_work(jobs_host[i], trace_tile);
   }
+  const long total = batch_items(jobs_host, n_jobs, trace_tile);
+  if (workspace_bytes < ss_batch_workspace_bytes(precision, jobs_host, n_jobs, trace_tile))
+    return fail(SS_EINVAL, "workspace smaller than ss_batch_workspace_bytes()");
   hipStream_t st = (hipStream_t)stream;
   for (int i = 0; i < n_jobs; ++i) {
     const ss_job& j = jobs_host[i];
@@ -864,8 +1136,8 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   hipError_t e = hipMemcpyAsync(jobs_device, jobs_host, sizeof(ss_job) * (size_t)n_jobs,
                                 hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return fail(SS_EHIP, std::string("job upload: ") + hipGetErrorString(e));
-  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(counter_device);
-  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, st, ctr);
+  unsigned long long* hdr = static_cast<unsigned long long*>(workspace);
+  hipLaunchKernelGGL(reset_workspace_kernel, dim3(1), dim3(64), 0, st, hdr);
   if (total == 0) {
     g_last_error.clear();
     return SS_OK;
@@ -880,14 +1152,23 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   const ss_job* jd = (const ss_job*)jobs_device;
   dim3 grid((unsigned)n_blocks), block(kBlock);
   const bool fp64 = precision == SS_PREC_FP64;
-#define SS_LAUNCH_BATCH(P, T) \
-  hipLaunchKernelGGL((batch_kernel<P, T>), grid, block, 0, st, jd, n_jobs, total, ctr)
+  const Redo rd = fp64 ? Redo{} : redo_view(workspace, total);
+  dim3 rgrid((unsigned)(total < 256 ? total : 256));
+#define SS_LAUNCH_BATCH(T)                                                                 \
+  if (fp64) {                                                                              \
+    hipLaunchKernelGGL((batch_kernel<SS_PREC_FP64, T>), grid, block, 0, st, jd, n_jobs,    \
+                       total, hdr, rd);                                                    \
+  } else {                                                                                 \
+    hipLaunchKernelGGL((batch_kernel<SS_PREC_MIXED, T>), grid, block, 0, st, jd, n_jobs,   \
+                       total, hdr, rd);                                                    \
+    hipLaunchKernelGGL((batch_redo_kernel<T>), rgrid, block, 0, st, jd, n_jobs, rd);       \
+  }
   if (trace_tile == 4) {
-    if (fp64) SS_LAUNCH_BATCH(SS_PREC_FP64, 4); else SS_LAUNCH_BATCH(SS_PREC_MIXED, 4);
+    SS_LAUNCH_BATCH(4);
   } else if (trace_tile == 8) {
-    if (fp64) SS_LAUNCH_BATCH(SS_PREC_FP64, 8); else SS_LAUNCH_BATCH(SS_PREC_MIXED, 8);
+    SS_LAUNCH_BATCH(8);
   } else {
-    if (fp64) SS_LAUNCH_BATCH(SS_PREC_FP64, 16); else SS_LAUNCH_BATCH(SS_PREC_MIXED, 16);
+    SS_LAUNCH_BATCH(16);
   }
 #undef SS_LAUNCH_BATCH
   e = hipGetLastError();

@@ -197,9 +197,12 @@ SS_HD T f_factor(T m, T a0, T edge, T l, const double* cst) {
 }
 
 // D_Phi, clamped at 0 (exactly 0 when (sigma K)^2 == 0).
-template <typename T>
+// RM (compile-time roughness mode): 1 area-only terms, 2 finite-facet
+// Gaussian terms, -1 decided by ``area_only`` at run time.
+template <typename T, int RM = -1>
 SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
-              bool area_only, int acf, const double* cst) {
+              bool area_only_rt, int acf, const double* cst) {
+  const bool area_only = RM == 1 ? true : (RM == 2 ? false : area_only_rt);
   T sk = sigma * K;
   T x = sk * sk;
   if (!(x > (T)0)) return (T)0;
@@ -210,20 +213,22 @@ SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
     T mf = (T)m;
     T logp = mf * logx - (T)cst[SS_CONST_LGAMMA + m] - x;
     T term;
-    if (area_only && acf == SS_ACF_EXPONENTIAL) {
+    if (RM != 2 && area_only && acf == SS_ACF_EXPONENTIAL) {
       T kb2 = ab2;
       T q = (l / mf) * (l / mf);
       T base = (T)1 + kb2 * q;
       // base^-1.5
       T rb = (T)1 / t_sqrt<T>(base);
       term = t_exp<T>(logp) * ((T)kTwoPi * q * (rb * rb * rb)) * Lx * Ly;
-    } else if (area_only) {
+    } else if (RM != 2 && area_only) {
       term = t_exp<T>(logp - ab2 * (l * l) / ((T)4 * mf)) *
              ((T)kPi * (l * l) / mf) * Lx * Ly;
-    } else {
+    } else if (RM != 1) {
       T l4 = (l * l) * (l * l);
       term = t_exp<T>(logp) * (l4 / (mf * mf)) * f_factor<T>(mf, A0, Lx, l, cst) *
              f_factor<T>(mf, B0, Ly, l, cst);
+    } else {
+      term = (T)0;
     }
     acc += term;
   }
@@ -261,10 +266,11 @@ SS_HD T element_field(T c, T q, bool enabled) {
 }
 
 // dh: departure unit vector; pv0/pv1: trace pattern vectors.
-template <typename T>
+// PAT: compile-time pattern kind (SS_PAT_*), or -1 to dispatch on pa.kind.
+template <typename T, int PAT = -1>
 SS_HD T antenna_gain(const PatternArgs& pa, const T dh[3], const T pv0[3],
                      const T pv1[3]) {
-  switch (pa.kind) {
+  switch (PAT >= 0 ? PAT : pa.kind) {
     case SS_PAT_DIPOLE: {
       T ca = (dh[0] * pv0[0] + dh[1] * pv0[1]) + dh[2] * pv0[2];
       T s2 = (T)1 - ca * ca;
@@ -382,6 +388,8 @@ struct PairOut {
   int right;       // split sides: 1 when the facet is right of track
   int raw_bin;     // diagnostic: floor bin of the pair (or SS_BIN_INVALID)
   bool valid;      // path validity (refracted); always true for the surface
+  bool redo;       // reduced-precision path unresolved: the float64 redo
+                   // kernel owns this pair (it contributes nothing here)
   double pwr_invalid;  // |contrib|^2 when the path is invalid
 };
 
@@ -394,10 +402,13 @@ SS_HD double exact_bin_pos(double r_or_opl, double c, double t0, double dt) {
 // ---------------------------------------------------------------------------
 // Surface pair, float64 (kernels/coherent.py:107-168, 216-236)
 // ---------------------------------------------------------------------------
+template <int RM = -1, int PAT = -1>
 SS_HD void surface_pair_fp64(const SurfFacet64& f, const TraceRec& t,
                              const SurfaceConst& sc, PairOut& o) {
 #pragma clang fp contract(off)
   const int flags = sc.flags;
+  const bool rough = RM != 0 && (flags & SS_ROUGH);
+  const bool pattern = PAT != 0 && sc.pat.kind != SS_PAT_ISOTROPIC;
   double dx = t.p[0] - f.c[0], dy = t.p[1] - f.c[1], dz = t.p[2] - f.c[2];
   double r = sqrt((dx * dx + dy * dy) + dz * dz);
   double cs = ((dx * f.n[0] + dy * f.n[1]) + dz * f.n[2]) / r;
@@ -415,11 +426,11 @@ SS_HD void surface_pair_fp64(const SurfFacet64& f, const TraceRec& t,
   }
   double phase = (-2.0 * sc.k) * (r - t.rref);
   double cre = amp, cim = 0.0;
-  if (flags & SS_ROUGH) {
+  if (rough) {
     double kk = 2.0 * sc.k * cs;
     double dp = 0.0;
     if (f.A > 0.0)
-      dp = d_phi<double>(sc.sigma, sc.lc, kk, 2.0 * sc.k * d1 / f.l1,
+      dp = d_phi<double, RM>(sc.sigma, sc.lc, kk, 2.0 * sc.k * d1 / f.l1,
                          2.0 * sc.k * d2 / f.l2, f.l1, f.l2, sc.n_terms,
                          (flags & SS_D_PHI_AREA) != 0, sc.acf, sc.cst);
     double amp_i = f.A > 0.0 ? (sc.k / kTwoPi) * gam * cs / (r * r) * sqrt(dp) : 0.0;
@@ -432,9 +443,9 @@ SS_HD void surface_pair_fp64(const SurfFacet64& f, const TraceRec& t,
   // j (cre + j cim) (cp + j sp)
   double re = -(cre * sp + cim * cp);
   double im = cre * cp - cim * sp;
-  if (sc.pat.kind != SS_PAT_ISOTROPIC) {
+  if (pattern) {
     double dh[3] = {-dx / r, -dy / r, -dz / r};
-    double g = antenna_gain<double>(sc.pat, dh, t.pv0, t.pv1);
+    double g = antenna_gain<double, PAT>(sc.pat, dh, t.pv0, t.pv1);
     double g2 = g * g;
     re *= g2;
     im *= g2;
@@ -469,10 +480,13 @@ SS_HD void surface_pair_fp64(const SurfFacet64& f, const TraceRec& t,
 // ---------------------------------------------------------------------------
 constexpr double kBinGuard = 1e-7;   // bin units; fast-path error is ~1e-10
 
+template <int RM = -1, int PAT = -1>
 SS_HD void surface_pair_mixed(const SurfFacet32& f, const SurfFacet64* f64,
                               const TraceRec& t, const SurfaceConst& sc,
                               PairOut& o) {
   const int flags = sc.flags;
+  const bool rough = RM != 0 && (flags & SS_ROUGH);
+  const bool pattern = PAT != 0 && sc.pat.kind != SS_PAT_ISOTROPIC;
   double dx = t.p[0] - f.c[0], dy = t.p[1] - f.c[1], dz = t.p[2] - f.c[2];
   double r2 = dx * dx + dy * dy + dz * dz;
   float r0 = f_sqrt((float)r2);
@@ -510,12 +524,12 @@ SS_HD void surface_pair_mixed(const SurfFacet32& f, const SurfFacet64* f64,
     amp *= f_exp(-(1.0f - c2) * f_rcp(c2 * (2.0f * ts * ts)));
   }
   float cre = amp, cim = 0.0f;
-  if (flags & SS_ROUGH) {
+  if (rough) {
     float k2 = 2.0f * (float)sc.k;
     float kk = k2 * cs;
     float dp = 0.0f;
     if (f.A > 0.0f)
-      dp = d_phi<float>((float)sc.sigma, (float)sc.lc, kk, k2 * d1 * f_rcp(f.l1),
+      dp = d_phi<float, RM>((float)sc.sigma, (float)sc.lc, kk, k2 * d1 * f_rcp(f.l1),
                         k2 * d2 * f_rcp(f.l2), f.l1, f.l2, sc.n_terms,
                         (flags & SS_D_PHI_AREA) != 0, sc.acf, sc.cst);
     float amp_i = f.A > 0.0f ? kg * cs * ir2 * f_sqrt(dp) : 0.0f;
@@ -528,11 +542,11 @@ SS_HD void surface_pair_mixed(const SurfFacet32& f, const SurfFacet64* f64,
   float sp = -f_sin_rev(ph), cp = f_cos_rev(ph);
   float re = -(cre * sp + cim * cp);
   float im = cre * cp - cim * sp;
-  if (sc.pat.kind != SS_PAT_ISOTROPIC) {
+  if (pattern) {
     float dh[3] = {-rx, -ry, -rz};
     float pv0[3] = {(float)t.pv0[0], (float)t.pv0[1], (float)t.pv0[2]};
     float pv1[3] = {(float)t.pv1[0], (float)t.pv1[1], (float)t.pv1[2]};
-    float g = antenna_gain<float>(sc.pat, dh, pv0, pv1);
+    float g = antenna_gain<float, PAT>(sc.pat, dh, pv0, pv1);
     float g2 = g * g;
     re *= g2;
     im *= g2;

@@ -267,6 +267,31 @@ struct RefrFacet {
   double c[3], n[3], A, e1[3], e2[3], gam, ph_re, ph_im, l1, l2, dif, dph_re, dph_im;
 };
 
+// Mixed-tier facet record: float64 centre (path lengths need it), the rest
+// float32 (radiometry only).
+struct RefrFacet32 {
+  double c[3];
+  float n[3], A, e1[3], e2[3], gam, ph_re, ph_im, l1, l2, dif, dph_re, dph_im;
+};
+
+SS_HD void to_f32(const RefrFacet& a, RefrFacet32& b) {
+  for (int j = 0; j < 3; ++j) {
+    b.c[j] = a.c[j];
+    b.n[j] = (float)a.n[j];
+    b.e1[j] = (float)a.e1[j];
+    b.e2[j] = (float)a.e2[j];
+  }
+  b.A = (float)a.A;
+  b.gam = (float)a.gam;
+  b.ph_re = (float)a.ph_re;
+  b.ph_im = (float)a.ph_im;
+  b.l1 = (float)a.l1;
+  b.l2 = (float)a.l2;
+  b.dif = (float)a.dif;
+  b.dph_re = (float)a.dph_re;
+  b.dph_im = (float)a.dph_im;
+}
+
 SS_HD void lookup_cell(const RefrConst& rc, int i, int row, int col, double pt[3],
                        double nr[3]) {
 #pragma clang fp contract(off)
@@ -365,6 +390,7 @@ SS_HD void seq_path_fp64(const RefrFacet& f, const TraceRec& t, const RefrConst&
 }
 
 // Float64 tier, final leg + radiometry + bin (kernels/multilayer.py:500-583).
+template <int RM = -1, int PAT = -1>
 SS_HD void radiometry_fp64(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
                            const PathSums& ps, PairOut& o) {
 #pragma clang fp contract(off)
@@ -392,12 +418,12 @@ SS_HD void radiometry_fp64(const RefrFacet& f, const TraceRec& t, const RefrCons
   double att_f = pow(10.0, -loss_db / 10.0);
   double cos_t = (rh[0] * f.n[0] + rh[1] * f.n[1]) + rh[2] * f.n[2];
   double spread = tau2 * flux / (l_par * l_perp);
-  if (rc.pat.kind != SS_PAT_ISOTROPIC) {
+  if (PAT != 0 && rc.pat.kind != SS_PAT_ISOTROPIC) {
     double d0[3] = {x_first[0] - t.p[0], x_first[1] - t.p[1], x_first[2] - t.p[2]};
     double dm = sqrt((d0[0] * d0[0] + d0[1] * d0[1]) + d0[2] * d0[2]);
     dm = dm > 1e-30 ? dm : 1e-30;
     double dh[3] = {d0[0] / dm, d0[1] / dm, d0[2] / dm};
-    double g = antenna_gain<double>(rc.pat, dh, t.pv0, t.pv1);
+    double g = antenna_gain<double, PAT>(rc.pat, dh, t.pv0, t.pv1);
     spread = spread * (g * g);
   }
   double re, im, pwr;
@@ -415,13 +441,13 @@ SS_HD void radiometry_fp64(const RefrFacet& f, const TraceRec& t, const RefrCons
       amp = amp * exp(-(1.0 - ct2) / (ct2 * (2.0 * rc.taper_s * rc.taper_s)));
     }
     double are = amp, aim = 0.0;
-    if (flags & SS_ROUGH) {
+    if (RM != 0 && (flags & SS_ROUGH)) {
       double kk = 2.0 * kj * cos_t;
       double sk = rc.sigma * kk;
       double matt = exp(-0.5 * (sk * sk));
       are = amp * matt;
       if (f.A > 0.0) {
-        double dp = d_phi<double>(rc.sigma, rc.lc, kk, 2.0 * kj * d1 / f.l1,
+        double dp = d_phi<double, RM>(rc.sigma, rc.lc, kk, 2.0 * kj * d1 / f.l1,
                                   2.0 * kj * d2 / f.l2, f.l1, f.l2, rc.n_terms,
                                   (flags & SS_D_PHI_AREA) != 0, rc.acf, rc.cst);
         double ai = (kj / kTwoPi) * gam * cos_t * spread * att_f * sqrt(dp);
@@ -465,7 +491,8 @@ SS_HD void radiometry_fp64(const RefrFacet& f, const TraceRec& t, const RefrCons
 
 // Mixed tier, sequential chain.  Returns false when the pair must be redone
 // on the float64 tier (ambiguous Snell residual).
-SS_HD bool seq_path_mixed(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
+template <typename FT>
+SS_HD bool seq_path_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
                           PathSums& ps) {
   const int flags = rc.flags;
   const double* nl = rc.legs;
@@ -562,7 +589,8 @@ SS_HD bool seq_path_mixed(const RefrFacet& f, const TraceRec& t, const RefrConst
 // Mixed tier, final leg + radiometry + bin.  Returns false when the bin
 // position lies within kBinGuard of an edge (the caller redoes the pair on
 // the float64 tier).
-SS_HD bool radiometry_mixed(const RefrFacet& f, const TraceRec& t, const RefrConst& rc,
+template <int RM = -1, int PAT = -1, typename FT = RefrFacet>
+SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
                             const PathSums& ps, PairOut& o) {
   const int flags = rc.flags;
   const double* nl = rc.legs;
@@ -598,7 +626,7 @@ SS_HD bool radiometry_mixed(const RefrFacet& f, const TraceRec& t, const RefrCon
   float att_f = f_exp2(-loss_db * (3.321928094887362f / 10.0f));  // 10^(-loss/10)
   float cos_t = rh[0] * (float)f.n[0] + rh[1] * (float)f.n[1] + rh[2] * (float)f.n[2];
   float spread = tau2 * flux / (l_par * l_perp);
-  if (rc.pat.kind != SS_PAT_ISOTROPIC) {
+  if (PAT != 0 && rc.pat.kind != SS_PAT_ISOTROPIC) {
     float d0[3] = {(float)(x_first[0] - t.p[0]), (float)(x_first[1] - t.p[1]),
                    (float)(x_first[2] - t.p[2])};
     float dm = fmaxf(f_sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]), 1e-30f);
@@ -606,7 +634,7 @@ SS_HD bool radiometry_mixed(const RefrFacet& f, const TraceRec& t, const RefrCon
     float dh[3] = {d0[0] * idm, d0[1] * idm, d0[2] * idm};
     float pv0[3] = {(float)t.pv0[0], (float)t.pv0[1], (float)t.pv0[2]};
     float pv1[3] = {(float)t.pv1[0], (float)t.pv1[1], (float)t.pv1[2]};
-    float g = antenna_gain<float>(rc.pat, dh, pv0, pv1);
+    float g = antenna_gain<float, PAT>(rc.pat, dh, pv0, pv1);
     spread *= g * g;
   }
   float re, im, pwr;
@@ -627,13 +655,13 @@ SS_HD bool radiometry_mixed(const RefrFacet& f, const TraceRec& t, const RefrCon
       amp *= f_exp(-(1.0f - ct2) * f_rcp(ct2 * (2.0f * ts * ts)));
     }
     float are = amp, aim = 0.0f;
-    if (flags & SS_ROUGH) {
+    if (RM != 0 && (flags & SS_ROUGH)) {
       float kk = 2.0f * kj * cos_t;
       float sk = (float)rc.sigma * kk;
       are = amp * f_exp(-0.5f * sk * sk);
       if (f.A > 0.0) {
         float l1 = (float)f.l1, l2 = (float)f.l2;
-        float dp = d_phi<float>((float)rc.sigma, (float)rc.lc, kk, 2.0f * kj * d1 / l1,
+        float dp = d_phi<float, RM>((float)rc.sigma, (float)rc.lc, kk, 2.0f * kj * d1 / l1,
                                 2.0f * kj * d2 / l2, l1, l2, rc.n_terms,
                                 (flags & SS_D_PHI_AREA) != 0, rc.acf, rc.cst);
         float ai = base * f_sqrt(dp);

@@ -313,12 +313,14 @@ class DeviceInterfaces:
 def refracted_params(target, ifaces, eps_leg, att_leg, *, mode, t0, dt, n_samples, c,
                      gamma=0.0, k0=None, split_sides=False, pattern=None, roughness=None,
                      taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
-                     joint_newton=6, joint_backtrack=4):
+                     joint_newton=6, joint_backtrack=4, extra_flags=0):
     """(RefractedParams, legs tensor, pattern table or None) for the
     refracted kernel over the (possibly padded) stack ``ifaces``;
-    ``eps_leg``/``att_leg`` are the unpadded per-leg media."""
+    ``eps_leg``/``att_leg`` are the unpadded per-leg media.  ``extra_flags``
+    is ORed into the flags (e.g. SS_REDO_ALL in tests)."""
     coherent = mode == "coherent"
     flags = _lib.SS_COHERENT if coherent else 0
+    flags |= int(extra_flags)
     flags |= _lib.SS_SPLIT_SIDES if split_sides else 0
     flags |= _lib.SS_TAPER if taper_s is not None else 0
     flags |= _lib.SS_D_PHI_AREA if d_phi_area else 0
@@ -362,16 +364,20 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
                     split_sides=False, pattern=None, roughness=None,
                     taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
                     joint_newton=6, joint_backtrack=4, out=None,
-                    dropped=None, pair_bins=False, stream=None):
-    """Enqueue the refracted kernel (sequential chain or joint solve).
-    Returns (out (T, n_seg, 2) float64, dropped (T,)[, pair_bins],
-    keep-alive) device tensors."""
+                    dropped=None, pair_bins=False, workspace=None, extra_flags=0,
+                    stream=None):
+    """Enqueue the refracted kernel (sequential chain or joint solve) and,
+    for the mixed tier, its float64 redo launch.  ``workspace``: a uint8
+    device tensor of at least ``refracted_workspace_bytes`` (allocated here
+    when None).  Returns (out (T, n_seg, 2) float64, dropped (T,)
+    [, pair_bins], keep-alive) device tensors."""
     dev = target.device
     p, legs_t, table = refracted_params(
         target, ifaces, eps_leg, att_leg, mode=mode, t0=t0, dt=dt, n_samples=n_samples,
         c=c, gamma=gamma, k0=k0, split_sides=split_sides, pattern=pattern,
         roughness=roughness, taper_s=taper_s, d_phi_area=d_phi_area, n_exp=n_exp,
-        refraction=refraction, joint_newton=joint_newton, joint_backtrack=joint_backtrack)
+        refraction=refraction, joint_newton=joint_newton, joint_backtrack=joint_backtrack,
+        extra_flags=extra_flags)
     n_seg = (2 if split_sides else 1) * int(n_samples)
     T = traces.n
     if out is None:
@@ -382,16 +388,26 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
             if pair_bins else None)
     cst = constants(dev)
     prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
+    need = refracted_workspace_bytes(precision, target.n, T)
+    if workspace is None and need > 0:
+        workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+    ws_bytes = 0 if workspace is None else workspace.numel()
     rc = _lib.load().ss_refracted_field(
         prec, ctypes.byref(p), _ptr(target.buf), target.stride, target.n,
         _ptr(traces.buf), traces.stride, T, _ptr(ifaces.desc), _ptr(ifaces.tables),
         _ptr(legs_t), _ptr(table), _ptr(cst), _ptr(out), _ptr(dropped), _ptr(pb_t),
-        _stream_handle(stream))
+        _ptr(workspace), ws_bytes, _stream_handle(stream))
     _lib.check(rc, "ss_refracted_field")
-    keep = (legs_t, table)
+    keep = (legs_t, table, workspace)
     if pair_bins:
         return out, dropped, pb_t, keep
     return out, dropped, keep
+
+
+def refracted_workspace_bytes(precision, n_facets, n_traces):
+    """Device workspace bytes of one refracted launch (0 for fp64)."""
+    prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
+    return int(_lib.load().ss_refracted_workspace_bytes(prec, int(n_facets), int(n_traces)))
 
 
 def range_compress(field, pulse, m, out=None, stream=None):

@@ -52,6 +52,12 @@ class ResidentScene:
                                     wl.waveform.tukey_alpha)
             self.pulse = (p, m)
         self._keep = []
+        # float64-redo workspaces of the mixed refracted launches, one per layer
+        self.workspace = [
+            None if ifc is None else torch.empty(
+                max(1, dv.refracted_workspace_bytes("mixed", fac.n, T)),
+                dtype=torch.uint8, device=self.device)
+            for _, fac, ifc, _ in self.layers]
 
     def pairs_per_layer(self):
         T = self.traces.n
@@ -59,6 +65,7 @@ class ResidentScene:
 
     def nbytes(self):
         b = self.traces.buf.numel() * 8 + self.raw.numel() * 16 + self.dropped.numel() * 8
+        b += sum(w.numel() for w in self.workspace if w is not None)
         for _, fac, ifc, _ in self.layers:
             b += fac.nbytes() + (ifc.nbytes() if ifc is not None else 0)
         return b
@@ -80,7 +87,8 @@ class ResidentScene:
                 gamma=0.0 if fac.has_gamma else L.gamma, k0=wl.k0,
                 precision=precision, split_sides=wl.split_sides, pattern=wl.pattern,
                 roughness=rough, taper_s=wl.taper_s, d_phi_area=wl.d_phi_area,
-                out=self.raw[j], dropped=self.dropped[j], stream=stream)
+                out=self.raw[j], dropped=self.dropped[j], workspace=self.workspace[j],
+                stream=stream)
         self._keep.append(res[-1])
 
     def compress(self, stream=None):

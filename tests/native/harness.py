@@ -18,7 +18,7 @@ def build():
     os.makedirs(os.path.dirname(SO), exist_ok=True)
     src = os.path.join(HERE, "pair_host.cpp")
     deps = [src] + [os.path.join(HERE, "..", "..", "radar_sounder_simulator_amd", "csrc", h)
-                    for h in ("ss_math.h", "ss_refract.h")]
+                    for h in ("ss_math.h", "ss_refract.h", "ss_joint.h")]
     if os.path.exists(SO) and all(os.path.getmtime(SO) >= os.path.getmtime(d) for d in deps):
         return SO
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared",

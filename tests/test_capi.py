@@ -56,17 +56,42 @@ def test_invalid_arguments_rejected_on_host():
     r = _lib.RefractedParams()
     r.n_samples, r.dt, r.c, r.n_crossed = 64, 1e-8, 3e8, 0
     rargs = [ctypes.byref(r), None, 0, 0, None, 0, 0, None, None, None, None, None, None,
-             None, None, None]
-    assert lib.ss_refracted_field(0, *rargs) == -3          # n_crossed out of range
+             None, None, None, 0]
+    assert lib.ss_refracted_field(0, *rargs, None) == -3          # n_crossed out of range
     r.n_crossed, r.flags = 1, _lib.SS_ROUGH                  # incoherent + roughness
-    assert lib.ss_refracted_field(0, *rargs) == -1
+    assert lib.ss_refracted_field(0, *rargs, None) == -1
+    # mixed tier without its redo workspace: rejected before any device work
+    r.flags = _lib.SS_COHERENT
+    dummy = 4096                                            # never dereferenced
+    wargs = [ctypes.byref(r), dummy, 10, 10, dummy, 4, 4, dummy, dummy, dummy, None, None,
+             dummy, dummy, None]
+    assert lib.ss_refracted_field(1, *wargs, None, 0, None) == -1
+    assert b"workspace" in lib.ss_last_error()
+    assert lib.ss_refracted_field(1, *wargs, dummy, 100, None) == -1
+    assert lib.ss_refracted_field(1, *wargs, dummy + 8, 1 << 20, None) == -1   # alignment
+    assert b"aligned" in lib.ss_last_error()
     assert lib.ss_range_compress(None, 1, 8, None, 1, None, None) == -1
-    assert lib.ss_batch_field(0, None, 0, None, None, 8, 0, None) == 0   # empty batch
-    assert lib.ss_batch_field(0, None, 1, None, None, 3, 0, None) == -1  # bad tile
+    assert lib.ss_batch_field(0, None, 0, None, None, 0, 8, 0, None) == 0   # empty batch
+    assert lib.ss_batch_field(0, None, 1, None, None, 0, 3, 0, None) == -1  # bad tile
     j = _lib.Job()
     j.kind = 7
-    assert lib.ss_batch_field(0, ctypes.byref(j), 1, 1, 1, 8, 0, None) == -1
+    assert lib.ss_batch_field(0, ctypes.byref(j), 1, 4096, 4096, 1 << 20, 8, 0, None) == -1
     assert b"kind" in lib.ss_last_error()
+
+
+def test_workspace_sizes():
+    lib = _lib.load()
+    # fp64 needs none; mixed: 256-byte header + (8 + 4 * 8 * 8) bytes per
+    # (8-trace, 256-facet) work item -- 1000 facets x 20 traces = 4 x 3 items
+    assert lib.ss_refracted_workspace_bytes(0, 1000, 20) == 0
+    assert lib.ss_refracted_workspace_bytes(1, 1000, 20) == 256 + 12 * (8 + 256)
+    assert lib.ss_refracted_workspace_bytes(1, 0, 20) == 0
+    j = _lib.Job()
+    j.kind, j.n_facets, j.n_traces = 1, 1000, 20
+    assert lib.ss_batch_workspace_bytes(0, ctypes.byref(j), 1, 8) == 256
+    assert lib.ss_batch_workspace_bytes(1, ctypes.byref(j), 1, 8) == 256 + 12 * (8 + 256)
+    assert lib.ss_batch_workspace_bytes(1, ctypes.byref(j), 1, 16) == 256 + 8 * (8 + 512)
+    assert lib.ss_batch_workspace_bytes(1, ctypes.byref(j), 1, 5) == -1
 
 
 def test_fill_constants():

@@ -1,0 +1,84 @@
+"""GPU: the batched persistent launch (ss_batch_field) against one launch
+per layer (ss_surface_field / ss_refracted_field) on the same resident
+scenes, both tiers, every trace tile, and with the float64 redo forced."""
+
+import numpy as np
+import pytest
+import torch
+
+from radar_sounder_simulator_amd import _lib, batch, workloads
+from radar_sounder_simulator_amd.pipeline import ResidentScene
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes():
+    wls = [workloads.config2(n=24, n_traces=9, n_samples=4096, seed=3),
+           workloads.config1(n=40, n_traces=5, n_samples=512, seed=8),
+           workloads.config2(n=17, n_traces=21, n_samples=4096, seed=5)]
+    return [ResidentScene(w, device="cuda") for w in wls]
+
+
+def _single(scenes, precision):
+    outs = []
+    for rs in scenes:
+        for j in range(len(rs.layers)):
+            rs.run_layer(j, precision)
+        torch.cuda.synchronize()
+        outs.append((rs.raw.cpu().numpy().copy(), rs.dropped.cpu().numpy().copy()))
+    return outs
+
+
+def _batched(scenes, precision, tt, redo_all=False):
+    br = batch.BatchRunner(device="cuda")
+    jobs, keep = [], []
+    for rs in scenes:
+        j, k = rs.jobs(precision)
+        jobs += j
+        keep += k
+    if redo_all:
+        for jb in jobs:
+            if jb.kind == 1:
+                jb.refracted.flags |= _lib.SS_REDO_ALL
+    br.launch(jobs, precision, trace_tile=tt)
+    torch.cuda.synchronize()
+    return [(rs.raw.cpu().numpy().copy(), rs.dropped.cpu().numpy().copy()) for rs in scenes]
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("tt", [4, 8, 16])
+def test_batch_fp64_matches_single(tt):
+    print(f"case batch/fp64/tt{tt}", flush=True)
+    scenes = _scenes()
+    ref = _single(scenes, "fp64")
+    got = _batched(scenes, "fp64", tt)
+    for (o, d), (ro, rd) in zip(got, ref):
+        assert all(np.linalg.norm(ro[j]) > 0 for j in range(ro.shape[0]))
+        assert _rel(o, ro) <= 1e-12
+        np.testing.assert_allclose(d, rd, rtol=1e-12, atol=1e-30)
+
+
+@pytest.mark.parametrize("tt", [4, 8, 16])
+def test_batch_mixed_matches_single(tt):
+    print(f"case batch/mixed/tt{tt}", flush=True)
+    scenes = _scenes()
+    ref = _single(scenes, "mixed")
+    got = _batched(scenes, "mixed", tt)
+    for (o, d), (ro, rd) in zip(got, ref):
+        assert _rel(o, ro) <= 1e-5       # same pair arithmetic, FP32 LDS sum order
+        np.testing.assert_allclose(d, rd, rtol=1e-5, atol=1e-30)
+
+
+def test_batch_redo_all_matches_fp64():
+    print("case batch/redo_all", flush=True)
+    scenes = _scenes()
+    ref = _single(scenes, "fp64")
+    got = _batched(scenes, "mixed", 8, redo_all=True)
+    for (o, d), (ro, rd) in zip(got, ref):
+        # refracted layers redone in float64; surface layers stay mixed
+        for j in range(o.shape[0]):
+            tol = 1e-12 if j > 0 else 1e-5
+            assert _rel(o[j], ro[j]) <= tol, (j, _rel(o[j], ro[j]))

@@ -193,3 +193,79 @@ def test_range_compress_vs_fft():
         y = y.cpu().numpy()
         y = y[..., 0] + 1j * y[..., 1]
         assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-12
+
+
+REDO_CASES = ["smooth", "rough_tab", "split", "incoherent", "joint2"]
+
+
+@pytest.mark.parametrize("name", REDO_CASES)
+def test_mixed_redo_path_matches_fp64(name):
+    """Every mixed-tier pair forced through the float64 redo launch
+    (SS_REDO_ALL): the queue, lane masks and global accumulation must give
+    the float64 tier's bins exactly and its field to rounding."""
+    from radar_sounder_simulator_amd import _lib
+    from radar_sounder_simulator_amd import device as dv
+    surf, bed, mid, tr = _bed_scene()
+    C = 299792458.0
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=2 * np.pi * 60e6 / C)
+    crossed, eps, att, joint = [surf], [1.0, 3.15], [0.0, 10.0], False
+    phasors = None
+    if name == "rough_tab":
+        phasors = speckle_phasors(len(bed), 3)
+        kw.update(roughness=(0.5, 20.0, 24, "exponential"), d_phi_area=True, taper_s=0.05,
+                  pattern=A.pattern_args(A.cos_power_table(2.0), tr.u_at, tr.u_ct))
+    elif name == "split":
+        kw.update(split_sides=True)
+    elif name == "incoherent":
+        kw.update(mode="incoherent", gamma=0.0, k0=None)
+    elif name == "joint2":
+        crossed, eps, att, joint = [surf, mid], [1.0, 2.0, 3.15], [0.0, 5.0, 10.0], True
+        kw.update(refraction="joint")
+    print(f"case redo/{name}", flush=True)
+    fac = dv.DeviceFacets(bed.centers, bed.normals, bed.areas, bed.e1, bed.e2,
+                          phasors=phasors, device="cuda")
+    ifc = dv.DeviceInterfaces(crossed, device="cuda", pad_to=len(crossed) if joint else None,
+                              z_platform=tr.positions[:, 2].min())
+    pv = None if kw.get("pattern") is None else kw["pattern"][1]
+    trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, pv, device="cuda")
+    res = {}
+    for key, prec, fl in (("fp64", "fp64", 0), ("redo", "mixed", _lib.SS_REDO_ALL),
+                          ("mixed", "mixed", 0)):
+        out, drop, bins, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision=prec,
+                                                pair_bins=True, extra_flags=fl, **kw)
+        torch.cuda.synchronize()
+        res[key] = (out.cpu().numpy(), drop.cpu().numpy(), bins.cpu().numpy())
+    (o64, d64, b64), (orr, drr, brr), (om_, dm, bm) = res["fp64"], res["redo"], res["mixed"]
+    np.testing.assert_array_equal(brr, b64)
+    np.testing.assert_array_equal(bm, b64)
+    assert np.linalg.norm(orr - o64) <= 1e-12 * np.linalg.norm(o64)
+    np.testing.assert_allclose(drr, d64, rtol=1e-12, atol=1e-30)
+    assert np.linalg.norm(o64) > 0
+
+
+def test_workspace_reuse_second_smaller_call():
+    """One workspace serves two mixed launches in a row, the second smaller:
+    the queue is reset on the stream, so stale entries never replay."""
+    from radar_sounder_simulator_amd import _lib
+    from radar_sounder_simulator_amd import device as dv
+    surf, bed, mid, tr = _bed_scene()
+    C = 299792458.0
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=2 * np.pi * 60e6 / C)
+    fac = dv.DeviceFacets(bed.centers, bed.normals, bed.areas, bed.e1, bed.e2, device="cuda")
+    ifc = dv.DeviceInterfaces([surf], device="cuda")
+    trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cuda")
+    trs2 = dv.DeviceTraces(tr.positions[:2], tr.u_ct[:2], None, None, device="cuda")
+    ws = torch.empty(dv.refracted_workspace_bytes("mixed", fac.n, trs.n), dtype=torch.uint8,
+                     device="cuda")
+    a, _, _ = dv.refracted_field(fac, trs, ifc, [1.0, 3.15], [0.0, 10.0], precision="mixed",
+                                 workspace=ws, extra_flags=_lib.SS_REDO_ALL, **kw)
+    b, _, _ = dv.refracted_field(fac, trs2, ifc, [1.0, 3.15], [0.0, 10.0], precision="mixed",
+                                 workspace=ws, extra_flags=_lib.SS_REDO_ALL, **kw)
+    c, _, _ = dv.refracted_field(fac, trs2, ifc, [1.0, 3.15], [0.0, 10.0], precision="fp64",
+                                 **kw)
+    torch.cuda.synchronize()
+    b, c = b.cpu().numpy(), c.cpu().numpy()
+    assert np.linalg.norm(b - c) <= 1e-12 * np.linalg.norm(c)
+    assert np.linalg.norm(a.cpu().numpy()[:2] - c) <= 1e-12 * np.linalg.norm(c)
