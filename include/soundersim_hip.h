@@ -219,11 +219,12 @@ int ss_surface_field(int precision, const ss_surface_params* params,
  * (:333-591), refraction_joint.py:249-322 (joint_crossings).  iface_desc: (n_crossed, SS_IDESC_COUNT) float64 device;
  * tables: concatenated (sum ny*nx, 6) float64 [centre, normal] device;
  * legs: (3, n_crossed + 1) float64 device rows [n, eps, att_db_per_km].
- * workspace: caller-owned device memory, 256-byte aligned, of at least
+ * workspace: caller-owned device memory, 8-byte aligned, of at least
  * ss_refracted_workspace_bytes(precision, n_facets, n_traces) bytes (may be
- * NULL when that is 0, i.e. for SS_PREC_FP64).  It holds the queue of pairs
- * the mixed tier hands to its float64 redo launch; the call resets it on the
- * stream, so concurrent calls need distinct workspaces. */
+ * NULL when that is 0, i.e. for SS_PREC_FP64).  It holds the per-work-item
+ * flags and lane masks of the pairs the mixed tier hands to its float64 redo
+ * pass (rewritten by every call, nothing to clear), so concurrent calls need
+ * distinct workspaces. */
 int ss_refracted_field(int precision, const ss_refracted_params* params,
                        const double* facets, int64_t facet_stride,
                        int64_t n_facets, const double* traces,
@@ -235,9 +236,9 @@ int ss_refracted_field(int precision, const ss_refracted_params* params,
                        void* stream);
 
 /* Workspace bytes ss_refracted_field needs: 0 for SS_PREC_FP64 or an empty
- * call; for SS_PREC_MIXED 256 + 264 bytes per (8-trace, 256-facet) work
- * item (the float64 redo queue and its lane masks).  No counterpart in the
- * reference (its kernels run one precision). */
+ * call; for SS_PREC_MIXED 256 + one flag byte per (8-trace, 256-facet) work
+ * item (rounded up to 8) + 256 bytes of lane masks per work item.  No
+ * counterpart in the reference (its kernels run one precision). */
 int64_t ss_refracted_workspace_bytes(int precision, int64_t n_facets, int64_t n_traces);
 
 /* Batched persistent launch over uneven jobs: every (job, trace tile, facet
@@ -245,10 +246,10 @@ int64_t ss_refracted_workspace_bytes(int precision, int64_t n_facets, int64_t n_
  * by ``n_blocks`` persistent workgroups (0 = a default of 4 per CU).  The
  * job table is copied from host memory into ``jobs_device`` (caller-owned
  * device buffer of at least n_jobs * sizeof(ss_job) bytes).  ``workspace``
- * (caller-owned device memory, 256-byte aligned, at least
+ * (caller-owned device memory, 8-byte aligned, at least
  * ss_batch_workspace_bytes() bytes) holds the work counter and, for
- * SS_PREC_MIXED, the float64 redo queue; it is reset on the stream, so the
- * call allocates nothing.  trace_tile: traces per workgroup (4, 8 or 16;
+ * SS_PREC_MIXED, the float64 redo flags and lane masks; the counter is reset
+ * on the stream, so the call allocates nothing.  trace_tile: traces per workgroup (4, 8 or 16;
  * chosen by the autotuner).  No counterpart in the reference (its requests
  * run one jit call after another); serves batched and continuous-batching
  * callers. */
@@ -257,9 +258,9 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
                    void* stream);
 
 /* Workspace bytes of ss_batch_field for these jobs (host table) and trace
- * tile: 256 for SS_PREC_FP64, plus 8 + 32 * trace_tile bytes per work item
- * for SS_PREC_MIXED; -1 for an invalid trace_tile.  No counterpart in the
- * reference. */
+ * tile: 256 for SS_PREC_FP64; for SS_PREC_MIXED plus one flag byte per work
+ * item (rounded up to 8) and 32 * trace_tile bytes of lane masks per work
+ * item; -1 for an invalid trace_tile.  No counterpart in the reference. */
 int64_t ss_batch_workspace_bytes(int precision, const ss_job* jobs_host, int n_jobs,
                                  int trace_tile);
 

@@ -5,6 +5,7 @@ The shared library ``libsoundersim_hip.so`` is built in-tree by
 fallback: if the library is missing, every entry point raises.
 """
 
+import contextlib
 import ctypes
 import os
 
@@ -85,18 +86,38 @@ class Job(ctypes.Structure):
 
 
 _lib = None
+_override = None
+
+
+@contextlib.contextmanager
+def use(lib):
+    """Route every call of the package to ``lib`` (a handle prepared by
+    configure(), e.g. a CPU build of the same kernels) inside the block."""
+    global _override
+    prev, _override = _override, lib
+    try:
+        yield lib
+    finally:
+        _override = prev
 
 
 def load():
     """Load (once) and return the ctypes library; raises if it is missing."""
     global _lib
+    if _override is not None:
+        return _override
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"HIP library not built: {LIB_PATH} is missing (run "
             "radar_sounder_simulator_amd.build.build_library())")
-    lib = ctypes.CDLL(LIB_PATH)
+    _lib = configure(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def configure(lib):
+    """Declare the C-ABI signatures on a loaded library handle."""
     vp, i64, i32, dp = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p
     lib.ss_version.restype = ctypes.c_char_p
     lib.ss_last_error.restype = ctypes.c_char_p
@@ -119,7 +140,6 @@ def load():
     lib.ss_sizeof_job.restype = ctypes.c_int64
     lib.ss_range_compress.argtypes = [dp, i64, i64, dp, i64, dp, vp]
     lib.ss_range_compress.restype = ctypes.c_int
-    _lib = lib
     return lib
 
 

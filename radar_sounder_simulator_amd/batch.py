@@ -54,7 +54,7 @@ class BatchRunner:
         rc = lib.ss_batch_field(prec, arr, n, ctypes.c_void_p(self.jobs_dev.data_ptr()),
                                 ctypes.c_void_p(self.workspace.data_ptr()),
                                 self.workspace.numel(), int(trace_tile), int(n_blocks),
-                                dv._stream_handle(stream))
+                                dv._stream_handle(stream, self.device))
         _lib.check(rc, "ss_batch_field")
 
     def run_scenes(self, scenes, precision="mixed", trace_tile=8, n_blocks=0, stream=None,

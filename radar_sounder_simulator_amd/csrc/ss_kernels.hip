@@ -23,12 +23,13 @@
 // precision path cannot be resolved to the float64 bin/cell decisions (a
 // residual in the ambiguous band, a bin position within the guard of an
 // edge) contributes nothing in the hot kernel; each wave records the lanes
-// concerned per trace as a 64-bit ballot in LDS, and a work item with any
-// such lane appends its id and masks to a queue in the caller's workspace.
-// A second launch (redo_kernel) evaluates exactly those pairs on the float64
-// tier and adds them with global atomics.  The queue holds one slot per work
-// item, so it cannot overflow, and the hot kernel never carries the float64
-// path's registers (about 130 vs 280 VGPRs).
+// concerned per trace as a 64-bit ballot in LDS, and every work item writes
+// one flag (any such lane or not) and, when flagged, its lane masks to the
+// caller's workspace -- plain stores, one slot per work item, nothing to
+// reset.  A second launch (redo_kernel / the batch redo pass) walks the
+// flags and runs the float64 trace loop on the flagged items only, masked
+// to the recorded lanes.  The hot kernel never carries the float64 path's
+// registers (about 170 vs 300 VGPRs).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -95,27 +96,27 @@ struct Shared {
 };
 
 // Workspace header of the entry points that take a workspace.
-constexpr long kWsHeader = 256;  // [0] work counter, [1] redo queue length
+constexpr long kWsHeader = 256;  // [0] work counter of a batched launch
 
-// Redo queue of one launch: ``queue`` holds work-item ids, ``masks`` holds
-// TT x 4 lane masks per work item (indexed by the item id).
+// Redo record of one launch: ``flags`` holds one byte per work item (1 =
+// float64 lanes pending), ``masks`` TT x 4 lane masks per flagged item.
 struct Redo {
-  unsigned long long* count;
-  long long* queue;
+  unsigned char* flags;
   unsigned long long* masks;
 };
+
+__host__ __device__ inline long redo_flag_bytes(long n_items) { return (n_items + 7) & ~7L; }
 
 __host__ __device__ inline Redo redo_view(void* ws, long n_items) {
   Redo r;
   char* b = static_cast<char*>(ws);
-  r.count = reinterpret_cast<unsigned long long*>(b) + 1;
-  r.queue = reinterpret_cast<long long*>(b + kWsHeader);
-  r.masks = reinterpret_cast<unsigned long long*>(b + kWsHeader + 8 * n_items);
+  r.flags = reinterpret_cast<unsigned char*>(b + kWsHeader);
+  r.masks = reinterpret_cast<unsigned long long*>(b + kWsHeader + redo_flag_bytes(n_items));
   return r;
 }
 
 __host__ __device__ inline long redo_bytes(long n_items, int tt) {
-  return kWsHeader + 8 * n_items + 8L * 4 * tt * n_items;
+  return kWsHeader + redo_flag_bytes(n_items) + 8L * 4 * tt * n_items;
 }
 
 template <typename ACC, int TT>
@@ -139,9 +140,10 @@ __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int b
   }
 }
 
-// Trace loop shared by both kernel families.  PAIR(t, o) evaluates this
-// lane's facet against trace record t into o.  REDO: pairs with o.redo set
-// are queued under work item ``item`` of ``rd`` instead of accumulated.
+// Trace loop shared by both kernel families.  PAIR(t, o, it) evaluates this
+// lane's facet against trace record t (the it-th of the tile) into o; a pair
+// left with o.redo set contributes nothing and writes no pair bin.  REDO:
+// such pairs are recorded under work item ``item`` of ``rd``.
 template <typename ACC, int TT, bool REDO = false, typename PairFn>
 __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
                                   bool live, long f, int n_samples, bool split,
@@ -168,7 +170,8 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     o.nparts = 0;
     o.valid = false;
     o.redo = false;
-    if (live) pair(sh.tr[it], o);
+    o.pwr_invalid = 0.0;
+    if (live) pair(sh.tr[it], o, it);
     if constexpr (REDO) {
       const unsigned long long m = __ballot(live && o.redo);
       if (lane == 0) sh.redo[it][wave] = m;
@@ -243,42 +246,9 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     bool any = false;
     for (int i = 0; i < nT; ++i)
       any |= ((sh.redo[i][0] | sh.redo[i][1]) | (sh.redo[i][2] | sh.redo[i][3])) != 0ULL;
-    if (any) {
-      if (tid < TT * 4)
-        rd.masks[item * (TT * 4) + tid] = (tid >> 2) < nT ? sh.redo[tid >> 2][tid & 3] : 0ULL;
-      if (tid == 0) rd.queue[atomicAdd(rd.count, 1ULL)] = item;
-    }
-  }
-}
-
-__device__ inline TraceRec load_trace_rec(const KCommon& k, long t) {
-  TraceRec r;
-  double* d = reinterpret_cast<double*>(&r);
-  for (int i = 0; i < SS_TROW_COUNT; ++i) d[i] = k.traces[(long)i * k.ts + t];
-  return r;
-}
-
-// Adds one pair to the field / dropped power with global atomics (the redo
-// path; same bookkeeping as the LDS path of trace_loop).
-__device__ inline void accumulate_global(const KCommon& k, long t, const PairOut& o,
-                                         int n_samples, bool split) {
-  if (!o.valid) {
-    if (o.pwr_invalid != 0.0) atomicAdd(&k.dropped[t], o.pwr_invalid);
-    return;
-  }
-#pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    if (p >= o.nparts) break;
-    const int b = o.bin[p];
-    const double re = o.re[p], im = o.im[p];
-    if (b >= 0 && b < n_samples) {
-      const long idx = (t * k.n_seg + (long)(split ? o.right : 0) * n_samples + b) * 2;
-      if (re != 0.0) atomicAdd(&k.field[idx], re);
-      if (im != 0.0) atomicAdd(&k.field[idx + 1], im);
-    } else {
-      const double pw = re * re + im * im;
-      if (pw != 0.0) atomicAdd(&k.dropped[t], pw);
-    }
+    if (any && tid < TT * 4)
+      rd.masks[item * (TT * 4) + tid] = (tid >> 2) < nT ? sh.redo[tid >> 2][tid & 3] : 0ULL;
+    if (tid == 0) rd.flags[item] = any ? 1 : 0;
   }
 }
 
@@ -439,14 +409,14 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
   const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
   if constexpr (PREC == SS_PREC_FP64) {
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
-                             [&](const TraceRec& t, PairOut& o) {
+                             [&](const TraceRec& t, PairOut& o, int) {
                                surface_pair_fp64<RM, PAT>(f64, t, sc, o);
                              });
   } else {
     SurfFacet32 f32;
     to_f32(f64, f32);
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
-                             [&](const TraceRec& t, PairOut& o) {
+                             [&](const TraceRec& t, PairOut& o, int) {
                                surface_pair_mixed<RM, PAT>(f32, nullptr, t, sc, o);
                              });
   }
@@ -488,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
     RefrFacet fr;
     load_refr_facet(k, fi, rc.flags, fr);
     trace_loop<ACC, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
-                             [&](const TraceRec& t, PairOut& o) {
+                             [&](const TraceRec& t, PairOut& o, int) {
                                refracted_pair_fp64<NMAX, RM, PAT>(fr, t, rc, o);
                              });
   } else {
@@ -499,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
       to_f32(fr, f32);
     }
     trace_loop<ACC, kTTRefr, true>(sh, k, w, live, f, rc.n_samples, split,
-                                   [&](const TraceRec& t, PairOut& o) {
+                                   [&](const TraceRec& t, PairOut& o, int) {
                                      if ((rc.flags & SS_REDO_ALL) ||
                                          !refracted_pair_mixed<NMAX, RM, PAT>(f32, t, rc, o)) {
                                        o.valid = false;
@@ -511,50 +481,58 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
   }
 }
 
-// Float64 evaluation of the queued pairs of work item w (TT traces x 256
-// facets; mask word [trace][wave], bit = lane), accumulated with global
-// atomics.  Every lane of the block runs this; only flagged lanes work.
-template <int NMAX, int RM, int PAT, int TT>
-__device__ inline void redo_item(const KCommon& k, const RefrConst& rc, long w,
-                                 const unsigned long long* m) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const long f = (w % k.n_ftiles) * kBlock + tid;
-  const long t_base = (w / k.n_ftiles) * TT;
-  unsigned bits = 0;
-  for (int it = 0; it < TT; ++it) bits |= (unsigned)((m[it * 4 + wave] >> lane) & 1ULL) << it;
-  if (bits == 0 || f >= k.nf) return;
-  const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
-  RefrFacet fr;
-  load_refr_facet(k, f, rc.flags, fr);
-  for (int it = 0; it < TT; ++it) {
-    if (!((bits >> it) & 1u)) continue;
-    const long t = t_base + it;
-    if (t >= k.nt) break;
-    const TraceRec tr = load_trace_rec(k, t);
-    PairOut o;
-    o.nparts = 0;
-    o.valid = false;
-    o.redo = false;
-    refracted_pair_fp64<NMAX, RM, PAT>(fr, tr, rc, o);
-    accumulate_global(k, t, o, rc.n_samples, split);
-    if (k.pair_bins) k.pair_bins[t * k.nf + f] = o.raw_bin;
-  }
-}
-
-// Drains the redo queue of one refracted_kernel launch.  The queue length is
-// read on the device, so the launch needs no host synchronisation; with an
-// empty queue every block exits at once.
+// Float64 pass over the work items the mixed refracted_kernel flagged: the
+// float64 trace loop of refracted_kernel, masked to the recorded lanes.
+// Grid-stride over work items (the flag test is uniform per block).
 template <int NMAX, int RM, int PAT>
 __global__ __launch_bounds__(kBlock) void redo_kernel(KCommon k, RefrConst rc, Redo rd) {
-  const long n = (long)*rd.count;
-  for (long i = blockIdx.x; i < n; i += gridDim.x) {
-    const long w = rd.queue[i];
-    redo_item<NMAX, RM, PAT, kTTRefr>(k, rc, w, rd.masks + w * (kTTRefr * 4));
+  __shared__ Shared<double, kTTRefr> sh;
+  const long n_items = (long)k.n_ftiles * k.n_ttiles;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
+  for (long w = blockIdx.x; w < n_items; w += gridDim.x) {
+    if (!rd.flags[w]) continue;
+    const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
+    const bool live = f < k.nf;
+    RefrFacet fr;
+    load_refr_facet(k, live ? f : 0, rc.flags, fr);
+    const unsigned long long* m = rd.masks + w * (kTTRefr * 4);
+    trace_loop<double, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
+                                [&](const TraceRec& t, PairOut& o, int it) {
+                                  if (((m[it * 4 + wave] >> lane) & 1ULL) == 0ULL) {
+                                    o.redo = true;  // resolved by the mixed pass
+                                    return;
+                                  }
+                                  refracted_pair_fp64<NMAX, RM, PAT>(fr, t, rc, o);
+                                });
   }
 }
 
 __global__ void reset_workspace_kernel(unsigned long long* hdr) {
-  if (threadIdx.x < 2) hdr[threadIdx.x] = 0ULL;
+  if (threadIdx.x == 0) hdr[0] = 0ULL;
+}
+
+// fold_kernel over every float64 surface job of a batch: blockIdx.y = job.
+__global__ void batch_fold_kernel(const ss_job* __restrict__ jobs) {
+  const ss_job& jb = jobs[blockIdx.y];
+  if (jb.kind != 0) return;
+  const int n_seg = ((jb.surface.flags & SS_SPLIT_SIDES) ? 2 : 1) * jb.surface.n_samples;
+  const long n = jb.n_traces * (long)n_seg;
+  const double k = jb.surface.k;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long t = i / n_seg;
+    const double rref = jb.traces[(long)SS_TROW_RREF * jb.trace_stride + t];
+    double th;
+    {
+#pragma clang fp contract(off)
+      th = fmod(2.0 * k * rref, kTwoPi);
+    }
+    const double c = cos(-th), s = sin(-th);
+    const double a = jb.field[2 * i], b = jb.field[2 * i + 1];
+    jb.field[2 * i] = a * c - b * s;
+    jb.field[2 * i + 1] = a * s + b * c;
+  }
 }
 
 // ------------------------------------------------------------ batched launch
@@ -646,16 +624,17 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
       load_surface_facet(k, live ? f : 0, sc.flags, f64);
       if constexpr (PREC == SS_PREC_FP64) {
         trace_loop<ACC, TT>(sh, k, w, live, f, sc.n_samples, split,
-                            [&](const TraceRec& t, PairOut& o) {
+                            [&](const TraceRec& t, PairOut& o, int) {
                               surface_pair_fp64(f64, t, sc, o);
                             });
       } else {
         SurfFacet32 f32;
         to_f32(f64, f32);
         trace_loop<ACC, TT>(sh, k, w, live, f, sc.n_samples, split,
-                            [&](const TraceRec& t, PairOut& o) {
+                            [&](const TraceRec& t, PairOut& o, int) {
                               surface_pair_mixed(f32, nullptr, t, sc, o);
                             });
+        if (tid == 0) rd.flags[gw] = 0;  // surface items never need the redo
       }
     } else {
       const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
@@ -666,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
         RefrFacet fr;
         load_refr_facet(k, fi, rc.flags, fr);
         trace_loop<ACC, TT>(sh, k, w, live, f, rc.n_samples, split,
-                            [&](const TraceRec& t, PairOut& o) {
+                            [&](const TraceRec& t, PairOut& o, int) {
                               refracted_pair_fp64<0>(fr, t, rc, o);
                             });
       } else {
@@ -677,7 +656,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
           to_f32(fr, f32);
         }
         trace_loop<ACC, TT, true>(sh, k, w, live, f, rc.n_samples, split,
-                                  [&](const TraceRec& t, PairOut& o) {
+                                  [&](const TraceRec& t, PairOut& o, int) {
                                     if ((rc.flags & SS_REDO_ALL) ||
                                         !refracted_pair_mixed<0>(f32, t, rc, o)) {
                                       o.valid = false;
@@ -691,22 +670,38 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
   }
 }
 
-// Drains the redo queue of one mixed batch_kernel launch (global work ids).
+// Float64 pass over the refracted work items a mixed batch_kernel launch
+// flagged (global work ids), masked to the recorded lanes.
 template <int TT>
 __global__ __launch_bounds__(kBlock) void batch_redo_kernel(const ss_job* __restrict__ jobs,
-                                                           int n_jobs, Redo rd) {
+                                                           int n_jobs, long total, Redo rd) {
+  __shared__ Shared<double, TT> sh;
   __shared__ long begins[kMaxJobs + 1];
-  const long n = (long)*rd.count;
-  if ((long)blockIdx.x >= n) return;  // uniform per block: before any barrier
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   job_prefix<TT>(jobs, n_jobs, begins);
-  for (long i = blockIdx.x; i < n; i += gridDim.x) {
-    const long gw = rd.queue[i];
+  for (long gw = blockIdx.x; gw < total; gw += gridDim.x) {
+    if (!rd.flags[gw]) continue;
     const int j = find_job(begins, n_jobs, gw);
     const ss_job& jb = jobs[j];
+    if (jb.kind != 1) continue;
+    const long w = gw - begins[j];
     const KCommon k = job_common<TT>(jb);
+    const long f = (w % k.n_ftiles) * kBlock + tid;
+    const bool live = f < k.nf;
     const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
                                          jb.pattern_table, jb.constants);
-    redo_item<0, -1, -1, TT>(k, rc, gw - begins[j], rd.masks + gw * (TT * 4));
+    const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
+    RefrFacet fr;
+    load_refr_facet(k, live ? f : 0, rc.flags, fr);
+    const unsigned long long* m = rd.masks + gw * (TT * 4);
+    trace_loop<double, TT>(sh, k, w, live, f, rc.n_samples, split,
+                           [&](const TraceRec& t, PairOut& o, int it) {
+                             if (((m[it * 4 + wave] >> lane) & 1ULL) == 0ULL) {
+                               o.redo = true;
+                               return;
+                             }
+                             refracted_pair_fp64<0>(fr, t, rc, o);
+                           });
   }
 }
 
@@ -967,8 +962,8 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   const int64_t need = ss_refracted_workspace_bytes(precision, n_facets, n_traces);
   if (need > 0 && (!workspace || workspace_bytes < need))
     return fail(SS_EINVAL, "workspace smaller than ss_refracted_workspace_bytes()");
-  if (need > 0 && (reinterpret_cast<uintptr_t>(workspace) & 255) != 0)
-    return fail(SS_EINVAL, "workspace must be 256-byte aligned");
+  if (need > 0 && (reinterpret_cast<uintptr_t>(workspace) & 7) != 0)
+    return fail(SS_EINVAL, "workspace must be 8-byte aligned");
   const bool split = (prm->flags & SS_SPLIT_SIDES) != 0;
   const int n_seg = (split ? 2 : 1) * prm->n_samples;
   hipStream_t st = (hipStream_t)stream;
@@ -989,14 +984,9 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   const bool fp64 = precision == SS_PREC_FP64;
   const int rm = rough_mode(prm->flags), pm = pattern_mode(prm->pattern);
   const long n_items = (long)k.n_ftiles * k.n_ttiles;
-  Redo rd{};
-  if (!fp64) {
-    rd = redo_view(workspace, n_items);
-    hipLaunchKernelGGL(reset_workspace_kernel, dim3(1), dim3(64), 0, st,
-                       static_cast<unsigned long long*>(workspace));
-  }
-  // Redo grid: enough blocks for the rare queued items, never idle-heavy.
-  dim3 rgrid((unsigned)(n_items < 256 ? n_items : 256));
+  const Redo rd = fp64 ? Redo{} : redo_view(workspace, n_items);
+  // Redo pass: a grid-stride sweep of the item flags, a few per block.
+  dim3 rgrid((unsigned)(n_items < 1024 ? n_items : 1024));
 #define SS_REFR(P, N, R, A) \
   hipLaunchKernelGGL((refracted_kernel<P, N, R, A>), grid, block, 0, st, k, rc, rd)
 #define SS_REDO(N, R, A) \
@@ -1113,8 +1103,8 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
     return SS_OK;
   }
   if (!jobs_host || !jobs_device || !workspace) return fail(SS_EINVAL, "null pointer");
-  if ((reinterpret_cast<uintptr_t>(workspace) & 255) != 0)
-    return fail(SS_EINVAL, "workspace must be 256-byte aligned");
+  if ((reinterpret_cast<uintptr_t>(workspace) & 7) != 0)
+    return fail(SS_EINVAL, "workspace must be 8-byte aligned");
   for (int i = 0; i < n_jobs; ++i) {
     int r = validate_job(jobs_host[i], i);
     if (r) return r;
@@ -1153,7 +1143,7 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   dim3 grid((unsigned)n_blocks), block(kBlock);
   const bool fp64 = precision == SS_PREC_FP64;
   const Redo rd = fp64 ? Redo{} : redo_view(workspace, total);
-  dim3 rgrid((unsigned)(total < 256 ? total : 256));
+  dim3 rgrid((unsigned)(total < 1024 ? total : 1024));
 #define SS_LAUNCH_BATCH(T)                                                                 \
   if (fp64) {                                                                              \
     hipLaunchKernelGGL((batch_kernel<SS_PREC_FP64, T>), grid, block, 0, st, jd, n_jobs,    \
@@ -1161,7 +1151,7 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   } else {                                                                                 \
     hipLaunchKernelGGL((batch_kernel<SS_PREC_MIXED, T>), grid, block, 0, st, jd, n_jobs,   \
                        total, hdr, rd);                                                    \
-    hipLaunchKernelGGL((batch_redo_kernel<T>), rgrid, block, 0, st, jd, n_jobs, rd);       \
+    hipLaunchKernelGGL((batch_redo_kernel<T>), rgrid, block, 0, st, jd, n_jobs, total, rd); \
   }
   if (trace_tile == 4) {
     SS_LAUNCH_BATCH(4);
@@ -1171,6 +1161,12 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
     SS_LAUNCH_BATCH(16);
   }
 #undef SS_LAUNCH_BATCH
+  if (fp64) {  // reference-phase fold-back of the float64 surface jobs
+    bool any_surface = false;
+    for (int i = 0; i < n_jobs; ++i) any_surface |= jobs_host[i].kind == 0;
+    if (any_surface)
+      hipLaunchKernelGGL(batch_fold_kernel, dim3(64, (unsigned)n_jobs), dim3(256), 0, st, jd);
+  }
   e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
   g_last_error.clear();

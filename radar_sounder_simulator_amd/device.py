@@ -34,7 +34,15 @@ def _dev(device):
     return torch.device(device)
 
 
-def _stream_handle(stream):
+def _stream_handle(stream, dev=None):
+    """HIP stream handle for a launch on tensors of ``dev``.  Host tensors
+    are accepted only by a library build that runs on host memory (the CPU
+    emulation the tests use); the HIP library refuses them here, before any
+    host pointer could reach a kernel."""
+    if dev is not None and torch.device(dev).type != "cuda":
+        if not getattr(_lib.load(), "host_memory", False):
+            raise RuntimeError(f"the HIP kernels need GPU tensors, got {dev} tensors")
+        return ctypes.c_void_p(0)
     if stream is None:
         return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     return ctypes.c_void_p(stream.cuda_stream)
@@ -235,7 +243,7 @@ def surface_field(facets, traces, *, k, gamma, t0, dt, n_samples, c,
     rc = _lib.load().ss_surface_field(
         prec, ctypes.byref(p), _ptr(facets.buf), facets.stride, facets.n,
         _ptr(traces.buf), traces.stride, T, _ptr(table), _ptr(cst), _ptr(out),
-        _ptr(dropped), _ptr(pb_t), _stream_handle(stream))
+        _ptr(dropped), _ptr(pb_t), _stream_handle(stream, dev))
     _lib.check(rc, "ss_surface_field")
     if pair_bins:
         return out, dropped, pb_t, table
@@ -396,7 +404,7 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
         prec, ctypes.byref(p), _ptr(target.buf), target.stride, target.n,
         _ptr(traces.buf), traces.stride, T, _ptr(ifaces.desc), _ptr(ifaces.tables),
         _ptr(legs_t), _ptr(table), _ptr(cst), _ptr(out), _ptr(dropped), _ptr(pb_t),
-        _ptr(workspace), ws_bytes, _stream_handle(stream))
+        _ptr(workspace), ws_bytes, _stream_handle(stream, dev))
     _lib.check(rc, "ss_refracted_field")
     keep = (legs_t, table, workspace)
     if pair_bins:
@@ -430,6 +438,6 @@ def range_compress(field, pulse, m, out=None, stream=None):
         rc = lib.ss_range_compress(ctypes.c_void_p(src.data_ptr() + off * 8), nr, n,
                                    _ptr(pt), int(m),
                                    ctypes.c_void_p(out.data_ptr() + off * 8),
-                                   _stream_handle(stream))
+                                   _stream_handle(stream, field.device))
         _lib.check(rc, "ss_range_compress")
     return out, pt

@@ -68,7 +68,7 @@ def test_invalid_arguments_rejected_on_host():
     assert lib.ss_refracted_field(1, *wargs, None, 0, None) == -1
     assert b"workspace" in lib.ss_last_error()
     assert lib.ss_refracted_field(1, *wargs, dummy, 100, None) == -1
-    assert lib.ss_refracted_field(1, *wargs, dummy + 8, 1 << 20, None) == -1   # alignment
+    assert lib.ss_refracted_field(1, *wargs, dummy + 4, 1 << 20, None) == -1   # alignment
     assert b"aligned" in lib.ss_last_error()
     assert lib.ss_range_compress(None, 1, 8, None, 1, None, None) == -1
     assert lib.ss_batch_field(0, None, 0, None, None, 0, 8, 0, None) == 0   # empty batch
@@ -81,16 +81,17 @@ def test_invalid_arguments_rejected_on_host():
 
 def test_workspace_sizes():
     lib = _lib.load()
-    # fp64 needs none; mixed: 256-byte header + (8 + 4 * 8 * 8) bytes per
-    # (8-trace, 256-facet) work item -- 1000 facets x 20 traces = 4 x 3 items
+    # fp64 needs none; mixed: 256-byte header + one flag byte per (8-trace,
+    # 256-facet) work item rounded up to 8 + 4 * 8 * 8 mask bytes per item
+    # -- 1000 facets x 20 traces = 4 x 3 items
     assert lib.ss_refracted_workspace_bytes(0, 1000, 20) == 0
-    assert lib.ss_refracted_workspace_bytes(1, 1000, 20) == 256 + 12 * (8 + 256)
+    assert lib.ss_refracted_workspace_bytes(1, 1000, 20) == 256 + 16 + 12 * 256
     assert lib.ss_refracted_workspace_bytes(1, 0, 20) == 0
     j = _lib.Job()
     j.kind, j.n_facets, j.n_traces = 1, 1000, 20
     assert lib.ss_batch_workspace_bytes(0, ctypes.byref(j), 1, 8) == 256
-    assert lib.ss_batch_workspace_bytes(1, ctypes.byref(j), 1, 8) == 256 + 12 * (8 + 256)
-    assert lib.ss_batch_workspace_bytes(1, ctypes.byref(j), 1, 16) == 256 + 8 * (8 + 512)
+    assert lib.ss_batch_workspace_bytes(1, ctypes.byref(j), 1, 8) == 256 + 16 + 12 * 256
+    assert lib.ss_batch_workspace_bytes(1, ctypes.byref(j), 1, 16) == 256 + 8 + 8 * 512
     assert lib.ss_batch_workspace_bytes(1, ctypes.byref(j), 1, 5) == -1
 
 

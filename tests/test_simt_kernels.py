@@ -1,0 +1,157 @@
+"""The gfx950 kernel source executed on the CPU through the SIMT emulation
+(tests/native/simt/hip/hip_runtime.h): one host thread per lane, real
+barriers, wavefront shuffles/ballots and atomics.  This checks the kernels'
+own bookkeeping -- trace tiles, LDS bin windows and flushes, the deferred
+float64 redo (item flags and lane masks), batched persistent launches and
+the float64 fold-back -- against the oracle and against each other without
+a GPU.  Float32 hardware primitives run as libm here, so mixed-tier values
+are checked to the mixed tolerance only.  Sizes are tiny."""
+
+import numpy as np
+import pytest
+import torch
+
+import simt_harness as sh
+from oracle import coherent as oc
+from oracle import multilayer as om
+from radar_sounder_simulator_amd import _lib
+from radar_sounder_simulator_amd import batch as B
+from radar_sounder_simulator_amd import device as dv
+from radar_sounder_simulator_amd import workloads as W
+from radar_sounder_simulator_amd.pipeline import ResidentScene
+from radar_sounder_simulator_amd.roughness import speckle_phasors
+from radar_sounder_simulator_amd.scene import gaussian_surface, grid_facets, straight_track
+
+C = 299792458.0
+
+
+@pytest.fixture(scope="module")
+def sim():
+    return sh.lib()
+
+
+def _bed(seed=5, n=20):
+    rng = np.random.default_rng(seed)
+    dx = 25.0
+    surf = grid_facets(3.0 * gaussian_surface(n + 1, dx, 100.0, rng), dx)
+    bed = grid_facets(-300.0 + 30.0 * gaussian_surface(n + 1, dx, 200.0, rng), dx)
+    tr = straight_track(5, 40.0, 3000.0)
+    return surf, bed, tr
+
+
+def _refr(sim, precision, extra=0, rough=False):
+    surf, bed, tr = _bed()
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=2 * np.pi * 60e6 / C)
+    ph = None
+    if rough:
+        ph = speckle_phasors(len(bed), 3)
+        kw.update(roughness=(0.5, 20.0, 24, "exponential"), d_phi_area=True, taper_s=0.05)
+    with _lib.use(sim):
+        fac = dv.DeviceFacets(bed.centers, bed.normals, bed.areas, bed.e1, bed.e2,
+                              phasors=ph, device="cpu")
+        ifc = dv.DeviceInterfaces([surf], device="cpu", z_platform=tr.positions[:, 2].min())
+        trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
+        out, drop, bins, _ = dv.refracted_field(fac, trs, ifc, [1.0, 3.15], [0.0, 10.0],
+                                                precision=precision, pair_bins=True,
+                                                extra_flags=extra, **kw)
+    o = out.numpy()
+    ref_kw = dict(kw)
+    if rough:
+        ref_kw["roughness"] = (0.5, 20.0, ph, 24, "exponential")
+    return (o[..., 0] + 1j * o[..., 1], drop.numpy(), fac.unpermute_pairs(bins.numpy()),
+            (tr, surf, bed, ref_kw))
+
+
+@pytest.mark.parametrize("rough", [False, True])
+def test_refracted_fp64_kernel_vs_oracle(sim, rough):
+    out, drop, bins, (tr, surf, bed, kw) = _refr(sim, "fp64", rough=rough)
+    ref, dref, bref, vref = om.refracted_cluttergram(tr.positions, tr.u_ct, bed, [surf],
+                                                     [1.0, 3.15], [0.0, 10.0],
+                                                     return_bins=True, **kw)
+    np.testing.assert_array_equal(bins, np.where(vref, bref, _lib.BIN_INVALID))
+    assert np.linalg.norm(out - ref) <= 1e-9 * np.linalg.norm(ref)
+    np.testing.assert_allclose(drop, dref, rtol=1e-9, atol=1e-30)
+
+
+def test_mixed_redo_pass_reproduces_fp64(sim):
+    """Every pair forced through the deferred float64 pass: flags, lane masks
+    and the masked float64 trace loop give the float64 kernel's result."""
+    o64, d64, b64, _ = _refr(sim, "fp64", rough=True)
+    orr, drr, brr, _ = _refr(sim, "mixed", extra=_lib.SS_REDO_ALL, rough=True)
+    om_, dm, bm, _ = _refr(sim, "mixed", rough=True)
+    np.testing.assert_array_equal(brr, b64)
+    np.testing.assert_array_equal(bm, b64)
+    assert np.linalg.norm(orr - o64) <= 1e-12 * np.linalg.norm(o64)
+    np.testing.assert_allclose(drr, d64, rtol=1e-12, atol=1e-30)
+    pk = np.abs(o64).max(axis=-1, keepdims=True)
+    m = np.abs(o64) > pk * 10 ** (-30 / 20)
+    assert np.abs(20 * np.log10(np.abs(om_[m]) / np.abs(o64[m]))).max() <= 0.05
+    assert np.abs(np.angle(om_[m] / o64[m])).max() * 1e3 <= 2.0
+
+
+def test_surface_fp64_kernel_vs_oracle(sim):
+    wl = W.config1(n=24, n_traces=5, n_samples=512, seed=11)
+    f, tr = wl.layers[0].facets, wl.track
+    kw = dict(k=wl.k0, gamma=wl.layers[0].gamma, t0=wl.t0, dt=wl.dt,
+              n_samples=wl.n_samples, c=wl.c)
+    ref, dref, bref, _ = oc.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals,
+                                                 f.areas, f.e1, f.e2, return_bins=True, **kw)
+    from radar_sounder_simulator_amd.kernels import _reference_range
+    with _lib.use(sim):
+        fac = dv.DeviceFacets(f.centers, f.normals, f.areas, f.e1, f.e2, device="cpu")
+        trs = dv.DeviceTraces(tr.positions, tr.u_ct, _reference_range(tr.positions, f.centers),
+                              None, device="cpu")
+        out, drop, bins, _ = dv.surface_field(fac, trs, precision="fp64", pair_bins=True, **kw)
+    o = out.numpy()
+    o = o[..., 0] + 1j * o[..., 1]
+    np.testing.assert_array_equal(fac.unpermute_pairs(bins.numpy()), bref)
+    assert np.linalg.norm(o - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
+def _scenes():
+    return [W.config2(n=9, n_traces=9, n_samples=4096, seed=3),
+            W.config1(n=17, n_traces=5, n_samples=512, seed=8)]
+
+
+def _run_single(sim, wls, precision):
+    outs = []
+    with _lib.use(sim):
+        for w in wls:
+            rs = ResidentScene(w, device="cpu")
+            for j in range(len(rs.layers)):
+                rs.run_layer(j, precision)
+            outs.append(rs.raw.numpy().copy())
+    return outs
+
+
+def _run_batch(sim, wls, precision, tt, redo_all=False):
+    with _lib.use(sim):
+        scenes = [ResidentScene(w, device="cpu") for w in wls]
+        jobs, keep = [], []
+        for rs in scenes:
+            j, k = rs.jobs(precision)
+            jobs += j
+            keep += k
+        if redo_all:
+            for jb in jobs:
+                if jb.kind == 1:
+                    jb.refracted.flags |= _lib.SS_REDO_ALL
+        B.BatchRunner(device="cpu").launch(jobs, precision, trace_tile=tt, n_blocks=3)
+        return [rs.raw.numpy().copy() for rs in scenes]
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("tt", [4, 16])
+def test_batched_launch_matches_single_launches(sim, tt):
+    wls = _scenes()
+    ref64 = _run_single(sim, wls, "fp64")
+    for o, r in zip(_run_batch(sim, wls, "fp64", tt), ref64):
+        assert all(np.linalg.norm(r[j]) > 0 for j in range(r.shape[0]))
+        assert _rel(o, r) <= 1e-12
+    for o, r in zip(_run_batch(sim, wls, "mixed", tt, redo_all=True), ref64):
+        assert _rel(o[1:], r[1:]) <= 1e-12          # refracted layers: float64 redo
+        assert _rel(o[:1], r[:1]) <= 1e-4           # surface layer: mixed tier
