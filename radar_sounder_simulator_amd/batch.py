@@ -71,46 +71,64 @@ class BatchRunner:
         return keep
 
 
+def draw_scene(rng, srng, scale=1.0, max_facets=None):
+    """One scene of the uneven mix: (layers, deepest mean depth).  Facets
+    log-uniform in [1e4, 2e6] x scale per interface, 1-4 interfaces: a
+    surface and internal layers 150-800 m apart (eps 3.15, 10 dB/km ice;
+    internal layers -40 dB, the last one a bed over eps 8)."""
+    F = math.exp(rng.uniform(math.log(1e4), math.log(2e6))) * scale
+    if max_facets:
+        F = min(F, max_facets)
+    L = int(rng.integers(1, 5))
+    side = max(8, int(round(math.sqrt(F))))
+    dx = 25.0
+    surf = grid_facets(2.0 * gaussian_surface(side + 1, dx, 200.0, srng), dx)
+    eps_ice = 3.15
+    layers = [Layer("surface", surf, fresnel_normal(1.0, eps_ice))]
+    crossed = [surf]
+    eps = [1.0, eps_ice]
+    att = [0.0, 10.0]
+    depth = 0.0
+    for j in range(1, L):
+        depth += rng.uniform(150.0, 800.0)
+        h = -depth + 20.0 * gaussian_surface(side + 1, dx, 1000.0, srng)
+        f = grid_facets(h, dx)
+        eps_below = eps_ice if j < L - 1 else 8.0
+        layers.append(Layer(f"layer{j}", f, fresnel_normal(eps[-1], eps_below)
+                            if j == L - 1 else -10 ** (-40 / 20), crossed=list(crossed),
+                            eps_leg=list(eps), att_leg=list(att)))
+        crossed.append(f)
+        eps = eps + [eps_ice]
+        att = att + [10.0]
+    return layers, depth
+
+
+def draw_instrument(rng, name, layers, depth, scale=1.0, n_samples=None, cross_offset=0.0):
+    """A Workload observing ``layers`` with a freshly drawn instrument and
+    trajectory: traces uniform in [50, 2000] x scale at 25 m along a track
+    shifted ``cross_offset`` m across, centre frequency 30-330 MHz,
+    bandwidth 10-30 MHz, sampling 2.5 B, a window reaching past the deepest
+    layer."""
+    T = int(max(4, round(rng.uniform(50, 2000) * scale)))
+    f0 = rng.uniform(30e6, 330e6)
+    B = rng.uniform(10e6, 30e6)
+    alt = 20000.0
+    dt = 1.0 / (2.5 * B)
+    n = n_samples or int(2 ** math.ceil(math.log2(2.0 * (depth * 1.78 + 2000.0) / C / dt)))
+    track = straight_track(T, 25.0, alt, cross_offset=cross_offset)
+    F = len(layers[0].facets)
+    return Workload(
+        name=name, track=track, layers=layers, k0=2 * math.pi * f0 / C,
+        t0=2.0 * (alt - 100.0) / C, dt=dt, n_samples=n, c=C,
+        waveform=Waveform(kind="chirp", bandwidth=B, pulse_length=10e-6),
+        meta=dict(f0=f0, B=B, F=F, T=T, L=len(layers)))
+
+
 def uneven_workloads(n_scenes=64, seed=4, scale=1.0, n_samples=None, max_facets=None):
     """The uneven 64-scene mix (module docstring) as a list of Workloads."""
     rng = np.random.default_rng(seed)
     out = []
     for s in range(n_scenes):
-        F = math.exp(rng.uniform(math.log(1e4), math.log(2e6))) * scale
-        if max_facets:
-            F = min(F, max_facets)
-        T = int(max(4, round(rng.uniform(50, 2000) * scale)))
-        L = int(rng.integers(1, 5))
-        f0 = rng.uniform(30e6, 330e6)
-        B = rng.uniform(10e6, 30e6)
-        side = max(8, int(round(math.sqrt(F))))
-        dx = 25.0
-        srng = np.random.default_rng([seed, s])
-        alt = 20000.0
-        surf = grid_facets(2.0 * gaussian_surface(side + 1, dx, 200.0, srng), dx)
-        eps_ice = 3.15
-        layers = [Layer("surface", surf, fresnel_normal(1.0, eps_ice))]
-        crossed = [surf]
-        eps = [1.0, eps_ice]
-        att = [0.0, 10.0]
-        depth = 0.0
-        for j in range(1, L):
-            depth += rng.uniform(150.0, 800.0)
-            h = -depth + 20.0 * gaussian_surface(side + 1, dx, 1000.0, srng)
-            f = grid_facets(h, dx)
-            eps_below = eps_ice if j < L - 1 else 8.0
-            layers.append(Layer(f"layer{j}", f, fresnel_normal(eps[-1], eps_below)
-                                if j == L - 1 else -10 ** (-40 / 20), crossed=list(crossed),
-                                eps_leg=list(eps), att_leg=list(att)))
-            crossed.append(f)
-            eps = eps + [eps_ice]
-            att = att + [10.0]
-        dt = 1.0 / (2.5 * B)
-        n = n_samples or int(2 ** math.ceil(math.log2(2.0 * (depth * 1.78 + 2000.0) / C / dt)))
-        track = straight_track(T, 25.0, alt)
-        out.append(Workload(
-            name=f"scene{s}", track=track, layers=layers, k0=2 * math.pi * f0 / C,
-            t0=2.0 * (alt - 100.0) / C, dt=dt, n_samples=n, c=C,
-            waveform=Waveform(kind="chirp", bandwidth=B, pulse_length=10e-6),
-            meta=dict(f0=f0, B=B, F=side * side, T=T, L=L)))
+        layers, depth = draw_scene(rng, np.random.default_rng([seed, s]), scale, max_facets)
+        out.append(draw_instrument(rng, f"scene{s}", layers, depth, scale, n_samples))
     return out

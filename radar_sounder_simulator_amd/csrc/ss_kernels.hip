@@ -53,6 +53,16 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
+// Wave-uniform copy of a value every lane of the workgroup holds (an LDS
+// broadcast, a table entry indexed by one), so that loop exits and job
+// selection in the persistent kernels compile to scalar branches.
+__device__ inline int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline long uniform_l(long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(v & 0xffffffffL));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)((unsigned long)v >> 32));
+  return (long)(((unsigned long)hi << 32) | lo);
+}
+
 __device__ inline int wave_min_i(int v) {
   for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
   return v;
@@ -491,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void redo_kernel(KCommon k, RefrConst rc, R
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
   for (long w = blockIdx.x; w < n_items; w += gridDim.x) {
-    if (!rd.flags[w]) continue;
+    if (!uniform_i(rd.flags[w])) continue;
     const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
     const bool live = f < k.nf;
     RefrFacet fr;
@@ -608,16 +618,16 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
   for (;;) {
     if (tid == 0) s_w = (long)atomicAdd(counter, 1ULL);
     __syncthreads();
-    const long gw = s_w;
+    const long gw = uniform_l(s_w);
     __syncthreads();
     if (gw >= total) break;
-    const int j = find_job(begins, n_jobs, gw);
+    const int j = uniform_i(find_job(begins, n_jobs, gw));
     const ss_job& jb = jobs[j];
     const long w = gw - begins[j];
     const KCommon k = job_common<TT>(jb);
     const long f = (w % k.n_ftiles) * kBlock + tid;
     const bool live = f < k.nf;
-    if (jb.kind == 0) {
+    if (uniform_i(jb.kind) == 0) {
       const SurfaceConst sc = make_surface_const(jb.surface, jb.pattern_table, jb.constants);
       const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
       SurfFacet64 f64;
@@ -667,6 +677,11 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
                                   rd, gw);
       }
     }
+    // Reconverge before the next work id: without this barrier the compiler
+    // merges the ``tid == 0`` flag stores above with the ``tid == 0``
+    // counter update at the loop top and lets the other lanes loop back to
+    // the work-id read on their own, re-reading a stale id forever.
+    __syncthreads();
   }
 }
 
@@ -680,10 +695,10 @@ __global__ __launch_bounds__(kBlock) void batch_redo_kernel(const ss_job* __rest
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   job_prefix<TT>(jobs, n_jobs, begins);
   for (long gw = blockIdx.x; gw < total; gw += gridDim.x) {
-    if (!rd.flags[gw]) continue;
-    const int j = find_job(begins, n_jobs, gw);
+    if (!uniform_i(rd.flags[gw])) continue;
+    const int j = uniform_i(find_job(begins, n_jobs, gw));
     const ss_job& jb = jobs[j];
-    if (jb.kind != 1) continue;
+    if (uniform_i(jb.kind) != 1) continue;
     const long w = gw - begins[j];
     const KCommon k = job_common<TT>(jb);
     const long f = (w % k.n_ftiles) * kBlock + tid;

@@ -1,0 +1,443 @@
+"""Request loop with continuous batching over resident scenes.
+
+A ``Server`` keeps scenes -- every layer's facets and crossed-interface
+tables -- resident in HBM and admits requests by their HBM footprint
+against a byte budget (by default the device's memory less a reserve).
+Admitted requests run together: every step is one persistent batched
+launch (``ss_batch_field``) over trace chunks of all runnable requests,
+sized to a pair budget shared evenly between them, so a request that
+arrives while others are on the GPU joins the next step rather than
+waiting for them to finish, and its work overlaps theirs on the device.
+A request's last step is followed by its range compression and an
+asynchronous copy to pinned host memory; its latency runs from arrival to
+the end of that copy.  Memory is returned when the copy completes; scenes
+no request uses are evicted least-recently-used first when an admission
+needs their room.
+
+Each trace of a request is computed by exactly one work item set per layer
+whichever step carries it, so a result is the same alone and in a mix up to
+the order of the floating-point sums.
+
+The reference has no request loop: its requests are one ``simulate()``
+call after another (``simulate.py:172``).  This module is the serving
+stream's host side; the kernels are the batched ones of ``batch.py``.
+"""
+
+import collections
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as dv
+from .batch import BatchRunner, draw_instrument, draw_scene
+from .kernels import _reference_range
+from .waveform import compressed_pulse
+
+
+@dataclass
+class Request:
+    """One simulation request: the scene it observes (``scene_id``; the
+    workload's layers are that scene's), the instrument and trajectory
+    (the rest of ``workload``) and its arrival time (s from stream start)."""
+    req_id: int
+    scene_id: str
+    workload: object
+    arrival: float = 0.0
+
+
+@dataclass
+class Result:
+    req_id: int
+    scene_id: str
+    field: np.ndarray          # (L, T, n_seg) complex128, range-compressed
+    dropped: np.ndarray        # (L, T)
+    pairs: int
+    arrival: float
+    admitted: float
+    done: float
+    uploaded: bool
+    steps: int = 1             # batched steps that carried this request
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+class DeviceScene:
+    """A scene's layers resident in HBM (facets + crossed-interface tables)."""
+
+    def __init__(self, layers, device):
+        self.layers = []
+        for L in layers:
+            ph = None if L.roughness is None else L.roughness[2]
+            g = np.asarray(L.gamma, np.float64)
+            fac = dv.DeviceFacets.from_facets(L.facets, gamma=g if g.ndim else None,
+                                              phasors=ph, device=device)
+            ifc = dv.DeviceInterfaces(L.crossed, device=device) if L.crossed else None
+            self.layers.append((L, fac, ifc))
+        self.refs = 0
+
+    def nbytes(self):
+        return sum(f.nbytes() + (i.nbytes() if i is not None else 0)
+                   for _, f, i in self.layers)
+
+    @staticmethod
+    def estimate(layers):
+        """HBM bytes DeviceScene(layers) will take (facet SoA rows + tables)."""
+        b = 0
+        for L in layers:
+            b += _lib.FROW["COUNT"] * len(L.facets) * 8
+            for c in (L.crossed or []):
+                b += (len(c) * 6 + 16) * 8
+        return b
+
+
+class ActiveRequest:
+    """An admitted request: its traces, output buffers, per-layer job
+    templates and how far each layer's traces have been launched."""
+
+    def __init__(self, req, scene, device, precision, admitted):
+        self.req = req
+        self.scene = scene
+        self.admitted = admitted
+        wl = req.workload
+        tr = wl.track
+        pv = None if wl.pattern is None else wl.pattern[1]
+        surf = wl.layers[0].facets
+        self.traces = dv.DeviceTraces(tr.positions, tr.u_ct,
+                                      _reference_range(tr.positions, surf.centers), pv,
+                                      device=device)
+        self.T = T = tr.positions.shape[0]
+        self.n_seg = n_seg = (2 if wl.split_sides else 1) * wl.n_samples
+        nl = len(scene.layers)
+        self.raw = torch.empty((nl, T, n_seg, 2), dtype=torch.float64, device=device)
+        self.out = torch.empty_like(self.raw)
+        self.dropped = torch.empty((nl, T), dtype=torch.float64, device=device)
+        self.pulse = None
+        if wl.waveform is not None and wl.waveform.kind != "delta":
+            self.pulse = compressed_pulse(wl.waveform.bandwidth, wl.waveform.pulse_length,
+                                          wl.dt, wl.waveform.window,
+                                          wl.waveform.construction, wl.waveform.tukey_alpha)
+        self.keep = []
+        self.templates = []
+        cst = dv.constants(device)
+        for L, fac, ifc in scene.layers:
+            jb = _lib.Job()
+            jb.facet_stride, jb.n_facets = fac.stride, fac.n
+            jb.trace_stride = self.traces.stride
+            jb.facets = fac.buf.data_ptr()
+            jb.constants = cst.data_ptr()
+            g = 0.0 if fac.has_gamma else L.gamma
+            rough = (None if L.roughness is None else
+                     (L.roughness[0], L.roughness[1], L.roughness[3],
+                      L.roughness[4] if len(L.roughness) > 4 else "gaussian"))
+            if ifc is None:
+                p, table = dv.surface_params(
+                    fac, k=wl.k0, gamma=g, t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples,
+                    c=wl.c, split_sides=wl.split_sides, pattern=wl.pattern, roughness=rough,
+                    taper_s=wl.taper_s, d_phi_area=wl.d_phi_area)
+                jb.kind = 0
+                jb.surface = p
+            else:
+                p, legs, table = dv.refracted_params(
+                    fac, ifc, L.eps_leg, L.att_leg, mode="coherent", t0=wl.t0, dt=wl.dt,
+                    n_samples=wl.n_samples, c=wl.c, gamma=g, k0=wl.k0,
+                    split_sides=wl.split_sides, pattern=wl.pattern, roughness=rough,
+                    taper_s=wl.taper_s, d_phi_area=wl.d_phi_area)
+                jb.kind = 1
+                jb.refracted = p
+                jb.iface_desc, jb.tables = ifc.desc.data_ptr(), ifc.tables.data_ptr()
+                jb.legs = legs.data_ptr()
+                self.keep.append(legs)
+            jb.pattern_table = 0 if table is None else table.data_ptr()
+            self.keep.append(table)
+            self.templates.append(jb)
+        self.cursor = [0] * nl
+        self.n_steps = 0
+        self.layer_facets = [fac.n for _, fac, _ in scene.layers]
+        self.pairs = T * sum(self.layer_facets)
+        self.event = None
+        self.host_out = None
+        self.host_drop = None
+
+    @staticmethod
+    def estimate(req):
+        """HBM bytes an admitted request holds (traces, raw + compressed
+        fields, dropped power)."""
+        wl = req.workload
+        T = wl.track.positions.shape[0]
+        n_seg = (2 if wl.split_sides else 1) * wl.n_samples
+        nl = len(wl.layers)
+        return _lib.TROW["COUNT"] * T * 8 + 2 * nl * T * n_seg * 16 + nl * T * 8
+
+    def remaining(self):
+        return sum(F * (self.T - c) for F, c in zip(self.layer_facets, self.cursor))
+
+    def take(self, budget, tt):
+        """Jobs for up to ``budget`` pairs of this request's next traces
+        (whole trace tiles, at least one tile), layer by layer."""
+        jobs = []
+        T = self.T
+        self.n_steps += 1
+        for j, F in enumerate(self.layer_facets):
+            if budget <= 0:
+                break
+            c = self.cursor[j]
+            if c >= T:
+                continue
+            n = max(tt, int(budget // max(F, 1)) // tt * tt)
+            n = min(n, T - c)
+            jobs.append(self._chunk(j, c, c + n))
+            self.cursor[j] = c + n
+            budget -= n * F
+        return jobs
+
+    def _chunk(self, j, t0, t1):
+        jb = _lib.Job.from_buffer_copy(self.templates[j])
+        jb.traces = self.traces.buf.data_ptr() + 8 * t0
+        jb.n_traces = t1 - t0
+        jb.field = self.raw.data_ptr() + 16 * (j * self.T + t0) * self.n_seg
+        jb.dropped = self.dropped.data_ptr() + 8 * (j * self.T + t0)
+        return jb
+
+
+class Server:
+    """Continuous-batching request loop on one device (see module docstring).
+
+    ``budget_bytes``: HBM the server may hold for scenes and requests
+    (default: total device memory less 8 GiB of reserve).  ``step_pairs``:
+    pair budget of one batched step.  ``trace_tile``/``n_blocks``: the
+    batched launch's tile and grid (autotuned values from ``autotune``)."""
+
+    def __init__(self, device=None, precision="mixed", budget_bytes=None, step_pairs=4e8,
+                 trace_tile=8, n_blocks=0, max_inflight=2, max_jobs=1024):
+        self.device = dv._dev(device)
+        self.cuda = self.device.type == "cuda"
+        self.precision = precision
+        if budget_bytes is None:
+            if not self.cuda:
+                raise ValueError("budget_bytes is required off the GPU")
+            total = torch.cuda.get_device_properties(self.device).total_memory
+            budget_bytes = total - (8 << 30)
+        self.budget = int(budget_bytes)
+        self.step_pairs = float(step_pairs)
+        self.tt = int(trace_tile)
+        self.n_blocks = int(n_blocks)
+        self.max_inflight = int(max_inflight)
+        self.max_jobs = int(max_jobs)
+        self.runner = BatchRunner(device=self.device, max_jobs=max_jobs)
+        self.scenes = collections.OrderedDict()    # scene_id -> DeviceScene (LRU order)
+        self.scene_bytes = {}
+        self.used = 0
+        self.peak = 0
+        self.steps = 0
+
+    # ------------------------------------------------------------ residency
+    def add_scene(self, scene_id, layers):
+        """Upload a scene and keep it resident (counted against the budget)."""
+        if scene_id in self.scenes:
+            self.scenes.move_to_end(scene_id)
+            return self.scenes[scene_id]
+        need = DeviceScene.estimate(layers)
+        if not self._make_room(need):
+            raise MemoryError(f"scene {scene_id} ({need} B) does not fit the budget")
+        sc = DeviceScene(layers, self.device)
+        self.scenes[scene_id] = sc
+        self.scene_bytes[scene_id] = sc.nbytes()
+        self._account(sc.nbytes())
+        return sc
+
+    def _account(self, delta):
+        self.used += delta
+        self.peak = max(self.peak, self.used)
+
+    def _make_room(self, need):
+        """Evict idle scenes (least recently used first) until ``need``
+        bytes fit; False if they cannot."""
+        if self.used + need <= self.budget:
+            return True
+        for sid in list(self.scenes):
+            if self.used + need <= self.budget:
+                break
+            sc = self.scenes[sid]
+            if sc.refs == 0:
+                del self.scenes[sid]
+                self._account(-self.scene_bytes.pop(sid))
+        return self.used + need <= self.budget
+
+    def _admit(self, req, now):
+        resident = req.scene_id in self.scenes
+        need = ActiveRequest.estimate(req)
+        if not resident:
+            need += DeviceScene.estimate(req.workload.layers)
+        if need > self.budget:
+            raise MemoryError(f"request {req.req_id} needs {need} B > budget {self.budget} B")
+        if resident:
+            self.scenes[req.scene_id].refs += 1   # pin before making room
+        ok = self._make_room(need)
+        if resident:
+            self.scenes[req.scene_id].refs -= 1
+        if not ok:
+            return None
+        sc = self.add_scene(req.scene_id, req.workload.layers)
+        sc.refs += 1
+        a = ActiveRequest(req, sc, self.device, self.precision, now)
+        a.uploaded = not resident
+        a.bytes = ActiveRequest.estimate(req)
+        self._account(a.bytes)
+        return a
+
+    # ----------------------------------------------------------------- steps
+    def _step(self, runnable):
+        share = self.step_pairs / len(runnable)
+        jobs = []
+        for a in runnable:
+            if len(jobs) + len(a.layer_facets) > self.max_jobs:
+                break                       # the rest run in the next step
+            jobs += a.take(share, self.tt)
+        self.runner.launch(jobs, self.precision, trace_tile=self.tt, n_blocks=self.n_blocks)
+        self.steps += 1
+        return jobs
+
+    def _finish_launch(self, a):
+        """Queue range compression and the copy to pinned host memory after
+        the request's last step (same stream, so it follows that step)."""
+        if a.pulse is None:
+            a.out.copy_(a.raw)
+        else:
+            p, m = a.pulse
+            n = a.req.workload.n_samples
+            _, pt = dv.range_compress(a.raw.view(-1, n, 2), p, m, out=a.out.view(-1, n, 2))
+            a.keep.append(pt)
+        if self.cuda:
+            a.host_out = torch.empty(a.out.shape, dtype=torch.float64, pin_memory=True)
+            a.host_drop = torch.empty(a.dropped.shape, dtype=torch.float64, pin_memory=True)
+            a.host_out.copy_(a.out, non_blocking=True)
+            a.host_drop.copy_(a.dropped, non_blocking=True)
+            a.event = torch.cuda.Event()
+            a.event.record()
+        else:
+            a.host_out = a.out.clone()
+            a.host_drop = a.dropped.clone()
+
+    def _done(self, a):
+        return a.event is None or a.event.query()
+
+    def _release(self, a, now, results):
+        o = a.host_out.numpy()
+        results.append(Result(req_id=a.req.req_id, scene_id=a.req.scene_id,
+                              field=o[..., 0] + 1j * o[..., 1] if self.keep_fields else None,
+                              dropped=a.host_drop.numpy().copy(), pairs=a.pairs,
+                              arrival=a.req.arrival if self.realtime else 0.0,
+                              admitted=a.admitted, done=now, uploaded=a.uploaded,
+                              steps=a.n_steps))
+        a.host_out = a.host_drop = None
+        a.scene.refs -= 1
+        self._account(-a.bytes)
+        a.raw = a.out = a.dropped = a.traces = None
+        a.keep = []
+
+    # ------------------------------------------------------------------ loop
+    def serve(self, requests, realtime=True, clock=time.perf_counter, keep_fields=True):
+        """Run a request stream to completion.  ``realtime``: requests arrive
+        at their ``arrival`` times on ``clock``; otherwise all are present
+        at once (in arrival order, latency counted from the start).
+        ``keep_fields``: keep each result's field on the host (else only its
+        dropped power and timings).  Returns the results in completion
+        order."""
+        self.realtime = realtime
+        self.keep_fields = keep_fields
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        active, finishing, steps_in_flight = [], [], collections.deque()
+        results = []
+        t0 = clock()
+
+        def now():
+            return clock() - t0 if realtime else math.inf
+
+        while pending or waiting or active or finishing:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.popleft())
+            while waiting:
+                a = self._admit(waiting[0], clock() - t0)
+                if a is None:
+                    break
+                waiting.popleft()
+                active.append(a)
+            runnable = [a for a in active if a.remaining() > 0]
+            if runnable:
+                while self.cuda and len(steps_in_flight) >= self.max_inflight:
+                    steps_in_flight.popleft().synchronize()
+                self._step(runnable)
+                if self.cuda:
+                    ev = torch.cuda.Event()
+                    ev.record()
+                    steps_in_flight.append(ev)
+                for a in runnable:
+                    if a.remaining() == 0:
+                        self._finish_launch(a)
+                        active.remove(a)
+                        finishing.append(a)
+            progressed = bool(runnable)
+            for a in list(finishing):
+                if self._done(a):
+                    finishing.remove(a)
+                    self._release(a, clock() - t0, results)
+                    progressed = True
+            if not progressed:
+                if finishing and self.cuda:
+                    time.sleep(0.0002)
+                elif pending and realtime and not waiting:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
+                elif waiting and not active and not finishing:
+                    raise MemoryError("a waiting request can never be admitted")
+        return results
+
+
+def latency_stats(results):
+    lat = np.array([r.latency for r in results])
+    if not len(lat):
+        return {}
+    span = max(r.done for r in results) - min(r.arrival for r in results)
+    pairs = sum(r.pairs for r in results)
+    return {"n": int(len(lat)), "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+            "max_s": float(lat.max()), "pairs": int(pairs),
+            "sustained_pairs_per_s": float(pairs / span) if span > 0 else None}
+
+
+def request_stream(n_requests, lam, seed=7, scale=1.0, n_resident=8, reuse=0.5,
+                   n_samples=None, max_facets=None):
+    """The serving stream: Poisson arrivals at ``lam`` requests/s, sizes
+    drawn as in the uneven batched mix; a fraction ``reuse`` observes one of
+    ``n_resident`` scenes (kept resident) with a new instrument and
+    trajectory, the rest bring a new scene.  Returns (resident scenes
+    {scene_id: layers}, requests)."""
+    rng = np.random.default_rng(seed)
+    resident = {}
+    depths = {}
+    for s in range(n_resident):
+        layers, depth = draw_scene(rng, np.random.default_rng([seed, 1000 + s]), scale,
+                                   max_facets)
+        resident[f"resident{s}"] = layers
+        depths[f"resident{s}"] = depth
+    reqs = []
+    t = 0.0
+    for i in range(n_requests):
+        t += rng.exponential(1.0 / lam)
+        if rng.uniform() < reuse:
+            sid = f"resident{int(rng.integers(n_resident))}"
+            layers, depth = resident[sid], depths[sid]
+        else:
+            sid = f"new{i}"
+            layers, depth = draw_scene(rng, np.random.default_rng([seed, i]), scale, max_facets)
+        wl = draw_instrument(rng, sid, layers, depth, scale, n_samples,
+                             cross_offset=float(rng.uniform(-200.0, 200.0)))
+        reqs.append(Request(req_id=i, scene_id=sid, workload=wl, arrival=t))
+    return resident, reqs

@@ -215,6 +215,8 @@ inline unsigned long long __ballot(int pred) {
   return m;
 }
 
+inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
+
 inline float atomicAdd(float* p, float v) { return sim::atomic_add_fp(p, v); }
 inline double atomicAdd(double* p, double v) { return sim::atomic_add_fp(p, v); }
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {

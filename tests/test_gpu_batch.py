@@ -72,10 +72,6 @@ def test_batch_mixed_matches_single(tt):
         np.testing.assert_allclose(d, rd, rtol=1e-5, atol=1e-30)
 
 
-@pytest.mark.skip(reason="kept off the GPU: this case hung the GPU once under an earlier "
-                         "queue-and-atomics redo design whose cause was never shown; the "
-                         "flag-and-mask redo that replaced it is checked on the CPU "
-                         "(test_simt_kernels.py) and, single-launch, by test_gpu_parity.py")
 def test_batch_redo_all_matches_fp64():
     print("case batch/redo_all", flush=True)
     scenes = _scenes()
