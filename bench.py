@@ -17,6 +17,19 @@ Output: one JSON line (rank 0).  ``roofline`` covers the dominant kernel
 (the refracted bed kernel) with its duration measured by HIP events on the
 launch stream; ``cpu_baseline`` times this project's float64 C restatement
 (oracle/csrc/oracle.c, OpenMP over traces) on a bounded sample of traces.
+
+Two further sections ride on the same line:
+  ``batched``  the uneven 64-scene mix (facets log-uniform in [1e4, 2e6],
+               traces uniform in [50, 2000], 1-4 layers, 30-330 MHz), sizes
+               times ``--batch-scale``, in one persistent launch with the
+               autotuned tile/grid, against the same scenes run one launch
+               per layer, scene after scene;
+  ``serving``  a Poisson request stream (``--lam`` requests/s, sizes as the
+               batched mix times ``--serve-scale``, half of the requests on 8
+               resident scenes with a new instrument and trajectory), served by
+               the continuous-batching loop on the mixed tier and on the
+               float64 tier: p50/p99 latency (arrival to result in pinned
+               host memory) and sustained pairs/s.
 """
 
 import argparse
@@ -84,6 +97,81 @@ def cpu_baseline(wl, n_traces):
                       f"restatement, OpenMP {nth} threads on {cpu}; {el:.1f} s"}
 
 
+def bench_batched(scale, dev, seed=4):
+    """One persistent launch over the 64-scene mix vs one launch per layer."""
+    import torch
+    from radar_sounder_simulator_amd import autotune
+    from radar_sounder_simulator_amd.batch import BatchRunner, uneven_workloads
+    from radar_sounder_simulator_amd.pipeline import ResidentScene
+    wls = uneven_workloads(64, seed=seed, scale=scale)
+    scenes = [ResidentScene(w, device=dev) for w in wls]
+    pairs = sum(sum(rs.pairs_per_layer()) for rs in scenes)
+    st = torch.cuda.current_stream()
+    out = {"scenes": 64, "scale": scale, "pairs": pairs,
+           "layers": sum(len(w.layers) for w in wls)}
+    runner = BatchRunner(device=dev)
+    tuner = autotune.Tuner()
+    for prec in ("mixed", "fp64"):
+        jobs, keep = [], []
+        for rs in scenes:
+            j, k = rs.jobs(prec)
+            jobs += j
+            keep += k
+        tt, nb = tuner.batch_shape(runner, jobs, prec)
+
+        def batched():
+            runner.launch(jobs, prec, trace_tile=tt, n_blocks=nb)
+            for rs in scenes:
+                rs.compress()
+
+        def sequential():
+            for rs in scenes:
+                rs.run(prec)
+
+        res = {"trace_tile": tt, "n_blocks": nb}
+        for name, fn in (("batched", batched), ("sequential", sequential)):
+            fn()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            fn()
+            e1.record(st)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            res[name] = {"ms": ms, "pairs_per_s": pairs / (ms * 1e-3)}
+        res["speedup"] = res["sequential"]["ms"] / res["batched"]["ms"]
+        out[prec] = res
+        log(f"bench: batched {prec}: {res['batched']['ms']:.1f} ms vs sequential "
+            f"{res['sequential']['ms']:.1f} ms (tile {tt}, {nb} blocks)")
+    return out
+
+
+def bench_serving(n_requests, lam, scale, dev, seed=7):
+    """The request stream on the mixed and on the float64 tier."""
+    import torch
+    from radar_sounder_simulator_amd import serve as S
+    resident, reqs = S.request_stream(n_requests, lam=lam, seed=seed, scale=scale,
+                                      n_resident=8, reuse=0.5)
+    out = {"requests": n_requests, "lambda_per_s": lam, "scale": scale,
+           "resident_scenes": 8, "reuse": 0.5,
+           "pairs": int(sum(r.workload.pairs() for r in reqs))}
+    for prec in ("mixed", "fp64"):
+        srv = S.Server(device=dev, precision=prec, step_pairs=4e8)
+        for sid, layers in resident.items():
+            srv.add_scene(sid, layers)
+        res = srv.serve(reqs, realtime=True, keep_fields=False)
+        torch.cuda.synchronize()
+        st = S.latency_stats(res)
+        st.update({"steps": srv.steps, "peak_hbm_bytes": srv.peak, "budget_bytes": srv.budget})
+        out[prec] = st
+        log(f"bench: serving {prec}: p50 {st['p50_s'] * 1e3:.1f} ms p99 "
+            f"{st['p99_s'] * 1e3:.1f} ms sustained {st['sustained_pairs_per_s']:.3e} pairs/s")
+        del srv
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +184,12 @@ def main():
     ap.add_argument("--cpu-traces", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--batch-scale", type=float, default=0.25)
+    ap.add_argument("--serve-scale", type=float, default=0.5)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--lam", type=float, default=8.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the coherent facet kernel shards by "
@@ -170,9 +264,17 @@ def main():
     f_bed = roofline.refracted_pair_flops(1, "tabulated", True, b.roughness[3])
     f_surf = roofline.surface_pair_flops("tabulated", True, s.roughness[3])
     achieved = f_bed * pairs_l[1] / (ms_b * 1e-3) / 1e12
+    kname = "refracted_kernel<1, 0, 1, 5>"      # mixed, sequential, area-only D_Phi, tabulated
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(kname, {}).get("bytes")
     roof = {"bound": "valu", "achieved": achieved, "peak": roofline.PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / roofline.PEAK_FP32_TFLOPS,
-            "traffic": None, "kernel": "refracted_kernel<mixed>",
+            "traffic": traffic, "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE "
+            "(calibrated by a 1 GiB copy), profiles/refracted_kernel_mixed_traffic_surface_bed.md",
+            "kernel": kname,
             "flops_per_pair": f_bed, "kernel_ms": ms_b,
             "surface_kernel": {"flops_per_pair": f_surf, "kernel_ms": ms_s,
                                "achieved": f_surf * pairs_l[0] / (ms_s * 1e-3) / 1e12}}
@@ -194,6 +296,15 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(extra)
+    del rs
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        log(f"bench: batched uneven mix (scale {args.batch_scale})")
+        line["batched"] = bench_batched(args.batch_scale, dev)
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        log(f"bench: serving stream ({args.serve_requests} requests at {args.lam}/s)")
+        line["serving"] = bench_serving(args.serve_requests, args.lam, args.serve_scale, dev)
     print(json.dumps(line), flush=True)
 
 

