@@ -277,6 +277,26 @@ int ss_range_compress(const double* in, int64_t n_rows, int64_t n_samples,
                       const double* pulse, int64_t m, double* out,
                       void* stream);
 
+/* FFT size ss_range_compress_fft uses for rows of n_samples and a (2m+1)-tap
+ * pulse: a power of two in [256, 8192], or 0 when the pulse is too long for
+ * the LDS-resident transform (use ss_range_compress).  No counterpart in
+ * the reference (numpy picks its FFT length itself, waveform.py:134). */
+int64_t ss_compress_fft_size(int64_t n_samples, int64_t m);
+
+/* Spectrum of a (2m+1)-tap complex128 pulse for ss_range_compress_fft:
+ * nfft complex128 values (device), in the transform's internal
+ * (bit-reversed) order, scaled by 1/nfft.  Computed once per pulse. */
+int ss_pulse_spectrum(const double* pulse, int64_t m, int64_t nfft, double* spectrum,
+                      void* stream);
+
+/* The same 'same' convolution as ss_range_compress by FFT overlap-save in
+ * float64 (the reference's method, waveform.py:126-138): each workgroup
+ * transforms nfft input samples in LDS, multiplies by the pulse spectrum
+ * and keeps the nfft - 2m wrap-free outputs.  in/out may not alias. */
+int ss_range_compress_fft(const double* in, int64_t n_rows, int64_t n_samples,
+                          const double* spectrum, int64_t m, int64_t nfft, double* out,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

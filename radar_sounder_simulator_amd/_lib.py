@@ -43,7 +43,8 @@ CONST_COUNT = 33 + MAX_TERMS + 1
 EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_fill_constants", "ss_surface_field", "ss_refracted_field",
             "ss_refracted_workspace_bytes", "ss_batch_field", "ss_batch_workspace_bytes",
-            "ss_sizeof_job", "ss_range_compress")
+            "ss_sizeof_job", "ss_range_compress", "ss_compress_fft_size", "ss_pulse_spectrum",
+            "ss_range_compress_fft")
 
 
 class SurfaceParams(ctypes.Structure):
@@ -140,6 +141,12 @@ def configure(lib):
     lib.ss_sizeof_job.restype = ctypes.c_int64
     lib.ss_range_compress.argtypes = [dp, i64, i64, dp, i64, dp, vp]
     lib.ss_range_compress.restype = ctypes.c_int
+    lib.ss_compress_fft_size.argtypes = [i64, i64]
+    lib.ss_compress_fft_size.restype = ctypes.c_int64
+    lib.ss_pulse_spectrum.argtypes = [dp, i64, i64, dp, vp]
+    lib.ss_pulse_spectrum.restype = ctypes.c_int
+    lib.ss_range_compress_fft.argtypes = [dp, i64, i64, dp, i64, i64, dp, vp]
+    lib.ss_range_compress_fft.restype = ctypes.c_int
     return lib
 
 

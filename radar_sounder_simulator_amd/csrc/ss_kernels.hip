@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -106,7 +107,7 @@ struct Shared {
 };
 
 // Workspace header of the entry points that take a workspace.
-constexpr long kWsHeader = 256;  // [0] work counter of a batched launch
+constexpr long kWsHeader = 256;  // [0], [1] work counters of a batched launch (per kind)
 
 // Redo record of one launch: ``flags`` holds one byte per work item (1 =
 // float64 lanes pending), ``masks`` TT x 4 lane masks per flagged item.
@@ -491,17 +492,42 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
   }
 }
 
+// Runs ITEM(w) for every flagged work item w < n_items: each block takes
+// chunks of kBlock items, one flag per lane, collects the waves' ballots in
+// LDS and walks the set bits in order (a uniform loop; ITEM may contain
+// barriers).  One pass over the flags costs one byte load per item per lane.
+template <typename ItemFn>
+__device__ inline void for_flagged_items(const unsigned char* __restrict__ flags, long n_items,
+                                         unsigned long long* sh_mask, ItemFn item) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (long base = (long)blockIdx.x * kBlock; base < n_items; base += (long)gridDim.x * kBlock) {
+    const long w = base + tid;
+    const unsigned long long m = __ballot(w < n_items && flags[w] != 0);
+    if (lane == 0) sh_mask[wave] = m;
+    __syncthreads();
+    for (int wv = 0; wv < kBlock / 64; ++wv) {
+      unsigned long long mm = (unsigned long long)uniform_l((long)sh_mask[wv]);
+      while (mm != 0ULL) {
+        const int bit = __builtin_ctzll(mm);
+        mm &= mm - 1ULL;
+        item(base + wv * 64 + bit);
+      }
+    }
+    __syncthreads();  // the masks are rewritten by the next chunk
+  }
+}
+
 // Float64 pass over the work items the mixed refracted_kernel flagged: the
 // float64 trace loop of refracted_kernel, masked to the recorded lanes.
 // Grid-stride over work items (the flag test is uniform per block).
 template <int NMAX, int RM, int PAT>
 __global__ __launch_bounds__(kBlock) void redo_kernel(KCommon k, RefrConst rc, Redo rd) {
   __shared__ Shared<double, kTTRefr> sh;
+  __shared__ unsigned long long sh_mask[kBlock / 64];
   const long n_items = (long)k.n_ftiles * k.n_ttiles;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
-  for (long w = blockIdx.x; w < n_items; w += gridDim.x) {
-    if (!uniform_i(rd.flags[w])) continue;
+  for_flagged_items(rd.flags, n_items, sh_mask, [&](long w) {
     const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
     const bool live = f < k.nf;
     RefrFacet fr;
@@ -515,11 +541,11 @@ __global__ __launch_bounds__(kBlock) void redo_kernel(KCommon k, RefrConst rc, R
                                   }
                                   refracted_pair_fp64<NMAX, RM, PAT>(fr, t, rc, o);
                                 });
-  }
+  });
 }
 
 __global__ void reset_workspace_kernel(unsigned long long* hdr) {
-  if (threadIdx.x == 0) hdr[0] = 0ULL;
+  if (threadIdx.x < 2) hdr[threadIdx.x] = 0ULL;
 }
 
 // fold_kernel over every float64 surface job of a batch: blockIdx.y = job.
@@ -556,10 +582,12 @@ __host__ __device__ inline long job_work(const ss_job& j, int tt) {
 }
 
 // LDS prefix of per-job work counts: begins[i] = first work id of job i.
-template <int TT>
+// KIND >= 0 counts only the jobs of that kind (the others get no items).
+template <int TT, int KIND = -1>
 __device__ inline void job_prefix(const ss_job* jobs, int n_jobs, long* begins) {
   const int tid = threadIdx.x;
-  for (int i = tid; i < n_jobs; i += kBlock) begins[i + 1] = job_work(jobs[i], TT);
+  for (int i = tid; i < n_jobs; i += kBlock)
+    begins[i + 1] = (KIND < 0 || jobs[i].kind == KIND) ? job_work(jobs[i], TT) : 0;
   __syncthreads();
   if (tid == 0) {
     begins[0] = 0;
@@ -568,7 +596,8 @@ __device__ inline void job_prefix(const ss_job* jobs, int n_jobs, long* begins) 
   __syncthreads();
 }
 
-// Job owning global work id gw (begins[lo] <= gw < begins[lo + 1]).
+// Job owning global work id gw (begins[lo] <= gw < begins[lo + 1]; the
+// last such index, so jobs without items are never chosen).
 __device__ inline int find_job(const long* begins, int n_jobs, long gw) {
   int lo = 0, hi = n_jobs;
   while (hi - lo > 1) {
@@ -604,9 +633,16 @@ __device__ inline KCommon job_common(const ss_job& jb) {
 
 // Persistent workgroups drain (job, trace tile, facet tile) items from one
 // counter; every workgroup exits once the counter passes the total.  Mixed
-// refracted items queue their unresolved pairs under their global work id.
-template <int PREC, int TT>
-__global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict__ jobs,
+// refracted items flag their unresolved pairs under their global work id.
+// RM/PAT: the widest roughness mode / antenna kind of the job mix (as in
+// surface_kernel), chosen on the host.  KIND: the launch runs the surface
+// (0) or the refracted (1) jobs of the table, so each instantiation carries
+// one pair function's registers.  The mixed instantiations are held to 3
+// waves per SIMD (168 VGPRs), where the single-launch kernels sit; the
+// job lookup would otherwise push them just past it.
+template <int PREC, int TT, int RM, int PAT, int KIND,
+          int WAVES = (PREC == SS_PREC_MIXED ? 3 : 1)>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void batch_kernel(const ss_job* __restrict__ jobs,
                                                       int n_jobs, long total,
                                                       unsigned long long* counter, Redo rd) {
   using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
@@ -614,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
   __shared__ long begins[kMaxJobs + 1];
   __shared__ long s_w;
   const int tid = threadIdx.x;
-  job_prefix<TT>(jobs, n_jobs, begins);
+  job_prefix<TT, KIND>(jobs, n_jobs, begins);
   for (;;) {
     if (tid == 0) s_w = (long)atomicAdd(counter, 1ULL);
     __syncthreads();
@@ -627,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
     const KCommon k = job_common<TT>(jb);
     const long f = (w % k.n_ftiles) * kBlock + tid;
     const bool live = f < k.nf;
-    if (uniform_i(jb.kind) == 0) {
+    if constexpr (KIND == 0) {
       const SurfaceConst sc = make_surface_const(jb.surface, jb.pattern_table, jb.constants);
       const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
       SurfFacet64 f64;
@@ -635,16 +671,15 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
       if constexpr (PREC == SS_PREC_FP64) {
         trace_loop<ACC, TT>(sh, k, w, live, f, sc.n_samples, split,
                             [&](const TraceRec& t, PairOut& o, int) {
-                              surface_pair_fp64(f64, t, sc, o);
+                              surface_pair_fp64<RM, PAT>(f64, t, sc, o);
                             });
       } else {
         SurfFacet32 f32;
         to_f32(f64, f32);
         trace_loop<ACC, TT>(sh, k, w, live, f, sc.n_samples, split,
                             [&](const TraceRec& t, PairOut& o, int) {
-                              surface_pair_mixed(f32, nullptr, t, sc, o);
+                              surface_pair_mixed<RM, PAT>(f32, nullptr, t, sc, o);
                             });
-        if (tid == 0) rd.flags[gw] = 0;  // surface items never need the redo
       }
     } else {
       const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
@@ -656,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
         load_refr_facet(k, fi, rc.flags, fr);
         trace_loop<ACC, TT>(sh, k, w, live, f, rc.n_samples, split,
                             [&](const TraceRec& t, PairOut& o, int) {
-                              refracted_pair_fp64<0>(fr, t, rc, o);
+                              refracted_pair_fp64<0, RM, PAT>(fr, t, rc, o);
                             });
       } else {
         RefrFacet32 f32;
@@ -668,7 +703,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ss_job* __restrict_
         trace_loop<ACC, TT, true>(sh, k, w, live, f, rc.n_samples, split,
                                   [&](const TraceRec& t, PairOut& o, int) {
                                     if ((rc.flags & SS_REDO_ALL) ||
-                                        !refracted_pair_mixed<0>(f32, t, rc, o)) {
+                                        !refracted_pair_mixed<0, RM, PAT>(f32, t, rc, o)) {
                                       o.valid = false;
                                       o.pwr_invalid = 0.0;
                                       o.redo = true;
@@ -692,13 +727,13 @@ __global__ __launch_bounds__(kBlock) void batch_redo_kernel(const ss_job* __rest
                                                            int n_jobs, long total, Redo rd) {
   __shared__ Shared<double, TT> sh;
   __shared__ long begins[kMaxJobs + 1];
+  __shared__ unsigned long long sh_mask[kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  job_prefix<TT>(jobs, n_jobs, begins);
-  for (long gw = blockIdx.x; gw < total; gw += gridDim.x) {
-    if (!uniform_i(rd.flags[gw])) continue;
+  job_prefix<TT, 1>(jobs, n_jobs, begins);
+  for_flagged_items(rd.flags, total, sh_mask, [&](long gw) {
     const int j = uniform_i(find_job(begins, n_jobs, gw));
     const ss_job& jb = jobs[j];
-    if (uniform_i(jb.kind) != 1) continue;
+    if (uniform_i(jb.kind) != 1) return;
     const long w = gw - begins[j];
     const KCommon k = job_common<TT>(jb);
     const long f = (w % k.n_ftiles) * kBlock + tid;
@@ -717,7 +752,7 @@ __global__ __launch_bounds__(kBlock) void batch_redo_kernel(const ss_job* __rest
                              }
                              refracted_pair_fp64<0>(fr, t, rc, o);
                            });
-  }
+  });
 }
 
 // ------------------------------------------------------- range compression
@@ -755,6 +790,124 @@ __global__ __launch_bounds__(kBlock) void compress_kernel(const double2* __restr
     __syncthreads();
   }
   if (i0 + tid < n) out[row * n + i0 + tid] = make_double2(ar, ai);
+}
+
+// FFT range compression (overlap-save).  The 'same' convolution
+// out[i] = sum_j p[j] x[i + m - j] (2m+1 taps, zero lag at tap m) is the
+// linear convolution y = p * x read from y[m].  One workgroup computes L =
+// N - K + 1 consecutive outputs of one row (K = 2m+1): it loads the N input
+// samples x[i0 + m - (K-1) ...] (zeros outside the row) into LDS, transforms
+// them with a decimation-in-frequency FFT (natural order in, bit-reversed
+// out), multiplies by the pulse spectrum (same bit-reversed order, scaled by
+// 1/N), transforms back with a decimation-in-time inverse (bit-reversed in,
+// natural out) and keeps the circular-convolution samples K-1 .. N-1, the
+// ones free of wrap-around.  The whole N-point float64 transform lives in
+// LDS (N <= 8192: 128 KiB); twiddles come from sincospi per butterfly, so
+// no table is needed.  This is the reference's own method
+// (waveform.py:126-138 multiplies FFTs), blocked so any row length fits.
+constexpr int kFftMax = 8192;
+constexpr int kFftThreads = 512;
+
+struct cd {
+  double x, y;
+};
+__device__ inline cd cmul(cd a, cd b) { return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+// Forward DIF over a[0..n), n = 2^logn: natural order in, bit-reversed out.
+__device__ inline void fft_dif(cd* a, int n, int logn) {
+  for (int s = 0; s < logn; ++s) {
+    const int lh = logn - 1 - s, h = 1 << lh;  // butterfly span
+    for (int b = threadIdx.x; b < (n >> 1); b += blockDim.x) {
+      const int j = b & (h - 1);
+      const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+      const cd u = a[i0], v = a[i1];
+      double sn, cs;
+      sincospi(-(double)j / (double)h, &sn, &cs);
+      a[i0] = cd{u.x + v.x, u.y + v.y};
+      a[i1] = cmul(cd{u.x - v.x, u.y - v.y}, cd{cs, sn});
+    }
+    __syncthreads();
+  }
+}
+
+// Inverse DIT over a[0..n): bit-reversed in, natural order out (unscaled).
+__device__ inline void ifft_dit(cd* a, int n, int logn) {
+  for (int lh = 0; lh < logn; ++lh) {
+    const int h = 1 << lh;
+    for (int b = threadIdx.x; b < (n >> 1); b += blockDim.x) {
+      const int j = b & (h - 1);
+      const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
+      double sn, cs;
+      sincospi((double)j / (double)h, &sn, &cs);
+      const cd u = a[i0], t = cmul(a[i1], cd{cs, sn});
+      a[i0] = cd{u.x + t.x, u.y + t.y};
+      a[i1] = cd{u.x - t.x, u.y - t.y};
+    }
+    __syncthreads();
+  }
+}
+
+// Pulse spectrum for compress_fft_kernel: DIF of the K taps zero-padded to
+// n, scaled by 1/n (the inverse's normalisation), bit-reversed order.
+__global__ __launch_bounds__(kFftThreads) void pulse_spectrum_kernel(const cd* __restrict__ pulse,
+                                                                    long K, int n, int logn,
+                                                                    cd* __restrict__ spec) {
+  __shared__ cd a[kFftMax];
+  for (int t = threadIdx.x; t < n; t += blockDim.x) a[t] = t < K ? pulse[t] : cd{0.0, 0.0};
+  __syncthreads();
+  fft_dif(a, n, logn);
+  const double sc = 1.0 / (double)n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) spec[t] = cd{a[t].x * sc, a[t].y * sc};
+}
+
+// grid (blocks per row, rows): block bx of row r produces out[i0, i0 + L).
+// NCAP: LDS capacity in points (nfft <= NCAP); smaller capacities let more
+// workgroups share a CU's 160 KiB of LDS.
+template <int NCAP>
+__global__ __launch_bounds__(kFftThreads) void compress_fft_kernel(const cd* __restrict__ in,
+                                                                  long n, const cd* __restrict__ spec,
+                                                                  long m, int nfft, int logn,
+                                                                  cd* __restrict__ out) {
+  __shared__ cd a[NCAP];
+  const long K = 2 * m + 1, L = nfft - K + 1;
+  const long row = blockIdx.y;
+  const long i0 = (long)blockIdx.x * L;
+  const long base = i0 + m - (K - 1);
+  const cd* x = in + row * n;
+  for (int t = threadIdx.x; t < nfft; t += blockDim.x) {
+    const long xi = base + t;
+    a[t] = (xi >= 0 && xi < n) ? x[xi] : cd{0.0, 0.0};
+  }
+  __syncthreads();
+  fft_dif(a, nfft, logn);
+  for (int t = threadIdx.x; t < nfft; t += blockDim.x) a[t] = cmul(a[t], spec[t]);
+  __syncthreads();
+  ifft_dit(a, nfft, logn);
+  cd* o = out + row * n;
+  for (long r = threadIdx.x; r < L; r += blockDim.x) {
+    const long i = i0 + r;
+    if (i < n) o[i] = a[K - 1 + r];
+  }
+}
+
+// FFT size for rows of n samples and 2m+1 taps: the power of two in
+// [2K, kFftMax] (at least 256) with the fewest butterflies per output
+// sample, or 0 when the pulse is too long for an LDS-resident transform.
+__host__ inline int compress_fft_size(long n, long m) {
+  const long K = 2 * m + 1;
+  int best = 0;
+  double best_cost = 0.0;
+  for (int N = 256; N <= kFftMax; N <<= 1) {
+    const long L = N - K + 1;
+    if (L < N / 4) continue;  // mostly wrap-around: too small
+    const long blocks = (n + L - 1) / L;
+    const double cost = (double)blocks * N * std::log2((double)N);
+    if (best == 0 || cost < best_cost) {
+      best = N;
+      best_cost = cost;
+    }
+  }
+  return best;
 }
 
 __global__ void fill_kernel(double* p, long n, double v) {
@@ -1000,8 +1153,9 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   const int rm = rough_mode(prm->flags), pm = pattern_mode(prm->pattern);
   const long n_items = (long)k.n_ftiles * k.n_ttiles;
   const Redo rd = fp64 ? Redo{} : redo_view(workspace, n_items);
-  // Redo pass: a grid-stride sweep of the item flags, a few per block.
-  dim3 rgrid((unsigned)(n_items < 1024 ? n_items : 1024));
+  // Redo pass: each block sweeps chunks of kBlock item flags.
+  const long rchunks = (n_items + kBlock - 1) / kBlock;
+  dim3 rgrid((unsigned)(rchunks < 1024 ? rchunks : 1024));
 #define SS_REFR(P, N, R, A) \
   hipLaunchKernelGGL((refracted_kernel<P, N, R, A>), grid, block, 0, st, k, rc, rd)
 #define SS_REDO(N, R, A) \
@@ -1128,6 +1282,10 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   if (workspace_bytes < ss_batch_workspace_bytes(precision, jobs_host, n_jobs, trace_tile))
     return fail(SS_EINVAL, "workspace smaller than ss_batch_workspace_bytes()");
   hipStream_t st = (hipStream_t)stream;
+  long items[2] = {0, 0};
+  // feature class of the mix: 0 = no roughness, isotropic; 1 = at most
+  // area-only D_Phi and a tabulated pattern; 2 = anything
+  int cls = 0;
   for (int i = 0; i < n_jobs; ++i) {
     const ss_job& j = jobs_host[i];
     int n_seg_i = j.kind == 0
@@ -1137,6 +1295,12 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
     if (r) return r;
     r = fill(j.dropped, j.n_traces, st);
     if (r) return r;
+    items[j.kind] += job_work(j, trace_tile);
+    const int fl = j.kind == 0 ? j.surface.flags : j.refracted.flags;
+    const int pk = j.kind == 0 ? j.surface.pattern : j.refracted.pattern;
+    const int rm = rough_mode(fl), pm = pattern_mode(pk);
+    const int c = (rm == 0 && pm == 0) ? 0 : ((rm <= 1 && pm != -1) ? 1 : 2);
+    cls = c > cls ? c : cls;
   }
   hipError_t e = hipMemcpyAsync(jobs_device, jobs_host, sizeof(ss_job) * (size_t)n_jobs,
                                 hipMemcpyHostToDevice, st);
@@ -1153,29 +1317,48 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     n_blocks = 4 * cus;
   }
-  if ((long)n_blocks > total) n_blocks = (int)total;
   const ss_job* jd = (const ss_job*)jobs_device;
-  dim3 grid((unsigned)n_blocks), block(kBlock);
+  dim3 block(kBlock);
   const bool fp64 = precision == SS_PREC_FP64;
-  const Redo rd = fp64 ? Redo{} : redo_view(workspace, total);
-  dim3 rgrid((unsigned)(total < 1024 ? total : 1024));
-#define SS_LAUNCH_BATCH(T)                                                                 \
-  if (fp64) {                                                                              \
-    hipLaunchKernelGGL((batch_kernel<SS_PREC_FP64, T>), grid, block, 0, st, jd, n_jobs,    \
-                       total, hdr, rd);                                                    \
-  } else {                                                                                 \
-    hipLaunchKernelGGL((batch_kernel<SS_PREC_MIXED, T>), grid, block, 0, st, jd, n_jobs,   \
-                       total, hdr, rd);                                                    \
-    hipLaunchKernelGGL((batch_redo_kernel<T>), rgrid, block, 0, st, jd, n_jobs, total, rd); \
+  // the redo flags and masks index the refracted launch's work ids
+  const Redo rd = fp64 ? Redo{} : redo_view(workspace, items[1]);
+  const long rchunks = (items[1] + kBlock - 1) / kBlock;
+  dim3 rgrid((unsigned)(rchunks < 1024 ? rchunks : 1024));
+  for (int kind = 0; kind < 2; ++kind) {
+    const long tot = items[kind];
+    if (tot == 0) continue;
+    dim3 grid((unsigned)((long)n_blocks < tot ? n_blocks : tot));
+    unsigned long long* ctr = hdr + kind;
+#define SS_BATCH_K(P, T, R, A)                                                                \
+  if (kind == 0)                                                                              \
+    hipLaunchKernelGGL((batch_kernel<P, T, R, A, 0>), grid, block, 0, st, jd, n_jobs, tot, ctr, \
+                       rd);                                                                   \
+  else                                                                                        \
+    hipLaunchKernelGGL((batch_kernel<P, T, R, A, 1>), grid, block, 0, st, jd, n_jobs, tot, ctr, \
+                       rd)
+#define SS_BATCH_C(P, T)                                                                      \
+  if (cls == 0) { SS_BATCH_K(P, T, 0, 0); }                                                   \
+  else if (cls == 1) { SS_BATCH_K(P, T, 1, 5); }                                              \
+  else { SS_BATCH_K(P, T, -1, -1); }
+#define SS_LAUNCH_BATCH(T)                                                                    \
+  if (fp64) {                                                                                 \
+    SS_BATCH_C(SS_PREC_FP64, T)                                                               \
+  } else {                                                                                    \
+    SS_BATCH_C(SS_PREC_MIXED, T)                                                              \
+    if (kind == 1)                                                                            \
+      hipLaunchKernelGGL((batch_redo_kernel<T>), rgrid, block, 0, st, jd, n_jobs, tot, rd);   \
   }
-  if (trace_tile == 4) {
-    SS_LAUNCH_BATCH(4);
-  } else if (trace_tile == 8) {
-    SS_LAUNCH_BATCH(8);
-  } else {
-    SS_LAUNCH_BATCH(16);
-  }
+    if (trace_tile == 4) {
+      SS_LAUNCH_BATCH(4);
+    } else if (trace_tile == 8) {
+      SS_LAUNCH_BATCH(8);
+    } else {
+      SS_LAUNCH_BATCH(16);
+    }
 #undef SS_LAUNCH_BATCH
+#undef SS_BATCH_C
+#undef SS_BATCH_K
+  }
   if (fp64) {  // reference-phase fold-back of the float64 surface jobs
     bool any_surface = false;
     for (int i = 0; i < n_jobs; ++i) any_surface |= jobs_host[i].kind == 0;
@@ -1184,6 +1367,63 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   }
   e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int64_t ss_compress_fft_size(int64_t n_samples, int64_t m) {
+  if (n_samples <= 0 || m < 0) return 0;
+  return compress_fft_size((long)n_samples, (long)m);
+}
+
+int ss_pulse_spectrum(const double* pulse, int64_t m, int64_t nfft, double* spectrum,
+                      void* stream) {
+  if (!pulse || !spectrum) return fail(SS_EINVAL, "null pointer");
+  if (m < 0) return fail(SS_EINVAL, "negative m");
+  if (nfft < 256 || nfft > kFftMax || (nfft & (nfft - 1)) != 0)
+    return fail(SS_ERANGE, "nfft must be a power of two in [256, 8192]");
+  if (2 * m + 1 > nfft) return fail(SS_ERANGE, "pulse longer than nfft");
+  const int logn = (int)std::lround(std::log2((double)nfft));
+  hipLaunchKernelGGL(pulse_spectrum_kernel, dim3(1), dim3(kFftThreads), 0, (hipStream_t)stream,
+                     (const cd*)pulse, (long)(2 * m + 1), (int)nfft, logn, (cd*)spectrum);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("spectrum launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_range_compress_fft(const double* in, int64_t n_rows, int64_t n_samples,
+                          const double* spectrum, int64_t m, int64_t nfft, double* out,
+                          void* stream) {
+  if (n_rows < 0 || n_samples < 0 || m < 0) return fail(SS_EINVAL, "negative sizes");
+  if (n_rows == 0 || n_samples == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  if (!in || !out || !spectrum) return fail(SS_EINVAL, "null pointer");
+  if (in == out) return fail(SS_EINVAL, "in and out must not alias");
+  if (nfft < 256 || nfft > kFftMax || (nfft & (nfft - 1)) != 0)
+    return fail(SS_ERANGE, "nfft must be a power of two in [256, 8192]");
+  const long L = nfft - (2 * m + 1) + 1;
+  if (L < 1) return fail(SS_ERANGE, "pulse longer than nfft");
+  if (n_rows > 65535) return fail(SS_ERANGE, "n_rows > 65535: split the call");
+  const long blocks = (n_samples + L - 1) / L;
+  const int logn = (int)std::lround(std::log2((double)nfft));
+  const dim3 grid((unsigned)blocks, (unsigned)n_rows);
+  hipStream_t st = (hipStream_t)stream;
+#define SS_FFT(CAP)                                                                            \
+  hipLaunchKernelGGL((compress_fft_kernel<CAP>), grid, dim3(kFftThreads), 0, st, (const cd*)in, \
+                     (long)n_samples, (const cd*)spectrum, (long)m, (int)nfft, logn, (cd*)out)
+  if (nfft <= 2048) {
+    SS_FFT(2048);
+  } else if (nfft <= 4096) {
+    SS_FFT(4096);
+  } else {
+    SS_FFT(kFftMax);
+  }
+#undef SS_FFT
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("compress launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
 }

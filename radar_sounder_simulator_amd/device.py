@@ -418,26 +418,64 @@ def refracted_workspace_bytes(precision, n_facets, n_traces):
     return int(_lib.load().ss_refracted_workspace_bytes(prec, int(n_facets), int(n_traces)))
 
 
-def range_compress(field, pulse, m, out=None, stream=None):
+class PulseSpectrum:
+    """A pulse prepared for range compression on ``device``: its taps as a
+    device tensor and, when the FFT path applies to rows of ``n_samples``,
+    its spectrum (computed once on the device)."""
+
+    def __init__(self, pulse, m, n_samples, device, method="auto", stream=None):
+        p = np.asarray(pulse)
+        self.m = int(m)
+        self.n_samples = int(n_samples)
+        self.taps = torch.as_tensor(
+            np.stack([p.real, p.imag], -1).astype(np.float64) if np.iscomplexobj(p) else
+            np.stack([p, np.zeros_like(p)], -1).astype(np.float64),
+            device=_dev(device)).contiguous()
+        lib = _lib.load()
+        self.nfft = 0 if method == "direct" else int(lib.ss_compress_fft_size(self.n_samples,
+                                                                              self.m))
+        if method == "fft" and self.nfft == 0:
+            raise ValueError("pulse too long for the FFT path")
+        self.spectrum = None
+        if self.nfft:
+            self.spectrum = torch.empty((self.nfft, 2), dtype=torch.float64,
+                                        device=self.taps.device)
+            rc = lib.ss_pulse_spectrum(_ptr(self.taps), self.m, self.nfft,
+                                       _ptr(self.spectrum),
+                                       _stream_handle(stream, self.taps.device))
+            _lib.check(rc, "ss_pulse_spectrum")
+
+
+def range_compress(field, pulse, m=None, out=None, stream=None, method="auto"):
     """'Same' convolution of every row of a (..., n, 2) float64 device tensor
-    (complex interleaved) with a complex pulse of 2m+1 taps."""
+    (complex interleaved) with a complex pulse of 2m+1 taps (or a prepared
+    PulseSpectrum): FFT overlap-save when the pulse fits the LDS transform
+    (``method`` "auto"/"fft"), else direct summation ("direct").  Returns
+    (out, the PulseSpectrum used)."""
     n = field.shape[-2]
     rows = field.numel() // (2 * n)
-    p = np.asarray(pulse)
-    pt = torch.as_tensor(np.stack([p.real, p.imag], -1).astype(np.float64)
-                         if np.iscomplexobj(p) else
-                         np.stack([p, np.zeros_like(p)], -1).astype(np.float64),
-                         device=field.device).contiguous()
+    ps = pulse if isinstance(pulse, PulseSpectrum) else PulseSpectrum(
+        pulse, m, n, field.device, method=method, stream=stream)
+    if ps.n_samples != n:
+        raise ValueError(f"spectrum prepared for {ps.n_samples} samples, rows have {n}")
     src = field.contiguous()
     if out is None:
         out = torch.empty_like(src)
     lib = _lib.load()
+    use_fft = ps.nfft > 0 and method != "direct"
     for r0 in range(0, rows, 65535):
         nr = min(65535, rows - r0)
         off = r0 * n * 2
-        rc = lib.ss_range_compress(ctypes.c_void_p(src.data_ptr() + off * 8), nr, n,
-                                   _ptr(pt), int(m),
-                                   ctypes.c_void_p(out.data_ptr() + off * 8),
-                                   _stream_handle(stream, field.device))
-        _lib.check(rc, "ss_range_compress")
-    return out, pt
+        if use_fft:
+            rc = lib.ss_range_compress_fft(ctypes.c_void_p(src.data_ptr() + off * 8), nr, n,
+                                           _ptr(ps.spectrum), ps.m, ps.nfft,
+                                           ctypes.c_void_p(out.data_ptr() + off * 8),
+                                           _stream_handle(stream, field.device))
+            _lib.check(rc, "ss_range_compress_fft")
+        else:
+            rc = lib.ss_range_compress(ctypes.c_void_p(src.data_ptr() + off * 8), nr, n,
+                                       _ptr(ps.taps), ps.m,
+                                       ctypes.c_void_p(out.data_ptr() + off * 8),
+                                       _stream_handle(stream, field.device))
+            _lib.check(rc, "ss_range_compress")
+    return out, ps

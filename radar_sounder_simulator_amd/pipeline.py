@@ -51,6 +51,7 @@ class ResidentScene:
                                     wl.dt, wl.waveform.window, wl.waveform.construction,
                                     wl.waveform.tukey_alpha)
             self.pulse = (p, m)
+            self.pulse_prep = dv.PulseSpectrum(p, m, wl.n_samples, self.device)
         self._keep = []
         # float64-redo workspaces of the mixed refracted launches, one per layer
         self.workspace = [
@@ -95,11 +96,9 @@ class ResidentScene:
         if self.pulse is None:
             self.out.copy_(self.raw)
             return
-        p, m = self.pulse
         n = self.wl.n_samples
-        _, pt = dv.range_compress(self.raw.view(-1, n, 2), p, m,
-                                  out=self.out.view(-1, n, 2), stream=stream)
-        self._keep.append(pt)
+        dv.range_compress(self.raw.view(-1, n, 2), self.pulse_prep,
+                          out=self.out.view(-1, n, 2), stream=stream)
 
     def run(self, precision, stream=None):
         self._keep = []

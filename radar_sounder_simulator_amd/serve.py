@@ -119,9 +119,10 @@ class ActiveRequest:
         self.dropped = torch.empty((nl, T), dtype=torch.float64, device=device)
         self.pulse = None
         if wl.waveform is not None and wl.waveform.kind != "delta":
-            self.pulse = compressed_pulse(wl.waveform.bandwidth, wl.waveform.pulse_length,
-                                          wl.dt, wl.waveform.window,
-                                          wl.waveform.construction, wl.waveform.tukey_alpha)
+            p, m = compressed_pulse(wl.waveform.bandwidth, wl.waveform.pulse_length,
+                                    wl.dt, wl.waveform.window,
+                                    wl.waveform.construction, wl.waveform.tukey_alpha)
+            self.pulse = dv.PulseSpectrum(p, m, wl.n_samples, device)
         self.keep = []
         self.templates = []
         cst = dv.constants(device)
@@ -309,10 +310,8 @@ class Server:
         if a.pulse is None:
             a.out.copy_(a.raw)
         else:
-            p, m = a.pulse
             n = a.req.workload.n_samples
-            _, pt = dv.range_compress(a.raw.view(-1, n, 2), p, m, out=a.out.view(-1, n, 2))
-            a.keep.append(pt)
+            dv.range_compress(a.raw.view(-1, n, 2), a.pulse, out=a.out.view(-1, n, 2))
         if self.cuda:
             a.host_out = torch.empty(a.out.shape, dtype=torch.float64, pin_memory=True)
             a.host_drop = torch.empty(a.dropped.shape, dtype=torch.float64, pin_memory=True)

@@ -217,6 +217,18 @@ inline unsigned long long __ballot(int pred) {
 
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
 
+// sin(pi x), cos(pi x) with the exact values at multiples of 1/2 that the
+// device routine returns.
+inline void sincospi(double x, double* s, double* c) {
+  double r = std::fmod(x, 2.0);
+  if (r == 0.0) { *s = 0.0 * (x < 0 ? -1.0 : 1.0); *c = 1.0; return; }
+  if (std::fabs(r) == 1.0) { *s = 0.0; *c = -1.0; return; }
+  if (r == 0.5 || r == -1.5) { *s = 1.0; *c = 0.0; return; }
+  if (r == -0.5 || r == 1.5) { *s = -1.0; *c = 0.0; return; }
+  *s = std::sin(M_PI * x);
+  *c = std::cos(M_PI * x);
+}
+
 inline float atomicAdd(float* p, float v) { return sim::atomic_add_fp(p, v); }
 inline double atomicAdd(double* p, double v) { return sim::atomic_add_fp(p, v); }
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {

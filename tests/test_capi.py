@@ -71,6 +71,12 @@ def test_invalid_arguments_rejected_on_host():
     assert lib.ss_refracted_field(1, *wargs, dummy + 4, 1 << 20, None) == -1   # alignment
     assert b"aligned" in lib.ss_last_error()
     assert lib.ss_range_compress(None, 1, 8, None, 1, None, None) == -1
+    assert lib.ss_range_compress_fft(None, 1, 8, None, 1, 512, None, None) == -1
+    assert lib.ss_range_compress_fft(4096, 1, 8, 4096, 1, 500, 8192, None) == -3   # nfft
+    assert lib.ss_pulse_spectrum(4096, 300, 512, 4096, None) == -3                  # 601 taps
+    assert lib.ss_compress_fft_size(4096, 960) == 4096      # 1921 taps: two 4096-point blocks
+    assert lib.ss_compress_fft_size(4096, 5000) == 0        # too long: direct summation
+    assert lib.ss_compress_fft_size(300, 20) == 256
     assert lib.ss_batch_field(0, None, 0, None, None, 0, 8, 0, None) == 0   # empty batch
     assert lib.ss_batch_field(0, None, 1, None, None, 0, 3, 0, None) == -1  # bad tile
     j = _lib.Job()

@@ -180,16 +180,23 @@ def test_empty_inputs():
     assert out.shape == (2, 64) and not np.any(out) and not np.any(drop)
 
 
-def test_range_compress_vs_fft():
+@pytest.mark.parametrize("method", ["direct", "fft"])
+@pytest.mark.parametrize("n,B,Tp,dt", [(1024, 15e6, 20e-6, 1 / 48e6), (4096, 15e6, 20e-6, 1 / 48e6),
+                                       (20000, 20e6, 4e-6, 1 / 50e6), (150, 10e6, 2e-6, 1 / 25e6)])
+def test_range_compress_vs_fft(method, n, B, Tp, dt):
+    """Both compression kernels against the oracle's FFT convolution (the
+    reference's method), analytic and chirp pulses, 1e-12 relative."""
     from oracle import waveform as ow
     from radar_sounder_simulator_amd import device as dv
-    rng = np.random.default_rng(0)
-    x = rng.standard_normal((7, 1024)) + 1j * rng.standard_normal((7, 1024))
+    print(f"case compress/{method}/n{n}", flush=True)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((7, n)) + 1j * rng.standard_normal((7, n))
     for construction in ("analytic", "chirp"):
-        p, m = ow.compressed_pulse(15e6, 20e-6, 1 / 48e6, "hann", construction)
+        p, m = ow.compressed_pulse(B, Tp, dt, "hann", construction)
         ref = ow.convolve_fast_time(x, p, m)
         t = torch.as_tensor(np.stack([x.real, x.imag], -1), device="cuda")
-        y, _ = dv.range_compress(t, p, m)
+        y, ps = dv.range_compress(t, p, m, method=method)
+        assert (ps.nfft > 0) == (method == "fft")
         y = y.cpu().numpy()
         y = y[..., 0] + 1j * y[..., 1]
         assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-12

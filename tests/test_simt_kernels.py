@@ -155,3 +155,26 @@ def test_batched_launch_matches_single_launches(sim, tt):
     for o, r in zip(_run_batch(sim, wls, "mixed", tt, redo_all=True), ref64):
         assert _rel(o[1:], r[1:]) <= 1e-12          # refracted layers: float64 redo
         assert _rel(o[:1], r[:1]) <= 1e-4           # surface layer: mixed tier
+
+
+@pytest.mark.parametrize("n,m", [(300, 20), (1000, 130)])
+def test_fft_compression_kernel_vs_oracle(sim, n, m):
+    """Overlap-save FFT compression (several blocks per row) against the
+    oracle's FFT convolution, and against the direct-summation kernel."""
+    from oracle import waveform as ow
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    p = rng.standard_normal(2 * m + 1) + 1j * rng.standard_normal(2 * m + 1)
+    ref = ow.convolve_fast_time(x, p, m)
+    t = torch.as_tensor(np.stack([x.real, x.imag], -1))
+    with _lib.use(sim):
+        y, ps = dv.range_compress(t, p, m, method="fft")
+        d, _ = dv.range_compress(t, p, m, method="direct")
+    L = ps.nfft - 2 * m
+    assert ps.nfft >= 256 and (n + L - 1) // L >= 2        # several blocks per row
+    y = y.numpy()
+    y = y[..., 0] + 1j * y[..., 1]
+    d = d.numpy()
+    d = d[..., 0] + 1j * d[..., 1]
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+    assert np.linalg.norm(d - ref) <= 1e-12 * np.linalg.norm(ref)
