@@ -264,6 +264,36 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
 int64_t ss_batch_workspace_bytes(int precision, const ss_job* jobs_host, int n_jobs,
                                  int trace_tile);
 
+/* Device-resident batched step (capturable in a HIP graph).  A job table in
+ * device memory -- SS_BATCH_TABLE_HEADER bytes holding the int32 job count,
+ * then the ss_job records -- is read by the kernels themselves: outputs are
+ * zeroed, both job kinds run as persistent launches, the float64 redo and
+ * the fold-back follow, all with grids fixed at the call, so the launch
+ * sequence can be captured once and replayed after each refresh of the
+ * table (e.g. a copy from a pinned host table written by ss_batch_pack).
+ * max_items bounds the refracted work items a table may hold (the redo
+ * record's size); feature_class (from ss_batch_pack) must cover the jobs.
+ * No counterpart in the reference; serves the request loop. */
+#define SS_BATCH_TABLE_HEADER 256
+
+/* Bytes of a job table for up to max_jobs jobs (-1 if out of range). */
+int64_t ss_batch_table_bytes(int max_jobs);
+
+/* Validate n_jobs jobs and write the table (header + records) into
+ * table_host (at least ss_batch_table_bytes(n_jobs) bytes; pinned memory
+ * for captured replays).  items[0], items[1]: work items of the surface and
+ * refracted jobs at this trace_tile; feature_class: 0 (no roughness,
+ * isotropic), 1 (at most area-only D_Phi, tabulated pattern) or 2. */
+int ss_batch_pack(const ss_job* jobs_host, int n_jobs, int trace_tile, void* table_host,
+                  int64_t table_bytes, int64_t* items, int* feature_class);
+
+/* Workspace bytes of ss_batch_step for up to max_items refracted work items. */
+int64_t ss_batch_step_workspace_bytes(int precision, int64_t max_items, int trace_tile);
+
+int ss_batch_step(int precision, const void* table_device, void* workspace,
+                  int64_t workspace_bytes, int64_t max_items, int trace_tile, int n_blocks,
+                  int feature_class, void* stream);
+
 /* sizeof(ss_job) as compiled into the library (for sizing jobs_device and
  * checking a binding's struct mirror).  No counterpart in the reference. */
 int64_t ss_sizeof_job(void);

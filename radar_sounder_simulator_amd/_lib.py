@@ -37,6 +37,7 @@ IDESC = dict(NY=0, NX=1, COEF=2, MP=8, MN=11, SIGC=14, TABOFF=15, COUNT=16)
 BIN_INVALID = -2147483648
 MAX_CROSSED = 64
 MAX_JOINT = 32
+BATCH_TABLE_HEADER = 256
 MAX_TERMS = 300
 CONST_COUNT = 33 + MAX_TERMS + 1
 
@@ -44,7 +45,8 @@ EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_fill_constants", "ss_surface_field", "ss_refracted_field",
             "ss_refracted_workspace_bytes", "ss_batch_field", "ss_batch_workspace_bytes",
             "ss_sizeof_job", "ss_range_compress", "ss_compress_fft_size", "ss_pulse_spectrum",
-            "ss_range_compress_fft")
+            "ss_range_compress_fft", "ss_batch_table_bytes", "ss_batch_pack",
+            "ss_batch_step_workspace_bytes", "ss_batch_step")
 
 
 class SurfaceParams(ctypes.Structure):
@@ -147,6 +149,15 @@ def configure(lib):
     lib.ss_pulse_spectrum.restype = ctypes.c_int
     lib.ss_range_compress_fft.argtypes = [dp, i64, i64, dp, i64, i64, dp, vp]
     lib.ss_range_compress_fft.restype = ctypes.c_int
+    lib.ss_batch_table_bytes.argtypes = [i32]
+    lib.ss_batch_table_bytes.restype = ctypes.c_int64
+    lib.ss_batch_pack.argtypes = [ctypes.POINTER(Job), i32, i32, dp, i64,
+                                  ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]
+    lib.ss_batch_pack.restype = ctypes.c_int
+    lib.ss_batch_step_workspace_bytes.argtypes = [i32, i64, i32]
+    lib.ss_batch_step_workspace_bytes.restype = ctypes.c_int64
+    lib.ss_batch_step.argtypes = [i32, dp, dp, i64, i64, i32, i32, i32, vp]
+    lib.ss_batch_step.restype = ctypes.c_int
     return lib
 
 

@@ -71,6 +71,90 @@ class BatchRunner:
         return keep
 
 
+class StepGraph:
+    """One batched step captured in a HIP graph (``ss_batch_step``): a copy
+    of a pinned host job table into a device table, then the step's kernels,
+    all reading the table in device memory.  ``launch(jobs)`` repacks the
+    pinned table (``ss_batch_pack``) and replays the graph, so a step costs
+    one graph launch whatever the number of jobs.  A graph is specific to
+    (precision, trace tile, feature class) and holds at most ``max_jobs``
+    jobs and ``max_items`` refracted work items.  The pinned table is reused
+    by the next launch: wait for ``event`` (recorded after each launch)
+    before launching the same StepGraph again.  ``capture=False`` (or a host
+    device) enqueues the same launches without a graph."""
+
+    def __init__(self, device, precision, trace_tile, feature_class, max_jobs=1024,
+                 max_items=1 << 16, n_blocks=0, capture=True):
+        self.device = dv._dev(device)
+        cuda = self.device.type == "cuda"
+        self.precision = precision
+        self.prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
+        self.tt, self.cls = int(trace_tile), int(feature_class)
+        self.max_jobs, self.max_items, self.n_blocks = int(max_jobs), int(max_items), int(n_blocks)
+        lib = _lib.load()
+        tb = int(lib.ss_batch_table_bytes(self.max_jobs))
+        if tb < 0:
+            raise ValueError(f"max_jobs out of range: {max_jobs}")
+        self.table_host = torch.zeros(tb, dtype=torch.uint8, pin_memory=cuda)
+        self.table_dev = torch.zeros(tb, dtype=torch.uint8, device=self.device)
+        ws = int(lib.ss_batch_step_workspace_bytes(self.prec, self.max_items, self.tt))
+        self.workspace = torch.empty(max(ws, 256), dtype=torch.uint8, device=self.device)
+        self._arr = None
+        self.event = None
+        self.graph = None
+        if capture and cuda:
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self._enqueue()
+
+    def _enqueue(self):
+        self.table_dev.copy_(self.table_host, non_blocking=True)
+        rc = _lib.load().ss_batch_step(
+            self.prec, ctypes.c_void_p(self.table_dev.data_ptr()),
+            ctypes.c_void_p(self.workspace.data_ptr()), self.workspace.numel(),
+            self.max_items, self.tt, self.n_blocks, self.cls,
+            dv._stream_handle(None, self.device))
+        _lib.check(rc, "ss_batch_step")
+
+    def fits(self, jobs):
+        """(fits, items, class) of ``jobs`` for this graph."""
+        n = len(jobs)
+        arr = (_lib.Job * max(n, 1))(*jobs)
+        items = (ctypes.c_int64 * 2)()
+        cls = ctypes.c_int()
+        scratch = np.zeros(int(_lib.load().ss_batch_table_bytes(max(n, 1))), np.uint8)
+        rc = _lib.load().ss_batch_pack(arr, n, self.tt, scratch.ctypes.data, scratch.size,
+                                       items, ctypes.byref(cls))
+        _lib.check(rc, "ss_batch_pack")
+        ok = n <= self.max_jobs and items[1] <= self.max_items and cls.value <= self.cls
+        return ok, (int(items[0]), int(items[1])), cls.value
+
+    def launch(self, jobs):
+        n = len(jobs)
+        if n > self.max_jobs:
+            raise ValueError(f"{n} jobs > {self.max_jobs}")
+        if self.event is not None:
+            self.event.synchronize()          # the previous copy has read the table
+        arr = (_lib.Job * max(n, 1))(*jobs)
+        items = (ctypes.c_int64 * 2)()
+        cls = ctypes.c_int()
+        rc = _lib.load().ss_batch_pack(arr, n, self.tt,
+                                       ctypes.c_void_p(self.table_host.data_ptr()),
+                                       self.table_host.numel(), items, ctypes.byref(cls))
+        _lib.check(rc, "ss_batch_pack")
+        if items[1] > self.max_items or cls.value > self.cls:
+            raise ValueError(f"step exceeds the graph: {items[1]} refracted items "
+                             f"(max {self.max_items}), class {cls.value} (graph {self.cls})")
+        self._arr = arr
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._enqueue()
+        if self.device.type == "cuda":
+            self.event = torch.cuda.Event()
+            self.event.record()
+
+
 def draw_scene(rng, srng, scale=1.0, max_facets=None):
     """One scene of the uneven mix: (layers, deepest mean depth).  Facets
     log-uniform in [1e4, 2e6] x scale per interface, 1-4 interfaces: a

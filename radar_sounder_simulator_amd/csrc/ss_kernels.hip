@@ -114,6 +114,7 @@ constexpr long kWsHeader = 256;  // [0], [1] work counters of a batched launch (
 struct Redo {
   unsigned char* flags;
   unsigned long long* masks;
+  long cap;  // work items the flags/masks can hold
 };
 
 __host__ __device__ inline long redo_flag_bytes(long n_items) { return (n_items + 7) & ~7L; }
@@ -123,6 +124,7 @@ __host__ __device__ inline Redo redo_view(void* ws, long n_items) {
   char* b = static_cast<char*>(ws);
   r.flags = reinterpret_cast<unsigned char*>(b + kWsHeader);
   r.masks = reinterpret_cast<unsigned long long*>(b + kWsHeader + redo_flag_bytes(n_items));
+  r.cap = n_items;
   return r;
 }
 
@@ -257,9 +259,11 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     bool any = false;
     for (int i = 0; i < nT; ++i)
       any |= ((sh.redo[i][0] | sh.redo[i][1]) | (sh.redo[i][2] | sh.redo[i][3])) != 0ULL;
-    if (any && tid < TT * 4)
-      rd.masks[item * (TT * 4) + tid] = (tid >> 2) < nT ? sh.redo[tid >> 2][tid & 3] : 0ULL;
-    if (tid == 0) rd.flags[item] = any ? 1 : 0;
+    if (item < rd.cap) {  // the caller sizes the record for every item
+      if (any && tid < TT * 4)
+        rd.masks[item * (TT * 4) + tid] = (tid >> 2) < nT ? sh.redo[tid >> 2][tid & 3] : 0ULL;
+      if (tid == 0) rd.flags[item] = any ? 1 : 0;
+    }
   }
 }
 
@@ -548,31 +552,66 @@ __global__ void reset_workspace_kernel(unsigned long long* hdr) {
   if (threadIdx.x < 2) hdr[threadIdx.x] = 0ULL;
 }
 
-// fold_kernel over every float64 surface job of a batch: blockIdx.y = job.
-__global__ void batch_fold_kernel(const ss_job* __restrict__ jobs) {
-  const ss_job& jb = jobs[blockIdx.y];
-  if (jb.kind != 0) return;
-  const int n_seg = ((jb.surface.flags & SS_SPLIT_SIDES) ? 2 : 1) * jb.surface.n_samples;
-  const long n = jb.n_traces * (long)n_seg;
-  const double k = jb.surface.k;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    const long t = i / n_seg;
-    const double rref = jb.traces[(long)SS_TROW_RREF * jb.trace_stride + t];
-    double th;
-    {
-#pragma clang fp contract(off)
-      th = fmod(2.0 * k * rref, kTwoPi);
+// ------------------------------------------------------------ batched launch
+constexpr int kMaxJobs = 1024;
+
+// A job table in device memory: the job count either from the host (n_host)
+// or from device memory (n_dev, the table header of ss_batch_step).
+struct BatchArgs {
+  const ss_job* jobs;
+  const int* n_dev;
+  int n_host;
+};
+
+__device__ inline int batch_n(const BatchArgs& ba) {
+  int n = ba.n_dev ? uniform_i(*ba.n_dev) : ba.n_host;
+  return n < 0 ? 0 : (n > kMaxJobs ? kMaxJobs : n);
+}
+
+// Zeroes every job's field and dropped power (blockIdx.y strides jobs).
+__global__ void batch_zero_kernel(BatchArgs ba) {
+  const int n = batch_n(ba);
+  for (int j = blockIdx.y; j < n; j += gridDim.y) {
+    const ss_job& jb = ba.jobs[j];
+    const int ns = jb.kind == 0 ? jb.surface.n_samples : jb.refracted.n_samples;
+    const int fl = jb.kind == 0 ? jb.surface.flags : jb.refracted.flags;
+    const long nf = jb.n_traces * (long)(((fl & SS_SPLIT_SIDES) ? 2 : 1) * ns) * 2;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nf + jb.n_traces;
+         i += (long)gridDim.x * blockDim.x) {
+      if (i < nf)
+        jb.field[i] = 0.0;
+      else
+        jb.dropped[i - nf] = 0.0;
     }
-    const double c = cos(-th), s = sin(-th);
-    const double a = jb.field[2 * i], b = jb.field[2 * i + 1];
-    jb.field[2 * i] = a * c - b * s;
-    jb.field[2 * i + 1] = a * s + b * c;
   }
 }
 
-// ------------------------------------------------------------ batched launch
-constexpr int kMaxJobs = 1024;
+// fold_kernel over every float64 surface job of a batch (blockIdx.y strides
+// jobs).
+__global__ void batch_fold_kernel(BatchArgs ba) {
+  const int n_jobs = batch_n(ba);
+  for (int jj = blockIdx.y; jj < n_jobs; jj += gridDim.y) {
+    const ss_job& jb = ba.jobs[jj];
+    if (jb.kind != 0) continue;
+    const int n_seg = ((jb.surface.flags & SS_SPLIT_SIDES) ? 2 : 1) * jb.surface.n_samples;
+    const long n = jb.n_traces * (long)n_seg;
+    const double k = jb.surface.k;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+      const long t = i / n_seg;
+      const double rref = jb.traces[(long)SS_TROW_RREF * jb.trace_stride + t];
+      double th;
+      {
+#pragma clang fp contract(off)
+        th = fmod(2.0 * k * rref, kTwoPi);
+      }
+      const double c = cos(-th), s = sin(-th);
+      const double a = jb.field[2 * i], b = jb.field[2 * i + 1];
+      jb.field[2 * i] = a * c - b * s;
+      jb.field[2 * i + 1] = a * s + b * c;
+    }
+  }
+}
 
 // Work items of one job under trace tile TT.
 __host__ __device__ inline long job_work(const ss_job& j, int tt) {
@@ -642,15 +681,17 @@ __device__ inline KCommon job_common(const ss_job& jb) {
 // job lookup would otherwise push them just past it.
 template <int PREC, int TT, int RM, int PAT, int KIND,
           int WAVES = (PREC == SS_PREC_MIXED ? 3 : 1)>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void batch_kernel(const ss_job* __restrict__ jobs,
-                                                      int n_jobs, long total,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void batch_kernel(BatchArgs ba,
                                                       unsigned long long* counter, Redo rd) {
   using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
   __shared__ Shared<ACC, TT> sh;
   __shared__ long begins[kMaxJobs + 1];
   __shared__ long s_w;
   const int tid = threadIdx.x;
+  const ss_job* __restrict__ jobs = ba.jobs;
+  const int n_jobs = batch_n(ba);
   job_prefix<TT, KIND>(jobs, n_jobs, begins);
+  const long total = uniform_l(begins[n_jobs]);
   for (;;) {
     if (tid == 0) s_w = (long)atomicAdd(counter, 1ULL);
     __syncthreads();
@@ -723,13 +764,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
 // Float64 pass over the refracted work items a mixed batch_kernel launch
 // flagged (global work ids), masked to the recorded lanes.
 template <int TT>
-__global__ __launch_bounds__(kBlock) void batch_redo_kernel(const ss_job* __restrict__ jobs,
-                                                           int n_jobs, long total, Redo rd) {
+__global__ __launch_bounds__(kBlock) void batch_redo_kernel(BatchArgs ba, Redo rd) {
   __shared__ Shared<double, TT> sh;
   __shared__ long begins[kMaxJobs + 1];
   __shared__ unsigned long long sh_mask[kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const ss_job* __restrict__ jobs = ba.jobs;
+  const int n_jobs = batch_n(ba);
   job_prefix<TT, 1>(jobs, n_jobs, begins);
+  long total = uniform_l(begins[n_jobs]);
+  total = total < rd.cap ? total : rd.cap;
   for_flagged_items(rd.flags, total, sh_mask, [&](long gw) {
     const int j = uniform_i(find_job(begins, n_jobs, gw));
     const ss_job& jb = jobs[j];
@@ -1251,6 +1295,80 @@ static int64_t batch_items(const ss_job* jobs_host, int n_jobs, int trace_tile) 
   return total;
 }
 
+// Feature class of a job mix: 0 = no roughness, isotropic antenna; 1 = at
+// most area-only D_Phi and a tabulated pattern; 2 = anything.
+static int batch_class(const ss_job* jobs_host, int n_jobs) {
+  int cls = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    const ss_job& j = jobs_host[i];
+    const int fl = j.kind == 0 ? j.surface.flags : j.refracted.flags;
+    const int pk = j.kind == 0 ? j.surface.pattern : j.refracted.pattern;
+    const int rm = rough_mode(fl), pm = pattern_mode(pk);
+    const int c = (rm == 0 && pm == 0) ? 0 : ((rm <= 1 && pm != -1) ? 1 : 2);
+    cls = c > cls ? c : cls;
+  }
+  return cls;
+}
+
+static int device_cus() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return cus > 0 ? cus : 1;
+}
+
+static int default_blocks(int n_blocks) { return n_blocks > 0 ? n_blocks : 4 * device_cus(); }
+
+// The launches of one batched step on a job table already in device memory:
+// zero the outputs, reset the per-kind work counters, one persistent launch
+// per job kind (grid[kind] workgroups; 0 = skip), the float64 redo sweep
+// (mixed) and the fold-back of float64 surface jobs.  Nothing here depends
+// on the table's contents on the host side when grids are fixed, so the
+// sequence can be captured once and replayed with new tables.
+static void launch_batch(bool fp64, const BatchArgs& ba, void* workspace, const Redo& rd,
+                         int tt, const unsigned grid[2], unsigned rblocks, int cls,
+                         bool fold, hipStream_t st) {
+  unsigned long long* hdr = static_cast<unsigned long long*>(workspace);
+  const dim3 block(kBlock);
+  const unsigned cus = (unsigned)device_cus();
+  hipLaunchKernelGGL(batch_zero_kernel, dim3(cus, 8), dim3(256), 0, st, ba);
+  hipLaunchKernelGGL(reset_workspace_kernel, dim3(1), dim3(64), 0, st, hdr);
+  for (int kind = 0; kind < 2; ++kind) {
+    if (grid[kind] == 0) continue;
+    const dim3 g(grid[kind]);
+    unsigned long long* ctr = hdr + kind;
+#define SS_BATCH_K(P, T, R, A)                                                                \
+  if (kind == 0)                                                                              \
+    hipLaunchKernelGGL((batch_kernel<P, T, R, A, 0>), g, block, 0, st, ba, ctr, rd);          \
+  else                                                                                        \
+    hipLaunchKernelGGL((batch_kernel<P, T, R, A, 1>), g, block, 0, st, ba, ctr, rd)
+#define SS_BATCH_C(P, T)                                                                      \
+  if (cls == 0) { SS_BATCH_K(P, T, 0, 0); }                                                   \
+  else if (cls == 1) { SS_BATCH_K(P, T, 1, 5); }                                              \
+  else { SS_BATCH_K(P, T, -1, -1); }
+#define SS_LAUNCH_BATCH(T)                                                                    \
+  if (fp64) {                                                                                 \
+    SS_BATCH_C(SS_PREC_FP64, T)                                                               \
+  } else {                                                                                    \
+    SS_BATCH_C(SS_PREC_MIXED, T)                                                              \
+    if (kind == 1 && rblocks > 0)                                                             \
+      hipLaunchKernelGGL((batch_redo_kernel<T>), dim3(rblocks), block, 0, st, ba, rd);        \
+  }
+    if (tt == 4) {
+      SS_LAUNCH_BATCH(4);
+    } else if (tt == 8) {
+      SS_LAUNCH_BATCH(8);
+    } else {
+      SS_LAUNCH_BATCH(16);
+    }
+#undef SS_LAUNCH_BATCH
+#undef SS_BATCH_C
+#undef SS_BATCH_K
+  }
+  if (fp64 && fold)
+    hipLaunchKernelGGL(batch_fold_kernel, dim3(cus > 4 ? cus / 4 : 1, 16), dim3(256), 0, st, ba);
+}
+
 int64_t ss_batch_workspace_bytes(int precision, const ss_job* jobs_host, int n_jobs,
                                  int trace_tile) {
   if (n_jobs <= 0 || !jobs_host) return kWsHeader;
@@ -1278,95 +1396,107 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
     int r = validate_job(jobs_host[i], i);
     if (r) return r;
   }
-  const long total = batch_items(jobs_host, n_jobs, trace_tile);
   if (workspace_bytes < ss_batch_workspace_bytes(precision, jobs_host, n_jobs, trace_tile))
     return fail(SS_EINVAL, "workspace smaller than ss_batch_workspace_bytes()");
-  hipStream_t st = (hipStream_t)stream;
   long items[2] = {0, 0};
-  // feature class of the mix: 0 = no roughness, isotropic; 1 = at most
-  // area-only D_Phi and a tabulated pattern; 2 = anything
-  int cls = 0;
+  bool any_surface = false;
   for (int i = 0; i < n_jobs; ++i) {
-    const ss_job& j = jobs_host[i];
-    int n_seg_i = j.kind == 0
-                      ? ((j.surface.flags & SS_SPLIT_SIDES) ? 2 : 1) * j.surface.n_samples
-                      : ((j.refracted.flags & SS_SPLIT_SIDES) ? 2 : 1) * j.refracted.n_samples;
-    int r = fill(j.field, j.n_traces * (long)n_seg_i * 2, st);
-    if (r) return r;
-    r = fill(j.dropped, j.n_traces, st);
-    if (r) return r;
-    items[j.kind] += job_work(j, trace_tile);
-    const int fl = j.kind == 0 ? j.surface.flags : j.refracted.flags;
-    const int pk = j.kind == 0 ? j.surface.pattern : j.refracted.pattern;
-    const int rm = rough_mode(fl), pm = pattern_mode(pk);
-    const int c = (rm == 0 && pm == 0) ? 0 : ((rm <= 1 && pm != -1) ? 1 : 2);
-    cls = c > cls ? c : cls;
+    items[jobs_host[i].kind] += job_work(jobs_host[i], trace_tile);
+    any_surface |= jobs_host[i].kind == 0;
   }
+  hipStream_t st = (hipStream_t)stream;
   hipError_t e = hipMemcpyAsync(jobs_device, jobs_host, sizeof(ss_job) * (size_t)n_jobs,
                                 hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return fail(SS_EHIP, std::string("job upload: ") + hipGetErrorString(e));
-  unsigned long long* hdr = static_cast<unsigned long long*>(workspace);
-  hipLaunchKernelGGL(reset_workspace_kernel, dim3(1), dim3(64), 0, st, hdr);
-  if (total == 0) {
-    g_last_error.clear();
-    return SS_OK;
-  }
-  if (n_blocks <= 0) {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    n_blocks = 4 * cus;
-  }
-  const ss_job* jd = (const ss_job*)jobs_device;
-  dim3 block(kBlock);
   const bool fp64 = precision == SS_PREC_FP64;
+  const int nb = default_blocks(n_blocks);
+  unsigned grid[2];
+  for (int k = 0; k < 2; ++k) grid[k] = (unsigned)(items[k] < nb ? items[k] : nb);
   // the redo flags and masks index the refracted launch's work ids
   const Redo rd = fp64 ? Redo{} : redo_view(workspace, items[1]);
-  const long rchunks = (items[1] + kBlock - 1) / kBlock;
-  dim3 rgrid((unsigned)(rchunks < 1024 ? rchunks : 1024));
-  for (int kind = 0; kind < 2; ++kind) {
-    const long tot = items[kind];
-    if (tot == 0) continue;
-    dim3 grid((unsigned)((long)n_blocks < tot ? n_blocks : tot));
-    unsigned long long* ctr = hdr + kind;
-#define SS_BATCH_K(P, T, R, A)                                                                \
-  if (kind == 0)                                                                              \
-    hipLaunchKernelGGL((batch_kernel<P, T, R, A, 0>), grid, block, 0, st, jd, n_jobs, tot, ctr, \
-                       rd);                                                                   \
-  else                                                                                        \
-    hipLaunchKernelGGL((batch_kernel<P, T, R, A, 1>), grid, block, 0, st, jd, n_jobs, tot, ctr, \
-                       rd)
-#define SS_BATCH_C(P, T)                                                                      \
-  if (cls == 0) { SS_BATCH_K(P, T, 0, 0); }                                                   \
-  else if (cls == 1) { SS_BATCH_K(P, T, 1, 5); }                                              \
-  else { SS_BATCH_K(P, T, -1, -1); }
-#define SS_LAUNCH_BATCH(T)                                                                    \
-  if (fp64) {                                                                                 \
-    SS_BATCH_C(SS_PREC_FP64, T)                                                               \
-  } else {                                                                                    \
-    SS_BATCH_C(SS_PREC_MIXED, T)                                                              \
-    if (kind == 1)                                                                            \
-      hipLaunchKernelGGL((batch_redo_kernel<T>), rgrid, block, 0, st, jd, n_jobs, tot, rd);   \
-  }
-    if (trace_tile == 4) {
-      SS_LAUNCH_BATCH(4);
-    } else if (trace_tile == 8) {
-      SS_LAUNCH_BATCH(8);
-    } else {
-      SS_LAUNCH_BATCH(16);
-    }
-#undef SS_LAUNCH_BATCH
-#undef SS_BATCH_C
-#undef SS_BATCH_K
-  }
-  if (fp64) {  // reference-phase fold-back of the float64 surface jobs
-    bool any_surface = false;
-    for (int i = 0; i < n_jobs; ++i) any_surface |= jobs_host[i].kind == 0;
-    if (any_surface)
-      hipLaunchKernelGGL(batch_fold_kernel, dim3(64, (unsigned)n_jobs), dim3(256), 0, st, jd);
-  }
+  const long rchunks = fp64 ? 0 : (items[1] + kBlock - 1) / kBlock;
+  const BatchArgs ba{(const ss_job*)jobs_device, nullptr, n_jobs};
+  launch_batch(fp64, ba, workspace, rd, trace_tile, grid,
+               (unsigned)(rchunks < 1024 ? rchunks : 1024), batch_class(jobs_host, n_jobs),
+               any_surface, st);
   e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int64_t ss_batch_table_bytes(int max_jobs) {
+  if (max_jobs < 1 || max_jobs > kMaxJobs) return -1;
+  return SS_BATCH_TABLE_HEADER + (int64_t)sizeof(ss_job) * max_jobs;
+}
+
+int ss_batch_pack(const ss_job* jobs_host, int n_jobs, int trace_tile, void* table_host,
+                  int64_t table_bytes, int64_t* items, int* feature_class) {
+  if (n_jobs < 0 || n_jobs > kMaxJobs) return fail(SS_ERANGE, "n_jobs out of range [0, 1024]");
+  if (trace_tile != 4 && trace_tile != 8 && trace_tile != 16)
+    return fail(SS_EINVAL, "trace_tile must be 4, 8 or 16");
+  if (!table_host || (n_jobs > 0 && !jobs_host)) return fail(SS_EINVAL, "null pointer");
+  if (table_bytes < ss_batch_table_bytes(n_jobs > 0 ? n_jobs : 1))
+    return fail(SS_EINVAL, "table smaller than ss_batch_table_bytes(n_jobs)");
+  for (int i = 0; i < n_jobs; ++i) {
+    int r = validate_job(jobs_host[i], i);
+    if (r) return r;
+  }
+  char* t = static_cast<char*>(table_host);
+  std::memset(t, 0, SS_BATCH_TABLE_HEADER);
+  std::memcpy(t, &n_jobs, sizeof(int));
+  if (n_jobs > 0) std::memcpy(t + SS_BATCH_TABLE_HEADER, jobs_host, sizeof(ss_job) * n_jobs);
+  long refr = 0, all = 0;
+  for (int i = 0; i < n_jobs; ++i) {
+    const long w = job_work(jobs_host[i], trace_tile);
+    all += w;
+    if (jobs_host[i].kind == 1) refr += w;
+  }
+  if (items) {
+    items[0] = all - refr;
+    items[1] = refr;
+  }
+  if (feature_class) *feature_class = batch_class(jobs_host, n_jobs);
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int64_t ss_batch_step_workspace_bytes(int precision, int64_t max_items, int trace_tile) {
+  if (trace_tile != 4 && trace_tile != 8 && trace_tile != 16) return -1;
+  if (max_items < 0) return -1;
+  if (precision != SS_PREC_MIXED) return kWsHeader;
+  return redo_bytes(max_items, trace_tile);
+}
+
+int ss_batch_step(int precision, const void* table_device, void* workspace,
+                  int64_t workspace_bytes, int64_t max_items, int trace_tile, int n_blocks,
+                  int feature_class, void* stream) {
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED)
+    return fail(SS_EINVAL, "precision must be SS_PREC_FP64 or SS_PREC_MIXED");
+  if (trace_tile != 4 && trace_tile != 8 && trace_tile != 16)
+    return fail(SS_EINVAL, "trace_tile must be 4, 8 or 16");
+  if (feature_class < 0 || feature_class > 2)
+    return fail(SS_EINVAL, "feature_class must be 0, 1 or 2");
+  if (!table_device || !workspace) return fail(SS_EINVAL, "null pointer");
+  if ((reinterpret_cast<uintptr_t>(workspace) & 7) != 0 ||
+      (reinterpret_cast<uintptr_t>(table_device) & 7) != 0)
+    return fail(SS_EINVAL, "table and workspace must be 8-byte aligned");
+  if (max_items < 1) return fail(SS_EINVAL, "max_items must be >= 1");
+  if (workspace_bytes < ss_batch_step_workspace_bytes(precision, max_items, trace_tile))
+    return fail(SS_EINVAL, "workspace smaller than ss_batch_step_workspace_bytes()");
+  const bool fp64 = precision == SS_PREC_FP64;
+  const int nb = default_blocks(n_blocks);
+  const unsigned grid[2] = {(unsigned)nb, (unsigned)nb};
+  const Redo rd = fp64 ? Redo{} : redo_view(workspace, max_items);
+  const long rchunks = fp64 ? 0 : (max_items + kBlock - 1) / kBlock;
+  const char* t = static_cast<const char*>(table_device);
+  const BatchArgs ba{reinterpret_cast<const ss_job*>(t + SS_BATCH_TABLE_HEADER),
+                     reinterpret_cast<const int*>(t), 0};
+  launch_batch(fp64, ba, workspace, rd, trace_tile, grid,
+               (unsigned)(rchunks < 1024 ? rchunks : 1024), feature_class, true,
+               (hipStream_t)stream);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("batch step launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
 }

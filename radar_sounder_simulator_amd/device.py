@@ -317,6 +317,18 @@ class DeviceInterfaces:
     def nbytes(self):
         return (self.desc.numel() + self.tables.numel()) * 8
 
+    def prefix(self, n):
+        """View of the top ``n`` interfaces (a layer higher in the same
+        stack): the descriptor rows' table offsets stay valid, so the tables
+        are shared, not rebuilt.  Sequential (unpadded) stacks only."""
+        if self.k_pad or not 1 <= n <= self.n:
+            raise ValueError("prefix views need an unpadded stack and 1 <= n <= its depth")
+        v = object.__new__(DeviceInterfaces)
+        v.device, v.k_pad, v.n, v.has_sigma = self.device, 0, n, self.has_sigma
+        v.desc = self.desc[:n]
+        v.tables = self.tables
+        return v
+
 
 def refracted_params(target, ifaces, eps_leg, att_leg, *, mode, t0, dt, n_samples, c,
                      gamma=0.0, k0=None, split_sides=False, pattern=None, roughness=None,

@@ -8,9 +8,12 @@ launch (``ss_batch_field``) over trace chunks of all runnable requests,
 sized to a pair budget shared evenly between them, so a request that
 arrives while others are on the GPU joins the next step rather than
 waiting for them to finish, and its work overlaps theirs on the device.
-A request's last step is followed by its range compression and an
-asynchronous copy to pinned host memory; its latency runs from arrival to
-the end of that copy.  Memory is returned when the copy completes; scenes
+A request that brings a new scene is admitted with the scene's footprint
+reserved, and the scene is prepared and uploaded by a worker thread on its
+own stream while the device keeps running the other requests; the request
+joins the steps once the upload's event has completed.  A request's last
+step is followed by its range compression and an asynchronous copy to
+pinned host memory; its latency runs from arrival to the end of that copy.  Memory is returned when the copy completes; scenes
 no request uses are evicted least-recently-used first when an admission
 needs their room.
 
@@ -26,6 +29,7 @@ stream's host side; the kernels are the batched ones of ``batch.py``.
 import collections
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -50,6 +54,14 @@ class Request:
 
 
 @dataclass
+class _Pending:
+    req: Request
+    admitted: float
+    bytes: int
+    uploaded: bool
+
+
+@dataclass
 class Result:
     req_id: int
     scene_id: str
@@ -68,26 +80,41 @@ class Result:
 
 
 class DeviceScene:
-    """A scene's layers resident in HBM (facets + crossed-interface tables)."""
+    """A scene's layers resident in HBM (facets + crossed-interface tables).
+    When every layer's crossed interfaces are the top of one stack (the
+    layered scenes here), the stack's tables are built once and each layer
+    uses a prefix view of them."""
 
     def __init__(self, layers, device):
         self.layers = []
+        deepest = max((L.crossed for L in layers if L.crossed), key=len, default=None)
+        stack = None
+        if deepest and all(not L.crossed or all(a is b for a, b in zip(L.crossed, deepest))
+                           for L in layers):
+            stack = dv.DeviceInterfaces(deepest, device=device)
+        self.stack = stack
         for L in layers:
             ph = None if L.roughness is None else L.roughness[2]
             g = np.asarray(L.gamma, np.float64)
             fac = dv.DeviceFacets.from_facets(L.facets, gamma=g if g.ndim else None,
                                               phasors=ph, device=device)
-            ifc = dv.DeviceInterfaces(L.crossed, device=device) if L.crossed else None
+            ifc = None
+            if L.crossed:
+                ifc = (stack.prefix(len(L.crossed)) if stack is not None else
+                       dv.DeviceInterfaces(L.crossed, device=device))
             self.layers.append((L, fac, ifc))
         self.refs = 0
 
     def nbytes(self):
-        return sum(f.nbytes() + (i.nbytes() if i is not None else 0)
-                   for _, f, i in self.layers)
+        b = sum(f.nbytes() for _, f, _ in self.layers)
+        if self.stack is not None:
+            return b + self.stack.nbytes()
+        return b + sum(i.nbytes() for _, _, i in self.layers if i is not None)
 
     @staticmethod
     def estimate(layers):
-        """HBM bytes DeviceScene(layers) will take (facet SoA rows + tables)."""
+        """HBM bytes DeviceScene(layers) will take (facet SoA rows + tables;
+        an upper bound when the layers share one stack)."""
         b = 0
         for L in layers:
             b += _lib.FROW["COUNT"] * len(L.facets) * 8
@@ -215,7 +242,7 @@ class Server:
     batched launch's tile and grid (autotuned values from ``autotune``)."""
 
     def __init__(self, device=None, precision="mixed", budget_bytes=None, step_pairs=4e8,
-                 trace_tile=8, n_blocks=0, max_inflight=2, max_jobs=1024):
+                 trace_tile=8, n_blocks=0, max_inflight=2, max_jobs=1024, upload_workers=2):
         self.device = dv._dev(device)
         self.cuda = self.device.type == "cuda"
         self.precision = precision
@@ -233,6 +260,9 @@ class Server:
         self.runner = BatchRunner(device=self.device, max_jobs=max_jobs)
         self.scenes = collections.OrderedDict()    # scene_id -> DeviceScene (LRU order)
         self.scene_bytes = {}
+        self.uploads = {}                           # scene_id -> (future, reserved bytes)
+        self.pool = ThreadPoolExecutor(max_workers=upload_workers)
+        self.upload_stream = torch.cuda.Stream(self.device) if self.cuda else None
         self.used = 0
         self.peak = 0
         self.steps = 0
@@ -258,7 +288,8 @@ class Server:
 
     def _make_room(self, need):
         """Evict idle scenes (least recently used first) until ``need``
-        bytes fit; False if they cannot."""
+        bytes fit; False if they cannot.  (Scenes still uploading are not
+        in the store yet, so they are never evicted.)"""
         if self.used + need <= self.budget:
             return True
         for sid in list(self.scenes):
@@ -270,26 +301,65 @@ class Server:
                 self._account(-self.scene_bytes.pop(sid))
         return self.used + need <= self.budget
 
+    def _build_scene(self, layers):
+        """Worker thread: prepare and upload a scene on the upload stream."""
+        if not self.cuda:
+            return DeviceScene(layers, self.device), None
+        with torch.cuda.stream(self.upload_stream):
+            sc = DeviceScene(layers, self.device)
+            ev = torch.cuda.Event()
+            ev.record(self.upload_stream)
+        return sc, ev
+
     def _admit(self, req, now):
-        resident = req.scene_id in self.scenes
+        """Reserve the request's memory; returns a _Pending (its scene may
+        still be uploading) or None when it does not fit yet."""
+        sid = req.scene_id
+        resident = sid in self.scenes
+        uploading = sid in self.uploads
         need = ActiveRequest.estimate(req)
-        if not resident:
-            need += DeviceScene.estimate(req.workload.layers)
-        if need > self.budget:
-            raise MemoryError(f"request {req.req_id} needs {need} B > budget {self.budget} B")
+        scene_need = 0 if (resident or uploading) else DeviceScene.estimate(req.workload.layers)
+        if need + scene_need > self.budget:
+            raise MemoryError(f"request {req.req_id} needs {need + scene_need} B > budget "
+                              f"{self.budget} B")
         if resident:
-            self.scenes[req.scene_id].refs += 1   # pin before making room
-        ok = self._make_room(need)
+            self.scenes[sid].refs += 1   # pin before making room
+        ok = self._make_room(need + scene_need)
         if resident:
-            self.scenes[req.scene_id].refs -= 1
+            self.scenes[sid].refs -= 1
         if not ok:
             return None
-        sc = self.add_scene(req.scene_id, req.workload.layers)
-        sc.refs += 1
-        a = ActiveRequest(req, sc, self.device, self.precision, now)
-        a.uploaded = not resident
-        a.bytes = ActiveRequest.estimate(req)
-        self._account(a.bytes)
+        if resident:
+            self.scenes.move_to_end(sid)
+            self.scenes[sid].refs += 1
+        elif not uploading:
+            self._account(scene_need)
+            self.uploads[sid] = (self.pool.submit(self._build_scene, req.workload.layers),
+                                 scene_need)
+        self._account(need)
+        return _Pending(req, now, need, uploaded=not resident)
+
+    def _activate(self, p):
+        """A pending request whose scene is resident: build its buffers."""
+        sid = p.req.scene_id
+        if sid not in self.scenes:
+            fut, reserved = self.uploads[sid]
+            if not fut.done():
+                return None
+            sc, ev = fut.result()
+            del self.uploads[sid]
+            if ev is not None:
+                torch.cuda.current_stream(self.device).wait_event(ev)
+            self.scenes[sid] = sc
+            self.scene_bytes[sid] = sc.nbytes()
+            self._account(sc.nbytes() - reserved)
+            sc.refs += 1
+        elif p.uploaded:
+            self.scenes[sid].refs += 1   # a later request for a scene uploaded meanwhile
+        sc = self.scenes[sid]
+        a = ActiveRequest(p.req, sc, self.device, self.precision, p.admitted)
+        a.uploaded = p.uploaded
+        a.bytes = p.bytes
         return a
 
     # ----------------------------------------------------------------- steps
@@ -359,16 +429,22 @@ class Server:
         def now():
             return clock() - t0 if realtime else math.inf
 
-        while pending or waiting or active or finishing:
+        staged = []
+        while pending or waiting or staged or active or finishing:
             t = now()
             while pending and pending[0].arrival <= t:
                 waiting.append(pending.popleft())
             while waiting:
-                a = self._admit(waiting[0], clock() - t0)
-                if a is None:
+                p = self._admit(waiting[0], clock() - t0)
+                if p is None:
                     break
                 waiting.popleft()
-                active.append(a)
+                staged.append(p)
+            for p in list(staged):
+                a = self._activate(p)
+                if a is not None:
+                    staged.remove(p)
+                    active.append(a)
             runnable = [a for a in active if a.remaining() > 0]
             if runnable:
                 while self.cuda and len(steps_in_flight) >= self.max_inflight:
@@ -390,7 +466,7 @@ class Server:
                     self._release(a, clock() - t0, results)
                     progressed = True
             if not progressed:
-                if finishing and self.cuda:
+                if (finishing and self.cuda) or staged:
                     time.sleep(0.0002)
                 elif pending and realtime and not waiting:
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))

@@ -98,11 +98,12 @@ struct Block {
   Barrier bar;
   std::vector<Barrier> wave_bar;
   std::vector<uint64_t> slots;
-  explicit Block(int nt) : nthreads(nt), wave_bar((nt + 63) / 64), slots(nt) {
-    bar.init(nt);
+  explicit Block(int nt) : nthreads(nt), wave_bar((nt + 63) / 64), slots(nt) { reset(); }
+  void reset() {
+    bar.init(nthreads);
     for (size_t w = 0; w < wave_bar.size(); ++w) {
       int first = (int)w * 64;
-      wave_bar[w].init(std::min(64, nt - first));
+      wave_bar[w].init(std::min(64, nthreads - first));
     }
   }
 };
@@ -127,6 +128,7 @@ inline uint64_t exchange_raw(uint64_t mine, int src) {
 template <class F, class... A>
 void launch(F f, dim3 grid, dim3 block, A... args) {
   const int nt = (int)(block.x * block.y * block.z);
+  const long nblocks = (long)grid.x * grid.y;
   g_grid = grid;
   g_block = block;
   std::atomic<bool> done{false};
@@ -143,24 +145,29 @@ void launch(F f, dim3 grid, dim3 block, A... args) {
       std::this_thread::sleep_for(std::chrono::milliseconds(50));
     }
   });
-  for (unsigned by = 0; by < grid.y; ++by)
-    for (unsigned bx = 0; bx < grid.x; ++bx) {
-      Block blk(nt);
-      blk.bx = bx;
-      blk.by = by;
-      std::vector<std::thread> th;
-      th.reserve(nt);
-      for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-          ctx.tid = dim3((unsigned)t, 0, 0);
-          ctx.bid = dim3(bx, by, 0);
-          ctx.blk = &blk;
-          f(args...);
-          blk.bar.leave();
-          blk.wave_bar[t / 64].leave();
-        });
-      for (auto& x : th) x.join();
-    }
+  // One host thread per lane for the whole launch; the workgroups run one
+  // after another, separated by a launch-wide barrier (lane 0 re-arms the
+  // workgroup's barriers in between).
+  Block blk(nt);
+  Barrier step;
+  step.init(nt);
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (long b = 0; b < nblocks; ++b) {
+        ctx.tid = dim3((unsigned)t, 0, 0);
+        ctx.bid = dim3((unsigned)(b % grid.x), (unsigned)(b / grid.x), 0);
+        ctx.blk = &blk;
+        f(args...);
+        blk.bar.leave();
+        blk.wave_bar[t / 64].leave();
+        step.wait("end of workgroup");
+        if (t == 0) blk.reset();
+        step.wait("next workgroup");
+      }
+    });
+  for (auto& x : th) x.join();
   done = true;
   watchdog.join();
 }

@@ -82,3 +82,39 @@ def test_batch_redo_all_matches_fp64():
         for j in range(o.shape[0]):
             tol = 1e-12 if j > 0 else 1e-4   # surface layer stays on the mixed tier
             assert _rel(o[j], ro[j]) <= tol, (j, _rel(o[j], ro[j]))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_captured_step_replays_match_eager(precision):
+    """The HIP-graph-captured step (pinned table -> device table + the step's
+    kernels) replayed several times, with eager work and synchronisations in
+    between and a different job table each time, equals the eager batched
+    launch every time."""
+    print(f"case batch/graph/{precision}", flush=True)
+    scenes = _scenes()
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    ref = _batched(scenes, precision, 8)
+    jobs, keep = [], []
+    for rs in scenes:
+        j, k = rs.jobs(precision)
+        jobs += j
+        keep += k
+    sg = batch.StepGraph("cuda", precision, 8, feature_class=2, max_jobs=16,
+                         max_items=1 << 14)
+    assert sg.graph is not None and sg.fits(jobs)[0]
+    for rep in range(3):
+        subset = jobs if rep != 1 else jobs[:3]
+        for rs in scenes:
+            rs.raw.fill_(-1.0)                   # eager work between replays
+        torch.cuda.synchronize()
+        sg.launch(subset)
+        torch.cuda.synchronize()
+        done = 0
+        for rs, (r, _) in zip(scenes, ref):
+            for j in range(rs.raw.shape[0]):
+                got = rs.raw[j].cpu().numpy()
+                if done < len(subset):
+                    assert _rel(got, r[j]) <= tol, (rep, done)
+                else:
+                    assert np.all(got == -1.0)
+                done += 1

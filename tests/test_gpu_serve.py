@@ -30,7 +30,7 @@ def _serve(reqs, resident, precision, **kw):
 def test_results_same_alone_and_in_mix(precision):
     print(f"case serve/alone_vs_mix/{precision}", flush=True)
     resident, reqs = _stream(8, seed=2)
-    srv, mix = _serve(reqs, resident, precision, step_pairs=2e6, trace_tile=8)
+    srv, mix = _serve(reqs, resident, precision, step_pairs=2e4, trace_tile=8)
     assert sorted(mix) == [r.req_id for r in reqs]
     assert max(r.steps for r in mix.values()) > 1
     tol = 1e-12 if precision == "fp64" else 1e-5

@@ -178,3 +178,36 @@ def test_fft_compression_kernel_vs_oracle(sim, n, m):
     d = d[..., 0] + 1j * d[..., 1]
     assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
     assert np.linalg.norm(d - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_device_resident_step_matches_batched_launch(sim, precision):
+    """ss_batch_step reads the job count and table from device memory and
+    sizes nothing on the host: the same results as ss_batch_field, for a
+    repacked table of a different size too."""
+    wls = _scenes()
+    ref = _run_batch(sim, wls, precision, 8)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    with _lib.use(sim):
+        scenes = [ResidentScene(w, device="cpu") for w in wls]
+        jobs, keep = [], []
+        for rs in scenes:
+            j, k = rs.jobs(precision)
+            jobs += j
+            keep += k
+        sg = B.StepGraph("cpu", precision, 8, feature_class=1, max_jobs=8, max_items=4096,
+                         n_blocks=3, capture=False)
+        assert sg.fits(jobs)[0]
+        for subset in (jobs, jobs[:2]):          # second launch: fewer jobs
+            for rs in scenes:
+                rs.raw.fill_(7.0)
+            sg.launch(subset)
+            n_layers = [len(rs.layers) for rs in scenes]
+            done = 0
+            for rs, r, nl in zip(scenes, ref, n_layers):
+                for j in range(nl):
+                    if done < len(subset):
+                        assert _rel(rs.raw[j].numpy(), r[j]) <= tol
+                    else:
+                        assert np.all(rs.raw[j].numpy() == 7.0)   # not in this table
+                    done += 1
