@@ -163,7 +163,8 @@ def bench_serving(n_requests, lam, scale, dev, seed=7):
         res = srv.serve(reqs, realtime=True, keep_fields=False)
         torch.cuda.synchronize()
         st = S.latency_stats(res)
-        st.update({"steps": srv.steps, "peak_hbm_bytes": srv.peak, "budget_bytes": srv.budget})
+        st.update({"steps": srv.steps, "graph_steps": srv.graph_steps,
+                   "peak_hbm_bytes": srv.peak, "budget_bytes": srv.budget})
         out[prec] = st
         log(f"bench: serving {prec}: p50 {st['p50_s'] * 1e3:.1f} ms p99 "
             f"{st['p99_s'] * 1e3:.1f} ms sustained {st['sustained_pairs_per_s']:.3e} pairs/s")

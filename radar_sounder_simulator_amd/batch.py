@@ -71,6 +71,21 @@ class BatchRunner:
         return keep
 
 
+def pack_info(jobs, trace_tile):
+    """Validate ``jobs`` and return (surface items, refracted items, feature
+    class) at ``trace_tile`` (host only; ``ss_batch_pack`` into scratch)."""
+    n = len(jobs)
+    lib = _lib.load()
+    arr = (_lib.Job * max(n, 1))(*jobs)
+    items = (ctypes.c_int64 * 2)()
+    cls = ctypes.c_int()
+    scratch = np.zeros(int(lib.ss_batch_table_bytes(max(min(n, 1024), 1))), np.uint8)
+    rc = lib.ss_batch_pack(arr, n, int(trace_tile), scratch.ctypes.data, scratch.size, items,
+                           ctypes.byref(cls))
+    _lib.check(rc, "ss_batch_pack")
+    return int(items[0]), int(items[1]), cls.value
+
+
 class StepGraph:
     """One batched step captured in a HIP graph (``ss_batch_step``): a copy
     of a pinned host job table into a device table, then the step's kernels,
@@ -117,17 +132,10 @@ class StepGraph:
         _lib.check(rc, "ss_batch_step")
 
     def fits(self, jobs):
-        """(fits, items, class) of ``jobs`` for this graph."""
-        n = len(jobs)
-        arr = (_lib.Job * max(n, 1))(*jobs)
-        items = (ctypes.c_int64 * 2)()
-        cls = ctypes.c_int()
-        scratch = np.zeros(int(_lib.load().ss_batch_table_bytes(max(n, 1))), np.uint8)
-        rc = _lib.load().ss_batch_pack(arr, n, self.tt, scratch.ctypes.data, scratch.size,
-                                       items, ctypes.byref(cls))
-        _lib.check(rc, "ss_batch_pack")
-        ok = n <= self.max_jobs and items[1] <= self.max_items and cls.value <= self.cls
-        return ok, (int(items[0]), int(items[1])), cls.value
+        """(fits, (surface, refracted) items, class) of ``jobs`` here."""
+        si, ri, cls = pack_info(jobs, self.tt)
+        ok = len(jobs) <= self.max_jobs and ri <= self.max_items and cls <= self.cls
+        return ok, (si, ri), cls
 
     def launch(self, jobs):
         n = len(jobs)

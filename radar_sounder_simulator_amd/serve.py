@@ -37,7 +37,7 @@ import torch
 
 from . import _lib
 from . import device as dv
-from .batch import BatchRunner, draw_instrument, draw_scene
+from .batch import BatchRunner, StepGraph, draw_instrument, draw_scene, pack_info
 from .kernels import _reference_range
 from .waveform import compressed_pulse
 
@@ -242,7 +242,8 @@ class Server:
     batched launch's tile and grid (autotuned values from ``autotune``)."""
 
     def __init__(self, device=None, precision="mixed", budget_bytes=None, step_pairs=4e8,
-                 trace_tile=8, n_blocks=0, max_inflight=2, max_jobs=1024, upload_workers=2):
+                 trace_tile=8, n_blocks=0, max_inflight=2, max_jobs=1024, upload_workers=2,
+                 graphs=True):
         self.device = dv._dev(device)
         self.cuda = self.device.type == "cuda"
         self.precision = precision
@@ -263,6 +264,13 @@ class Server:
         self.uploads = {}                           # scene_id -> (future, reserved bytes)
         self.pool = ThreadPoolExecutor(max_workers=upload_workers)
         self.upload_stream = torch.cuda.Stream(self.device) if self.cuda else None
+        # captured steps: two StepGraphs per feature class, used alternately
+        # so a step's pinned table is never rewritten while in flight
+        self.graphs = graphs and self.cuda
+        self._graph_sets = {}
+        self._graph_turn = 0
+        self.graph_steps = 0
+        self.max_items = int(max(1 << 16, 4 * self.step_pairs / (256 * self.tt)))
         self.used = 0
         self.peak = 0
         self.steps = 0
@@ -370,9 +378,30 @@ class Server:
             if len(jobs) + len(a.layer_facets) > self.max_jobs:
                 break                       # the rest run in the next step
             jobs += a.take(share, self.tt)
-        self.runner.launch(jobs, self.precision, trace_tile=self.tt, n_blocks=self.n_blocks)
+        if self.graphs and self._graph_launch(jobs):
+            self.graph_steps += 1
+        else:
+            self.runner.launch(jobs, self.precision, trace_tile=self.tt, n_blocks=self.n_blocks)
         self.steps += 1
         return jobs
+
+    def _graph_launch(self, jobs):
+        """Replay a captured step for ``jobs``; False when they exceed the
+        graphs' capacity (the caller then launches eagerly)."""
+        _, refracted_items, cls = pack_info(jobs, self.tt)
+        if len(jobs) > self.max_jobs or refracted_items > self.max_items:
+            return False
+        pair = self._graph_sets.get(cls)
+        if pair is None:
+            pair = self._graph_sets.setdefault(cls, [self._capture(cls), self._capture(cls)])
+        sg = pair[self._graph_turn & 1]
+        self._graph_turn += 1
+        sg.launch(jobs)
+        return True
+
+    def _capture(self, cls):
+        return StepGraph(self.device, self.precision, self.tt, cls, max_jobs=self.max_jobs,
+                         max_items=self.max_items, n_blocks=self.n_blocks)
 
     def _finish_launch(self, a):
         """Queue range compression and the copy to pinned host memory after

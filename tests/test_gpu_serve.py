@@ -33,6 +33,7 @@ def test_results_same_alone_and_in_mix(precision):
     srv, mix = _serve(reqs, resident, precision, step_pairs=2e4, trace_tile=8)
     assert sorted(mix) == [r.req_id for r in reqs]
     assert max(r.steps for r in mix.values()) > 1
+    assert srv.graph_steps == srv.steps > 0          # every step a captured-graph replay
     tol = 1e-12 if precision == "fp64" else 1e-5
     for r in reqs:
         _, alone = _serve([r], resident, precision, step_pairs=1e12, trace_tile=8)
