@@ -155,7 +155,8 @@ typedef struct ss_refracted_params {
   int32_t acf;
   int32_t pattern;
   int32_t pat_n;
-  int32_t refraction;       /* 0 sequential chain, 1 joint solve          */
+  int32_t refraction;       /* 0 sequential chain, 1 joint solve, 2 joint
+                               solve over a flat stack (see below)       */
   int32_t k_pad;            /* joint: index-matched pads at the top       */
   int32_t joint_newton;     /* joint: damped Newton steps (reference 6)   */
   int32_t joint_backtrack;  /* joint: step halvings (reference 4)         */
@@ -214,7 +215,13 @@ int ss_surface_field(int precision, const ss_surface_params* params,
 
 /* Refracted-path field of one target interface under n_crossed interfaces,
  * sequential two-point chain or joint N-interface solve (refraction = 1,
- * stack incl. pads at most SS_MAX_JOINT deep).  Replaces
+ * stack incl. pads at most SS_MAX_JOINT deep).  refraction = 2 is the joint
+ * solve for a stack whose real interfaces below the surface (indices
+ * k_pad + 1 ..) are horizontal planes: every table cell of each has normal
+ * (0, 0, +nz) and the same z, which the caller guarantees (the kernel reads
+ * each plane's z from its first cell).  It gives the joint solve's
+ * stationary path with two unknowns instead of 2 * n_crossed, needs no
+ * per-interface storage and takes stacks up to SS_MAX_CROSSED deep.  Replaces
  * kernels/multilayer.py:594-770 (refracted_cluttergram) and _refracted_fn
  * (:333-591), refraction_joint.py:249-322 (joint_crossings).  iface_desc: (n_crossed, SS_IDESC_COUNT) float64 device;
  * tables: concatenated (sum ny*nx, 6) float64 [centre, normal] device;

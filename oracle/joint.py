@@ -207,12 +207,13 @@ def joint_consts(crossed, pad_to, z_platform):
     return tabs, coefs, shps, np.array(mps), np.array(mns), k
 
 
-def joint_path(p, fc, jc, n_leg, eps, att, k0, sig_c, n_newton, n_backtrack):
-    """Joint refracted path for one trace (``kernels/multilayer.py:408-457``);
-    returns the same tuple as ``multilayer.sequential_path``."""
+def anchor_planes(p, fc, jc, n_leg):
+    """Pass 1 of the joint path (``kernels/multilayer.py:420-433``): the
+    sequential chain against the mean planes (10 iterations), then the cell
+    lookup of every crossing.  Returns the local planes pt, nr (N, B, 3) and
+    the chain."""
     tabs, coefs, shps, mp, mn, k_pad = jc
     N = len(tabs)
-    p = np.asarray(p, np.float64)
     B = fc.shape[0]
     o1 = np.broadcast_to(mp[:, None, :], (N, B, 3))
     n1 = np.broadcast_to(mn[:, None, :], (N, B, 3))
@@ -230,8 +231,14 @@ def joint_path(p, fc, jc, n_leg, eps, att, k0, sig_c, n_newton, n_backtrack):
         pt[i] = g6[:, :3]
         v = g6[:, 3:]
         nr[i] = v / np.linalg.norm(v, axis=-1, keepdims=True)
-    r2 = joint_crossings(p, fc, pt, nr, n_leg, n_newton=n_newton,
-                         n_backtrack=n_backtrack)
+    return pt, nr, r1
+
+
+def path_sums(r2, k_pad, n_leg, eps, att, k0, sig_c):
+    """Path sums of a solved polyline (``kernels/multilayer.py:436-457``):
+    pads booked with the surface cosine; returns the tuple of
+    ``multilayer.sequential_path``."""
+    N = r2["x"].shape[0]
     s_up = r2["s"][:-1]
     c_inc = np.maximum(np.cos(r2["theta1"]), C_MIN)
     c_surf = c_inc[k_pad]
@@ -253,3 +260,13 @@ def joint_path(p, fc, jc, n_leg, eps, att, k0, sig_c, n_newton, n_backtrack):
     comp = lambda v: (v[:, 0], v[:, 1], v[:, 2])
     return (comp(x[-1]), r2["valid"], opl, loss, sum_par, sum_perp, tau2, c_surf, c_last,
             comp(x[0]))
+
+
+def joint_path(p, fc, jc, n_leg, eps, att, k0, sig_c, n_newton, n_backtrack):
+    """Joint refracted path for one trace (``kernels/multilayer.py:408-457``);
+    returns the same tuple as ``multilayer.sequential_path``."""
+    p = np.asarray(p, np.float64)
+    pt, nr, _ = anchor_planes(p, fc, jc, n_leg)
+    r2 = joint_crossings(p, fc, pt, nr, n_leg, n_newton=n_newton,
+                         n_backtrack=n_backtrack)
+    return path_sums(r2, jc[-1], n_leg, eps, att, k0, sig_c)

@@ -170,9 +170,12 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
                           crossed_sigma=None, diffuse=None, taper_s=None,
                           d_phi_area=False, refraction="sequential",
                           pad_to=None, joint_newton=6, joint_backtrack=4,
-                          return_bins=False):
+                          return_bins=False, flat_stack=False):
     """FP64 restatement of ``refracted_cluttergram`` (``:594-770``): out
-    (T, n[, 2]) complex128 (coherent) / float64 (incoherent), dropped (T,)."""
+    (T, n[, 2]) complex128 (coherent) / float64 (incoherent), dropped (T,).
+    ``flat_stack`` (not in the reference; joint only): solve the crossings
+    with ``flatstack.flat_path`` -- the oracle of the kernels' flat-stack
+    form, for stacks whose interfaces below the surface are horizontal."""
     coherent = mode == "coherent"
     if (roughness is not None or crossed_sigma is not None) and not coherent:
         raise ValueError("roughness requires coherent mode")
@@ -214,6 +217,7 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
     joint = refraction == "joint"
     if joint:
         from .joint import joint_consts, joint_path
+        from .flatstack import flat_path
         jc = joint_consts(crossed, pad_to or len(crossed), pos[:, 2].min())
         kp = int(jc[-1])
         n_leg = np.concatenate([np.full(kp, n_leg[0]), n_leg])
@@ -238,8 +242,8 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
     for t in range(nT):
         p = (pos[t, 0], pos[t, 1], pos[t, 2])
         if joint:
-            path = joint_path(p, fc, jc, n_leg, eps, att, k0, sig_c,
-                              joint_newton, joint_backtrack)
+            path = (flat_path if flat_stack else joint_path)(
+                p, fc, jc, n_leg, eps, att, k0, sig_c, joint_newton, joint_backtrack)
         else:
             path = sequential_path(p, q, consts, n_leg, eps, att, k0, sig_c)
         pt = None

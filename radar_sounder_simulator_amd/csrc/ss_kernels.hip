@@ -457,7 +457,8 @@ __global__ void fold_kernel(double* field, const double* traces, long ts, long n
 }
 
 // --------------------------------------------------------------- refracted
-// NMAX = 0: sequential chain only; NMAX > 0: joint solve for stacks <= NMAX.
+// NMAX = 0: sequential chain only; NMAX > 0: joint solve for stacks <= NMAX;
+// NMAX < 0: flat-stack joint solve.
 // The mixed instantiations leave unresolved pairs to redo_kernel (queue rd).
 template <int PREC, int NMAX, int RM, int PAT>
 __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc, Redo rd) {
@@ -1142,10 +1143,10 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   if (prm->n_crossed < 1 || prm->n_crossed > SS_MAX_CROSSED)
     return fail(SS_ERANGE, "n_crossed out of range [1, 64]");
   if (!iface_desc || !tables || !legs) return fail(SS_EINVAL, "null interface data");
-  if (prm->refraction != 0 && prm->refraction != 1)
-    return fail(SS_EINVAL, "refraction must be 0 (sequential) or 1 (joint)");
-  if (prm->refraction == 1) {
-    if (prm->n_crossed > SS_MAX_JOINT)
+  if (prm->refraction < 0 || prm->refraction > 2)
+    return fail(SS_EINVAL, "refraction must be 0 (sequential), 1 (joint) or 2 (flat-stack joint)");
+  if (prm->refraction != 0) {
+    if (prm->refraction == 1 && prm->n_crossed > SS_MAX_JOINT)
       return fail(SS_ERANGE, "joint stack deeper than SS_MAX_JOINT (32)");
     if (prm->k_pad < 0 || prm->k_pad >= prm->n_crossed)
       return fail(SS_EINVAL, "k_pad must be in [0, n_crossed)");
@@ -1210,7 +1211,9 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
 #define SS_REFR_PA(R) \
   if (pm == 0) { SS_BOTH(0, R, 0); } else if (pm == 5) { SS_BOTH(0, R, 5); } \
   else { SS_BOTH(0, R, -1); }
-  if (!rc.joint) {
+  if (prm->refraction == 2) {
+    SS_BOTH(-1, -1, -1);
+  } else if (!rc.joint) {
     if (rm == 0) {
       SS_REFR_PA(0);
     } else if (rm == 1) {
@@ -1312,8 +1315,9 @@ static int batch_class(const ss_job* jobs_host, int n_jobs) {
 
 static int device_cus() {
   int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess)
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
   return cus > 0 ? cus : 1;
 }
 

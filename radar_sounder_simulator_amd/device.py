@@ -250,11 +250,22 @@ def surface_field(facets, traces, *, k, gamma, t0, dt, n_samples, c,
     return out, dropped, table
 
 
+def is_horizontal(f):
+    """Facets ``f`` form one horizontal plane: every normal (0, 0, +nz) and
+    every centre at one z (exact comparisons)."""
+    return bool(np.all(f.normals[:, :2] == 0.0) and np.all(f.normals[:, 2] > 0.0)
+                and np.all(f.centers[:, 2] == f.centers[0, 2]))
+
+
 class DeviceInterfaces:
     """Crossed-interface lookup data for the refracted kernel: per interface
     an (ny*nx, 6) float64 [centre, normal] table in grid order, the affine
     (x, y, 1) -> (row, col) fit and the area-weighted mean plane
-    (``kernels/multilayer.py:269-283``, float64)."""
+    (``kernels/multilayer.py:269-283``, float64).  A horizontal plane is
+    stored as one cell: every lookup on it returns the same plane, so no
+    crossing depends on the cell (the float64 two-point solve reads a cell
+    only through n . (x - o), which for the exact +z normal is x_z - z, so
+    the sequential chain's result is bit-identical)."""
 
     def __init__(self, crossed, crossed_sigma=None, device=None, pad_to=None,
                  z_platform=None):
@@ -287,14 +298,22 @@ class DeviceInterfaces:
             desc[i, IDESC["TABOFF"]] = off
             tabs.append(np.array([[0.0, 0.0, z, 0.0, 0.0, 1.0]]))
             off += 1
+        horizontal = [is_horizontal(f) for f in crossed]
         for i0, f in enumerate(crossed):
             i = i0 + k
             ny, nx = f.grid_shape
-            tab = np.concatenate([f.centers.reshape(ny * nx, 3),
-                                  f.normals.reshape(ny * nx, 3)], axis=1)
-            a = np.column_stack([f.centers[:, 0], f.centers[:, 1],
-                                 np.ones(len(f.centers))])
-            coef, *_ = np.linalg.lstsq(a, f.cell.astype(np.float64), rcond=None)
+            if horizontal[i0]:
+                # every cell gives the same plane: one cell, fit (row, col) = 0
+                # (the crossing solvers read only z and the +z normal)
+                ny = nx = 1
+                tab = np.concatenate([f.centers[:1], f.normals[:1]], axis=1)
+                coef = np.zeros((3, 2))
+            else:
+                tab = np.concatenate([f.centers.reshape(ny * nx, 3),
+                                      f.normals.reshape(ny * nx, 3)], axis=1)
+                a = np.column_stack([f.centers[:, 0], f.centers[:, 1],
+                                     np.ones(len(f.centers))])
+                coef, *_ = np.linalg.lstsq(a, f.cell.astype(np.float64), rcond=None)
             w = (f.areas / f.areas.sum())[:, None]
             mp = (f.centers * w).sum(0)
             mn = (f.normals * w).sum(0)
@@ -309,6 +328,10 @@ class DeviceInterfaces:
             off += ny * nx
             tabs.append(tab)
         self.n = n
+        # per real interface: a horizontal plane (every cell normal (0, 0, +nz)
+        # and one z), the condition of the flat-stack joint solve
+        self.horizontal = horizontal
+        self.flat_below = all(self.horizontal[1:])
         self.desc = torch.as_tensor(desc, device=self.device)
         self.tables = torch.as_tensor(np.ascontiguousarray(np.concatenate(tabs)),
                                       device=self.device)
@@ -325,6 +348,8 @@ class DeviceInterfaces:
             raise ValueError("prefix views need an unpadded stack and 1 <= n <= its depth")
         v = object.__new__(DeviceInterfaces)
         v.device, v.k_pad, v.n, v.has_sigma = self.device, 0, n, self.has_sigma
+        v.horizontal = self.horizontal[:n]
+        v.flat_below = all(v.horizontal[1:])
         v.desc = self.desc[:n]
         v.tables = self.tables
         return v
@@ -333,11 +358,14 @@ class DeviceInterfaces:
 def refracted_params(target, ifaces, eps_leg, att_leg, *, mode, t0, dt, n_samples, c,
                      gamma=0.0, k0=None, split_sides=False, pattern=None, roughness=None,
                      taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
-                     joint_newton=6, joint_backtrack=4, extra_flags=0):
+                     joint_newton=6, joint_backtrack=4, extra_flags=0, flat_stack="auto"):
     """(RefractedParams, legs tensor, pattern table or None) for the
     refracted kernel over the (possibly padded) stack ``ifaces``;
     ``eps_leg``/``att_leg`` are the unpadded per-leg media.  ``extra_flags``
-    is ORed into the flags (e.g. SS_REDO_ALL in tests)."""
+    is ORed into the flags (e.g. SS_REDO_ALL in tests).  ``flat_stack``
+    (joint only): "auto" takes the flat-stack joint solve (refraction code
+    2) when every real interface below the surface is a horizontal plane,
+    False always the general one, True requires the flat one."""
     coherent = mode == "coherent"
     flags = _lib.SS_COHERENT if coherent else 0
     flags |= int(extra_flags)
@@ -354,7 +382,15 @@ def refracted_params(target, ifaces, eps_leg, att_leg, *, mode, t0, dt, n_sample
     p.n_exp = 0.0 if n_exp is None else float(n_exp)
     p.n_samples = int(n_samples)
     p.n_crossed = ifaces.n
-    p.refraction = 1 if refraction == "joint" else 0
+    if refraction not in ("sequential", "joint"):
+        raise ValueError(f"unknown refraction solver {refraction!r}")
+    if flat_stack not in ("auto", True, False):
+        raise ValueError("flat_stack must be 'auto', True or False")
+    if flat_stack is True and not ifaces.flat_below:
+        raise ValueError("flat_stack=True needs horizontal planes below the surface")
+    p.refraction = 0
+    if refraction == "joint":
+        p.refraction = 2 if (flat_stack is not False and ifaces.flat_below) else 1
     p.k_pad = ifaces.k_pad
     p.joint_newton, p.joint_backtrack = int(joint_newton), int(joint_backtrack)
     if roughness is not None:
@@ -385,8 +421,9 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
                     taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
                     joint_newton=6, joint_backtrack=4, out=None,
                     dropped=None, pair_bins=False, workspace=None, extra_flags=0,
-                    stream=None):
-    """Enqueue the refracted kernel (sequential chain or joint solve) and,
+                    stream=None, flat_stack="auto"):
+    """Enqueue the refracted kernel (sequential chain, joint solve or its
+    flat-stack form, see ``refracted_params``) and,
     for the mixed tier, its float64 redo launch.  ``workspace``: a uint8
     device tensor of at least ``refracted_workspace_bytes`` (allocated here
     when None).  Returns (out (T, n_seg, 2) float64, dropped (T,)
@@ -397,7 +434,7 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
         c=c, gamma=gamma, k0=k0, split_sides=split_sides, pattern=pattern,
         roughness=roughness, taper_s=taper_s, d_phi_area=d_phi_area, n_exp=n_exp,
         refraction=refraction, joint_newton=joint_newton, joint_backtrack=joint_backtrack,
-        extra_flags=extra_flags)
+        extra_flags=extra_flags, flat_stack=flat_stack)
     n_seg = (2 if split_sides else 1) * int(n_samples)
     T = traces.n
     if out is None:

@@ -109,12 +109,16 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
                           d_phi_area=False, block_size=None,
                           refraction="sequential", pad_to=None,
                           joint_newton=6, joint_backtrack=4, window_cull=True,
-                          precision="mixed", device=None, return_bins=False):
+                          precision="mixed", device=None, return_bins=False,
+                          flat_stack="auto"):
     """Binned refracted-path contributions of one target interface.
 
     Same conventions as the reference (module docstring); returns
     ``(out, dropped)`` with out complex (coherent) or real power
-    (incoherent), (T, n[, 2])."""
+    (incoherent), (T, n[, 2]).  ``flat_stack`` (joint only): "auto" runs
+    the joint solve in its flat-stack form when every crossed interface
+    below the surface is a horizontal plane (then stacks up to 64 deep are
+    accepted), False always in its general form."""
     coherent = mode == "coherent"
     if mode not in ("coherent", "incoherent"):
         raise ValueError(f"unknown mode {mode!r}")
@@ -143,7 +147,8 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
     if refraction not in ("sequential", "joint"):
         raise ValueError(f"unknown refraction solver {refraction!r}")
     joint = refraction == "joint"
-    if joint and (pad_to or len(crossed)) > 32:
+    flat = joint and flat_stack is not False and all(_dv.is_horizontal(f) for f in crossed[1:])
+    if joint and not flat and (pad_to or len(crossed)) > 32:
         raise ValueError("joint refraction supports stacks of at most 32 interfaces")
     if precision not in ("mixed", "fp64"):
         raise ValueError("precision must be 'mixed' or 'fp64'")
@@ -165,7 +170,7 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
         k0=k0, precision=precision, split_sides=split_sides, pattern=pattern,
         roughness=rough, taper_s=taper_s, d_phi_area=d_phi_area, n_exp=n_exp,
         refraction=refraction, joint_newton=joint_newton, joint_backtrack=joint_backtrack,
-        pair_bins=return_bins)
+        pair_bins=return_bins, flat_stack=flat_stack)
     torch.cuda.synchronize(tgt.device)
     raw = res[0].cpu().numpy()
     drop = res[1].cpu().numpy()

@@ -270,7 +270,10 @@ class Server:
         self._graph_sets = {}
         self._graph_turn = 0
         self.graph_steps = 0
-        self.max_items = int(max(1 << 16, 4 * self.step_pairs / (256 * self.tt)))
+        # captured-step capacity in (trace tile, 256-facet) work items: four
+        # times a step's share, at most 1 << 22 (a mixed-tier workspace of
+        # about 1 GiB per graph); a larger step runs as an eager launch
+        self.max_items = int(min(max(1 << 16, 4 * self.step_pairs / (256 * self.tt)), 1 << 22))
         self.used = 0
         self.peak = 0
         self.steps = 0

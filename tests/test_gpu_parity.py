@@ -100,7 +100,13 @@ def _bed_scene(seed=5):
 
 REFR_CASES = ["smooth", "incoherent", "rough_exp", "rough_gauss", "cross_sigma", "diffuse",
               "split", "tabulated", "gamma_facet", "two_crossings", "shadow", "joint2",
-        "joint_pad", "joint3_sigma"]
+        "joint_pad", "joint3_sigma", "flat_stack", "flat_stack_pad", "flat_stack_deep",
+        "flat_stack_general"]
+
+
+def _planes(zs, like):
+    n = like.grid_shape[0]
+    return [grid_facets(np.full((n + 1, n + 1), z), like.e1[0, 0]) for z in zs]
 
 
 @pytest.mark.parametrize("name", REFR_CASES)
@@ -143,16 +149,40 @@ def test_refracted_vs_oracle(name, precision):
                            mid.e1[0, 0])
         crossed, eps, att = [surf, mid, mid2], [1.0, 1.6, 2.0, 3.15], [0.0, 2.0, 5.0, 10.0]
         kw.update(refraction="joint", crossed_sigma=[0.1, 0.05, 0.0])
+    elif name in ("flat_stack", "flat_stack_pad", "flat_stack_general"):
+        # rough surface over horizontal planes: the flat-stack joint solve
+        # ("general": the same stack through the general joint kernel)
+        crossed = [surf] + _planes([-30.0, -80.0, -150.0], surf)
+        eps, att = [1.0, 1.6, 2.1, 2.6, 3.15], [0.0, 5.0, 5.0, 5.0, 10.0]
+        kw.update(refraction="joint", crossed_sigma=[0.2, 0.05, 0.05, 0.05],
+                  roughness=(0.5, 20.0, ph, 24, "exponential"), d_phi_area=True)
+        if name == "flat_stack_pad":
+            kw.update(pad_to=8)
+    elif name == "flat_stack_deep":
+        # 40 planes: deeper than the general joint kernel takes
+        zs = -5.0 * np.arange(1, 41)
+        crossed = [surf] + _planes(zs, surf)
+        eps = list(np.concatenate([[1.0], np.linspace(1.3, 3.1, 40), [3.15]]))
+        att = [0.0] + [5.0] * 40 + [10.0]
+        kw.update(refraction="joint")
     elif name == "shadow":
         n = 20
         h = np.where(np.arange(n + 1)[None, :] > 10, 20.0, -50.0) * np.ones((n + 1, 1))
         target = grid_facets(h, 25.0)
     print(f"case refracted/{name}/{precision}", flush=True)
+    # flat stacks: the oracle of the flat-stack form (pinned against the
+    # general joint oracle in test_oracle_flatstack.py)
+    flat = name.startswith("flat_stack") and name != "flat_stack_general"
     ref, dref, bref, vref = om.refracted_cluttergram(tr.positions, tr.u_ct, target, crossed,
-                                                     eps, att, return_bins=True, **kw)
+                                                     eps, att, return_bins=True,
+                                                     flat_stack=flat, **kw)
     bref = np.where(vref, bref, K.BIN_INVALID)
+    if name == "flat_stack_general":
+        kw["flat_stack"] = False
     out, drop, bins = K.refracted_cluttergram(tr.positions, tr.u_ct, target, crossed, eps,
                                               att, precision=precision, return_bins=True, **kw)
+    if name.startswith("flat_stack"):
+        assert vref.mean() > 0.9
     np.testing.assert_array_equal(bins, bref)
     if precision == "fp64":
         rel = np.linalg.norm(out - ref) / max(np.linalg.norm(ref), 1e-300)

@@ -72,3 +72,15 @@ def test_admission_respects_budget(sim):
                for r in res.values())
     with pytest.raises(MemoryError):
         _run(sim, reqs[:1], {}, "mixed", budget=min(need) // 2, step_pairs=5000)
+
+
+def test_captured_step_capacity_is_bounded(sim):
+    """A huge step share (one request per step) must not size the captured
+    steps' redo workspace from the share: it is capped, and larger steps run
+    as eager launches."""
+    with _lib.use(sim):
+        srv = S.Server(device="cpu", precision="mixed", budget_bytes=64 << 20,
+                       step_pairs=1e12, trace_tile=8, n_blocks=2)
+        assert srv.max_items == 1 << 22
+        ws = _lib.load().ss_batch_step_workspace_bytes(_lib.SS_PREC_MIXED, srv.max_items, 8)
+        assert ws <= 1100 << 20

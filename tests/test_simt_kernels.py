@@ -109,6 +109,133 @@ def test_surface_fp64_kernel_vs_oracle(sim):
     assert np.linalg.norm(o - ref) <= 1e-9 * np.linalg.norm(ref)
 
 
+def _flat_stack(seed=4, n=16):
+    """Rough surface over three horizontal planes and a rough bed."""
+    rng = np.random.default_rng(seed)
+    dx = 25.0
+    surf = grid_facets(3.0 * gaussian_surface(n + 1, dx, 100.0, rng), dx)
+    planes = [grid_facets(np.full((n + 1, n + 1), z), dx) for z in (-30.0, -80.0, -150.0)]
+    bed = grid_facets(-300.0 + 30.0 * gaussian_surface(n + 1, dx, 200.0, rng), dx)
+    tr = straight_track(5, 40.0, 3000.0)
+    return surf, planes, bed, tr
+
+
+@pytest.mark.parametrize("pad_to", [None, 8])
+@pytest.mark.parametrize("rough", [False, True])
+def test_flat_stack_joint_kernel_vs_oracle(sim, pad_to, rough):
+    """refraction code 2 (the joint solve specialised to horizontal planes
+    below the surface) against the oracle's general joint path, with and
+    without the no-op pads; the general joint kernel on the same stack."""
+    surf, planes, bed, tr = _flat_stack()
+    crossed = [surf] + planes
+    eps, att = [1.0, 1.6, 2.1, 2.6, 3.15], [0.0, 5.0, 5.0, 5.0, 10.0]
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=2 * np.pi * 60e6 / C)
+    ph = sig = None
+    if rough:
+        ph = speckle_phasors(len(bed), 5)
+        sig = [0.2, 0.05, 0.05, 0.05]
+        kw.update(roughness=(0.5, 20.0, 24, "exponential"), d_phi_area=True)
+    outs = {}
+    with _lib.use(sim):
+        fac = dv.DeviceFacets(bed.centers, bed.normals, bed.areas, bed.e1, bed.e2,
+                              phasors=ph, device="cpu")
+        ifc = dv.DeviceInterfaces(crossed, sig, device="cpu", pad_to=pad_to or len(crossed),
+                                  z_platform=tr.positions[:, 2].min())
+        assert ifc.flat_below and ifc.horizontal == [False, True, True, True]
+        trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
+        for flat in (True, False):
+            out, drop, bins, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision="fp64",
+                                                    pair_bins=True, refraction="joint",
+                                                    flat_stack=flat, **kw)
+            o = out.numpy()
+            outs[flat] = (o[..., 0] + 1j * o[..., 1], drop.numpy(),
+                          fac.unpermute_pairs(bins.numpy()))
+    ref_kw = dict(kw)
+    if rough:
+        ref_kw["roughness"] = (0.5, 20.0, ph, 24, "exponential")
+    ref, dref, bref, vref = om.refracted_cluttergram(
+        tr.positions, tr.u_ct, bed, crossed, eps, att, refraction="joint", pad_to=pad_to,
+        crossed_sigma=sig, return_bins=True, **ref_kw)
+    assert vref.mean() > 0.9
+    for flat in (True, False):
+        out, drop, bins = outs[flat]
+        np.testing.assert_array_equal(bins, np.where(vref, bref, _lib.BIN_INVALID))
+        assert np.linalg.norm(out - ref) <= 1e-9 * np.linalg.norm(ref)
+        np.testing.assert_allclose(drop, dref, rtol=1e-9, atol=1e-30)
+    # mixed tier (float64 flat solve, mixed radiometry) and its float64 redo
+    with _lib.use(sim):
+        for extra in (0, _lib.SS_REDO_ALL):
+            out, drop, bins, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision="mixed",
+                                                    pair_bins=True, refraction="joint",
+                                                    extra_flags=extra, **kw)
+            o = out.numpy()
+            o = o[..., 0] + 1j * o[..., 1]
+            np.testing.assert_array_equal(fac.unpermute_pairs(bins.numpy()), outs[True][2])
+            if extra:
+                assert np.linalg.norm(o - outs[True][0]) <= 1e-12 * np.linalg.norm(ref)
+            else:
+                pk = np.abs(ref).max(axis=-1, keepdims=True)
+                m = np.abs(ref) > pk * 10 ** (-30 / 20)
+                assert np.abs(20 * np.log10(np.abs(o[m]) / np.abs(ref[m]))).max() <= 0.05
+                assert np.abs(np.angle(o[m] / ref[m])).max() * 1e3 <= 2.0
+
+
+def test_deep_flat_stack_kernel_vs_flat_oracle(sim):
+    """41 interfaces (deeper than the general joint kernel takes): the
+    flat-stack kernel against the oracle of the flat-stack form."""
+    surf, _, bed, tr = _flat_stack()
+    n = surf.grid_shape[0]
+    crossed = [surf] + [grid_facets(np.full((n + 1, n + 1), z), 25.0)
+                        for z in -5.0 * np.arange(1, 41)]
+    eps = list(np.concatenate([[1.0], np.linspace(1.3, 3.1, 40), [3.15]]))
+    att = [0.0] + [5.0] * 40 + [10.0]
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=2 * np.pi * 60e6 / C)
+    with _lib.use(sim):
+        fac = dv.DeviceFacets(bed.centers, bed.normals, bed.areas, bed.e1, bed.e2, device="cpu")
+        ifc = dv.DeviceInterfaces(crossed, device="cpu", pad_to=len(crossed),
+                                  z_platform=tr.positions[:, 2].min())
+        trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
+        out, drop, bins, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision="fp64",
+                                                pair_bins=True, refraction="joint", **kw)
+    o = out.numpy()
+    o = o[..., 0] + 1j * o[..., 1]
+    ref, dref, bref, vref = om.refracted_cluttergram(tr.positions, tr.u_ct, bed, crossed, eps,
+                                                     att, return_bins=True, refraction="joint",
+                                                     flat_stack=True, **kw)
+    assert vref.mean() > 0.9
+    np.testing.assert_array_equal(fac.unpermute_pairs(bins.numpy()),
+                                  np.where(vref, bref, _lib.BIN_INVALID))
+    assert np.linalg.norm(o - ref) <= 1e-9 * np.linalg.norm(ref)
+    np.testing.assert_allclose(drop.numpy(), dref, rtol=1e-9, atol=1e-30)
+
+
+def test_sequential_chain_over_horizontal_planes_vs_oracle(sim):
+    """Horizontal planes are stored as one cell; the sequential chain
+    through them matches the oracle's (full-table) lookups bit for bit in
+    its bins."""
+    surf, planes, bed, tr = _flat_stack()
+    crossed = [surf] + planes
+    eps, att = [1.0, 1.6, 2.1, 2.6, 3.15], [0.0, 5.0, 5.0, 5.0, 10.0]
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=2 * np.pi * 60e6 / C)
+    with _lib.use(sim):
+        fac = dv.DeviceFacets(bed.centers, bed.normals, bed.areas, bed.e1, bed.e2, device="cpu")
+        ifc = dv.DeviceInterfaces(crossed, device="cpu")
+        assert ifc.tables.shape[0] == len(surf) + 3
+        trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
+        out, drop, bins, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision="fp64",
+                                                pair_bins=True, **kw)
+    o = out.numpy()
+    o = o[..., 0] + 1j * o[..., 1]
+    ref, dref, bref, vref = om.refracted_cluttergram(tr.positions, tr.u_ct, bed, crossed, eps,
+                                                     att, return_bins=True, **kw)
+    np.testing.assert_array_equal(fac.unpermute_pairs(bins.numpy()),
+                                  np.where(vref, bref, _lib.BIN_INVALID))
+    assert np.linalg.norm(o - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
 def _scenes():
     return [W.config2(n=9, n_traces=9, n_samples=4096, seed=3),
             W.config1(n=17, n_traces=5, n_samples=512, seed=8)]
