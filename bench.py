@@ -29,7 +29,10 @@ Two further sections ride on the same line:
                resident scenes with a new instrument and trajectory), served by
                the continuous-batching loop on the mixed tier and on the
                float64 tier: p50/p99 latency (arrival to result in pinned
-               host memory) and sustained pairs/s.
+               host memory) and sustained pairs/s;
+  ``multilayer`` the 25-interface firn/horizon/bed scene (workloads.config3)
+               over ``--ml-traces`` of its traces: both tiers, per-layer
+               kernel times and the mixed tier's error per layer.
 """
 
 import argparse
@@ -147,6 +150,62 @@ def bench_batched(scale, dev, seed=4):
     return out
 
 
+def bench_multilayer(n_traces, dev, steps=2):
+    """The multilayer scene (2048 x 1024 facets per interface, 25 target
+    interfaces: rough surface, 20 firn sublayers, 3 horizons, rough bed;
+    8192 samples, 150/30 MHz Tukey chirp) over ``n_traces`` of its 1000
+    traces: mixed and float64 passes, per-layer kernel times and the mixed
+    tier's error per layer against the float64 tier."""
+    import torch
+    from radar_sounder_simulator_amd import workloads
+    from radar_sounder_simulator_amd.pipeline import ResidentScene
+    t = time.time()
+    wl = workloads.config3(n_traces=n_traces)
+    rs = ResidentScene(wl, device=dev)
+    log(f"bench: multilayer scene built and uploaded in {time.time() - t:.1f} s")
+    pairs_l = rs.pairs_per_layer()
+    pairs = sum(pairs_l)
+    st = torch.cuda.current_stream()
+    names = [L.name for L in wl.layers]
+    out = {"workload": "multilayer, 2048x1024 facets at 30 m x 25 interfaces, "
+                       f"{n_traces} of 1000 traces at 20 km, 8192 samples, 150/30 MHz",
+           "traces": n_traces, "layers": len(names), "pairs_per_step": pairs}
+    fields = {}
+    for prec in ("mixed", "fp64"):
+        rs.run(prec)
+        torch.cuda.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 2)]
+              for _ in range(steps)]
+        for s in range(steps):
+            e = ev[s]
+            e[0].record(st)
+            for j in range(len(names)):
+                rs.run_layer(j, prec)
+                e[j + 1].record(st)
+            rs.compress()
+            e[-1].record(st)
+        torch.cuda.synchronize()
+        ms = np.mean([e[0].elapsed_time(e[-1]) for e in ev])
+        per = [float(np.mean([e[j].elapsed_time(e[j + 1]) for e in ev]))
+               for j in range(len(names))]
+        fields[prec] = rs.raw.cpu().numpy()
+        out[prec] = {"ms_per_step": ms, "pairs_per_s": pairs / (ms * 1e-3),
+                     "layer_ms": dict(zip(names, per))}
+        log(f"bench: multilayer {prec}: {ms:.1f} ms/step, {pairs / (ms * 1e-3):.3e} pairs/s")
+    errs = {}
+    for j, n in enumerate(names):
+        db, mrad, rel = errors_db_mrad(fields["mixed"][j:j + 1], fields["fp64"][j:j + 1])
+        errs[n] = {"max_db": db, "max_mrad": mrad, "rel_l2": rel}
+    out["mixed_vs_fp64_per_layer"] = errs
+    out["max_db"] = max(e["max_db"] for e in errs.values())
+    out["max_mrad"] = max(e["max_mrad"] for e in errs.values())
+    out["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    log(f"bench: multilayer mixed vs fp64 worst layer {out['max_db']:.2e} dB "
+        f"{out['max_mrad']:.3f} mrad")
+    del rs
+    return out
+
+
 def bench_serving(n_requests, lam, scale, dev, seed=7):
     """The request stream on the mixed and on the float64 tier."""
     import torch
@@ -191,6 +250,9 @@ def main():
     ap.add_argument("--lam", type=float, default=8.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--ml-traces", type=int, default=16,
+                    help="traces of the multilayer scene's 1000 in its section")
+    ap.add_argument("--no-multilayer", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the coherent facet kernel shards by "
@@ -306,6 +368,10 @@ def main():
     if not args.no_serve:
         log(f"bench: serving stream ({args.serve_requests} requests at {args.lam}/s)")
         line["serving"] = bench_serving(args.serve_requests, args.lam, args.serve_scale, dev)
+    if not args.no_multilayer:
+        torch.cuda.empty_cache()
+        log(f"bench: multilayer scene ({args.ml_traces} traces)")
+        line["multilayer"] = bench_multilayer(args.ml_traces, dev)
     print(json.dumps(line), flush=True)
 
 

@@ -257,6 +257,51 @@ def is_horizontal(f):
                 and np.all(f.centers[:, 2] == f.centers[0, 2]))
 
 
+def _iface_entry(f):
+    """Lookup data of one interface: (table (rows, 6), descriptor row with
+    TABOFF and SIGC unset, horizontal?)."""
+    desc = np.zeros(IDESC["COUNT"], np.float64)
+    ny, nx = f.grid_shape
+    horizontal = is_horizontal(f)
+    if horizontal:
+        # every cell gives the same plane: one cell, fit (row, col) = 0
+        ny = nx = 1
+        tab = np.concatenate([f.centers[:1], f.normals[:1]], axis=1)
+        coef = np.zeros((3, 2))
+    else:
+        tab = np.concatenate([f.centers.reshape(ny * nx, 3),
+                              f.normals.reshape(ny * nx, 3)], axis=1)
+        a = np.column_stack([f.centers[:, 0], f.centers[:, 1], np.ones(len(f.centers))])
+        coef, *_ = np.linalg.lstsq(a, f.cell.astype(np.float64), rcond=None)
+    w = (f.areas / f.areas.sum())[:, None]
+    mp = (f.centers * w).sum(0)
+    mn = (f.normals * w).sum(0)
+    mn = mn / np.linalg.norm(mn)
+    desc[IDESC["NY"]] = ny
+    desc[IDESC["NX"]] = nx
+    desc[IDESC["COEF"]:IDESC["COEF"] + 6] = coef.ravel()
+    desc[IDESC["MP"]:IDESC["MP"] + 3] = mp
+    desc[IDESC["MN"]:IDESC["MN"] + 3] = mn
+    return tab, desc, horizontal
+
+
+def _pad_planes(crossed, pad_to, z_platform):
+    """z of the index-matched no-op planes the joint solver pads a stack
+    with, spread through the air gap above the surface, top-down
+    (``kernels/multilayer.py:286-330``)."""
+    k = 0 if pad_to is None else int(pad_to) - len(crossed)
+    if k < 0:
+        raise ValueError(f"pad_to={pad_to} < {len(crossed)} crossed interfaces")
+    if not k:
+        return []
+    z_surf = float(crossed[0].centers[:, 2].max())
+    gap = float(z_platform) - z_surf
+    if gap <= 0:
+        raise ValueError("joint refraction padding needs the platform above the "
+                         f"surface (gap {gap:.1f} m)")
+    return list(z_surf + gap * (np.arange(k, 0, -1) / (k + 1.0)))
+
+
 class DeviceInterfaces:
     """Crossed-interface lookup data for the refracted kernel: per interface
     an (ny*nx, 6) float64 [centre, normal] table in grid order, the affine
@@ -265,77 +310,67 @@ class DeviceInterfaces:
     stored as one cell: every lookup on it returns the same plane, so no
     crossing depends on the cell (the float64 two-point solve reads a cell
     only through n . (x - o), which for the exact +z normal is x_z - z, so
-    the sequential chain's result is bit-identical)."""
+    the sequential chain's result is bit-identical).  ``stacks`` builds the
+    stacks of several target layers over one shared table."""
 
     def __init__(self, crossed, crossed_sigma=None, device=None, pad_to=None,
                  z_platform=None):
-        self.device = _dev(device)
-        n_real = len(crossed)
-        if n_real < 1 or n_real > _lib.MAX_CROSSED:
-            raise ValueError(f"need 1..{_lib.MAX_CROSSED} crossed interfaces, got {n_real}")
-        k = 0 if pad_to is None else int(pad_to) - n_real
-        if k < 0:
-            raise ValueError(f"pad_to={pad_to} < {n_real} crossed interfaces")
-        # joint solver: index-matched no-op planes spread through the air gap
-        # above the surface (kernels/multilayer.py:286-330), top-down first
-        pads = []
-        if k:
-            z_surf = float(crossed[0].centers[:, 2].max())
-            gap = float(z_platform) - z_surf
-            if gap <= 0:
-                raise ValueError("joint refraction padding needs the platform above the "
-                                 f"surface (gap {gap:.1f} m)")
-            pads = list(z_surf + gap * (np.arange(k, 0, -1) / (k + 1.0)))
-        self.k_pad = k
-        n = n_real + k
-        desc = np.zeros((n, IDESC["COUNT"]), np.float64)
-        tabs = []
+        built = DeviceInterfaces.stacks([crossed], [crossed_sigma], [pad_to], z_platform,
+                                        device)[0]
+        self.__dict__.update(built.__dict__)
+
+    @staticmethod
+    def stacks(stacks, sigmas=None, pad_tos=None, z_platform=None, device=None):
+        """One DeviceInterfaces per crossed stack, all over ONE device table:
+        each distinct Facets object (by identity) is tabulated and uploaded
+        once, the pads of every stack are rows of the same table."""
+        device = _dev(device)
+        sigmas = sigmas or [None] * len(stacks)
+        pad_tos = pad_tos or [None] * len(stacks)
+        entries, tabs, offs = {}, [], {}
         off = 0
-        for i, z in enumerate(pads):
-            desc[i, IDESC["NY"]] = desc[i, IDESC["NX"]] = 1
-            desc[i, IDESC["MP"] + 2] = z
-            desc[i, IDESC["MN"] + 2] = 1.0
-            desc[i, IDESC["TABOFF"]] = off
-            tabs.append(np.array([[0.0, 0.0, z, 0.0, 0.0, 1.0]]))
-            off += 1
-        horizontal = [is_horizontal(f) for f in crossed]
-        for i0, f in enumerate(crossed):
-            i = i0 + k
-            ny, nx = f.grid_shape
-            if horizontal[i0]:
-                # every cell gives the same plane: one cell, fit (row, col) = 0
-                # (the crossing solvers read only z and the +z normal)
-                ny = nx = 1
-                tab = np.concatenate([f.centers[:1], f.normals[:1]], axis=1)
-                coef = np.zeros((3, 2))
-            else:
-                tab = np.concatenate([f.centers.reshape(ny * nx, 3),
-                                      f.normals.reshape(ny * nx, 3)], axis=1)
-                a = np.column_stack([f.centers[:, 0], f.centers[:, 1],
-                                     np.ones(len(f.centers))])
-                coef, *_ = np.linalg.lstsq(a, f.cell.astype(np.float64), rcond=None)
-            w = (f.areas / f.areas.sum())[:, None]
-            mp = (f.centers * w).sum(0)
-            mn = (f.normals * w).sum(0)
-            mn = mn / np.linalg.norm(mn)
-            desc[i, IDESC["NY"]] = ny
-            desc[i, IDESC["NX"]] = nx
-            desc[i, IDESC["COEF"]:IDESC["COEF"] + 6] = coef.ravel()
-            desc[i, IDESC["MP"]:IDESC["MP"] + 3] = mp
-            desc[i, IDESC["MN"]:IDESC["MN"] + 3] = mn
-            desc[i, IDESC["SIGC"]] = 0.0 if crossed_sigma is None else crossed_sigma[i0]
-            desc[i, IDESC["TABOFF"]] = off
-            off += ny * nx
-            tabs.append(tab)
-        self.n = n
-        # per real interface: a horizontal plane (every cell normal (0, 0, +nz)
-        # and one z), the condition of the flat-stack joint solve
-        self.horizontal = horizontal
-        self.flat_below = all(self.horizontal[1:])
-        self.desc = torch.as_tensor(desc, device=self.device)
-        self.tables = torch.as_tensor(np.ascontiguousarray(np.concatenate(tabs)),
-                                      device=self.device)
-        self.has_sigma = crossed_sigma is not None
+        descs = []
+        for crossed, sig, pad_to in zip(stacks, sigmas, pad_tos):
+            n_real = len(crossed)
+            if n_real < 1 or n_real > _lib.MAX_CROSSED:
+                raise ValueError(f"need 1..{_lib.MAX_CROSSED} crossed interfaces, got {n_real}")
+            pads = _pad_planes(crossed, pad_to, z_platform)
+            k = len(pads)
+            desc = np.zeros((n_real + k, IDESC["COUNT"]), np.float64)
+            for i, z in enumerate(pads):
+                desc[i, IDESC["NY"]] = desc[i, IDESC["NX"]] = 1
+                desc[i, IDESC["MP"] + 2] = z
+                desc[i, IDESC["MN"] + 2] = 1.0
+                desc[i, IDESC["TABOFF"]] = off
+                tabs.append(np.array([[0.0, 0.0, z, 0.0, 0.0, 1.0]]))
+                off += 1
+            horizontal = []
+            for i0, f in enumerate(crossed):
+                key = id(f)
+                if key not in entries:
+                    entries[key] = _iface_entry(f)
+                    tabs.append(entries[key][0])
+                    offs[key] = off
+                    off += len(entries[key][0])
+                _, row, hz = entries[key]
+                desc[i0 + k] = row
+                desc[i0 + k, IDESC["SIGC"]] = 0.0 if sig is None else sig[i0]
+                desc[i0 + k, IDESC["TABOFF"]] = offs[key]
+                horizontal.append(hz)
+            descs.append((desc, k, horizontal, sig is not None))
+        tables = torch.as_tensor(np.ascontiguousarray(np.concatenate(tabs)), device=device)
+        out = []
+        for desc, k, horizontal, has_sigma in descs:
+            v = object.__new__(DeviceInterfaces)
+            v.device, v.k_pad, v.n, v.has_sigma = device, k, desc.shape[0], has_sigma
+            # per real interface: a horizontal plane (every cell normal
+            # (0, 0, +nz) and one z), the condition of the flat-stack joint solve
+            v.horizontal = horizontal
+            v.flat_below = all(horizontal[1:])
+            v.desc = torch.as_tensor(desc, device=device)
+            v.tables = tables
+            out.append(v)
+        return out
 
     def nbytes(self):
         return (self.desc.numel() + self.tables.numel()) * 8

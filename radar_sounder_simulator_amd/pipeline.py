@@ -28,13 +28,17 @@ class ResidentScene:
                                       _reference_range(tr.positions, surf.centers),
                                       pv, device=self.device)
         self.layers = []
+        # every layer's crossed stack over one shared lookup table
+        refr = [L for L in wl.layers if L.crossed]
+        stacks = iter(dv.DeviceInterfaces.stacks(
+            [L.crossed for L in refr], None, [L.pad_to for L in refr],
+            tr.positions[:, 2].min(), self.device) if refr else [])
         for L in wl.layers:
             ph = None if L.roughness is None else L.roughness[2]
             g = np.asarray(L.gamma, np.float64)
             fac = dv.DeviceFacets.from_facets(L.facets, gamma=g if g.ndim else None,
                                               phasors=ph, device=self.device)
-            ifc = (dv.DeviceInterfaces(L.crossed, device=self.device)
-                   if L.crossed else None)
+            ifc = next(stacks) if L.crossed else None
             rough = (None if L.roughness is None else
                      (L.roughness[0], L.roughness[1], L.roughness[3],
                       L.roughness[4] if len(L.roughness) > 4 else "gaussian"))
@@ -67,8 +71,14 @@ class ResidentScene:
     def nbytes(self):
         b = self.traces.buf.numel() * 8 + self.raw.numel() * 16 + self.dropped.numel() * 8
         b += sum(w.numel() for w in self.workspace if w is not None)
+        tables = set()
         for _, fac, ifc, _ in self.layers:
-            b += fac.nbytes() + (ifc.nbytes() if ifc is not None else 0)
+            b += fac.nbytes()
+            if ifc is not None:
+                b += ifc.desc.numel() * 8
+                if ifc.tables.data_ptr() not in tables:
+                    tables.add(ifc.tables.data_ptr())
+                    b += ifc.tables.numel() * 8
         return b
 
     def run_layer(self, j, precision, stream=None):
@@ -89,7 +99,7 @@ class ResidentScene:
                 precision=precision, split_sides=wl.split_sides, pattern=wl.pattern,
                 roughness=rough, taper_s=wl.taper_s, d_phi_area=wl.d_phi_area,
                 out=self.raw[j], dropped=self.dropped[j], workspace=self.workspace[j],
-                stream=stream)
+                refraction=L.refraction, stream=stream)
         self._keep.append(res[-1])
 
     def compress(self, stream=None):
@@ -118,6 +128,8 @@ class ResidentScene:
         jobs, keep = [], []
         cst = dv.constants(self.device)
         for j, (L, fac, ifc, rough) in enumerate(self.layers):
+            if L.refraction != "sequential":
+                raise ValueError(f"layer {L.name}: batched jobs take the sequential chain only")
             jb = _lib.Job()
             jb.facet_stride, jb.n_facets = fac.stride, fac.n
             jb.trace_stride, jb.n_traces = self.traces.stride, self.traces.n

@@ -16,17 +16,26 @@ the configuration text leaves open are recorded here:
 * trace spacing 25 m for the 256-trace surface+bed scene;
 * exponential sub-facet roughness implies the grazing fix (taper
   s_eff = 0.05 + area-only D_Phi), as the reference's config validator
-  requires (``config.py:360-365``).
+  requires (``config.py:360-365``);
+* multilayer scene (2048 x 1024 at 30 m): a gently rough surface (sigma
+  1 m, l 300 m); the 20 firn sublayers are 5 m thick over the top 100 m,
+  each with n = 1 + 0.845 rho at its top (rho(z) = 917 - 567 exp(-z/30 m),
+  kg/m^3), ice below (rho 917); every interface is a target, the firn ones
+  with their normal-incidence Fresnel coefficient; bed at mean depth
+  3000 m (sigma 400 m stays clear of the 1500 m horizon), -10 dB;
+  10 dB/km one-way in firn and ice; 1000 traces at 30 m; 60 MHz
+  sampling; deeper layers (j > 1) through the joint solve padded to
+  power-of-two stacks as the reference's simulate() does.
 """
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
 from . import antenna as _ant
 from .physics import C, fresnel_normal
 from .roughness import n_terms_for, speckle_phasors
-from .scene import gaussian_surface, grid_facets, straight_track
+from .scene import eps_kovacs, gaussian_surface, grid_facets, straight_track
 from .waveform import Waveform
 
 
@@ -41,6 +50,8 @@ class Layer:
     crossed: list = field(default_factory=list)
     eps_leg: list = None
     att_leg: list = None
+    refraction: str = "sequential"  # or "joint" (padded to pad_to planes)
+    pad_to: int = None
 
 
 @dataclass
@@ -122,3 +133,78 @@ def config2(n=512, n_traces=256, seed=2, n_samples=4096, trace_spacing=25.0):
         waveform=Waveform(kind="chirp", bandwidth=15e6, pulse_length=20e-6),
         meta=dict(f0=f0, facets=f"{n}x{n}@{dx}m x2", seed=seed,
                   n_terms=(nt_s, nt_b)))
+
+
+def _pad_bucket(j, n_max):
+    """Power-of-two joint padding bucket of a stack of j interfaces, at
+    most n_max = the scene's interface count - 1 (``simulate.py:253-260``)."""
+    b = 1
+    while b < j:
+        b *= 2
+    return min(b, n_max)
+
+
+def config3(nx=2048, ny=1024, n_traces=1000, seed=3, n_samples=8192, n_firn=20,
+            firn_depth=100.0, trace_spacing=30.0, horizons=(500.0, 1000.0, 1500.0)):
+    """Multilayer scene: 2048 x 1024 facets at 30 m; firn rho(z) = 917 -
+    567 exp(-z/30 m) in 20 horizontal sublayers with n = 1 + 0.845 rho; three
+    horizontal horizons at 500/1000/1500 m (-60 dB); rough bed (sigma
+    400 m, l 3 km); 1000 traces at 20 km, cos^2 pattern, 150 MHz centre,
+    30 MHz chirp with a Tukey(0.2) pedestal, 8192 samples.  Layers: surface,
+    the n_firn sublayer bases, the horizons, the bed (all targets); a layer
+    j >= 2 takes the joint solve over its crossed stack."""
+    rng_s = np.random.default_rng([seed, 0])
+    rng_b = np.random.default_rng([seed, 1])
+    dx = 30.0
+    hs = 1.0 * gaussian_surface(ny + 1, dx, 300.0, rng_s, m=nx + 1)
+    hb = -3000.0 + 400.0 * gaussian_surface(ny + 1, dx, 3000.0, rng_b, m=nx + 1)
+    surf = grid_facets(hs, dx)
+    bed = grid_facets(hb, dx)
+    flat0 = grid_facets(np.zeros((ny + 1, nx + 1)), dx)
+
+    def plane(z):
+        c = flat0.centers.copy()
+        c[:, 2] = z
+        return replace(flat0, centers=c)
+
+    tops = np.arange(n_firn) * (firn_depth / n_firn)
+    rho = 917.0 - 567.0 * np.exp(-tops / 30.0)
+    eps = [1.0] + list(eps_kovacs(rho)) + [float(eps_kovacs(917.0))]
+    zs = [-(k + 1) * firn_depth / n_firn for k in range(n_firn)] + [-h for h in horizons]
+    alt = 20000.0
+    track = straight_track(n_traces, trace_spacing, alt)
+    f0 = 150e6
+    k0 = 2.0 * np.pi * f0 / C
+    ant = _ant.cos_power_table(2.0)
+    pattern = _ant.pattern_args(ant, track.u_at, track.u_ct)
+    layers = [Layer("surface", surf, fresnel_normal(1.0, eps[1]))]
+    crossed = [surf]
+    eps_leg, att_leg = [1.0, eps[1]], [0.0, 10.0]
+    n_media = len(eps) - 1                     # media below the air
+    n_max = len(zs) + 1                        # interfaces - 1
+    for i, z in enumerate(zs):
+        e_above = eps[min(i + 1, n_media)]
+        e_below = eps[min(i + 2, n_media)]
+        if i < n_firn:
+            name, gam = f"firn{i + 1}", fresnel_normal(e_above, e_below)
+        else:
+            name, gam = f"horizon{int(horizons[i - n_firn])}", -10.0 ** (-60.0 / 20.0)
+        j = len(layers)
+        layers.append(Layer(name, plane(z), gam, crossed=list(crossed), eps_leg=list(eps_leg),
+                            att_leg=list(att_leg),
+                            refraction="joint" if j > 1 else "sequential",
+                            pad_to=_pad_bucket(len(crossed), n_max) if j > 1 else None))
+        crossed.append(layers[-1].facets)
+        eps_leg.append(e_below)
+        att_leg.append(10.0)
+    layers.append(Layer("bed", bed, -10.0 ** (-10.0 / 20.0), crossed=list(crossed),
+                        eps_leg=list(eps_leg), att_leg=list(att_leg), refraction="joint",
+                        pad_to=_pad_bucket(len(crossed), n_max)))
+    dt = 1.0 / 60e6
+    return Workload(
+        name="cfg3_multilayer", track=track, layers=layers, k0=k0,
+        t0=2.0 * (alt - 100.0) / C, dt=dt, n_samples=n_samples, c=C, pattern=pattern,
+        waveform=Waveform(kind="chirp", bandwidth=30e6, pulse_length=20e-6, window="tukey",
+                          tukey_alpha=0.2),
+        meta=dict(f0=f0, facets=f"{nx}x{ny}@{dx}m x{len(layers)}", seed=seed,
+                  n_layers=len(layers)))

@@ -306,3 +306,35 @@ def test_workspace_reuse_second_smaller_call():
     b, c = b.cpu().numpy(), c.cpu().numpy()
     assert np.linalg.norm(b - c) <= 1e-12 * np.linalg.norm(c)
     assert np.linalg.norm(a.cpu().numpy()[:2] - c) <= 1e-12 * np.linalg.norm(c)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_multilayer_scene_vs_oracle(precision):
+    """The multilayer workload (rough surface, 20 horizontal firn sublayers,
+    three horizons, rough bed) through the resident pipeline on the GPU:
+    every refracted layer against the oracle (flat-stack joint form for
+    j >= 2, padded as the reference's simulate() pads)."""
+    from radar_sounder_simulator_amd.pipeline import ResidentScene
+    print(f"case multilayer/{precision}", flush=True)
+    wl = W.config3(nx=48, ny=24, n_traces=3)
+    rs = ResidentScene(wl, device="cuda")
+    for j in range(len(rs.layers)):
+        rs.run_layer(j, precision)
+    torch.cuda.synchronize()
+    raw = rs.raw.cpu().numpy()
+    tr = wl.track
+    for j, L in enumerate(wl.layers[1:], start=1):
+        f = L.facets
+        ref, dref = om.refracted_cluttergram(
+            tr.positions, tr.u_ct, f, L.crossed, L.eps_leg, L.att_leg, mode="coherent",
+            t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c, gamma=L.gamma, k0=wl.k0,
+            pattern=wl.pattern, refraction=L.refraction, pad_to=L.pad_to,
+            flat_stack=L.refraction == "joint")
+        o = raw[j][..., 0] + 1j * raw[j][..., 1]
+        assert np.linalg.norm(ref) > 0, L.name
+        if precision == "fp64":
+            rel = np.linalg.norm(o - ref) / np.linalg.norm(ref)
+            assert rel <= REL_FP64, (L.name, rel)
+        else:
+            db, mrad = _err(o, ref)
+            assert db <= MAX_DB and mrad <= MAX_MRAD, (L.name, db, mrad)

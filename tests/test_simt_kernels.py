@@ -211,6 +211,32 @@ def test_deep_flat_stack_kernel_vs_flat_oracle(sim):
     np.testing.assert_allclose(drop.numpy(), dref, rtol=1e-9, atol=1e-30)
 
 
+def test_multilayer_scene_layers_vs_oracle(sim):
+    """The multilayer workload (rough surface, 20 firn sublayers, three
+    horizons, rough bed; every interface a target) through the resident
+    pipeline: each refracted layer against the oracle (sequential for the
+    first sublayer, the flat-stack joint form below it, padded as the
+    reference's simulate() pads), float64."""
+    wl = W.config3(nx=16, ny=8, n_traces=2)
+    with _lib.use(sim):
+        rs = ResidentScene(wl, device="cpu")
+        for j in range(len(rs.layers)):
+            rs.run_layer(j, "fp64")
+        raw = rs.raw.numpy().copy()
+    tr = wl.track
+    assert len(wl.layers) == 25 and wl.layers[-1].pad_to == 24
+    for j, L in enumerate(wl.layers[1:], start=1):
+        f = L.facets
+        ref, _ = om.refracted_cluttergram(
+            tr.positions, tr.u_ct, f, L.crossed, L.eps_leg, L.att_leg, mode="coherent",
+            t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c, gamma=L.gamma, k0=wl.k0,
+            pattern=wl.pattern, refraction=L.refraction, pad_to=L.pad_to,
+            flat_stack=L.refraction == "joint")
+        o = raw[j][..., 0] + 1j * raw[j][..., 1]
+        assert np.linalg.norm(ref) > 0, L.name
+        assert np.linalg.norm(o - ref) <= 1e-9 * np.linalg.norm(ref), L.name
+
+
 def test_sequential_chain_over_horizontal_planes_vs_oracle(sim):
     """Horizontal planes are stored as one cell; the sequential chain
     through them matches the oracle's (full-table) lookups bit for bit in
