@@ -383,18 +383,20 @@ SS_HD void joint_path(const double p[3], const double q[3], const RefrConst& rc,
 // joint solve runs the chain through the no-op pads (each a straight step
 // toward q, so the last pad crossing lies on the segment p -> q) and then
 // Snell against the surface mean plane (10 iterations) and the cell lookup,
-// exactly as joint_path does.  The stationary path then has two unknowns,
-// the surface crossing s on the anchored plane.  Below s the horizontal
-// slowness v is conserved; it solves H(v; s) = sum_k d_k v / c_k - (q_h -
-// s_h) = 0 (c_k = sqrt(n_k^2 - |v|^2), d_k the layer thicknesses under s)
-// along u = (q_h - s_h)/R in t = tan(theta) of the slowest layer, where each
-// term of g(t) is increasing and concave, so Newton from below converges
-// monotonically.  Snell at the surface, F(s) = B^T (n_0 (s-p)/|s-p| - k_1)
-// = 0, is solved by damped Newton with the implicit derivative dv/ds =
-// -M^-1 [I | v/c_1], M = dH/dv.  The pads lie on the straight air leg and
-// change none of the sums (index-matched, booked with the surface cosine),
-// so the air leg is booked as one segment.  Validity and path sums follow
-// joint_crossings / joint_path on the polyline.
+// exactly as joint_path does.  The stationary path is then fixed by the
+// horizontal slowness v, conserved below the surface: given v it is built
+// upward from q (c_k = sqrt(n_k^2 - |v|^2); x_top = q - sum_{k>=2} d_k
+// (v, -c_k)/c_k on the first horizontal plane; then up along u = (-v, c_1)
+// to the anchored plane, s = x_top + lam u), and Snell at the surface,
+// F(v) = B^T (n_0 (s - p)/|s - p| - (v, -c_1)) = 0, is solved by damped
+// Newton with the joint solve's budgets and an analytic Jacobian, from the
+// slowness of p -> s0 refracted at the anchored plane.  No inner solve: one
+// sqrt and one division per layer per evaluation.  The pads lie on the
+// straight air leg and change none of the sums (index-matched, booked with
+// the surface cosine), so the air leg is booked as one segment.  Validity
+// and sums follow joint_crossings / joint_path on the polyline; at the
+// horizontal planes the tangential residual vanishes by construction (both
+// sides carry the same v), so the residual test acts on the surface.
 
 // z of the horizontal interface with absolute index i: its first table cell.
 SS_HD double flat_z(const RefrConst& rc, int i) {
@@ -405,73 +407,76 @@ struct FlatStack {
   const RefrConst* rc;
   const double* n;  // media indices from the air leg down (N + 1)
   int kp, N;        // pads, real interfaces (surface first)
-  double nm;        // slowest medium below the surface
-  // thickness of real layer k (1..N) under a surface crossing at height sz
-  SS_HD double d(int k, double sz, double qz) const {
-    const double top = k == 1 ? sz : flat_z(*rc, kp + k - 1);
+  // height of real layer k (2..N) between its planes (the last ends at qz)
+  SS_HD double d(int k, double qz) const {
+    const double top = flat_z(*rc, kp + k - 1);
     const double bot = k == N ? qz : flat_z(*rc, kp + k);
     return top - bot;
   }
 };
 
-// Horizontal slowness v below s toward q.  t: tangent in the slowest layer
-// (in: start, clamped to >= 0; out: solution).  Also c_1 and M^-1.
-SS_HD void flat_slowness(const FlatStack& fs, const double s[3], const double q[3],
-                         int iters, double& t, double v[2], double& c1,
-                         double Minv[2][2]) {
-  const double Dx = q[0] - s[0], Dy = q[1] - s[1];
-  const double R = sqrt(Dx * Dx + Dy * Dy);
-  const double nm = fs.nm;
-  t = t > 0.0 ? t : 0.0;
-  for (int it = 0; it < iters; ++it) {
-    double g = -R, gp = 0.0;
-    for (int k = 1; k <= fs.N; ++k) {
-      const double dk = fs.d(k, s[2], q[2]), nk = fs.n[k];
-      const double den = nk * nk * (1.0 + t * t) - nm * nm * t * t;
-      const double sq = sqrt(den);
-      g += dk * nm * t / sq;
-      gp += dk * nm * nk * nk / (den * sq);
+// The path of slowness v: F(v) and (jac) dF/dv in the plane basis B, the
+// surface crossing s, its climb length factor lam, c_1, n.u; returns
+// feasibility (every c real, lam > 0, the climbing ray meets the plane from
+// below).
+SS_HD bool flat_eval(const FlatStack& fs, const double p[3], const double q[3],
+                     const double pt0[3], const double nr0[3], const double B[2][3],
+                     const double v[2], bool jac, double F[2], double J[2][2], double s[3],
+                     double& lam, double& c1, double& nu) {
+  const double* nl = fs.n;
+  const double vv = v[0] * v[0] + v[1] * v[1];
+  bool ok = true;
+  double a = 0.0, b = 0.0;  // sum d/c, sum d/c^3 over layers 2..N
+  double xt[3] = {q[0], q[1], q[2]};
+  if (fs.N > 1) {
+    for (int k = 2; k <= fs.N; ++k) {
+      const double ck2 = nl[k] * nl[k] - vv;
+      ok = ok && (ck2 > 0.0);
+      const double rk = 1.0 / sqrt(ck2);
+      const double dk = fs.d(k, q[2]);
+      a += dk * rk;
+      b += dk * (rk * rk * rk);
     }
-    const double tn = t - g / gp;
-    t = tn > 0.0 ? tn : 0.0;
+    xt[0] = q[0] - a * v[0];
+    xt[1] = q[1] - a * v[1];
+    xt[2] = flat_z(*fs.rc, fs.kp + 1);
   }
-  const double pr = nm * t / sqrt(1.0 + t * t);
-  const double ux = R > 0.0 ? Dx / R : 0.0, uy = R > 0.0 ? Dy / R : 0.0;
-  v[0] = pr * ux;
-  v[1] = pr * uy;
-  double a = 0.0, b = 0.0;  // sum d/c, sum d/c^3
-  for (int k = 1; k <= fs.N; ++k) {
-    const double dk = fs.d(k, s[2], q[2]), nk = fs.n[k];
-    const double ck = sqrt(nk * nk - pr * pr);
-    a += dk / ck;
-    b += dk / (ck * ck * ck);
-    if (k == 1) c1 = ck;
-  }
-  const double m00 = a + b * v[0] * v[0], m01 = b * v[0] * v[1], m11 = a + b * v[1] * v[1];
-  const double det = m00 * m11 - m01 * m01;
-  Minv[0][0] = m11 / det;
-  Minv[1][1] = m00 / det;
-  Minv[0][1] = Minv[1][0] = -m01 / det;
-}
-
-// Newton state at one surface crossing.
-struct FlatState {
-  double s[3], t, v[2], c1, Minv[2][2], dh[3], L, F[2];
-};
-
-SS_HD void flat_eval(const FlatStack& fs, const double p[3], const double q[3],
-                     const double B[2][3], int iters, FlatState& st) {
-  flat_slowness(fs, st.s, q, iters, st.t, st.v, st.c1, st.Minv);
-  const double n0 = fs.n[0];
-  const double ds[3] = {st.s[0] - p[0], st.s[1] - p[1], st.s[2] - p[2]};
+  const double c12 = nl[1] * nl[1] - vv;
+  ok = ok && (c12 > 0.0);
+  c1 = sqrt(c12);
+  const double u[3] = {-v[0], -v[1], c1};
+  nu = nr0[0] * u[0] + nr0[1] * u[1] + nr0[2] * u[2];
+  lam = (nr0[0] * (pt0[0] - xt[0]) + nr0[1] * (pt0[1] - xt[1]) + nr0[2] * (pt0[2] - xt[2])) / nu;
+  ok = ok && (nu > 0.0) && (lam > 0.0);
+  for (int c = 0; c < 3; ++c) s[c] = xt[c] + lam * u[c];
+  const double ds[3] = {s[0] - p[0], s[1] - p[1], s[2] - p[2]};
   double L = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
   L = L > kTiny ? L : kTiny;
-  st.L = L;
-  for (int c = 0; c < 3; ++c) st.dh[c] = ds[c] / L;
-  const double g[3] = {n0 * st.dh[0] - st.v[0], n0 * st.dh[1] - st.v[1],
-                       n0 * st.dh[2] + st.c1};
-  st.F[0] = B[0][0] * g[0] + B[0][1] * g[1] + B[0][2] * g[2];
-  st.F[1] = B[1][0] * g[0] + B[1][1] * g[1] + B[1][2] * g[2];
+  const double dh[3] = {ds[0] / L, ds[1] / L, ds[2] / L};
+  const double n0 = nl[0];
+  const double g[3] = {n0 * dh[0] - v[0], n0 * dh[1] - v[1], n0 * dh[2] + c1};
+  F[0] = B[0][0] * g[0] + B[0][1] * g[1] + B[0][2] * g[2];
+  F[1] = B[1][0] * g[0] + B[1][1] * g[1] + B[1][2] * g[2];
+  if (jac) {
+    for (int j = 0; j < 2; ++j) {
+      // dx_top/dv_j = -(a e_j + b v v_j); du/dv_j = (-e_j, -v_j / c_1)
+      const double dxt[3] = {-(a * (j == 0 ? 1.0 : 0.0) + b * v[0] * v[j]),
+                             -(a * (j == 1 ? 1.0 : 0.0) + b * v[1] * v[j]), 0.0};
+      const double du[3] = {j == 0 ? -1.0 : 0.0, j == 1 ? -1.0 : 0.0, -v[j] / c1};
+      const double dl = -((nr0[0] * dxt[0] + nr0[1] * dxt[1] + nr0[2] * dxt[2]) +
+                          lam * (nr0[0] * du[0] + nr0[1] * du[1] + nr0[2] * du[2])) / nu;
+      double dsv[3];
+      for (int c = 0; c < 3; ++c) dsv[c] = dxt[c] + u[c] * dl + lam * du[c];
+      const double hd = dh[0] * dsv[0] + dh[1] * dsv[1] + dh[2] * dsv[2];
+      // d(n0 dh)/dv_j - dk_1/dv_j, k_1 = (v, -c_1)
+      const double G[3] = {n0 * (dsv[0] - dh[0] * hd) / L - (j == 0 ? 1.0 : 0.0),
+                           n0 * (dsv[1] - dh[1] * hd) / L - (j == 1 ? 1.0 : 0.0),
+                           n0 * (dsv[2] - dh[2] * hd) / L - v[j] / c1};
+      J[0][j] = B[0][0] * G[0] + B[0][1] * G[1] + B[0][2] * G[2];
+      J[1][j] = B[1][0] * G[0] + B[1][1] * G[1] + B[1][2] * G[2];
+    }
+  }
+  return ok;
 }
 
 SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, int n_newton,
@@ -487,9 +492,6 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, 
   const double* nl = fs.n;
   const double* epsl = rc.legs + (NA + 1) + kp;
   const double* attl = rc.legs + 2 * (NA + 1) + kp;
-  double nm = nl[1];
-  for (int k = 2; k <= N; ++k) nm = nl[k] < nm ? nl[k] : nm;
-  fs.nm = nm;
   // pass 1: the pads step straight toward q, then the surface mean plane
   double c0[3] = {p[0], p[1], p[2]};
   if (kp > 0) {
@@ -499,8 +501,7 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, 
     c0[2] = zp;
   }
   const double* d0 = rc.desc + kp * SS_IDESC_COUNT;
-  double pt0[3], nr0[3];
-  FlatState st;
+  double pt0[3], nr0[3], s[3];
   {
     double mp[3] = {d0[SS_IDESC_MP], d0[SS_IDESC_MP + 1], d0[SS_IDESC_MP + 2]};
     double mn[3] = {d0[SS_IDESC_MN], d0[SS_IDESC_MN + 1], d0[SS_IDESC_MN + 2]};
@@ -520,72 +521,60 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, 
     }
     double h = (sn.x[0] - pt0[0]) * nr0[0] + (sn.x[1] - pt0[1]) * nr0[1] +
                (sn.x[2] - pt0[2]) * nr0[2];
-    for (int c = 0; c < 3; ++c) st.s[c] = sn.x[c] - h * nr0[c];
+    for (int c = 0; c < 3; ++c) s[c] = sn.x[c] - h * nr0[c];
   }
   double B[2][3];
   plane_basis(nr0, B);
-  const double z_top = N > 1 ? flat_z(rc, kp + 1) : q[2];
-  st.t = 0.0;
-  flat_eval(fs, p, q, B, 12, st);
-  // damped Newton on the surface crossing: a step is halved until it lowers
-  // |F|^2 and keeps s above the first horizontal plane; a step that never
-  // does ends the solve
+  // start: the slowness of p -> s refracted n0 -> n1 at the anchored plane
+  double v[2];
+  {
+    const double ds[3] = {s[0] - p[0], s[1] - p[1], s[2] - p[2]};
+    const double L = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+    const double i0 = ds[0] / L, i1 = ds[1] / L, i2 = ds[2] / L;
+    const double cosi = -(nr0[0] * i0 + nr0[1] * i1 + nr0[2] * i2);
+    const double r = nl[0] / nl[1];
+    const double cost = sqrt(1.0 - r * r * (1.0 - cosi * cosi));
+    const double m = nl[0] * cosi - nl[1] * cost;
+    v[0] = nl[0] * i0 + m * nr0[0];
+    v[1] = nl[0] * i1 + m * nr0[1];
+  }
+  // damped Newton on v: of lam = 1, 1/2, ... (n_backtrack steps) the first
+  // feasible step that lowers |F|^2 is taken; none ends the solve
+  double F[2], J[2][2], lam, c1, nu;
+  flat_eval(fs, p, q, pt0, nr0, B, v, true, F, J, s, lam, c1, nu);
   for (int it = 0; it < n_newton; ++it) {
-    const double m0 = st.F[0] * st.F[0] + st.F[1] * st.F[1];
+    const double m0 = F[0] * F[0] + F[1] * F[1];
     if (!(m0 > 1e-30)) break;
-    double dvds[2][3];
-    for (int r = 0; r < 2; ++r) {
-      dvds[r][0] = -st.Minv[r][0];
-      dvds[r][1] = -st.Minv[r][1];
-      dvds[r][2] = -(st.Minv[r][0] * st.v[0] + st.Minv[r][1] * st.v[1]) / st.c1;
-    }
-    double G[3][3];
-    const double n0 = nl[0];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) {
-        const double pm = ((r == c ? 1.0 : 0.0) - st.dh[r] * st.dh[c]) / st.L;
-        const double dk = r < 2 ? dvds[r][c]
-                                : (st.v[0] * dvds[0][c] + st.v[1] * dvds[1][c]) / st.c1;
-        G[r][c] = n0 * pm - dk;
-      }
-    double J[2][2];
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) {
-        double acc = 0.0;
-        for (int r = 0; r < 3; ++r) {
-          const double gb = G[r][0] * B[b][0] + G[r][1] * B[b][1] + G[r][2] * B[b][2];
-          acc += B[a][r] * gb;
-        }
-        J[a][b] = acc;
-      }
     const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const double st0 = (J[1][1] * st.F[0] - J[0][1] * st.F[1]) / det;
-    const double st1 = (J[0][0] * st.F[1] - J[1][0] * st.F[0]) / det;
-    double lam = 1.0;
+    const double st0 = (J[1][1] * F[0] - J[0][1] * F[1]) / det;
+    const double st1 = (J[0][0] * F[1] - J[1][0] * F[0]) / det;
+    double lm = 1.0;
     bool taken = false;
-    for (int bt = 0; bt < n_backtrack && !taken; ++bt, lam *= 0.5) {
-      FlatState tr = st;
-      for (int c = 0; c < 3; ++c) tr.s[c] = st.s[c] - lam * (B[0][c] * st0 + B[1][c] * st1);
-      flat_eval(fs, p, q, B, 6, tr);
-      const double mt = tr.F[0] * tr.F[0] + tr.F[1] * tr.F[1];
-      if (isfinite(mt) && mt < m0 && tr.s[2] > z_top) {
-        st = tr;
+    for (int bt = 0; bt < n_backtrack && !taken; ++bt, lm *= 0.5) {
+      const double vt[2] = {v[0] - lm * st0, v[1] - lm * st1};
+      double Ft[2], Jt[2][2], stt[3], lt, c1t, nut;
+      const bool okt = flat_eval(fs, p, q, pt0, nr0, B, vt, true, Ft, Jt, stt, lt, c1t, nut);
+      const double mt = Ft[0] * Ft[0] + Ft[1] * Ft[1];
+      if (okt && isfinite(mt) && mt < m0) {
+        v[0] = vt[0];
+        v[1] = vt[1];
+        F[0] = Ft[0];
+        F[1] = Ft[1];
+        for (int r = 0; r < 2; ++r)
+          for (int c = 0; c < 2; ++c) J[r][c] = Jt[r][c];
+        for (int c = 0; c < 3; ++c) s[c] = stt[c];
+        lam = lt;
+        c1 = c1t;
+        nu = nut;
         taken = true;
       }
     }
     if (!taken) break;
   }
-  double Minv[2][2];
-  flat_slowness(fs, st.s, q, 4, st.t, st.v, st.c1, Minv);
-  // polyline, streamed top-down: validity and sums as joint_path
-  const double* s = st.s;
-  const double* v = st.v;
-  double prev[3] = {p[0], p[1], p[2]};
-  double xi[3] = {s[0], s[1], s[2]};
-  double resid = 0.0, c_surf = 1.0, ct2_last = 1.0;
-  bool sides = true, evan = false, fin = true;
-  tau2 = 1.0;
-  opl = loss_db = sum_par = sum_perp = 0.0;
+  // the polyline from v: validity and sums as joint_path
+  const double vv = v[0] * v[0] + v[1] * v[1];
+  const double vm = sqrt(vv);
+  bool sides = true, evan = false, fin = isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]);
   for (int i = 0; i < kp; ++i) {
     const double zp = rc.desc[i * SS_IDESC_COUNT + SS_IDESC_MP + 2];
     sides = sides && ((p[2] - zp) * (s[2] - zp) < 0.0);
@@ -598,74 +587,80 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, 
   } else {
     for (int c = 0; c < 3; ++c) x_first[c] = s[c];
   }
-  const double pr2 = v[0] * v[0] + v[1] * v[1];
-  for (int i = 0; i < N; ++i) {
-    double nxt[3];
-    if (i + 1 < N) {
-      const double dk = fs.d(i + 1, s[2], q[2]), nk = nl[i + 1];
-      const double ck = sqrt(nk * nk - pr2);
-      nxt[0] = xi[0] + dk * v[0] / ck;
-      nxt[1] = xi[1] + dk * v[1] / ck;
-      nxt[2] = flat_z(rc, kp + i + 1);
-    } else {
-      nxt[0] = q[0];
-      nxt[1] = q[1];
-      nxt[2] = q[2];
-    }
-    const double n[3] = {i == 0 ? nr0[0] : 0.0, i == 0 ? nr0[1] : 0.0, i == 0 ? nr0[2] : 1.0};
-    const double a[3] = {xi[0] - prev[0], xi[1] - prev[1], xi[2] - prev[2]};
-    const double b[3] = {nxt[0] - xi[0], nxt[1] - xi[1], nxt[2] - xi[2]};
-    const double sa = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    const double sb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    const double sam = sa > kTiny ? sa : kTiny, sbm = sb > kTiny ? sb : kTiny;
-    const double din[3] = {a[0] / sam, a[1] / sam, a[2] / sam};
-    const double dout[3] = {b[0] / sbm, b[1] / sbm, b[2] / sbm};
-    const double ct1 = n[0] * din[0] + n[1] * din[1] + n[2] * din[2];
-    const double ct2 = n[0] * dout[0] + n[1] * dout[1] + n[2] * dout[2];
-    const double t1[3] = {din[0] - ct1 * n[0], din[1] - ct1 * n[1], din[2] - ct1 * n[2]};
-    const double t2[3] = {dout[0] - ct2 * n[0], dout[1] - ct2 * n[1], dout[2] - ct2 * n[2]};
-    const double st1 = sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
-    const double st2 = sqrt(t2[0] * t2[0] + t2[1] * t2[1] + t2[2] * t2[2]);
-    const double th1 = atan2(st1, fabs(ct1)), th2 = atan2(st2, fabs(ct2));
-    const double dG[3] = {nl[i] * din[0] - nl[i + 1] * dout[0],
-                          nl[i] * din[1] - nl[i + 1] * dout[1],
-                          nl[i] * din[2] - nl[i + 1] * dout[2]};
-    const double nd = n[0] * dG[0] + n[1] * dG[1] + n[2] * dG[2];
-    const double rt[3] = {dG[0] - nd * n[0], dG[1] - nd * n[1], dG[2] - nd * n[2]};
-    const double rn = sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
-    resid = rn > resid ? rn : resid;
-    fin = fin && isfinite(rn) && isfinite(sa) && isfinite(xi[0]) && isfinite(xi[1]) &&
-          isfinite(xi[2]);
-    const double hp = -(n[0] * a[0] + n[1] * a[1] + n[2] * a[2]);
-    const double hn = n[0] * b[0] + n[1] * b[1] + n[2] * b[2];
-    sides = sides && (hp * hn < 0.0);
-    evan = evan || (nl[i] * st1 > nl[i + 1] * (1.0 + 64.0 * kEps64));
-    const double ci = fmax(cos(th1), kCMin);
-    if (i == 0) c_surf = ci;
+  // surface: incidence from the air leg, refraction into layer 1
+  const double ds[3] = {s[0] - p[0], s[1] - p[1], s[2] - p[2]};
+  const double sa = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+  const double sam = sa > kTiny ? sa : kTiny;
+  const double din[3] = {ds[0] / sam, ds[1] / sam, ds[2] / sam};
+  const double dout[3] = {v[0] / nl[1], v[1] / nl[1], -c1 / nl[1]};
+  const double ct1 = nr0[0] * din[0] + nr0[1] * din[1] + nr0[2] * din[2];
+  const double ct2 = nr0[0] * dout[0] + nr0[1] * dout[1] + nr0[2] * dout[2];
+  const double t1[3] = {din[0] - ct1 * nr0[0], din[1] - ct1 * nr0[1], din[2] - ct1 * nr0[2]};
+  const double st1 = sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+  const double dG[3] = {nl[0] * din[0] - nl[1] * dout[0], nl[0] * din[1] - nl[1] * dout[1],
+                        nl[0] * din[2] - nl[1] * dout[2]};
+  const double nd = nr0[0] * dG[0] + nr0[1] * dG[1] + nr0[2] * dG[2];
+  const double rt[3] = {dG[0] - nd * nr0[0], dG[1] - nd * nr0[1], dG[2] - nd * nr0[2]};
+  const double resid = sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+  fin = fin && isfinite(resid) && isfinite(sa) && isfinite(c1);
+  sides = sides && ((nr0[0] * (p[0] - s[0]) + nr0[1] * (p[1] - s[1]) + nr0[2] * (p[2] - s[2])) *
+                        (-lam * nu) < 0.0);
+  evan = nl[0] * st1 > nl[1] * (1.0 + 64.0 * kEps64);
+  const bool rough = (rc.flags & SS_CROSS_ROUGH) != 0;
+  // interface i: TE Fresnel at incidence cosine ci (medium i -> i + 1) with
+  // transmission cosine ct (the crossing-roughness term uses both)
+  auto cross = [&](int i, double ci_raw, double ct_raw) {
+    const double ci = fmax(ci_raw, kCMin);
     const double e1 = epsl[i], e2 = epsl[i + 1];
-    const double n1 = sqrt(e1), n2 = sqrt(e2);
     const double sin2 = (e1 / e2) * (1.0 - ci * ci);
     const double cc2 = sqrt(fmax(1.0 - sin2, 0.0));
-    const double gm = (n1 * ci - n2 * cc2) / (n1 * ci + n2 * cc2);
+    const double gm = (nl[i] * ci - nl[i + 1] * cc2) / (nl[i] * ci + nl[i + 1] * cc2);
     tau2 *= (1.0 - gm * gm);
-    if (rc.flags & SS_CROSS_ROUGH) {
-      const double kt = rc.k0 * (nl[i] * cos(th1) - nl[i + 1] * cos(th2));
+    if (rough) {
+      const double kt = rc.k0 * (nl[i] * ci_raw - nl[i + 1] * ct_raw);
       const double sk = rc.desc[(kp + i) * SS_IDESC_COUNT + SS_IDESC_SIGC] * kt;
       tau2 *= exp(-2.0 * (sk * sk));
     }
-    opl += nl[i] * sa;
-    loss_db += sa * (attl[i] / 1000.0);
-    sum_perp += sa / nl[i];
-    sum_par += sa / (nl[i] * ci * ci);
-    ct2_last = cos(th2);
-    for (int c = 0; c < 3; ++c) prev[c] = xi[c];
-    if (i + 1 < N)
-      for (int c = 0; c < 3; ++c) xi[c] = nxt[c];
+    return ci;
+  };
+  tau2 = 1.0;
+  const double cs_raw = fabs(ct1);
+  const double c_surf = cross(0, cs_raw, fabs(ct2));
+  opl = nl[0] * sa;
+  loss_db = sa * (attl[0] / 1000.0);
+  sum_perp = sa / nl[0];
+  sum_par = sa / (nl[0] * c_surf * c_surf);
+  double ct_last = fabs(ct2);
+  double ck = c1;  // c of the current layer
+  for (int i = 1; i < N; ++i) {
+    // layer i above plane i: height h (layer 1: the climb lam c_1), segment
+    // length |h| n_i / c_i (a polyline length also where an unsolved lane
+    // left h < 0: the dropped power of invalid lanes is booked from it)
+    const double h = i == 1 ? lam * c1 : fs.d(i, q[2]);
+    const double ck1 = sqrt(nl[i + 1] * nl[i + 1] - vv);
+    sides = sides && (h > 0.0) && (fs.d(i + 1, q[2]) > 0.0);
+    evan = evan || (vm > nl[i + 1] * (1.0 + 64.0 * kEps64));
+    fin = fin && isfinite(ck1);
+    const double len = i == 1 ? fabs(lam) * nl[1] : fabs(h) * nl[i] / ck;
+    const double ci = cross(i, ck / nl[i], ck1 / nl[i + 1]);
+    opl += nl[i] * len;
+    loss_db += len * (attl[i] / 1000.0);
+    sum_perp += len / nl[i];
+    sum_par += len / (nl[i] * ci * ci);
+    ct_last = ck1 / nl[i + 1];
+    ck = ck1;
   }
   valid = (resid <= 1e-9) && fin && sides && !evan;
   c_first = c_surf;
-  c_last = fmax(ct2_last, kCMin);
-  for (int c = 0; c < 3; ++c) cur[c] = xi[c];
+  c_last = fmax(ct_last, kCMin);
+  if (N > 1) {
+    const double dN = fs.d(N, q[2]);
+    cur[0] = q[0] - dN * v[0] / ck;
+    cur[1] = q[1] - dN * v[1] / ck;
+    cur[2] = flat_z(rc, kp + N - 1);
+  } else {
+    for (int c = 0; c < 3; ++c) cur[c] = s[c];
+  }
 }
 
 // Top-level refracted pair: crossing solver (NMAX < 0: the flat-stack joint

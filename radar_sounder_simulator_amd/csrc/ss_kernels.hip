@@ -1212,7 +1212,15 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   if (pm == 0) { SS_BOTH(0, R, 0); } else if (pm == 5) { SS_BOTH(0, R, 5); } \
   else { SS_BOTH(0, R, -1); }
   if (prm->refraction == 2) {
-    SS_BOTH(-1, -1, -1);
+    // flat stacks: the multilayer scenes' smooth interfaces under a
+    // tabulated (or no) pattern get their own instantiations
+    if (rm == 0 && pm == 5) {
+      SS_BOTH(-1, 0, 5);
+    } else if (rm == 0 && pm == 0) {
+      SS_BOTH(-1, 0, 0);
+    } else {
+      SS_BOTH(-1, -1, -1);
+    }
   } else if (!rc.joint) {
     if (rm == 0) {
       SS_REFR_PA(0);

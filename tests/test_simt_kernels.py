@@ -183,8 +183,11 @@ def test_flat_stack_joint_kernel_vs_oracle(sim, pad_to, rough):
 
 def test_deep_flat_stack_kernel_vs_flat_oracle(sim):
     """41 interfaces (deeper than the general joint kernel takes): the
-    flat-stack kernel against the oracle of the flat-stack form."""
-    surf, _, bed, tr = _flat_stack()
+    flat-stack kernel against the oracle of the flat-stack form.  The rough
+    surface dips below the first plane (-5 m) in places: those lanes are
+    invalid and their power is booked in dropped from the unsolved
+    polyline."""
+    surf, _, bed, tr = _flat_stack(seed=5, n=20)
     n = surf.grid_shape[0]
     crossed = [surf] + [grid_facets(np.full((n + 1, n + 1), z), 25.0)
                         for z in -5.0 * np.arange(1, 41)]
@@ -204,7 +207,7 @@ def test_deep_flat_stack_kernel_vs_flat_oracle(sim):
     ref, dref, bref, vref = om.refracted_cluttergram(tr.positions, tr.u_ct, bed, crossed, eps,
                                                      att, return_bins=True, refraction="joint",
                                                      flat_stack=True, **kw)
-    assert vref.mean() > 0.9
+    assert 0.9 < vref.mean() < 1.0
     np.testing.assert_array_equal(fac.unpermute_pairs(bins.numpy()),
                                   np.where(vref, bref, _lib.BIN_INVALID))
     assert np.linalg.norm(o - ref) <= 1e-9 * np.linalg.norm(ref)
