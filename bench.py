@@ -250,7 +250,7 @@ def main():
     ap.add_argument("--lam", type=float, default=8.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--ml-traces", type=int, default=16,
+    ap.add_argument("--ml-traces", type=int, default=64,
                     help="traces of the multilayer scene's 1000 in its section")
     ap.add_argument("--no-multilayer", action="store_true")
     args = ap.parse_args()

@@ -334,6 +334,33 @@ int ss_range_compress_fft(const double* in, int64_t n_rows, int64_t n_samples,
                           const double* spectrum, int64_t m, int64_t nfft, double* out,
                           void* stream);
 
+/* Facets of a regular vertex lattice built on the device.  heights:
+ * (ny+1, nx+1) float64 device, row-major (rows along y, columns along x),
+ * vertices at ((j - nx/2) * spacing + x0, (i - ny/2) * spacing + y0).  Cell
+ * (r, c) becomes a facet by the reference's corner math (scene.py:158-176,
+ * compare/brute_force_layered.py:44-71: mean edges e1 (along +column) and
+ * e2 (along +row), centre = vertex mean, normal = e1 x e2 oriented +z,
+ * area = |e1 x e2|) written to facet rows SS_FROW_CX..SS_FROW_E2X + 2 at
+ * position ss order: 16x16-cell tiles, tiles row-major, cells row-major in
+ * a tile (r' = r mod 16, c' = c mod 16, h, w the tile's rows and columns:
+ * pos = (r/16)*16*nx + (c/16)*16*h + r'*w + c').  Rows from SS_FROW_GAMMA on
+ * are left to the caller.  table (nullable): (ny*nx, 6) float64 device
+ * [centre, normal] in grid order, the crossed-interface lookup table of
+ * ss_refracted_field.  stats: 10 float64 device values [sum A, sum A*centre
+ * (3), sum A*normal (3), sum centre (3)] (the area-weighted mean plane and
+ * the centre mean), summed in a fixed order.  workspace: caller memory of
+ * ss_facets_workspace_bytes(ny, nx) bytes, 8-byte aligned.  Replaces the
+ * host-side tessellation the reference runs before each kernel call; no
+ * counterpart as a device routine. */
+int ss_facets_from_heights(const double* heights, int64_t ny, int64_t nx, double spacing,
+                           double x0, double y0, double* facets, int64_t facet_stride,
+                           double* table, double* stats, void* workspace,
+                           int64_t workspace_bytes, void* stream);
+
+/* Workspace bytes of ss_facets_from_heights (per-block partial sums), or -1
+ * for an empty lattice. */
+int64_t ss_facets_workspace_bytes(int64_t ny, int64_t nx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,7 +46,8 @@ EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_refracted_workspace_bytes", "ss_batch_field", "ss_batch_workspace_bytes",
             "ss_sizeof_job", "ss_range_compress", "ss_compress_fft_size", "ss_pulse_spectrum",
             "ss_range_compress_fft", "ss_batch_table_bytes", "ss_batch_pack",
-            "ss_batch_step_workspace_bytes", "ss_batch_step")
+            "ss_batch_step_workspace_bytes", "ss_batch_step", "ss_facets_from_heights",
+            "ss_facets_workspace_bytes")
 
 
 class SurfaceParams(ctypes.Structure):
@@ -158,6 +159,11 @@ def configure(lib):
     lib.ss_batch_step_workspace_bytes.restype = ctypes.c_int64
     lib.ss_batch_step.argtypes = [i32, dp, dp, i64, i64, i32, i32, i32, vp]
     lib.ss_batch_step.restype = ctypes.c_int
+    lib.ss_facets_from_heights.argtypes = [dp, i64, i64, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, dp, i64, dp, dp, dp, i64, vp]
+    lib.ss_facets_from_heights.restype = ctypes.c_int
+    lib.ss_facets_workspace_bytes.argtypes = [i64, i64]
+    lib.ss_facets_workspace_bytes.restype = ctypes.c_int64
     return lib
 
 

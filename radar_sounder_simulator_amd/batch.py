@@ -23,7 +23,7 @@ import torch
 from . import _lib
 from . import device as dv
 from .physics import C, fresnel_normal
-from .scene import gaussian_surface, grid_facets, straight_track
+from .scene import Lattice, gaussian_surface, grid_facets, straight_track
 from .waveform import Waveform
 from .workloads import Layer, Workload
 
@@ -174,21 +174,22 @@ def draw_scene(rng, srng, scale=1.0, max_facets=None):
     L = int(rng.integers(1, 5))
     side = max(8, int(round(math.sqrt(F))))
     dx = 25.0
-    surf = grid_facets(2.0 * gaussian_surface(side + 1, dx, 200.0, srng), dx)
+    lat = Lattice(2.0 * gaussian_surface(side + 1, dx, 200.0, srng), dx)
+    surf = lat.facets()
     eps_ice = 3.15
-    layers = [Layer("surface", surf, fresnel_normal(1.0, eps_ice))]
+    layers = [Layer("surface", surf, fresnel_normal(1.0, eps_ice), lattice=lat)]
     crossed = [surf]
     eps = [1.0, eps_ice]
     att = [0.0, 10.0]
     depth = 0.0
     for j in range(1, L):
         depth += rng.uniform(150.0, 800.0)
-        h = -depth + 20.0 * gaussian_surface(side + 1, dx, 1000.0, srng)
-        f = grid_facets(h, dx)
+        lat = Lattice(-depth + 20.0 * gaussian_surface(side + 1, dx, 1000.0, srng), dx)
+        f = lat.facets()
         eps_below = eps_ice if j < L - 1 else 8.0
         layers.append(Layer(f"layer{j}", f, fresnel_normal(eps[-1], eps_below)
                             if j == L - 1 else -10 ** (-40 / 20), crossed=list(crossed),
-                            eps_leg=list(eps), att_leg=list(att)))
+                            eps_leg=list(eps), att_leg=list(att), lattice=lat))
         crossed.append(f)
         eps = eps + [eps_ice]
         att = att + [10.0]

@@ -479,20 +479,16 @@ SS_HD bool flat_eval(const FlatStack& fs, const double p[3], const double q[3],
   return ok;
 }
 
-SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, int n_newton,
-                     int n_backtrack, double cur[3], bool& valid, double& opl,
-                     double& loss_db, double& sum_par, double& sum_perp, double& tau2,
-                     double& c_first, double& c_last, double x_first[3]) {
-  const int NA = rc.n_crossed, kp = rc.k_pad, N = NA - kp;
-  FlatStack fs;
-  fs.rc = &rc;
-  fs.n = rc.legs + kp;
-  fs.kp = kp;
-  fs.N = N;
+// Pass 1: the pads step straight toward q, then Snell against the surface
+// mean plane and the cell lookup; the anchored plane (pt0, nr0) and the
+// crossing projected onto it (s0).  mixed: float32 anchor in mean-plane
+// relative coordinates, the float64 one only near a lookup rounding edge
+// (as the sequential chain's mixed tier).
+SS_HD void flat_anchor(const double p[3], const double q[3], const RefrConst& rc,
+                       const FlatStack& fs, bool mixed, double pt0[3], double nr0[3],
+                       double s0[3]) {
+  const int kp = fs.kp;
   const double* nl = fs.n;
-  const double* epsl = rc.legs + (NA + 1) + kp;
-  const double* attl = rc.legs + 2 * (NA + 1) + kp;
-  // pass 1: the pads step straight toward q, then the surface mean plane
   double c0[3] = {p[0], p[1], p[2]};
   if (kp > 0) {
     const double zp = rc.desc[(kp - 1) * SS_IDESC_COUNT + SS_IDESC_MP + 2];
@@ -501,46 +497,139 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, 
     c0[2] = zp;
   }
   const double* d0 = rc.desc + kp * SS_IDESC_COUNT;
-  double pt0[3], nr0[3], s[3];
-  {
-    double mp[3] = {d0[SS_IDESC_MP], d0[SS_IDESC_MP + 1], d0[SS_IDESC_MP + 2]};
-    double mn[3] = {d0[SS_IDESC_MN], d0[SS_IDESC_MN + 1], d0[SS_IDESC_MN + 2]};
+  const double* cf = d0 + SS_IDESC_COEF;
+  double mp[3] = {d0[SS_IDESC_MP], d0[SS_IDESC_MP + 1], d0[SS_IDESC_MP + 2]};
+  double mn[3] = {d0[SS_IDESC_MN], d0[SS_IDESC_MN + 1], d0[SS_IDESC_MN + 2]};
+  double x[3];
+  double rc0 = 0.0, rc1 = 0.0;
+  bool exact = !mixed;
+  if (mixed) {
+    float pr[3] = {(float)(c0[0] - mp[0]), (float)(c0[1] - mp[1]), (float)(c0[2] - mp[2])};
+    float qr[3] = {(float)(q[0] - mp[0]), (float)(q[1] - mp[1]), (float)(q[2] - mp[2])};
+    float mf[3] = {(float)mn[0], (float)mn[1], (float)mn[2]};
+    float xr[3];
+    snell_anchor32(pr, qr, mf, (float)nl[0], (float)nl[1], xr);
+    for (int c = 0; c < 3; ++c) x[c] = (double)xr[c] + mp[c];
+    rc0 = (x[0] * cf[0] + x[1] * cf[2]) + cf[4];
+    rc1 = (x[0] * cf[1] + x[1] * cf[3]) + cf[5];
+    const double f0 = rc0 - floor(rc0), f1 = rc1 - floor(rc1);
+    exact = fabs(f0 - 0.5) < kCellGuard || fabs(f1 - 0.5) < kCellGuard;
+  }
+  if (exact) {
     Snell64 sn;
     snell_fp64(c0, q, mp, mn, nl[0], nl[1], kMeanIters, sn);
-    const double* cf = d0 + SS_IDESC_COEF;
-    double rc0 = (sn.x[0] * cf[0] + sn.x[1] * cf[2]) + cf[4];
-    double rc1 = (sn.x[0] * cf[1] + sn.x[1] * cf[3]) + cf[5];
-    int row = clamp_round(rc0, (int)d0[SS_IDESC_NY] - 1);
-    int col = clamp_round(rc1, (int)d0[SS_IDESC_NX] - 1);
-    const double* g = rc.tables + ((long)d0[SS_IDESC_TABOFF] + (long)row * (int)d0[SS_IDESC_NX] +
-                                   col) * 6;
-    double nn = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
-    for (int c = 0; c < 3; ++c) {
-      pt0[c] = g[c];
-      nr0[c] = g[3 + c] / nn;
+    for (int c = 0; c < 3; ++c) x[c] = sn.x[c];
+    {
+#pragma clang fp contract(off)
+      rc0 = (x[0] * cf[0] + x[1] * cf[2]) + cf[4];
+      rc1 = (x[0] * cf[1] + x[1] * cf[3]) + cf[5];
     }
-    double h = (sn.x[0] - pt0[0]) * nr0[0] + (sn.x[1] - pt0[1]) * nr0[1] +
-               (sn.x[2] - pt0[2]) * nr0[2];
-    for (int c = 0; c < 3; ++c) s[c] = sn.x[c] - h * nr0[c];
   }
-  double B[2][3];
-  plane_basis(nr0, B);
-  // start: the slowness of p -> s refracted n0 -> n1 at the anchored plane
-  double v[2];
-  {
-    const double ds[3] = {s[0] - p[0], s[1] - p[1], s[2] - p[2]};
-    const double L = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
-    const double i0 = ds[0] / L, i1 = ds[1] / L, i2 = ds[2] / L;
-    const double cosi = -(nr0[0] * i0 + nr0[1] * i1 + nr0[2] * i2);
-    const double r = nl[0] / nl[1];
-    const double cost = sqrt(1.0 - r * r * (1.0 - cosi * cosi));
-    const double m = nl[0] * cosi - nl[1] * cost;
-    v[0] = nl[0] * i0 + m * nr0[0];
-    v[1] = nl[0] * i1 + m * nr0[1];
+  int row = clamp_round(rc0, (int)d0[SS_IDESC_NY] - 1);
+  int col = clamp_round(rc1, (int)d0[SS_IDESC_NX] - 1);
+  const double* g = rc.tables + ((long)d0[SS_IDESC_TABOFF] + (long)row * (int)d0[SS_IDESC_NX] +
+                                 col) * 6;
+  double nn = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+  for (int c = 0; c < 3; ++c) {
+    pt0[c] = g[c];
+    nr0[c] = g[3 + c] / nn;
   }
-  // damped Newton on v: of lam = 1, 1/2, ... (n_backtrack steps) the first
-  // feasible step that lowers |F|^2 is taken; none ends the solve
-  double F[2], J[2][2], lam, c1, nu;
+  double h = (x[0] - pt0[0]) * nr0[0] + (x[1] - pt0[1]) * nr0[1] + (x[2] - pt0[2]) * nr0[2];
+  for (int c = 0; c < 3; ++c) s0[c] = x[c] - h * nr0[c];
+}
+
+// Start slowness: the ray p -> s0 refracted n0 -> n1 at the plane nr0.
+SS_HD void flat_start(const double p[3], const double s0[3], const double nr0[3],
+                      const double* nl, double v[2]) {
+  const double ds[3] = {s0[0] - p[0], s0[1] - p[1], s0[2] - p[2]};
+  const double L = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+  const double i0 = ds[0] / L, i1 = ds[1] / L, i2 = ds[2] / L;
+  const double cosi = -(nr0[0] * i0 + nr0[1] * i1 + nr0[2] * i2);
+  const double r = nl[0] / nl[1];
+  const double cost = sqrt(1.0 - r * r * (1.0 - cosi * cosi));
+  const double m = nl[0] * cosi - nl[1] * cost;
+  v[0] = nl[0] * i0 + m * nr0[0];
+  v[1] = nl[0] * i1 + m * nr0[1];
+}
+
+// Float32 twin of flat_eval (F and J only) in coordinates relative to s0:
+// P, Q, O = p, q, pt0 minus s0; zt = first plane's z minus s0's; dN = the
+// last layer's height (z_{N-1} - q_z).
+// Returns 0 feasible, 1 clearly infeasible (some c^2 < -1e-4, n.u < -1e-4
+// or the plane more than 5 cm past the climbing ray's start: the float64
+// test agrees), 2 infeasible within those margins.
+SS_HD int flat_eval32(const FlatStack& fs, const float P[3], const float Q[3],
+                      const float O[3], const float nr[3], const float B[2][3], float zt,
+                      float dN, const float v[2], float F[2], float J[2][2]) {
+  const double* nl = fs.n;
+  const float vv = v[0] * v[0] + v[1] * v[1];
+  bool ok = true, clear = false;
+  float a = 0.0f, b = 0.0f;
+  float xt[3] = {Q[0], Q[1], Q[2]};
+  if (fs.N > 1) {
+    for (int k = 2; k <= fs.N; ++k) {
+      const float nk = (float)nl[k];
+      const float ck2 = nk * nk - vv;
+      ok = ok && (ck2 > 0.0f);
+      clear = clear || (ck2 < -1e-4f);
+      const float rk = f_rsq(fmaxf(ck2, 1e-30f));
+      const float dk = k == fs.N ? dN
+                                 : (float)(flat_z(*fs.rc, fs.kp + k - 1) - flat_z(*fs.rc, fs.kp + k));
+      a += dk * rk;
+      b += dk * (rk * rk * rk);
+    }
+    xt[0] = Q[0] - a * v[0];
+    xt[1] = Q[1] - a * v[1];
+    xt[2] = zt;
+  }
+  const float n1 = (float)nl[1];
+  const float c12 = n1 * n1 - vv;
+  ok = ok && (c12 > 0.0f);
+  clear = clear || (c12 < -1e-4f);
+  const float c1 = f_sqrt(fmaxf(c12, 1e-30f));
+  const float ic1 = f_rcp(c1);
+  const float u[3] = {-v[0], -v[1], c1};
+  const float nu = nr[0] * u[0] + nr[1] * u[1] + nr[2] * u[2];
+  const float inu = f_rcp(nu);
+  const float num = nr[0] * (O[0] - xt[0]) + nr[1] * (O[1] - xt[1]) + nr[2] * (O[2] - xt[2]);
+  const float lam = num * inu;
+  ok = ok && (nu > 0.0f) && (lam > 0.0f);
+  clear = clear || (nu < -1e-4f) || (nu > 1e-4f && num < -0.05f);
+  float sv[3];
+  for (int c = 0; c < 3; ++c) sv[c] = xt[c] + lam * u[c];
+  const float ds[3] = {sv[0] - P[0], sv[1] - P[1], sv[2] - P[2]};
+  const float L = f_sqrt(fmaxf(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2], 1e-30f));
+  const float iL = f_rcp(L);
+  const float dh[3] = {ds[0] * iL, ds[1] * iL, ds[2] * iL};
+  const float n0 = (float)nl[0];
+  const float g[3] = {n0 * dh[0] - v[0], n0 * dh[1] - v[1], n0 * dh[2] + c1};
+  F[0] = B[0][0] * g[0] + B[0][1] * g[1] + B[0][2] * g[2];
+  F[1] = B[1][0] * g[0] + B[1][1] * g[1] + B[1][2] * g[2];
+  for (int j = 0; j < 2; ++j) {
+    const float dxt[3] = {-(a * (j == 0 ? 1.0f : 0.0f) + b * v[0] * v[j]),
+                          -(a * (j == 1 ? 1.0f : 0.0f) + b * v[1] * v[j]), 0.0f};
+    const float du[3] = {j == 0 ? -1.0f : 0.0f, j == 1 ? -1.0f : 0.0f, -v[j] * ic1};
+    const float dl = -((nr[0] * dxt[0] + nr[1] * dxt[1] + nr[2] * dxt[2]) +
+                       lam * (nr[0] * du[0] + nr[1] * du[1] + nr[2] * du[2])) * inu;
+    float dsv[3];
+    for (int c = 0; c < 3; ++c) dsv[c] = dxt[c] + u[c] * dl + lam * du[c];
+    const float hd = dh[0] * dsv[0] + dh[1] * dsv[1] + dh[2] * dsv[2];
+    const float G[3] = {n0 * (dsv[0] - dh[0] * hd) * iL - (j == 0 ? 1.0f : 0.0f),
+                        n0 * (dsv[1] - dh[1] * hd) * iL - (j == 1 ? 1.0f : 0.0f),
+                        n0 * (dsv[2] - dh[2] * hd) * iL - v[j] * ic1};
+    J[0][j] = B[0][0] * G[0] + B[0][1] * G[1] + B[0][2] * G[2];
+    J[1][j] = B[1][0] * G[0] + B[1][1] * G[1] + B[1][2] * G[2];
+  }
+  return ok ? 0 : (clear ? 1 : 2);
+}
+
+// Float64 damped Newton on v with the joint solve's budgets: of lam = 1,
+// 1/2, ... (n_backtrack steps) the first feasible step that lowers |F|^2 is
+// taken; none ends the solve.
+SS_HD void flat_newton64(const FlatStack& fs, const double p[3], const double q[3],
+                         const double pt0[3], const double nr0[3], const double B[2][3],
+                         int n_newton, int n_backtrack, double v[2]) {
+  double F[2], J[2][2], s[3], lam, c1, nu;
   flat_eval(fs, p, q, pt0, nr0, B, v, true, F, J, s, lam, c1, nu);
   for (int it = 0; it < n_newton; ++it) {
     const double m0 = F[0] * F[0] + F[1] * F[1];
@@ -562,19 +651,100 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, 
         F[1] = Ft[1];
         for (int r = 0; r < 2; ++r)
           for (int c = 0; c < 2; ++c) J[r][c] = Jt[r][c];
-        for (int c = 0; c < 3; ++c) s[c] = stt[c];
-        lam = lt;
-        c1 = c1t;
-        nu = nut;
         taken = true;
       }
     }
     if (!taken) break;
   }
-  // the polyline from v: validity and sums as joint_path
+}
+
+// The polyline of slowness v in one pass over the layers (float64): the
+// joint path's validity and sums, the surface residual, cur and x_first.
+// Segment i (medium i above interface i): the air leg |s - p|, layer 1 the
+// climb |lam| n_1, layer k >= 2 |d_k| n_k / c_k (polyline lengths also where
+// an unsolved lane left a negative height: the dropped power of invalid
+// lanes is booked from them).  At a horizontal plane the incidence and
+// transmission cosines are c_k/n_k and c_{k+1}/n_{k+1}, so the TE
+// coefficient is (c_k - c_{k+1})/(c_k + c_{k+1}) (the reference's formula
+// with its clamp where c_k/n_k < C_MIN).
+SS_HD double flat_final(const FlatStack& fs, const double p[3], const double q[3],
+                        const double pt0[3], const double nr0[3], const double v[2],
+                        PathSums& ps) {
+  const RefrConst& rc = *fs.rc;
+  const int NA = rc.n_crossed, kp = fs.kp, N = fs.N;
+  const double* nl = fs.n;
+  const double* epsl = rc.legs + (NA + 1) + kp;
+  const double* attl = rc.legs + 2 * (NA + 1) + kp;
+  const bool rough = (rc.flags & SS_CROSS_ROUGH) != 0;
   const double vv = v[0] * v[0] + v[1] * v[1];
   const double vm = sqrt(vv);
-  bool sides = true, evan = false, fin = isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]);
+  bool sides = true, evan = false, fin = true;
+  double tau2 = 1.0, opl = 0.0, loss = 0.0, perp = 0.0, par = 0.0;
+  // interface i from medium i to i + 1 with cosines ci_raw -> ct_raw
+  auto cross = [&](int i, double ci_raw, double ct_raw, bool flat) {
+    const double ci = fmax(ci_raw, kCMin);
+    double gm;
+    if (flat && ci_raw >= kCMin) {
+      const double a0 = nl[i] * ci_raw, a1 = nl[i + 1] * ct_raw;   // c_i, c_{i+1}
+      gm = (a0 - a1) / (a0 + a1);
+    } else {
+      const double e1 = epsl[i], e2 = epsl[i + 1];
+      const double sin2 = (e1 / e2) * (1.0 - ci * ci);
+      const double cc2 = sqrt(fmax(1.0 - sin2, 0.0));
+      gm = (nl[i] * ci - nl[i + 1] * cc2) / (nl[i] * ci + nl[i + 1] * cc2);
+    }
+    tau2 *= (1.0 - gm * gm);
+    if (rough) {
+      const double kt = rc.k0 * (nl[i] * ci_raw - nl[i + 1] * ct_raw);
+      const double sk = rc.desc[(kp + i) * SS_IDESC_COUNT + SS_IDESC_SIGC] * kt;
+      tau2 *= exp(-2.0 * (sk * sk));
+    }
+    return ci;
+  };
+  const double c12 = nl[1] * nl[1] - vv;
+  const double c1 = sqrt(c12);
+  fin = fin && isfinite(c1);
+  // layers 2..N: x_top offset, the segments above planes 2..N-1 and the
+  // interfaces 1..N-1
+  double a = 0.0, c_prev = c1, c_last = 0.0;
+  for (int k = 2; k <= N; ++k) {
+    const double ck = sqrt(nl[k] * nl[k] - vv);
+    const double rk = 1.0 / ck;
+    const double dk = fs.d(k, q[2]);
+    a += dk * rk;
+    fin = fin && isfinite(rk);
+    sides = sides && (dk > 0.0);
+    evan = evan || (vm > nl[k] * (1.0 + 64.0 * kEps64));
+    // interface k-1 (medium k-1 -> k)
+    cross(k - 1, c_prev / nl[k - 1], ck / nl[k], true);
+    if (k < N) {
+      // segment k: length |d_k| n_k / c_k; incidence cosine c_k / n_k
+      const double ad = fabs(dk);
+      const double len = ad * nl[k] * rk;
+      const double cik = ck / nl[k];
+      const double ci = fmax(cik, kCMin);
+      opl += nl[k] * len;
+      loss += len * (attl[k] / 1000.0);
+      perp += ad * rk;
+      par += len / (nl[k] * ci * ci);
+    }
+    c_prev = ck;
+    c_last = ck;
+  }
+  double xt[3] = {q[0], q[1], q[2]};
+  if (N > 1) {
+    xt[0] = q[0] - a * v[0];
+    xt[1] = q[1] - a * v[1];
+    xt[2] = flat_z(rc, kp + 1);
+  }
+  const double u[3] = {-v[0], -v[1], c1};
+  const double nu = nr0[0] * u[0] + nr0[1] * u[1] + nr0[2] * u[2];
+  const double lam =
+      (nr0[0] * (pt0[0] - xt[0]) + nr0[1] * (pt0[1] - xt[1]) + nr0[2] * (pt0[2] - xt[2])) / nu;
+  double s[3];
+  for (int c = 0; c < 3; ++c) s[c] = xt[c] + lam * u[c];
+  fin = fin && isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]);
+  // pads: straddled by p and s; x_first on the top pad
   for (int i = 0; i < kp; ++i) {
     const double zp = rc.desc[i * SS_IDESC_COUNT + SS_IDESC_MP + 2];
     sides = sides && ((p[2] - zp) * (s[2] - zp) < 0.0);
@@ -582,12 +752,12 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, 
   if (kp > 0) {
     const double zp = rc.desc[SS_IDESC_MP + 2];
     const double f = (p[2] - zp) / (p[2] - s[2]);
-    for (int c = 0; c < 3; ++c) x_first[c] = p[c] + f * (s[c] - p[c]);
-    x_first[2] = zp;
+    for (int c = 0; c < 3; ++c) ps.x_first[c] = p[c] + f * (s[c] - p[c]);
+    ps.x_first[2] = zp;
   } else {
-    for (int c = 0; c < 3; ++c) x_first[c] = s[c];
+    for (int c = 0; c < 3; ++c) ps.x_first[c] = s[c];
   }
-  // surface: incidence from the air leg, refraction into layer 1
+  // surface: the honest residual from the air leg and the layer-1 direction
   const double ds[3] = {s[0] - p[0], s[1] - p[1], s[2] - p[2]};
   const double sa = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
   const double sam = sa > kTiny ? sa : kTiny;
@@ -602,65 +772,174 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc, 
   const double nd = nr0[0] * dG[0] + nr0[1] * dG[1] + nr0[2] * dG[2];
   const double rt[3] = {dG[0] - nd * nr0[0], dG[1] - nd * nr0[1], dG[2] - nd * nr0[2]};
   const double resid = sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
-  fin = fin && isfinite(resid) && isfinite(sa) && isfinite(c1);
-  sides = sides && ((nr0[0] * (p[0] - s[0]) + nr0[1] * (p[1] - s[1]) + nr0[2] * (p[2] - s[2])) *
-                        (-lam * nu) < 0.0);
-  evan = nl[0] * st1 > nl[1] * (1.0 + 64.0 * kEps64);
-  const bool rough = (rc.flags & SS_CROSS_ROUGH) != 0;
-  // interface i: TE Fresnel at incidence cosine ci (medium i -> i + 1) with
-  // transmission cosine ct (the crossing-roughness term uses both)
-  auto cross = [&](int i, double ci_raw, double ct_raw) {
-    const double ci = fmax(ci_raw, kCMin);
-    const double e1 = epsl[i], e2 = epsl[i + 1];
-    const double sin2 = (e1 / e2) * (1.0 - ci * ci);
-    const double cc2 = sqrt(fmax(1.0 - sin2, 0.0));
-    const double gm = (nl[i] * ci - nl[i + 1] * cc2) / (nl[i] * ci + nl[i + 1] * cc2);
-    tau2 *= (1.0 - gm * gm);
-    if (rough) {
-      const double kt = rc.k0 * (nl[i] * ci_raw - nl[i + 1] * ct_raw);
-      const double sk = rc.desc[(kp + i) * SS_IDESC_COUNT + SS_IDESC_SIGC] * kt;
-      tau2 *= exp(-2.0 * (sk * sk));
-    }
-    return ci;
-  };
-  tau2 = 1.0;
-  const double cs_raw = fabs(ct1);
-  const double c_surf = cross(0, cs_raw, fabs(ct2));
-  opl = nl[0] * sa;
-  loss_db = sa * (attl[0] / 1000.0);
-  sum_perp = sa / nl[0];
-  sum_par = sa / (nl[0] * c_surf * c_surf);
-  double ct_last = fabs(ct2);
-  double ck = c1;  // c of the current layer
-  for (int i = 1; i < N; ++i) {
-    // layer i above plane i: height h (layer 1: the climb lam c_1), segment
-    // length |h| n_i / c_i (a polyline length also where an unsolved lane
-    // left h < 0: the dropped power of invalid lanes is booked from it)
-    const double h = i == 1 ? lam * c1 : fs.d(i, q[2]);
-    const double ck1 = sqrt(nl[i + 1] * nl[i + 1] - vv);
-    sides = sides && (h > 0.0) && (fs.d(i + 1, q[2]) > 0.0);
-    evan = evan || (vm > nl[i + 1] * (1.0 + 64.0 * kEps64));
-    fin = fin && isfinite(ck1);
-    const double len = i == 1 ? fabs(lam) * nl[1] : fabs(h) * nl[i] / ck;
-    const double ci = cross(i, ck / nl[i], ck1 / nl[i + 1]);
-    opl += nl[i] * len;
-    loss_db += len * (attl[i] / 1000.0);
-    sum_perp += len / nl[i];
-    sum_par += len / (nl[i] * ci * ci);
-    ct_last = ck1 / nl[i + 1];
-    ck = ck1;
-  }
-  valid = (resid <= 1e-9) && fin && sides && !evan;
-  c_first = c_surf;
-  c_last = fmax(ct_last, kCMin);
+  fin = fin && isfinite(resid) && isfinite(sa);
+  sides = sides &&
+          ((nr0[0] * (p[0] - s[0]) + nr0[1] * (p[1] - s[1]) + nr0[2] * (p[2] - s[2])) *
+               (-lam * nu) < 0.0);
+  evan = evan || (nl[0] * st1 > nl[1] * (1.0 + 64.0 * kEps64));
+  const double c_surf = cross(0, fabs(ct1), fabs(ct2), false);
+  opl += nl[0] * sa;
+  loss += sa * (attl[0] / 1000.0);
+  perp += sa / nl[0];
+  par += sa / (nl[0] * c_surf * c_surf);
   if (N > 1) {
+    // layer 1: the climb
+    const double len = fabs(lam) * nl[1];
+    const double ci = fmax(c1 / nl[1], kCMin);
+    opl += nl[1] * len;
+    loss += len * (attl[1] / 1000.0);
+    perp += len / nl[1];
+    par += len / (nl[1] * ci * ci);
     const double dN = fs.d(N, q[2]);
-    cur[0] = q[0] - dN * v[0] / ck;
-    cur[1] = q[1] - dN * v[1] / ck;
-    cur[2] = flat_z(rc, kp + N - 1);
+    ps.cur[0] = q[0] - dN * v[0] / c_last;
+    ps.cur[1] = q[1] - dN * v[1] / c_last;
+    ps.cur[2] = flat_z(rc, kp + N - 1);
+    ps.c_last = fmax(c_last / nl[N], kCMin);
   } else {
-    for (int c = 0; c < 3; ++c) cur[c] = s[c];
+    for (int c = 0; c < 3; ++c) ps.cur[c] = s[c];
+    ps.c_last = fmax(fabs(ct2), kCMin);
   }
+  ps.valid = (resid <= 1e-9) && fin && sides && !evan;
+  ps.opl = opl;
+  ps.loss_db = loss;
+  ps.sum_par = par;
+  ps.sum_perp = perp;
+  ps.tau2 = tau2;
+  ps.c_first = c_surf;
+  return resid;
+}
+
+SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc,
+                     PathSums& ps) {
+  FlatStack fs;
+  fs.rc = &rc;
+  fs.kp = rc.k_pad;
+  fs.N = rc.n_crossed - rc.k_pad;
+  fs.n = rc.legs + rc.k_pad;
+  double pt0[3], nr0[3], s0[3], B[2][3], v[2];
+  flat_anchor(p, q, rc, fs, false, pt0, nr0, s0);
+  plane_basis(nr0, B);
+  flat_start(p, s0, nr0, fs.n, v);
+  flat_newton64(fs, p, q, pt0, nr0, B, rc.n_newton, rc.n_backtrack, v);
+  flat_final(fs, p, q, pt0, nr0, v, ps);
+}
+
+// Mixed tier: float32 anchor, float32 Newton in coordinates relative to s0,
+// one float64 Newton step, the float64 polyline.  Returns false (float64
+// redo of the pair) unless the float32 phase took only full steps and the
+// float64 result is valid with a surface residual <= 1e-12 -- the cases
+// where the float64 solve reaches the same stationary point -- or the lane
+// is stuck: no first step is feasible, its full step and every halving the
+// float64 solve would try being clearly infeasible, so that solve stays at
+// the start (the lane has no valid path: the anchored surface plane passes
+// below the first horizontal plane) and the lane is booked from the start
+// path.  One float32 evaluation site and one final pass keep the registers
+// of the float64 tier's kernel.
+SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst& rc,
+                           PathSums& ps) {
+  FlatStack fs;
+  fs.rc = &rc;
+  fs.kp = rc.k_pad;
+  fs.N = rc.n_crossed - rc.k_pad;
+  fs.n = rc.legs + rc.k_pad;
+  double pt0[3], nr0[3], s0[3], B[2][3], v[2];
+  flat_anchor(p, q, rc, fs, true, pt0, nr0, s0);
+  plane_basis(nr0, B);
+  flat_start(p, s0, nr0, fs.n, v);
+  bool stuck = false;
+  {
+    float P[3], Q[3], O[3], nr[3], Bf[2][3];
+    for (int c = 0; c < 3; ++c) {
+      P[c] = (float)(p[c] - s0[c]);
+      Q[c] = (float)(q[c] - s0[c]);
+      O[c] = (float)(pt0[c] - s0[c]);
+      nr[c] = (float)nr0[c];
+      Bf[0][c] = (float)B[0][c];
+      Bf[1][c] = (float)B[1][c];
+    }
+    const float zt = fs.N > 1 ? (float)(flat_z(rc, fs.kp + 1) - s0[2]) : Q[2];
+    const float dN = fs.N > 1 ? (float)fs.d(fs.N, q[2]) : 0.0f;
+    const float lm_min = ldexpf(1.0f, 1 - rc.n_backtrack);
+    float vf[2] = {(float)v[0], (float)v[1]}, vt[2] = {vf[0], vf[1]};
+    float F[2] = {0.0f, 0.0f}, J[2][2] = {{1.0f, 0.0f}, {0.0f, 1.0f}}, st[2] = {0.0f, 0.0f};
+    float m0 = 0.0f, lm = 1.0f;
+    int steps = 0;
+    bool base = true, halving = false, ok = false, fail = false;
+    for (int ev = 0; ev < 40; ++ev) {
+      float Ft[2], Jt[2][2];
+      const int ft = flat_eval32(fs, P, Q, O, nr, Bf, zt, dN, vt, Ft, Jt);
+      const float mt = Ft[0] * Ft[0] + Ft[1] * Ft[1];
+      bool take = false;
+      if (base) {
+        take = true;
+        ok = ft == 0;
+        base = false;
+      } else if (halving) {
+        // the float64 solve's halvings of the first step, all clearly infeasible?
+        if (ft != 1) {
+          fail = true;
+          break;
+        }
+        if (lm <= lm_min) {
+          stuck = true;
+          break;
+        }
+        lm *= 0.5f;
+        vt[0] = vf[0] - lm * st[0];
+        vt[1] = vf[1] - lm * st[1];
+        continue;
+      } else if (ft == 0 && mt < m0) {
+        take = true;
+        ok = true;
+        ++steps;
+      } else if (!(m0 > 1e-10f)) {
+        break;  // at the float32 floor the phase just ends
+      } else if (steps == 0 && ft == 1) {
+        if (rc.n_backtrack <= 1) {
+          stuck = true;
+          break;
+        }
+        halving = true;
+        lm = 0.5f;
+        vt[0] = vf[0] - lm * st[0];
+        vt[1] = vf[1] - lm * st[1];
+        continue;
+      } else {
+        fail = true;  // a damped step: the float64 solve decides
+        break;
+      }
+      if (take) {
+        vf[0] = vt[0];
+        vf[1] = vt[1];
+        F[0] = Ft[0];
+        F[1] = Ft[1];
+        for (int r = 0; r < 2; ++r)
+          for (int c = 0; c < 2; ++c) J[r][c] = Jt[r][c];
+      }
+      m0 = F[0] * F[0] + F[1] * F[1];
+      if (!(m0 > 1e-13f) || steps == 4) break;
+      const float idet = f_rcp(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+      st[0] = (J[1][1] * F[0] - J[0][1] * F[1]) * idet;
+      st[1] = (J[0][0] * F[1] - J[1][0] * F[0]) * idet;
+      vt[0] = vf[0] - st[0];
+      vt[1] = vf[1] - st[1];
+    }
+    if (fail || (!stuck && !ok)) return false;
+    if (!stuck) {
+      v[0] = vf[0];
+      v[1] = vf[1];
+    }
+  }
+  if (!stuck) {
+    // one float64 Newton step
+    double F[2], J[2][2], s[3], lam, c1, nu;
+    if (!flat_eval(fs, p, q, pt0, nr0, B, v, true, F, J, s, lam, c1, nu)) return false;
+    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    v[0] -= (J[1][1] * F[0] - J[0][1] * F[1]) / det;
+    v[1] -= (J[0][0] * F[1] - J[1][0] * F[0]) / det;
+  }
+  const double resid = flat_final(fs, p, q, pt0, nr0, v, ps);
+  return stuck ? !ps.valid : (ps.valid && resid <= 1e-12);
 }
 
 // Top-level refracted pair: crossing solver (NMAX < 0: the flat-stack joint
@@ -670,8 +949,7 @@ template <int NMAX, typename FT>
 SS_HD void joint_sums(const FT& f, const TraceRec& t, const RefrConst& rc,
                       PathSums& ps) {
   if constexpr (NMAX < 0) {
-    flat_path(t.p, f.c, rc, rc.n_newton, rc.n_backtrack, ps.cur, ps.valid, ps.opl,
-              ps.loss_db, ps.sum_par, ps.sum_perp, ps.tau2, ps.c_first, ps.c_last, ps.x_first);
+    flat_path(t.p, f.c, rc, ps);
   } else {
     joint_path<NMAX>(t.p, f.c, rc, rc.k_pad, rc.n_newton, rc.n_backtrack, ps.cur, ps.valid,
                      ps.opl, ps.loss_db, ps.sum_par, ps.sum_perp, ps.tau2, ps.c_first,
@@ -699,7 +977,7 @@ SS_HD bool refracted_pair_mixed(const FT& f, const TraceRec& t, const RefrConst&
                                 PairOut& o) {
   PathSums ps;
   if constexpr (NMAX < 0) {
-    joint_sums<-1>(f, t, rc, ps);
+    if (!flat_path_mixed(t.p, f.c, rc, ps)) return false;
   } else {
     if (NMAX > 0 && rc.joint)
       joint_sums<(NMAX > 0 ? NMAX : 1)>(f, t, rc, ps);

@@ -460,10 +460,9 @@ __global__ void fold_kernel(double* field, const double* traces, long ts, long n
 // NMAX = 0: sequential chain only; NMAX > 0: joint solve for stacks <= NMAX;
 // NMAX < 0: flat-stack joint solve.
 // The mixed instantiations leave unresolved pairs to redo_kernel (queue rd).
-template <int PREC, int NMAX, int RM, int PAT>
-__global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc, Redo rd) {
-  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
-  __shared__ Shared<ACC, kTTRefr> sh;
+template <int PREC, int NMAX, int RM, int PAT, typename ACC>
+__device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k,
+                                      const RefrConst& rc, const Redo& rd) {
   const long w = work_id(k);
   if (w < 0) return;
   const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
@@ -497,6 +496,27 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
   }
 }
 
+template <int PREC, int NMAX, int RM, int PAT>
+__global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc, Redo rd) {
+  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  __shared__ Shared<ACC, kTTRefr> sh;
+  refracted_body<PREC, NMAX, RM, PAT, ACC>(sh, k, rc, rd);
+}
+
+// The mixed tier of the flat-stack joint solve (refracted_body with
+// NMAX < 0; the float64 tier is refracted_kernel<FP64, -1, ...>).  The
+// smooth instantiations are held to 3 waves per SIMD like the sequential
+// mixed kernel (the float32 Newton loop, the float64 step and the float64
+// pass would otherwise settle just past 168 VGPRs), the generic (rough)
+// ones to 2.
+template <int PREC, int RM, int PAT, int WAVES = (RM == 0 ? 3 : 2)>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void flat_kernel(
+    KCommon k, RefrConst rc, Redo rd) {
+  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  __shared__ Shared<ACC, kTTRefr> sh;
+  refracted_body<PREC, -1, RM, PAT, ACC>(sh, k, rc, rd);
+}
+
 // Runs ITEM(w) for every flagged work item w < n_items: each block takes
 // chunks of kBlock items, one flag per lane, collects the waves' ballots in
 // LDS and walks the set bits in order (a uniform loop; ITEM may contain
@@ -522,31 +542,104 @@ __device__ inline void for_flagged_items(const unsigned char* __restrict__ flags
   }
 }
 
-// Float64 pass over the work items the mixed refracted_kernel flagged: the
-// float64 trace loop of refracted_kernel, masked to the recorded lanes.
-// Grid-stride over work items (the flag test is uniform per block).
+// Books one pair's output straight into global memory, exactly as
+// trace_loop books it through its LDS window: a valid in-window part to its
+// field bin (per side), a valid out-of-window part's power and an invalid
+// pair's power to dropped; the raw bin to pair_bins.
+__device__ inline void book_pair_global(const KCommon& k, const PairOut& o, long t, long f,
+                                        int n_samples, bool split) {
+  if (!o.valid) {
+    if (o.pwr_invalid != 0.0) atomicAdd(&k.dropped[t], o.pwr_invalid);
+  } else {
+    for (int p = 0; p < 2; ++p) {
+      if (p >= o.nparts) break;
+      const int b = o.bin[p];
+      const double re = o.re[p], im = o.im[p];
+      if (b >= 0 && b < n_samples) {
+        const int side = split ? o.right : 0;
+        const long idx = (t * k.n_seg + (long)side * n_samples + b) * 2;
+        if (re != 0.0) atomicAdd(&k.field[idx], re);
+        if (im != 0.0) atomicAdd(&k.field[idx + 1], im);
+      } else {
+        const double pw = re * re + im * im;
+        if (pw != 0.0) atomicAdd(&k.dropped[t], pw);
+      }
+    }
+  }
+  if (k.pair_bins) k.pair_bins[t * k.nf + f] = o.raw_bin;
+}
+
+// Float64 pass over the pairs the mixed refracted_kernel flagged, compacted:
+// each block takes chunks of kBlock work items, counts the set bits of the
+// flagged items' lane masks, scans the counts in LDS and hands the chunk's
+// pairs out one per lane (item by binary search over the scan, pair by
+// rank among the item's set bits, masks in (trace, wave) order).  A pair is
+// evaluated with its facet and trace read from global memory and booked
+// with global atomics, so the cost follows the number of flagged pairs, not
+// of the items holding them.
 template <int NMAX, int RM, int PAT>
 __global__ __launch_bounds__(kBlock) void redo_kernel(KCommon k, RefrConst rc, Redo rd) {
-  __shared__ Shared<double, kTTRefr> sh;
-  __shared__ unsigned long long sh_mask[kBlock / 64];
+  constexpr int M = kTTRefr * 4;  // masks per work item
+  __shared__ int sh_cnt[kBlock + 1];
   const long n_items = (long)k.n_ftiles * k.n_ttiles;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
   const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
-  for_flagged_items(rd.flags, n_items, sh_mask, [&](long w) {
-    const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
-    const bool live = f < k.nf;
-    RefrFacet fr;
-    load_refr_facet(k, live ? f : 0, rc.flags, fr);
-    const unsigned long long* m = rd.masks + w * (kTTRefr * 4);
-    trace_loop<double, kTTRefr>(sh, k, w, live, f, rc.n_samples, split,
-                                [&](const TraceRec& t, PairOut& o, int it) {
-                                  if (((m[it * 4 + wave] >> lane) & 1ULL) == 0ULL) {
-                                    o.redo = true;  // resolved by the mixed pass
-                                    return;
-                                  }
-                                  refracted_pair_fp64<NMAX, RM, PAT>(fr, t, rc, o);
-                                });
-  });
+  for (long base = (long)blockIdx.x * kBlock; base < n_items; base += (long)gridDim.x * kBlock) {
+    const long w = base + tid;
+    int c = 0;
+    if (w < n_items && rd.flags[w] != 0)
+      for (int i = 0; i < M; ++i) c += __popcll(rd.masks[w * M + i]);
+    if (tid == 0) sh_cnt[0] = 0;
+    sh_cnt[tid + 1] = c;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {  // inclusive scan of sh_cnt[1..kBlock]
+      const int v = tid >= off ? sh_cnt[tid + 1 - off] : 0;
+      __syncthreads();
+      sh_cnt[tid + 1] += v;
+      __syncthreads();
+    }
+    const int total = uniform_i(sh_cnt[kBlock]);
+    for (int r0 = 0; r0 < total; r0 += kBlock) {
+      const int g = r0 + tid;
+      if (g < total) {
+        int lo = 0, hi = kBlock - 1;  // last item whose first pair is <= g
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sh_cnt[mid] <= g)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        const long wi = base + lo;
+        int rank = g - sh_cnt[lo];
+        int mi = 0;
+        unsigned long long m = 0ULL;
+        for (; mi < M; ++mi) {
+          m = rd.masks[wi * M + mi];
+          const int pc = __popcll(m);
+          if (rank < pc) break;
+          rank -= pc;
+        }
+        for (int j = 0; j < rank; ++j) m &= m - 1ULL;
+        const int bit = __builtin_ctzll(m);
+        const long f = (wi % k.n_ftiles) * kBlock + (mi & 3) * 64 + bit;
+        const long t = (wi / k.n_ftiles) * kTTRefr + (mi >> 2);
+        RefrFacet fr;
+        load_refr_facet(k, f, rc.flags, fr);
+        TraceRec tr;
+        for (int r = 0; r < SS_TROW_COUNT; ++r)
+          reinterpret_cast<double*>(&tr)[r] = k.traces[(long)r * k.ts + t];
+        PairOut o;
+        o.nparts = 0;
+        o.valid = false;
+        o.redo = false;
+        o.pwr_invalid = 0.0;
+        refracted_pair_fp64<NMAX, RM, PAT>(fr, tr, rc, o);
+        book_pair_global(k, o, t, f, rc.n_samples, split);
+      }
+    }
+    __syncthreads();  // sh_cnt is rewritten by the next chunk
+  }
 }
 
 __global__ void reset_workspace_kernel(unsigned long long* hdr) {
@@ -679,9 +772,11 @@ __device__ inline KCommon job_common(const ss_job& jb) {
 // (0) or the refracted (1) jobs of the table, so each instantiation carries
 // one pair function's registers.  The mixed instantiations are held to 3
 // waves per SIMD (168 VGPRs), where the single-launch kernels sit; the
-// job lookup would otherwise push them just past it.
+// job lookup would otherwise push them just past it.  The smooth float64
+// ones are held to 2 (the single-launch kernels' occupancy; a few spilled
+// dwords), the rough float64 ones keep 1 (2 would spill 100-200 dwords).
 template <int PREC, int TT, int RM, int PAT, int KIND,
-          int WAVES = (PREC == SS_PREC_MIXED ? 3 : 1)>
+          int WAVES = (PREC == SS_PREC_MIXED ? 3 : (RM == 0 ? 2 : 1))>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void batch_kernel(BatchArgs ba,
                                                       unsigned long long* counter, Redo rd) {
   using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
@@ -1016,6 +1111,108 @@ static KCommon make_common(const double* facets, int64_t fs, int64_t nf,
 
 using namespace ss;
 
+// ---------------------------------------------------------- scene builder
+// Facets of a regular vertex lattice built in HBM (the corner math of
+// scene.py:158-176 / compare/brute_force_layered.py:44-71, float64 in the
+// host restatement's operation order): a (ny+1, nx+1) height lattice at
+// `spacing` centred on (x0, y0) (x along columns, y along rows).  Facet
+// order: 16x16-cell tiles, tiles row-major, cells row-major in a tile, so a
+// 256-facet workgroup tile is one compact patch (edge tiles narrower).
+constexpr int kTileSide = 16;
+constexpr int kStatCount = 10;  // sum A, sum A c (3), sum A n (3), sum c (3)
+
+__host__ __device__ inline long tile_position(long r, long c, long ny, long nx) {
+  const long tr = r / kTileSide, tc = c / kTileSide;
+  const long h = (ny - tr * kTileSide) < kTileSide ? (ny - tr * kTileSide) : kTileSide;
+  const long w = (nx - tc * kTileSide) < kTileSide ? (nx - tc * kTileSide) : kTileSide;
+  return tr * kTileSide * nx + tc * kTileSide * h + (r % kTileSide) * w + (c % kTileSide);
+}
+
+__global__ __launch_bounds__(kBlock) void facets_from_heights_kernel(
+    const double* __restrict__ heights, long ny, long nx, double spacing, double x0, double y0,
+    double* __restrict__ facets, long fs, double* __restrict__ table,
+    double* __restrict__ partials) {
+#pragma clang fp contract(off)
+  __shared__ double red[kStatCount][kBlock];
+  const long n = ny * nx;
+  const long cell = (long)blockIdx.x * kBlock + threadIdx.x;
+  double st[kStatCount];
+  for (int i = 0; i < kStatCount; ++i) st[i] = 0.0;
+  if (cell < n) {
+    const long r = cell / nx, c = cell - r * nx;
+    const long nx1 = nx + 1, ny1 = ny + 1;
+    const double hx = (double)(nx1 - 1) / 2.0, hy = (double)(ny1 - 1) / 2.0;
+    const double xa = ((double)c - hx) * spacing + x0, xb = ((double)(c + 1) - hx) * spacing + x0;
+    const double ya = ((double)r - hy) * spacing + y0, yb = ((double)(r + 1) - hy) * spacing + y0;
+    const double v00[3] = {xa, ya, heights[r * nx1 + c]};
+    const double v01[3] = {xb, ya, heights[r * nx1 + c + 1]};
+    const double v10[3] = {xa, yb, heights[(r + 1) * nx1 + c]};
+    const double v11[3] = {xb, yb, heights[(r + 1) * nx1 + c + 1]};
+    double e1[3], e2[3], ce[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = ((v01[k] + v11[k]) - (v00[k] + v10[k])) / 2.0;
+      e2[k] = ((v10[k] + v11[k]) - (v00[k] + v01[k])) / 2.0;
+      ce[k] = (((v00[k] + v01[k]) + v10[k]) + v11[k]) / 4.0;
+    }
+    const double raw[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                           e1[0] * e2[1] - e1[1] * e2[0]};
+    const double mag = sqrt((raw[0] * raw[0] + raw[1] * raw[1]) + raw[2] * raw[2]);
+    double nr[3] = {raw[0] / mag, raw[1] / mag, raw[2] / mag};
+    const double sg = nr[2] < 0.0 ? -1.0 : 1.0;
+    for (int k = 0; k < 3; ++k) nr[k] = nr[k] * sg;
+    const long pos = tile_position(r, c, ny, nx);
+    for (int k = 0; k < 3; ++k) {
+      facets[(SS_FROW_CX + k) * fs + pos] = ce[k];
+      facets[(SS_FROW_NX + k) * fs + pos] = nr[k];
+      facets[(SS_FROW_E1X + k) * fs + pos] = e1[k];
+      facets[(SS_FROW_E2X + k) * fs + pos] = e2[k];
+    }
+    facets[SS_FROW_AREA * fs + pos] = mag;
+    if (table) {
+      for (int k = 0; k < 3; ++k) {
+        table[cell * 6 + k] = ce[k];
+        table[cell * 6 + 3 + k] = nr[k];
+      }
+    }
+    st[0] = mag;
+    for (int k = 0; k < 3; ++k) {
+      st[1 + k] = mag * ce[k];
+      st[4 + k] = mag * nr[k];
+      st[7 + k] = ce[k];
+    }
+  }
+  // block sums in a fixed tree order (deterministic)
+  for (int i = 0; i < kStatCount; ++i) red[i][threadIdx.x] = st[i];
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int i = 0; i < kStatCount; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + h];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < kStatCount)
+    partials[(long)blockIdx.x * kStatCount + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Sums the per-block partials in a fixed order: thread i strides the
+// blocks, then a tree over the threads.
+__global__ __launch_bounds__(kBlock) void facet_stats_kernel(const double* __restrict__ partials,
+                                                             long n_blocks,
+                                                             double* __restrict__ stats) {
+  __shared__ double red[kStatCount][kBlock];
+  double st[kStatCount];
+  for (int i = 0; i < kStatCount; ++i) st[i] = 0.0;
+  for (long b = threadIdx.x; b < n_blocks; b += kBlock)
+    for (int i = 0; i < kStatCount; ++i) st[i] += partials[b * kStatCount + i];
+  for (int i = 0; i < kStatCount; ++i) red[i][threadIdx.x] = st[i];
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int i = 0; i < kStatCount; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + h];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < kStatCount) stats[threadIdx.x] = red[threadIdx.x][0];
+}
+
 extern "C" {
 
 const char* ss_version(void) { return "soundersim-hip 0.1 (gfx950)"; }
@@ -1211,15 +1408,22 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
 #define SS_REFR_PA(R) \
   if (pm == 0) { SS_BOTH(0, R, 0); } else if (pm == 5) { SS_BOTH(0, R, 5); } \
   else { SS_BOTH(0, R, -1); }
+#define SS_FLAT(R, A)                                                                         \
+  if (fp64) {                                                                                 \
+    SS_REFR(SS_PREC_FP64, -1, R, A);                                                          \
+  } else {                                                                                    \
+    hipLaunchKernelGGL((flat_kernel<SS_PREC_MIXED, R, A>), grid, block, 0, st, k, rc, rd);    \
+    SS_REDO(-1, R, A);                                                                        \
+  }
   if (prm->refraction == 2) {
     // flat stacks: the multilayer scenes' smooth interfaces under a
     // tabulated (or no) pattern get their own instantiations
     if (rm == 0 && pm == 5) {
-      SS_BOTH(-1, 0, 5);
+      SS_FLAT(0, 5);
     } else if (rm == 0 && pm == 0) {
-      SS_BOTH(-1, 0, 0);
+      SS_FLAT(0, 0);
     } else {
-      SS_BOTH(-1, -1, -1);
+      SS_FLAT(-1, -1);
     }
   } else if (!rc.joint) {
     if (rm == 0) {
@@ -1234,6 +1438,7 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   } else {
     SS_BOTH(SS_MAX_JOINT, -1, -1);
   }
+#undef SS_FLAT
 #undef SS_REFR_PA
 #undef SS_BOTH
 #undef SS_REDO
@@ -1509,6 +1714,39 @@ int ss_batch_step(int precision, const void* table_device, void* workspace,
                (hipStream_t)stream);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("batch step launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int64_t ss_facets_workspace_bytes(int64_t ny, int64_t nx) {
+  if (ny < 1 || nx < 1) return -1;
+  const int64_t blocks = (ny * nx + kBlock - 1) / kBlock;
+  return blocks * kStatCount * (int64_t)sizeof(double);
+}
+
+int ss_facets_from_heights(const double* heights, int64_t ny, int64_t nx, double spacing,
+                           double x0, double y0, double* facets, int64_t facet_stride,
+                           double* table, double* stats, void* workspace,
+                           int64_t workspace_bytes, void* stream) {
+  if (ny < 1 || nx < 1) return fail(SS_EINVAL, "need at least one cell (ny, nx >= 1)");
+  if (ny > (1L << 20) || nx > (1L << 20) || ny * nx > (1L << 31))
+    return fail(SS_ERANGE, "lattice too large (ny * nx <= 2^31)");
+  if (!(spacing > 0.0)) return fail(SS_EINVAL, "spacing must be > 0");
+  if (!heights || !facets || !stats || !workspace) return fail(SS_EINVAL, "null pointer");
+  if (facet_stride < ny * nx) return fail(SS_EINVAL, "facet_stride smaller than ny * nx");
+  if (workspace_bytes < ss_facets_workspace_bytes(ny, nx))
+    return fail(SS_EINVAL, "workspace smaller than ss_facets_workspace_bytes()");
+  if ((reinterpret_cast<uintptr_t>(workspace) & 7) != 0)
+    return fail(SS_EINVAL, "workspace must be 8-byte aligned");
+  const long blocks = (long)((ny * nx + kBlock - 1) / kBlock);
+  hipStream_t st = (hipStream_t)stream;
+  double* partials = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(facets_from_heights_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                     heights, (long)ny, (long)nx, spacing, x0, y0, facets, (long)facet_stride,
+                     table, partials);
+  hipLaunchKernelGGL(facet_stats_kernel, dim3(1), dim3(kBlock), 0, st, partials, blocks, stats);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("facet build launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
 }

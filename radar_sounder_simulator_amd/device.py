@@ -130,6 +130,119 @@ class DeviceFacets:
         return out
 
 
+def tile_order(ny, nx):
+    """Cell (grid-order index) at each facet position of the device-built
+    order: 16x16-cell tiles, tiles row-major, cells row-major in a tile."""
+    r, c = np.divmod(np.arange(ny * nx, dtype=np.int64), nx)
+    tr, tc = r // 16, c // 16
+    h = np.minimum(16, ny - tr * 16)
+    w = np.minimum(16, nx - tc * 16)
+    pos = tr * 16 * nx + tc * 16 * h + (r % 16) * w + (c % 16)
+    order = np.empty(ny * nx, np.int64)
+    order[pos] = np.arange(ny * nx, dtype=np.int64)
+    return order
+
+
+class BuiltLattice:
+    """A lattice tessellated on the device: its facets (``fac``), the
+    (sum A, sum A c, sum A n, sum c) statistics and, when the lattice is a
+    crossed interface, its lookup-table rows (``table_off`` into the shared
+    table; one row for a horizontal plane)."""
+
+    def __init__(self, lat, fac, stats, table_off, horizontal):
+        self.lattice, self.fac, self.stats = lat, fac, stats
+        self.table_off, self.horizontal = table_off, horizontal
+        ny, nx = lat.grid_shape
+        self.ny, self.nx = ny, nx
+        self.center_mean = stats[7:10] / (ny * nx)
+
+    def desc_row(self):
+        """Descriptor row (TABOFF/SIGC unset): grid shape, the exact affine
+        (x, y, 1) -> (row, col) map of the regular grid (what the reference's
+        least-squares fit recovers), the area-weighted mean plane."""
+        lat = self.lattice
+        d = np.zeros(IDESC["COUNT"], np.float64)
+        A = self.stats[0]
+        mp = self.stats[1:4] / A
+        mn = self.stats[4:7] / np.linalg.norm(self.stats[4:7])
+        if self.horizontal:
+            d[IDESC["NY"]] = d[IDESC["NX"]] = 1
+        else:
+            s = float(lat.spacing)
+            x0, y0 = map(float, lat.origin)
+            hy, hx = self.ny / 2.0, self.nx / 2.0
+            coef = np.array([[0.0, 1.0 / s], [1.0 / s, 0.0],
+                             [hy - 0.5 - y0 / s, hx - 0.5 - x0 / s]])
+            d[IDESC["NY"]], d[IDESC["NX"]] = self.ny, self.nx
+            d[IDESC["COEF"]:IDESC["COEF"] + 6] = coef.ravel()
+        d[IDESC["MP"]:IDESC["MP"] + 3] = mp
+        d[IDESC["MN"]:IDESC["MN"] + 3] = mn
+        return d
+
+
+def build_lattices(lattices, crossed, device=None, gammas=None, phasors=None,
+                   n_pad_rows=0, stream=None):
+    """Tessellate every distinct lattice (by identity) once on the device
+    (``ss_facets_from_heights``): facets for each, and for those in
+    ``crossed`` (a set of ids) their lookup rows in one shared table with
+    ``n_pad_rows`` spare rows at its end.  ``gammas``/``phasors``: optional
+    {id: per-cell array in grid order}.  Returns ({id: BuiltLattice}, table
+    tensor)."""
+    dev = _dev(device)
+    lib = _lib.load()
+    uniq = {}
+    for lat in lattices:
+        uniq.setdefault(id(lat), lat)
+    hz = {k: lat.horizontal() for k, lat in uniq.items()}
+    rows, offs = 0, {}
+    for k, lat in uniq.items():
+        if k in crossed:
+            offs[k] = rows
+            rows += 1 if hz[k] else len(lat)
+    table = torch.zeros((max(rows + n_pad_rows, 1), 6), dtype=torch.float64, device=dev)
+    st = _stream_handle(stream, dev)
+    gammas, phasors = gammas or {}, phasors or {}
+    out, pending = {}, []
+    for k, lat in uniq.items():
+        ny, nx = lat.grid_shape
+        n = ny * nx
+        h = torch.as_tensor(np.ascontiguousarray(lat.heights, np.float64), device=dev)
+        buf = torch.zeros((FROW["COUNT"], n), dtype=torch.float64, device=dev)
+        stats = torch.zeros(16, dtype=torch.float64, device=dev)
+        ws = torch.empty(max(int(lib.ss_facets_workspace_bytes(ny, nx)), 8),
+                         dtype=torch.uint8, device=dev)
+        tab = None
+        if k in crossed and not hz[k]:
+            tab = table[offs[k]:offs[k] + n]
+        rc = lib.ss_facets_from_heights(_ptr(h), ny, nx, float(lat.spacing),
+                                        float(lat.origin[0]), float(lat.origin[1]), _ptr(buf),
+                                        n, _ptr(tab), _ptr(stats), _ptr(ws), ws.numel(), st)
+        _lib.check(rc, "ss_facets_from_heights")
+        order = tile_order(ny, nx)
+        fac = object.__new__(DeviceFacets)
+        fac.n, fac.order, fac.device, fac.buf, fac.stride = n, order, dev, buf, n
+        fac.has_gamma = k in gammas
+        fac.has_phasors = k in phasors
+        fac.has_diffuse = False
+        if fac.has_gamma:
+            buf[13] = torch.as_tensor(np.asarray(gammas[k], np.float64)[order], device=dev)
+        if fac.has_phasors:
+            ph = np.asarray(phasors[k])[order]
+            buf[14] = torch.as_tensor(np.ascontiguousarray(ph.real), device=dev)
+            buf[15] = torch.as_tensor(np.ascontiguousarray(ph.imag), device=dev)
+        pending.append((k, lat, fac, stats, h, ws))
+    for k, lat, fac, stats, h, ws in pending:
+        s_host = stats[:10].cpu().numpy()        # waits for the builds (stream order)
+        out[k] = BuiltLattice(lat, fac, s_host, offs.get(k), hz[k])
+        if k in crossed and hz[k]:
+            ny, nx = lat.grid_shape
+            z = float(lat.heights.flat[0])
+            x = (0.5 - nx / 2.0) * lat.spacing + lat.origin[0]
+            y = (0.5 - ny / 2.0) * lat.spacing + lat.origin[1]
+            table[offs[k]] = torch.tensor([x, y, z, 0.0, 0.0, 1.0], dtype=torch.float64)
+    return out, table
+
+
 class DeviceTraces:
     """Per-trace records resident in HBM (rows SS_TROW_*)."""
 
@@ -285,6 +398,20 @@ def _iface_entry(f):
     return tab, desc, horizontal
 
 
+def _pad_planes_z(z_surf, n_real, pad_to, z_platform):
+    """``_pad_planes`` from the surface's highest point."""
+    k = 0 if pad_to is None else int(pad_to) - n_real
+    if k < 0:
+        raise ValueError(f"pad_to={pad_to} < {n_real} crossed interfaces")
+    if not k:
+        return []
+    gap = float(z_platform) - z_surf
+    if gap <= 0:
+        raise ValueError("joint refraction padding needs the platform above the "
+                         f"surface (gap {gap:.1f} m)")
+    return list(z_surf + gap * (np.arange(k, 0, -1) / (k + 1.0)))
+
+
 def _pad_planes(crossed, pad_to, z_platform):
     """z of the index-matched no-op planes the joint solver pads a stack
     with, spread through the air gap above the surface, top-down
@@ -318,6 +445,55 @@ class DeviceInterfaces:
         built = DeviceInterfaces.stacks([crossed], [crossed_sigma], [pad_to], z_platform,
                                         device)[0]
         self.__dict__.update(built.__dict__)
+
+    @staticmethod
+    def lattice_stacks(stacks, built, table, pad_row0, pad_tos=None, sigmas=None,
+                       z_platform=None, device=None):
+        """One DeviceInterfaces per crossed stack of Lattices tessellated by
+        ``build_lattices`` (``built``, ``table``): descriptors point into the
+        shared table; pads take rows from ``pad_row0`` on."""
+        device = _dev(device)
+        sigmas = sigmas or [None] * len(stacks)
+        pad_tos = pad_tos or [None] * len(stacks)
+        row = pad_row0
+        pad_rows = []
+        out = []
+        for crossed, sig, pad_to in zip(stacks, sigmas, pad_tos):
+            n_real = len(crossed)
+            if n_real < 1 or n_real > _lib.MAX_CROSSED:
+                raise ValueError(f"need 1..{_lib.MAX_CROSSED} crossed interfaces, got {n_real}")
+            h = crossed[0].heights
+            # the surface's highest facet centre (the reference pads from it)
+            z_top = float(((((h[:-1, :-1] + h[:-1, 1:]) + h[1:, :-1]) + h[1:, 1:]) / 4.0).max())
+            pads = _pad_planes_z(z_top, n_real, pad_to, z_platform)
+            k = len(pads)
+            desc = np.zeros((n_real + k, IDESC["COUNT"]), np.float64)
+            for i, z in enumerate(pads):
+                desc[i, IDESC["NY"]] = desc[i, IDESC["NX"]] = 1
+                desc[i, IDESC["MP"] + 2] = z
+                desc[i, IDESC["MN"] + 2] = 1.0
+                desc[i, IDESC["TABOFF"]] = row
+                pad_rows.append((row, z))
+                row += 1
+            horizontal = []
+            for i0, lat in enumerate(crossed):
+                b = built[id(lat)]
+                desc[i0 + k] = b.desc_row()
+                desc[i0 + k, IDESC["SIGC"]] = 0.0 if sig is None else sig[i0]
+                desc[i0 + k, IDESC["TABOFF"]] = b.table_off
+                horizontal.append(b.horizontal)
+            v = object.__new__(DeviceInterfaces)
+            v.device, v.k_pad, v.n, v.has_sigma = device, k, desc.shape[0], sig is not None
+            v.horizontal = horizontal
+            v.flat_below = all(horizontal[1:])
+            v.desc = torch.as_tensor(desc, device=device)
+            v.tables = table
+            out.append(v)
+        if row > table.shape[0]:
+            raise ValueError("shared table has too few pad rows")
+        for r, z in pad_rows:
+            table[r] = torch.tensor([0.0, 0.0, z, 0.0, 0.0, 1.0], dtype=torch.float64)
+        return out
 
     @staticmethod
     def stacks(stacks, sigmas=None, pad_tos=None, z_platform=None, device=None):

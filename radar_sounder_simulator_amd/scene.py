@@ -85,6 +85,32 @@ def grid_facets(heights, spacing, origin=(0.0, 0.0)):
     return facets_from_vertices(np.stack([X, Y, h], axis=-1))
 
 
+@dataclass
+class Lattice:
+    """A regular height lattice: ``heights`` (ny+1, nx+1) vertex heights at
+    ``spacing`` m centred on ``origin`` (the input of ``grid_facets``).  The
+    device builds its facets itself (``device.build_lattices``); ``facets()``
+    is the host tessellation of the same lattice."""
+
+    heights: np.ndarray
+    spacing: float
+    origin: tuple = (0.0, 0.0)
+
+    @property
+    def grid_shape(self):
+        return (self.heights.shape[0] - 1, self.heights.shape[1] - 1)
+
+    def __len__(self):
+        ny, nx = self.grid_shape
+        return ny * nx
+
+    def horizontal(self):
+        return bool(np.all(self.heights == self.heights.flat[0]))
+
+    def facets(self):
+        return grid_facets(self.heights, self.spacing, self.origin)
+
+
 def gaussian_surface(n, dx, corr_len, rng, m=None):
     """(n, m) zero-mean unit-variance Gaussian-ACF field, C = exp(-rho^2/l^2)
     (``compare/haynes.py:94-112``, extended to rectangular grids)."""

@@ -87,6 +87,12 @@ class DeviceScene:
 
     def __init__(self, layers, device):
         self.layers = []
+        self.center_mean = None
+        lat_of = {id(L.facets): L.lattice for L in layers if L.lattice is not None}
+        if (all(L.lattice is not None for L in layers)
+                and all(id(c) in lat_of for L in layers for c in L.crossed)):
+            self._build_on_device(layers, lat_of, device)
+            return
         deepest = max((L.crossed for L in layers if L.crossed), key=len, default=None)
         stack = None
         if deepest and all(not L.crossed or all(a is b for a, b in zip(L.crossed, deepest))
@@ -105,8 +111,36 @@ class DeviceScene:
             self.layers.append((L, fac, ifc))
         self.refs = 0
 
+    def _build_on_device(self, layers, lat_of, device):
+        """Every layer a Lattice: tessellated in HBM (ss_facets_from_heights),
+        the heights the only per-facet upload; one shared lookup table."""
+        lats = [L.lattice for L in layers]
+        crossed = {id(lat_of[id(c)]) for L in layers for c in L.crossed}
+        gammas, phasors = {}, {}
+        for L in layers:
+            g = np.asarray(L.gamma, np.float64)
+            if g.ndim:
+                gammas[id(L.lattice)] = g
+            if L.roughness is not None:
+                phasors[id(L.lattice)] = L.roughness[2]
+        built, table = dv.build_lattices(lats, crossed, device=device, gammas=gammas,
+                                         phasors=phasors)
+        refr = [L for L in layers if L.crossed]
+        ifcs = iter(dv.DeviceInterfaces.lattice_stacks(
+            [[lat_of[id(c)] for c in L.crossed] for L in refr], built, table,
+            pad_row0=table.shape[0], device=device) if refr else [])
+        self.stack = None
+        self.table = table
+        for L in layers:
+            self.layers.append((L, built[id(L.lattice)].fac, next(ifcs) if L.crossed else None))
+        self.center_mean = built[id(lats[0])].center_mean
+        self.refs = 0
+
     def nbytes(self):
         b = sum(f.nbytes() for _, f, _ in self.layers)
+        if getattr(self, "table", None) is not None:
+            return b + self.table.numel() * 8 + sum(i.desc.numel() * 8
+                                                    for _, _, i in self.layers if i is not None)
         if self.stack is not None:
             return b + self.stack.nbytes()
         return b + sum(i.nbytes() for _, _, i in self.layers if i is not None)
@@ -123,6 +157,13 @@ class DeviceScene:
         return b
 
 
+def _range_to(positions, m):
+    """Per-trace |p - m| in float64 (the surface reference range from a
+    device-side centre mean)."""
+    d = np.asarray(positions, np.float64) - np.asarray(m, np.float64)
+    return np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+
+
 class ActiveRequest:
     """An admitted request: its traces, output buffers, per-layer job
     templates and how far each layer's traces have been launched."""
@@ -136,7 +177,9 @@ class ActiveRequest:
         pv = None if wl.pattern is None else wl.pattern[1]
         surf = wl.layers[0].facets
         self.traces = dv.DeviceTraces(tr.positions, tr.u_ct,
-                                      _reference_range(tr.positions, surf.centers), pv,
+                                      _reference_range(tr.positions, surf.centers)
+                                      if scene.center_mean is None else
+                                      _range_to(tr.positions, scene.center_mean), pv,
                                       device=device)
         self.T = T = tr.positions.shape[0]
         self.n_seg = n_seg = (2 if wl.split_sides else 1) * wl.n_samples

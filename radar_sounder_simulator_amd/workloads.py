@@ -52,6 +52,7 @@ class Layer:
     att_leg: list = None
     refraction: str = "sequential"  # or "joint" (padded to pad_to planes)
     pad_to: int = None
+    lattice: object = None          # scene.Lattice the facets tessellate, if any
 
 
 @dataclass

@@ -125,8 +125,19 @@ inline uint64_t exchange_raw(uint64_t mine, int src) {
   return r;
 }
 
+// One mutex for every kernel type (a static inside the launch template
+// would be one per instantiation).
+inline std::mutex& launch_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 template <class F, class... A>
 void launch(F f, dim3 grid, dim3 block, A... args) {
+  // one launch at a time: launches from several host threads (a serving
+  // loop's upload worker and its step loop) share the grid globals below,
+  // where the device runs them on separate streams
+  std::lock_guard<std::mutex> launch_lock(launch_mutex());
   const int nt = (int)(block.x * block.y * block.z);
   const long nblocks = (long)grid.x * grid.y;
   g_grid = grid;
@@ -223,6 +234,8 @@ inline unsigned long long __ballot(int pred) {
 }
 
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
+
+inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 
 // sin(pi x), cos(pi x) with the exact values at multiples of 1/2 that the
 // device routine returns.

@@ -212,6 +212,14 @@ def test_deep_flat_stack_kernel_vs_flat_oracle(sim):
                                   np.where(vref, bref, _lib.BIN_INVALID))
     assert np.linalg.norm(o - ref) <= 1e-9 * np.linalg.norm(ref)
     np.testing.assert_allclose(drop.numpy(), dref, rtol=1e-9, atol=1e-30)
+    # mixed tier: lanes with no feasible start step settle without the
+    # float64 redo and book their dropped power from the start path
+    with _lib.use(sim):
+        om_, dm, bm, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision="mixed",
+                                            pair_bins=True, refraction="joint", **kw)
+    np.testing.assert_array_equal(fac.unpermute_pairs(bm.numpy()),
+                                  np.where(vref, bref, _lib.BIN_INVALID))
+    np.testing.assert_allclose(dm.numpy(), dref, rtol=1e-4, atol=1e-30)
 
 
 def test_multilayer_scene_layers_vs_oracle(sim):
