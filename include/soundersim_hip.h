@@ -361,6 +361,25 @@ int ss_facets_from_heights(const double* heights, int64_t ny, int64_t nx, double
  * for an empty lattice. */
 int64_t ss_facets_workspace_bytes(int64_t ny, int64_t nx);
 
+/* Per-trace diagnostics over one interface's facets, float64: the facet
+ * whose centre is horizontally nearest the platform (nadir_index, its centre
+ * z in nadir_z) and the earliest exact arrival 2|p - centre|/c inside
+ * [t0, t0 + n_samples*dt) (first_index, first_twtt, its centre in first_xyz
+ * (T, 3)); ties go to the lowest caller index, as numpy's argmin.
+ * facet_index (nullable, device int64 (n_facets,)): the caller's index of
+ * each facet position (tie-breaks and the returned indices), else the
+ * position itself.  No facet in the window: first_index -1, first_twtt +inf,
+ * first_xyz NaN (the reference then reports facet 0, output.py:79-82).
+ * Replaces output.py:69-82 (first_return_*) and the nearest-facet step of
+ * simulate.py:398-412 (_nadir_twtt_layers), both O(T F) host arrays in the
+ * reference. */
+int ss_trace_diagnostics(const double* facets, int64_t facet_stride, int64_t n_facets,
+                         const int64_t* facet_index, const double* traces,
+                         int64_t trace_stride, int64_t n_traces, double t0, double dt,
+                         int64_t n_samples, double c, int64_t* nadir_index, double* nadir_z,
+                         int64_t* first_index, double* first_twtt, double* first_xyz,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

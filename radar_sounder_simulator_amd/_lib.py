@@ -47,7 +47,7 @@ EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_sizeof_job", "ss_range_compress", "ss_compress_fft_size", "ss_pulse_spectrum",
             "ss_range_compress_fft", "ss_batch_table_bytes", "ss_batch_pack",
             "ss_batch_step_workspace_bytes", "ss_batch_step", "ss_facets_from_heights",
-            "ss_facets_workspace_bytes")
+            "ss_facets_workspace_bytes", "ss_trace_diagnostics")
 
 
 class SurfaceParams(ctypes.Structure):
@@ -164,6 +164,10 @@ def configure(lib):
     lib.ss_facets_from_heights.restype = ctypes.c_int
     lib.ss_facets_workspace_bytes.argtypes = [i64, i64]
     lib.ss_facets_workspace_bytes.restype = ctypes.c_int64
+    d = ctypes.c_double
+    lib.ss_trace_diagnostics.argtypes = [dp, i64, i64, dp, dp, i64, i64, d, d, i64, d,
+                                         dp, dp, dp, dp, dp, vp]
+    lib.ss_trace_diagnostics.restype = ctypes.c_int
     return lib
 
 

@@ -1213,6 +1213,98 @@ __global__ __launch_bounds__(kBlock) void facet_stats_kernel(const double* __res
   if ((int)threadIdx.x < kStatCount) stats[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// ------------------------------------------------------ trace diagnostics
+// Per trace, over one interface's facets (output.py:69-82,
+// simulate.py:398-412): the horizontally nearest facet centre (nadir) and
+// the earliest in-window exact arrival 2|p - c|/c (first return), ties to
+// the lowest caller index as numpy's argmin.  kDiagTT traces per
+// workgroup, each lane a strided subset of the facets, then a fixed-order
+// LDS reduction per trace.
+constexpr int kDiagTT = 8;
+
+__device__ inline bool diag_better(double k, long i, double bk, long bi) {
+  return k < bk || (k == bk && i < bi);
+}
+
+__global__ __launch_bounds__(kBlock) void trace_diag_kernel(
+    const double* __restrict__ facets, long fs, long nf, const long* __restrict__ index,
+    const double* __restrict__ traces, long ts, long nt, double t0, double t_end, double c,
+    long* __restrict__ nadir_index, double* __restrict__ nadir_z,
+    long* __restrict__ first_index, double* __restrict__ first_twtt,
+    double* __restrict__ first_xyz) {
+#pragma clang fp contract(off)
+  __shared__ double sk[kBlock];
+  __shared__ long si[kBlock], sp[kBlock];
+  const int tid = threadIdx.x;
+  const long t_base = (long)blockIdx.x * kDiagTT;
+  const int nT = (int)((nt - t_base) < kDiagTT ? (nt - t_base) : kDiagTT);
+  double px[kDiagTT], py[kDiagTT], pz[kDiagTT];
+  double bn[kDiagTT], bf[kDiagTT];
+  long bni[kDiagTT], bnp[kDiagTT], bfi[kDiagTT], bfp[kDiagTT];
+  const double inf = __builtin_inf();
+  for (int t = 0; t < kDiagTT; ++t) {
+    const long tt = t_base + (t < nT ? t : 0);
+    px[t] = traces[(SS_TROW_PX + 0) * ts + tt];
+    py[t] = traces[(SS_TROW_PX + 1) * ts + tt];
+    pz[t] = traces[(SS_TROW_PX + 2) * ts + tt];
+    bn[t] = bf[t] = inf;
+    bni[t] = bfi[t] = LONG_MAX;
+    bnp[t] = bfp[t] = -1;
+  }
+  for (long f = tid; f < nf; f += kBlock) {
+    const double cx = facets[(SS_FROW_CX + 0) * fs + f];
+    const double cy = facets[(SS_FROW_CX + 1) * fs + f];
+    const double cz = facets[(SS_FROW_CX + 2) * fs + f];
+    const long oi = index ? index[f] : f;
+    for (int t = 0; t < kDiagTT; ++t) {
+      const double dx = cx - px[t], dy = cy - py[t], dz = cz - pz[t];
+      const double h2 = dx * dx + dy * dy;
+      if (diag_better(h2, oi, bn[t], bni[t])) {
+        bn[t] = h2;
+        bni[t] = oi;
+        bnp[t] = f;
+      }
+      const double ft = 2.0 * sqrt(h2 + dz * dz) / c;
+      if (ft >= t0 && ft < t_end && diag_better(ft, oi, bf[t], bfi[t])) {
+        bf[t] = ft;
+        bfi[t] = oi;
+        bfp[t] = f;
+      }
+    }
+  }
+  for (int t = 0; t < nT; ++t) {
+    for (int q = 0; q < 2; ++q) {
+      sk[tid] = q == 0 ? bn[t] : bf[t];
+      si[tid] = q == 0 ? bni[t] : bfi[t];
+      sp[tid] = q == 0 ? bnp[t] : bfp[t];
+      __syncthreads();
+      for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (tid < h && diag_better(sk[tid + h], si[tid + h], sk[tid], si[tid])) {
+          sk[tid] = sk[tid + h];
+          si[tid] = si[tid + h];
+          sp[tid] = sp[tid + h];
+        }
+        __syncthreads();
+      }
+      if (tid == 0) {
+        const long tt = t_base + t;
+        const long pos = sp[0];
+        if (q == 0) {
+          nadir_index[tt] = pos >= 0 ? si[0] : -1;
+          nadir_z[tt] = pos >= 0 ? facets[(SS_FROW_CX + 2) * fs + pos] : __builtin_nan("");
+        } else {
+          first_index[tt] = pos >= 0 ? si[0] : -1;
+          first_twtt[tt] = pos >= 0 ? sk[0] : inf;
+          for (int k = 0; k < 3; ++k)
+            first_xyz[tt * 3 + k] = pos >= 0 ? facets[(SS_FROW_CX + k) * fs + pos]
+                                             : __builtin_nan("");
+        }
+      }
+      __syncthreads();  // the LDS arrays are rewritten for the next reduction
+    }
+  }
+}
+
 extern "C" {
 
 const char* ss_version(void) { return "soundersim-hip 0.1 (gfx950)"; }
@@ -1747,6 +1839,37 @@ int ss_facets_from_heights(const double* heights, int64_t ny, int64_t nx, double
   hipLaunchKernelGGL(facet_stats_kernel, dim3(1), dim3(kBlock), 0, st, partials, blocks, stats);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("facet build launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_trace_diagnostics(const double* facets, int64_t facet_stride, int64_t n_facets,
+                         const int64_t* facet_index, const double* traces,
+                         int64_t trace_stride, int64_t n_traces, double t0, double dt,
+                         int64_t n_samples, double c, int64_t* nadir_index, double* nadir_z,
+                         int64_t* first_index, double* first_twtt, double* first_xyz,
+                         void* stream) {
+  if (n_facets < 0 || n_traces < 0) return fail(SS_EINVAL, "negative sizes");
+  if (facet_stride < n_facets || trace_stride < n_traces)
+    return fail(SS_EINVAL, "stride smaller than count");
+  if (!(dt > 0.0) || !(c > 0.0) || n_samples <= 0)
+    return fail(SS_EINVAL, "dt, c and n_samples must be > 0");
+  if (n_traces == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  if (!traces || !nadir_index || !nadir_z || !first_index || !first_twtt || !first_xyz ||
+      (n_facets > 0 && !facets))
+    return fail(SS_EINVAL, "null pointer");
+  const long blocks = (long)((n_traces + kDiagTT - 1) / kDiagTT);
+  hipLaunchKernelGGL(trace_diag_kernel, dim3((unsigned)blocks), dim3(kBlock), 0,
+                     (hipStream_t)stream, facets, (long)facet_stride, (long)n_facets,
+                     reinterpret_cast<const long*>(facet_index), traces, (long)trace_stride,
+                     (long)n_traces, t0, t0 + (double)n_samples * dt, c,
+                     reinterpret_cast<long*>(nadir_index), nadir_z,
+                     reinterpret_cast<long*>(first_index), first_twtt, first_xyz);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("diagnostics launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
 }

@@ -672,6 +672,30 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
     return out, dropped, keep
 
 
+def trace_diagnostics(fac, traces, *, t0, dt, n_samples, c, stream=None):
+    """Per-trace nadir facet and first return over ``fac``
+    (``ss_trace_diagnostics``): dict of device tensors nadir_index,
+    nadir_z, first_index, first_twtt, first_xyz (indices in the caller's
+    facet order)."""
+    dev = fac.device
+    T = traces.n
+    idx = getattr(fac, "_index_t", None)
+    if idx is None:
+        idx = fac._index_t = torch.as_tensor(np.asarray(fac.order, np.int64), device=dev)
+    out = {"nadir_index": torch.empty(T, dtype=torch.int64, device=dev),
+           "nadir_z": torch.empty(T, dtype=torch.float64, device=dev),
+           "first_index": torch.empty(T, dtype=torch.int64, device=dev),
+           "first_twtt": torch.empty(T, dtype=torch.float64, device=dev),
+           "first_xyz": torch.empty((T, 3), dtype=torch.float64, device=dev)}
+    rc = _lib.load().ss_trace_diagnostics(
+        _ptr(fac.buf), fac.stride, fac.n, _ptr(idx), _ptr(traces.buf), traces.stride, T,
+        float(t0), float(dt), int(n_samples), float(c), _ptr(out["nadir_index"]),
+        _ptr(out["nadir_z"]), _ptr(out["first_index"]), _ptr(out["first_twtt"]),
+        _ptr(out["first_xyz"]), _stream_handle(stream, dev))
+    _lib.check(rc, "ss_trace_diagnostics")
+    return out
+
+
 def refracted_workspace_bytes(precision, n_facets, n_traces):
     """Device workspace bytes of one refracted launch (0 for fp64)."""
     prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED

@@ -189,3 +189,35 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
     if return_bins:
         return out, drop, tgt.unpermute_pairs(res[2].cpu().numpy())
     return out, drop
+
+
+def trace_diagnostics(positions, interfaces, eps, *, t0, dt, n_samples, c, device=None):
+    """Per-trace output diagnostics on the device (``ss_trace_diagnostics``):
+    ``nadir_twtt`` (T, L) -- the vertical two-way time through the stack with
+    in-medium speeds via each interface's horizontally nearest facet
+    (``simulate.py:398-412``) -- and the surface's first return
+    (``output.py:77-82``): ``first_return_twtt`` (T,) and its facet centre
+    ``first_return_xyz`` (T, 3), local frame.  A trace with no facet in the
+    window reports facet 0 at +inf, as the reference."""
+    _check_gpu()
+    pos = np.asarray(positions, np.float64)
+    traces = _dv.DeviceTraces(pos, np.zeros_like(pos), None, None, device=device)
+    nadir = np.empty((len(pos), len(interfaces)))
+    prev = pos[:, 2].copy()
+    acc = np.zeros(len(pos))
+    first = None
+    for j, f in enumerate(interfaces):
+        fac = _dv.DeviceFacets(f.centers, f.normals, f.areas, f.e1, f.e2, device=device)
+        d = _dv.trace_diagnostics(fac, traces, t0=t0, dt=dt, n_samples=n_samples, c=c)
+        z = d["nadir_z"].cpu().numpy()
+        acc = acc + np.sqrt(eps[j]) * (prev - z)
+        nadir[:, j] = 2.0 * acc / c
+        prev = z
+        if j == 0:
+            fi = d["first_index"].cpu().numpy()
+            ftw = d["first_twtt"].cpu().numpy()
+            xyz = d["first_xyz"].cpu().numpy()
+            none = fi < 0
+            xyz[none] = np.asarray(f.centers, np.float64)[0]
+            first = (ftw, xyz)
+    return nadir, first[0], first[1]

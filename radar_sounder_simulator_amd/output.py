@@ -5,8 +5,12 @@ provenance attributes; ``combine`` sums layers (coherent fields add before
 squaring); ``save``/``load`` round-trip through a NumPy ``.npz`` with a JSON
 attribute record (NetCDF needs h5netcdf, which this library does not use).
 
-The reference's O(T F) ``first_return_*`` diagnostics are not materialised
-(they need a (T, F, 3) float64 array, ~50 GB at the 2.1 M-facet scale).
+The reference's ``first_return_*`` diagnostics (``output.py:77-82``, a
+(T, F, 3) float64 host array there, ~50 GB at the 2.1 M-facet scale) are
+computed on the device (``kernels.trace_diagnostics``):
+``first_return_twtt`` and the facet centre ``first_return_xyz`` in the
+local frame (the reference converts it to lat/lon; geodesy is not part of
+this library).
 """
 
 import json
@@ -26,6 +30,8 @@ class SimResult:
     positions: np.ndarray
     layer: list = None
     attrs: dict = field(default_factory=dict)
+    first_return_twtt: np.ndarray = None
+    first_return_xyz: np.ndarray = None
 
     @property
     def power(self):
@@ -33,14 +39,15 @@ class SimResult:
 
     @classmethod
     def build(cls, *, field, dropped, nadir_twtt, t0, dt, positions, layers=None,
-              config=None):
+              config=None, first_return_twtt=None, first_return_xyz=None):
         n = field.shape[1]
         attrs = {"soundersim_amd_version": VERSION, "mode": "coherent"}
         if config is not None:
             attrs["config"] = config.model_dump_json()
         nad = nadir_twtt if layers is not None else nadir_twtt[:, 0]
         return cls(field, dropped, nad, t0 + dt * np.arange(n), np.asarray(positions),
-                   list(layers) if layers is not None else None, attrs)
+                   list(layers) if layers is not None else None, attrs,
+                   first_return_twtt, first_return_xyz)
 
     def sel_layer(self, name):
         j = self.layer.index(name)
@@ -55,14 +62,20 @@ def combine(res, dim="layer"):
 
 
 def save(res, path):
+    extra = {}
+    if res.first_return_twtt is not None:
+        extra = dict(first_return_twtt=res.first_return_twtt,
+                     first_return_xyz=res.first_return_xyz)
     np.savez(path, field=res.field, dropped_power=res.dropped_power,
              nadir_twtt=res.nadir_twtt, twtt=res.twtt, positions=res.positions,
              meta=np.frombuffer(json.dumps({"layer": res.layer, "attrs": res.attrs}).encode(),
-                                dtype=np.uint8))
+                                dtype=np.uint8), **extra)
 
 
 def load(path):
     with np.load(path, allow_pickle=False) as z:
         meta = json.loads(z["meta"].tobytes().decode())
+        fr = ((z["first_return_twtt"], z["first_return_xyz"]) if "first_return_twtt" in z
+              else (None, None))
         return SimResult(z["field"], z["dropped_power"], z["nadir_twtt"], z["twtt"],
-                         z["positions"], meta["layer"], meta["attrs"])
+                         z["positions"], meta["layer"], meta["attrs"], *fr)

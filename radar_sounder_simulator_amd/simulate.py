@@ -23,7 +23,7 @@ import numpy as np
 
 from . import antenna as _ant
 from .config import DemInterface, FlatInterface, OffsetInterface, SimConfig
-from .kernels import coherent_cluttergram, refracted_cluttergram
+from .kernels import coherent_cluttergram, refracted_cluttergram, trace_diagnostics
 from .output import SimResult
 from .physics import fresnel_normal
 from .roughness import n_terms_for, speckle_phasors, speckle_phasors_for_keys
@@ -112,7 +112,8 @@ def build_interfaces(scene, cfg):
 
 def nadir_twtt_layers(pos, interfaces, eps, c):
     """Per-layer nadir two-way time via the horizontally nearest facet centre
-    per interface (``simulate.py:398-412``), O(T F) per interface."""
+    per interface (``simulate.py:398-412``), O(T F) per interface on the
+    host; ``simulate`` uses the device form (``kernels.trace_diagnostics``)."""
     nadir = np.empty((pos.shape[0], len(interfaces)))
     prev = pos[:, 2].copy()
     acc = np.zeros(pos.shape[0])
@@ -189,7 +190,10 @@ def simulate(scene: PlanarScene, sim_config: SimConfig, device=None):
     field = outs[0] if single else np.stack(outs, axis=-1)
     field = apply_waveform(field, rc, "coherent")
     dropped = drops[0] if single else np.stack(drops, axis=-1)
-    nadir = nadir_twtt_layers(track.positions, ifaces, eps, rc.c)
+    nadir, first_twtt, first_xyz = trace_diagnostics(
+        track.positions, ifaces, eps, t0=rc.t0, dt=rc.dt, n_samples=rc.n_samples, c=rc.c,
+        device=device)
     return SimResult.build(field=field, dropped=dropped, nadir_twtt=nadir,
                            t0=rc.t0, dt=rc.dt, positions=track.positions,
-                           layers=None if single else names, config=cfg)
+                           layers=None if single else names, config=cfg,
+                           first_return_twtt=first_twtt, first_return_xyz=first_xyz)

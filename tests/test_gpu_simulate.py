@@ -1,0 +1,63 @@
+"""GPU: simulate() end to end (interface resolution, per-layer kernels with
+the reference's refraction and padding choices, diagnostics) on a planar
+three-interface scene against the oracle chain, float64."""
+
+import numpy as np
+import pytest
+
+from oracle import coherent as oc
+from oracle import multilayer as om
+from oracle import output as oo
+from radar_sounder_simulator_amd import config as cf
+from radar_sounder_simulator_amd import simulate as sm
+from radar_sounder_simulator_amd.physics import fresnel_normal
+from radar_sounder_simulator_amd.scene import gaussian_surface, straight_track
+
+pytestmark = pytest.mark.gpu
+C = 299792458.0
+
+
+def test_simulate_three_interfaces_vs_oracle():
+    print("case simulate/three_interfaces", flush=True)
+    rng = np.random.default_rng(12)
+    n, dx = 24, 25.0
+    dems = [3.0 * gaussian_surface(n + 1, dx, 100.0, rng),
+            np.zeros((n + 1, n + 1)),                      # placeholder for the flat one
+            -300.0 + 30.0 * gaussian_surface(n + 1, dx, 200.0, rng)]
+    tr = straight_track(6, 40.0, 3000.0)
+    scene = sm.PlanarScene(dems=dems, spacing=dx, track=tr)
+    media = [("air", 1.0, 0.0), ("firn", 2.0, 5.0), ("ice", 3.15, 10.0), ("rock", 8.0, 0.0)]
+    cfg = cf.SimConfig(
+        mode="coherent", precision="fp64", refraction="joint",
+        radar=cf.RadarConfig(dt=1 / 48e6, n_samples=512, t0=2 * 2900 / C, f0=60e6),
+        media=[cf.Medium(name=m, eps_r=e, attenuation_db_per_km=a) for m, e, a in media],
+        interfaces=[cf.DemInterface(name="surface"), cf.FlatInterface(name="horizon", elevation=-60.0),
+                    cf.DemInterface(name="bed")])
+    res = sm.simulate(scene, cfg, device="cuda")
+    ifaces, names = sm.build_interfaces(scene, cfg)
+    eps = [m[1] for m in media]
+    att = [m[2] for m in media]
+    k0 = 2 * np.pi * 60e6 / C
+    win = dict(t0=cfg.radar.t0, dt=cfg.radar.dt, n_samples=512, c=C)
+    f = ifaces[0]
+    ref = [oc.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals, f.areas, f.e1,
+                                   f.e2, k=k0, gamma=fresnel_normal(eps[0], eps[1]), **win)[0]]
+    for j in (1, 2):
+        refr = "joint" if j > 1 else "sequential"
+        ref.append(om.refracted_cluttergram(
+            tr.positions, tr.u_ct, ifaces[j], ifaces[:j], eps[:j + 1], att[:j + 1],
+            mode="coherent", gamma=fresnel_normal(eps[j], eps[j + 1]), k0=k0, refraction=refr,
+            pad_to=sm.joint_pad_to(j, 2) if refr == "joint" else None,
+            flat_stack=refr == "joint", **win)[0])
+    assert res.layer == names == ["surface", "horizon", "bed"]
+    for j in range(3):
+        out = res.field[..., j]
+        assert np.linalg.norm(ref[j]) > 0
+        rel = np.linalg.norm(out - ref[j]) / np.linalg.norm(ref[j])
+        assert rel <= 1e-9, (names[j], rel)
+    # diagnostics: per-layer nadir delay and the surface's first return
+    np.testing.assert_array_equal(res.nadir_twtt, oo.nadir_twtt_layers(tr.positions, ifaces,
+                                                                       eps, C))
+    fi, ftw, fxyz = oo.first_return(tr.positions, f.centers, cfg.radar.t0, cfg.radar.dt, 512, C)
+    np.testing.assert_array_equal(res.first_return_twtt, ftw)
+    np.testing.assert_array_equal(res.first_return_xyz, fxyz)

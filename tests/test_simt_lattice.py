@@ -86,3 +86,35 @@ def test_refracted_field_through_device_built_scene(sim):
     od, oh = od.numpy(), oh.numpy()
     assert np.linalg.norm(od - oh) <= 1e-12 * np.linalg.norm(oh)
     np.testing.assert_allclose(dd.numpy(), dh.numpy(), rtol=1e-12, atol=1e-30)
+
+
+@pytest.mark.parametrize("built", [False, True])
+def test_trace_diagnostics_match_oracle(sim, built):
+    """ss_trace_diagnostics (nadir facet, first return) against the oracle,
+    indices and values exact, on a grid with equidistant ties (the track runs
+    along cell edges) in Morton and in tile order."""
+    from oracle import output as oo
+    rng = np.random.default_rng(7)
+    lat = Lattice(2.0 * gaussian_surface(33, 20.0, 100.0, rng, m=41), 20.0)
+    f = lat.facets()
+    tr = straight_track(13, 20.0, 2000.0)                 # y = 0: on a row of cell edges
+    t0, dt, n, c = 2 * 1980.0 / C, 1 / 48e6, 600, C
+    with _lib.use(sim):
+        if built:
+            fac = dv.build_lattices([lat], set(), device="cpu")[0][id(lat)].fac
+        else:
+            fac = dv.DeviceFacets(f.centers, f.normals, f.areas, f.e1, f.e2, device="cpu")
+        trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
+        d = dv.trace_diagnostics(fac, trs, t0=t0, dt=dt, n_samples=n, c=c)
+    ni, nz = oo.nadir(tr.positions, f.centers)
+    fi, ftw, fxyz = oo.first_return(tr.positions, f.centers, t0, dt, n, c)
+    np.testing.assert_array_equal(d["nadir_index"].numpy(), ni)
+    np.testing.assert_array_equal(d["nadir_z"].numpy(), nz)
+    np.testing.assert_array_equal(d["first_index"].numpy(), fi)
+    np.testing.assert_array_equal(d["first_twtt"].numpy(), ftw)
+    np.testing.assert_array_equal(d["first_xyz"].numpy(), fxyz)
+    # a window that no facet reaches: -1 / +inf on the device (the reference
+    # then reports facet 0)
+    with _lib.use(sim):
+        d = dv.trace_diagnostics(fac, trs, t0=1.0, dt=dt, n_samples=10, c=c)
+    assert np.all(d["first_index"].numpy() == -1) and np.all(np.isinf(d["first_twtt"].numpy()))
