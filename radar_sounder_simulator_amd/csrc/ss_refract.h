@@ -28,6 +28,9 @@ constexpr int kLocalIters32 = 10;  // mixed: float32 local iterations
 constexpr int kPolish64 = 2;       // mixed: float64 polish steps
 constexpr double kCellGuard = 2e-4;  // cells; float32 anchor error ~1e-5
 constexpr double kResLo = 1e-11, kResHi = 1e-7;  // ambiguous residual band
+// float32 Newton loops stop once a full step moves sigma by at most this
+// relative amount (4 ulp): settled, or alternating between neighbours
+constexpr float kF32Settle = 4.8e-7f;
 
 struct Snell64 {
   double x[3];
@@ -150,7 +153,10 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
     float Fp = arf * (q1 * q1 * q1) + (bdf * rf) * (q2 * q2 * q2);
     float sn = sig - F * f_rcp(fmaxf(Fp, 1e-30f));
     bool ok = (sn >= lo) && (sn <= hi) && (sn == sn);
-    sig = ok ? sn : 0.5f * (lo + hi);
+    const float nx = ok ? sn : 0.5f * (lo + hi);
+    const bool done = ok && fabsf(nx - sig) <= kF32Settle * sig;
+    sig = nx;
+    if (done) break;  // settled to a few ulp: the float64 polish takes it from here
   }
   // float64 polish (plain Newton, clamped into [0, hi0))
   double sg = (double)sig;
@@ -227,7 +233,10 @@ SS_HD void snell_anchor32(const float p[3], const float q[3], const float nrm[3]
     float Fp = ar * (q1 * q1 * q1) + (bd * ratio) * (q2 * q2 * q2);
     float sn = sig - F * f_rcp(fmaxf(Fp, 1e-30f));
     bool ok = (sn >= lo) && (sn <= hi) && (sn == sn);
-    sig = ok ? sn : 0.5f * (lo + hi);
+    const float nx = ok ? sn : 0.5f * (lo + hi);
+    const bool done = ok && fabsf(nx - sig) <= kF32Settle * sig;
+    sig = nx;
+    if (done) break;  // settled to a few ulp (the cell guard band is 2e-4 cells)
   }
   float c1 = fmaxf((1.0f - sig) * (1.0f + sig), 1e-30f);
   float tr = ar * sig * f_rsq(c1);
