@@ -17,6 +17,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define SS_HD __host__ __device__ inline
@@ -209,6 +211,30 @@ SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
   T logx = t_log<T>(x);
   T acc = (T)0;
   T ab2 = A0 * A0 + B0 * B0;
+  if constexpr (std::is_same<T, float>::value && RM != 2) {
+    // float32 tier, area-only ACFs: the hardware reciprocal / rsqrt for the
+    // per-term 1/m and base^-1/2 (IEEE float divisions otherwise cost ~10
+    // instructions each, twice per term)
+    if (area_only) {
+      const float ll = l * l;
+      const bool expo = acf == SS_ACF_EXPONENTIAL;
+      for (int m = 1; m <= n_terms; ++m) {
+        const float mf = (float)m;
+        const float im = f_rcp(mf);
+        const float logp = mf * logx - (float)cst[SS_CONST_LGAMMA + m] - x;
+        float term;
+        if (expo) {
+          const float q = ll * (im * im);
+          const float rb = f_rsq((T)1 + ab2 * q);
+          term = t_exp<T>(logp) * ((T)kTwoPi * q * (rb * rb * rb)) * Lx * Ly;
+        } else {
+          term = t_exp<T>(logp - ab2 * ll * (0.25f * im)) * ((T)kPi * ll * im) * Lx * Ly;
+        }
+        acc += term;
+      }
+      return acc > (T)0 ? acc : (T)0;
+    }
+  }
   for (int m = 1; m <= n_terms; ++m) {
     T mf = (T)m;
     T logp = mf * logx - (T)cst[SS_CONST_LGAMMA + m] - x;

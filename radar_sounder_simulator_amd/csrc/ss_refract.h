@@ -558,9 +558,9 @@ SS_HD bool seq_path_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
     {
       float e1 = (float)epsl[i], e2 = (float)epsl[i + 1];
       float n1 = (float)nl[i], n2 = (float)nl[i + 1];
-      float sin2 = (e1 / e2) * (1.0f - c_inc * c_inc);
+      float sin2 = (e1 * f_rcp(e2)) * (1.0f - c_inc * c_inc);
       float c2 = f_sqrt(fmaxf(1.0f - sin2, 0.0f));
-      float g = (n1 * c_inc - n2 * c2) / (n1 * c_inc + n2 * c2);
+      float g = (n1 * c_inc - n2 * c2) * f_rcp(n1 * c_inc + n2 * c2);
       tau2 *= (1.0f - g * g);
     }
     float co = s.c_out;
@@ -573,8 +573,9 @@ SS_HD bool seq_path_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
     float ni = (float)nl[i];
     opl = opl + nl[i] * s.s1;
     loss_db += s1f * ((float)attl[i] * 1e-3f);
-    sum_perp += s1f / ni;
-    sum_par += s1f / (ni * c_inc * c_inc);
+    const float ini = f_rcp(ni);
+    sum_perp += s1f * ini;
+    sum_par += s1f * ini * f_rcp(c_inc * c_inc);
     c_last = fmaxf(co, (float)kCMin);
     cur[0] = s.x[0];
     cur[1] = s.x[1];
@@ -626,15 +627,16 @@ SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
   float rh[3] = {(float)dd[0] * isj, (float)dd[1] * isj, (float)dd[2] * isj};
   float njf = (float)nj;
   loss_db += sjf * ((float)attl[J] * 1e-3f);
-  sum_perp += sjf / njf;
-  sum_par += sjf / (njf * c_last * c_last);
+  const float injf = f_rcp(njf);
+  sum_perp += sjf * injf;
+  sum_par += sjf * injf * f_rcp(c_last * c_last);
   float n0 = (float)nl[0];
   float l_perp = n0 * sum_perp;
   float l_par = n0 * c_first * c_first * sum_par;
-  float flux = (n0 * c_first) / (njf * c_last);
+  float flux = (n0 * c_first) * injf * f_rcp(c_last);
   float att_f = f_exp2(-loss_db * (3.321928094887362f / 10.0f));  // 10^(-loss/10)
   float cos_t = rh[0] * (float)f.n[0] + rh[1] * (float)f.n[1] + rh[2] * (float)f.n[2];
-  float spread = tau2 * flux / (l_par * l_perp);
+  float spread = tau2 * flux * f_rcp(l_par * l_perp);
   if (PAT != 0 && rc.pat.kind != SS_PAT_ISOTROPIC) {
     float d0[3] = {(float)(x_first[0] - t.p[0]), (float)(x_first[1] - t.p[1]),
                    (float)(x_first[2] - t.p[2])};
@@ -670,8 +672,8 @@ SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
       are = amp * f_exp(-0.5f * sk * sk);
       if (f.A > 0.0) {
         float l1 = (float)f.l1, l2 = (float)f.l2;
-        float dp = d_phi<float, RM>((float)rc.sigma, (float)rc.lc, kk, 2.0f * kj * d1 / l1,
-                                2.0f * kj * d2 / l2, l1, l2, rc.n_terms,
+        float dp = d_phi<float, RM>((float)rc.sigma, (float)rc.lc, kk, 2.0f * kj * d1 * f_rcp(l1),
+                                2.0f * kj * d2 * f_rcp(l2), l1, l2, rc.n_terms,
                                 (flags & SS_D_PHI_AREA) != 0, rc.acf, rc.cst);
         float ai = base * f_sqrt(dp);
         are += ai * (float)f.ph_re;
