@@ -32,7 +32,10 @@ Two further sections ride on the same line:
                host memory) and sustained pairs/s;
   ``multilayer`` the 25-interface firn/horizon/bed scene (workloads.config3)
                over ``--ml-traces`` of its traces: both tiers, per-layer
-               kernel times and the mixed tier's error per layer.
+               kernel times and the mixed tier's error per layer;
+  ``incoherent`` the power-summing kernel on the surface layer of the same
+               scene: both tiers, the mixed tier's error, roofline, and the
+               NumPy oracle timed on 8 traces.
 """
 
 import argparse
@@ -206,6 +209,62 @@ def bench_multilayer(n_traces, dev, steps=2):
     return out
 
 
+def bench_incoherent(wl, dev, steps=10, cpu_traces=8):
+    """The incoherent (power-summing) kernel on the surface layer of the
+    surface+bed scene (512 x 512 facets, 256 traces, 4096 samples, cos^2
+    pattern): both tiers, HIP events on the launch stream, the mixed tier's
+    error against the float64 tier, and the oracle timed on a few traces."""
+    import torch
+    from radar_sounder_simulator_amd import device as dv
+    from radar_sounder_simulator_amd import roofline
+    s = wl.layers[0].facets
+    tr = wl.track
+    fac = dv.DeviceFacets(s.centers, s.normals, s.areas, s.e1, s.e2, device=dev)
+    trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, wl.pattern[1], device=dev)
+    kw = dict(t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c, pattern=wl.pattern)
+    pairs = fac.n * trs.n
+    st = torch.cuda.current_stream()
+    out = {"workload": "incoherent surface, 512x512 facets at 25 m, 256 traces at 20 km, "
+                       "4096 samples, cos^2 pattern", "pairs_per_step": pairs}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        dv.incoherent_power(fac, trs, precision=prec, **kw)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        ev[0].record(st)
+        for i in range(steps):
+            p, d, _ = dv.incoherent_power(fac, trs, precision=prec, **kw)
+            ev[i + 1].record(st)
+        torch.cuda.synchronize()
+        ms = float(np.mean([ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]))
+        res[prec] = p.cpu().numpy()
+        out[prec] = {"kernel_ms": ms, "pairs_per_s": pairs / (ms * 1e-3)}
+        log(f"bench: incoherent {prec}: {ms:.3f} ms, {pairs / (ms * 1e-3):.3e} pairs/s")
+    ref = res["fp64"]
+    out["mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(res["mixed"] - ref) / np.linalg.norm(ref))
+    f = roofline.incoherent_pair_flops("tabulated")
+    ach = f * pairs / (out["mixed"]["kernel_ms"] * 1e-3) / 1e12
+    out["roofline"] = {"bound": "valu", "flops_per_pair": f, "achieved": ach,
+                       "peak": roofline.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                       "frac": ach / roofline.PEAK_FP32_TFLOPS,
+                       "kernel": "incoherent_kernel<1, 5>"}
+    out["speedup_vs_fp64"] = out["fp64"]["kernel_ms"] / out["mixed"]["kernel_ms"]
+    from oracle import incoherent as oi
+    t = time.time()
+    oi.incoherent_cluttergram(tr.positions[:cpu_traces], tr.u_ct[:cpu_traces], s.centers,
+                              s.normals, s.areas, t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples,
+                              c=wl.c, pattern=(wl.pattern[0], wl.pattern[1][:cpu_traces],
+                                               wl.pattern[2], wl.pattern[3]))
+    el = time.time() - t
+    out["cpu_baseline"] = {"value": cpu_traces * fac.n / el, "unit": "facet*trace/s",
+                           "cores": 1, "kind": "port",
+                           "sample": f"oracle/incoherent.py (NumPy float64) over {cpu_traces} "
+                                     "traces of the same scene"}
+    log(f"bench: incoherent mixed vs fp64 rel {out['mixed_vs_fp64_rel_l2']:.2e}; "
+        f"cpu {cpu_traces * fac.n / el:.3e} pairs/s")
+    return out
+
+
 def bench_serving(n_requests, lam, scale, dev, seed=7):
     """The request stream on the mixed and on the float64 tier."""
     import torch
@@ -253,6 +312,7 @@ def main():
     ap.add_argument("--ml-traces", type=int, default=64,
                     help="traces of the multilayer scene's 1000 in its section")
     ap.add_argument("--no-multilayer", action="store_true")
+    ap.add_argument("--no-incoherent", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the coherent facet kernel shards by "
@@ -359,6 +419,9 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(extra)
+    if not args.no_incoherent:
+        log("bench: incoherent kernel on the surface layer")
+        line["incoherent"] = bench_incoherent(wl, dev)
     del rs
     torch.cuda.empty_cache()
     if not args.no_batch:

@@ -163,6 +163,17 @@ typedef struct ss_refracted_params {
   int32_t reserved;
 } ss_refracted_params;
 
+/* Incoherent (power-summing) surface kernel parameters. */
+typedef struct ss_incoherent_params {
+  double t0, dt, c;
+  double pat_a;     /* antenna parameters as in ss_surface_params          */
+  double pat_b[4];
+  int32_t n_samples;
+  int32_t flags;    /* SS_SPLIT_SIDES only                                 */
+  int32_t pattern;  /* SS_PAT_*                                            */
+  int32_t pat_n;
+} ss_incoherent_params;
+
 /* One layer of one request in a batched launch (ss_batch_field).  kind 0:
  * surface layer (uses ``surface``), kind 1: refracted layer, sequential
  * chain (uses ``refracted`` and the interface pointers).  All pointers are
@@ -212,6 +223,22 @@ int ss_surface_field(int precision, const ss_surface_params* params,
                      const double* pattern_table, const double* constants,
                      double* field, double* dropped, int32_t* pair_bins,
                      void* stream);
+
+/* Incoherent surface power.  Replaces kernels/incoherent.py:66-105
+ * (incoherent_cluttergram) and its device program _incoherent_fn (:31-64):
+ * per pair (A cos)^2 / r^4, times g^4 for a non-isotropic antenna, binned at
+ * floor((2r/c - t0)/dt); out-of-window power goes to ``dropped``.  Reads
+ * the centre, normal and area rows of ``facets`` only.  power: (n_traces,
+ * n_seg) float64 (real, not interleaved), n_seg = n_samples (x2 with
+ * SS_SPLIT_SIDES: [left | right]); zeroed by the call.  pair_bins as for
+ * ss_surface_field.  SS_PREC_MIXED: float64 range, bin and side, float32
+ * power and gain (bins equal SS_PREC_FP64's). */
+int ss_incoherent_power(int precision, const ss_incoherent_params* params,
+                        const double* facets, int64_t facet_stride,
+                        int64_t n_facets, const double* traces,
+                        int64_t trace_stride, int64_t n_traces,
+                        const double* pattern_table, double* power,
+                        double* dropped, int32_t* pair_bins, void* stream);
 
 /* Refracted-path field of one target interface under n_crossed interfaces,
  * sequential two-point chain or joint N-interface solve (refraction = 1,

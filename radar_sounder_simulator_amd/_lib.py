@@ -47,7 +47,7 @@ EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_sizeof_job", "ss_range_compress", "ss_compress_fft_size", "ss_pulse_spectrum",
             "ss_range_compress_fft", "ss_batch_table_bytes", "ss_batch_pack",
             "ss_batch_step_workspace_bytes", "ss_batch_step", "ss_facets_from_heights",
-            "ss_facets_workspace_bytes", "ss_trace_diagnostics")
+            "ss_facets_workspace_bytes", "ss_trace_diagnostics", "ss_incoherent_power")
 
 
 class SurfaceParams(ctypes.Structure):
@@ -74,6 +74,14 @@ class RefractedParams(ctypes.Structure):
                 ("pat_n", ctypes.c_int32), ("refraction", ctypes.c_int32),
                 ("k_pad", ctypes.c_int32), ("joint_newton", ctypes.c_int32),
                 ("joint_backtrack", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class IncoherentParams(ctypes.Structure):
+    _fields_ = [("t0", ctypes.c_double), ("dt", ctypes.c_double),
+                ("c", ctypes.c_double), ("pat_a", ctypes.c_double),
+                ("pat_b", ctypes.c_double * 4),
+                ("n_samples", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("pattern", ctypes.c_int32), ("pat_n", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):
@@ -131,6 +139,9 @@ def configure(lib):
     lib.ss_surface_field.argtypes = [i32, ctypes.POINTER(SurfaceParams), dp, i64,
                                      i64, dp, i64, i64, dp, dp, dp, dp, dp, vp]
     lib.ss_surface_field.restype = ctypes.c_int
+    lib.ss_incoherent_power.argtypes = [i32, ctypes.POINTER(IncoherentParams), dp, i64,
+                                        i64, dp, i64, i64, dp, dp, dp, dp, vp]
+    lib.ss_incoherent_power.restype = ctypes.c_int
     lib.ss_refracted_field.argtypes = [i32, ctypes.POINTER(RefractedParams), dp,
                                        i64, i64, dp, i64, i64, dp, dp, dp, dp,
                                        dp, dp, dp, dp, dp, i64, vp]

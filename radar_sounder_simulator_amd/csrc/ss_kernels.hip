@@ -132,7 +132,9 @@ __host__ __device__ inline long redo_bytes(long n_items, int tt) {
   return kWsHeader + redo_flag_bytes(n_items) + 8L * 4 * tt * n_items;
 }
 
-template <typename ACC, int TT>
+// POWER: a real-valued output (n_traces, n_seg) float64 with only the re
+// slots of the window in use (incoherent kernel); else complex interleaved.
+template <typename ACC, int TT, bool POWER = false>
 __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int buf,
                                     long t, int base, int span, int n_samples,
                                     bool split) {
@@ -142,6 +144,14 @@ __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int b
   for (int s = threadIdx.x; s < nside * w; s += kBlock) {
     int side = s / w, slot = s - side * w;
     ACC* v = sh.win[buf][side * kWin + slot];
+    if constexpr (POWER) {
+      ACC pw = v[0];
+      if (pw != (ACC)0) {
+        atomicAdd(&k.field[t * k.n_seg + (long)side * n_samples + base + slot], (double)pw);
+        v[0] = (ACC)0;
+      }
+      continue;
+    }
     ACC re = v[0], im = v[1];
     if (re != (ACC)0 || im != (ACC)0) {
       long idx = (t * k.n_seg + (long)side * n_samples + base + slot) * 2;
@@ -156,8 +166,9 @@ __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int b
 // Trace loop shared by both kernel families.  PAIR(t, o, it) evaluates this
 // lane's facet against trace record t (the it-th of the tile) into o; a pair
 // left with o.redo set contributes nothing and writes no pair bin.  REDO:
-// such pairs are recorded under work item ``item`` of ``rd``.
-template <typename ACC, int TT, bool REDO = false, typename PairFn>
+// such pairs are recorded under work item ``item`` of ``rd``.  POWER: the
+// pairs carry powers into a real output (flush_window).
+template <typename ACC, int TT, bool REDO = false, bool POWER = false, typename PairFn>
 __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
                                   bool live, long f, int n_samples, bool split,
                                   PairFn pair, const Redo& rd = Redo{}, long item = 0) {
@@ -183,6 +194,7 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     o.nparts = 0;
     o.valid = false;
     o.redo = false;
+    o.power = false;
     o.pwr_invalid = 0.0;
     if (live) pair(sh.tr[it], o, it);
     if constexpr (REDO) {
@@ -209,8 +221,8 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     }
     __syncthreads();
     if (it > 0)
-      flush_window<ACC, TT>(sh, k, buf ^ 1, t_base + it - 1, prev_base, prev_span,
-                            n_samples, split);
+      flush_window<ACC, TT, POWER>(sh, k, buf ^ 1, t_base + it - 1, prev_base, prev_span,
+                                   n_samples, split);
     int base = min(min(sh.wmin[buf][0], sh.wmin[buf][1]),
                    min(sh.wmin[buf][2], sh.wmin[buf][3]));
     int top = max(max(sh.wmax[buf][0], sh.wmax[buf][1]),
@@ -229,7 +241,12 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
           if (b >= 0 && b < n_samples) {
             int off = b - base;
             int side = split ? o.right : 0;
-            if (off < kWin) {
+            if constexpr (POWER) {
+              if (off < kWin)
+                atomicAdd(&sh.win[buf][side * kWin + off][0], (ACC)re);
+              else if (re != 0.0)
+                atomicAdd(&k.field[t * k.n_seg + (long)side * n_samples + b], re);
+            } else if (off < kWin) {
               atomicAdd(&sh.win[buf][side * kWin + off][0], (ACC)re);
               atomicAdd(&sh.win[buf][side * kWin + off][1], (ACC)im);
             } else {
@@ -238,7 +255,7 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
               if (im != 0.0) atomicAdd(&k.field[idx + 1], im);
             }
           } else {
-            double pw = re * re + im * im;
+            double pw = o.power ? re : re * re + im * im;
             if (pw != 0.0) atomicAdd(&sh.drop[it], pw);
           }
         }
@@ -250,8 +267,8 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
   }
   __syncthreads();
   if (nT > 0)
-    flush_window<ACC, TT>(sh, k, (nT - 1) & 1, t_base + nT - 1, prev_base, prev_span,
-                          n_samples, split);
+    flush_window<ACC, TT, POWER>(sh, k, (nT - 1) & 1, t_base + nT - 1, prev_base, prev_span,
+                                 n_samples, split);
   __syncthreads();
   if (tid < nT && sh.drop[tid] != 0.0) atomicAdd(&k.dropped[t_base + tid], sh.drop[tid]);
   if constexpr (REDO) {
@@ -437,6 +454,38 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
   }
 }
 
+// ------------------------------------------------------------- incoherent
+// Power-summing surface kernel (kernels/incoherent.py:31-64): the surface
+// kernel's tiles and trace loop with a real output; only the centre,
+// normal and area rows of a facet are read.
+template <int PREC, int PAT>
+__global__ __launch_bounds__(kBlock) void incoherent_kernel(KCommon k, SurfaceConst sc) {
+  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  __shared__ Shared<ACC, kTTSurf> sh;
+  const long w = work_id(k);
+  if (w < 0) return;
+  const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
+  const bool live = f < k.nf;
+  IncohFacet fc;
+  {
+    const long fi = live ? f : 0;
+    const double* F = k.facets;
+    for (int j = 0; j < 3; ++j) {
+      fc.c[j] = F[(SS_FROW_CX + j) * k.fs + fi];
+      fc.n[j] = F[(SS_FROW_NX + j) * k.fs + fi];
+    }
+    fc.A = F[SS_FROW_AREA * k.fs + fi];
+  }
+  const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
+  trace_loop<ACC, kTTSurf, false, true>(sh, k, w, live, f, sc.n_samples, split,
+                                        [&](const TraceRec& t, PairOut& o, int) {
+                                          if constexpr (PREC == SS_PREC_FP64)
+                                            incoherent_pair_fp64<PAT>(fc, t, sc, o);
+                                          else
+                                            incoherent_pair_mixed<PAT>(fc, t, sc, o);
+                                        });
+}
+
 // Per-trace reference-phase fold-back exp(-j ((2 k r_ref) mod 2 pi)), float64
 // surface path (kernels/coherent.py:311-313, 359).
 __global__ void fold_kernel(double* field, const double* traces, long ts, long nt,
@@ -545,7 +594,8 @@ __device__ inline void for_flagged_items(const unsigned char* __restrict__ flags
 // Books one pair's output straight into global memory, exactly as
 // trace_loop books it through its LDS window: a valid in-window part to its
 // field bin (per side), a valid out-of-window part's power and an invalid
-// pair's power to dropped; the raw bin to pair_bins.
+// pair's power to dropped (an incoherent part's re is that power); the raw
+// bin to pair_bins.
 __device__ inline void book_pair_global(const KCommon& k, const PairOut& o, long t, long f,
                                         int n_samples, bool split) {
   if (!o.valid) {
@@ -561,7 +611,7 @@ __device__ inline void book_pair_global(const KCommon& k, const PairOut& o, long
         if (re != 0.0) atomicAdd(&k.field[idx], re);
         if (im != 0.0) atomicAdd(&k.field[idx + 1], im);
       } else {
-        const double pw = re * re + im * im;
+        const double pw = o.power ? re : re * re + im * im;
         if (pw != 0.0) atomicAdd(&k.dropped[t], pw);
       }
     }
@@ -633,6 +683,7 @@ __global__ __launch_bounds__(kBlock) void redo_kernel(KCommon k, RefrConst rc, R
         o.nparts = 0;
         o.valid = false;
         o.redo = false;
+        o.power = false;
         o.pwr_invalid = 0.0;
         refracted_pair_fp64<NMAX, RM, PAT>(fr, tr, rc, o);
         book_pair_global(k, o, t, f, rc.n_samples, split);
@@ -1412,6 +1463,70 @@ int ss_surface_field(int precision, const ss_surface_params* prm, const double* 
     e = hipGetLastError();
     if (e != hipSuccess) return fail(SS_EHIP, std::string("fold launch: ") + hipGetErrorString(e));
   }
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_incoherent_power(int precision, const ss_incoherent_params* prm, const double* facets,
+                        int64_t facet_stride, int64_t n_facets, const double* traces,
+                        int64_t trace_stride, int64_t n_traces, const double* pattern_table,
+                        double* power, double* dropped, int32_t* pair_bins, void* stream) {
+  if (!prm) return fail(SS_EINVAL, "null params");
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED)
+    return fail(SS_EINVAL, "precision must be SS_PREC_FP64 or SS_PREC_MIXED");
+  if (n_facets < 0 || n_traces < 0) return fail(SS_EINVAL, "negative sizes");
+  if (prm->n_samples <= 0) return fail(SS_EINVAL, "n_samples must be > 0");
+  if (!(prm->dt > 0.0) || !(prm->c > 0.0)) return fail(SS_EINVAL, "dt and c must be > 0");
+  if (prm->flags & ~SS_SPLIT_SIDES)
+    return fail(SS_EINVAL, "the incoherent kernel takes no flag but SS_SPLIT_SIDES");
+  if (n_traces > 0 && (!power || !dropped || !traces))
+    return fail(SS_EINVAL, "null trace/output pointer");
+  if (n_facets > 0 && !facets) return fail(SS_EINVAL, "null facets");
+  if (facet_stride < n_facets || trace_stride < n_traces)
+    return fail(SS_EINVAL, "stride smaller than count");
+  int rcp = check_pattern(prm->pattern, prm->pat_n, pattern_table);
+  if (rcp) return rcp;
+  const bool split = (prm->flags & SS_SPLIT_SIDES) != 0;
+  const int n_seg = (split ? 2 : 1) * prm->n_samples;
+  if ((long)n_seg * 2 > (long)INT_MAX) return fail(SS_ERANGE, "n_samples too large");
+  hipStream_t st = (hipStream_t)stream;
+  int r = fill(power, (long)n_traces * n_seg, st);
+  if (r) return r;
+  r = fill(dropped, n_traces, st);
+  if (r) return r;
+  if (n_traces == 0 || n_facets == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  ss_surface_params sp = {};
+  sp.t0 = prm->t0;
+  sp.dt = prm->dt;
+  sp.c = prm->c;
+  sp.pat_a = prm->pat_a;
+  for (int i = 0; i < 4; ++i) sp.pat_b[i] = prm->pat_b[i];
+  sp.n_samples = prm->n_samples;
+  sp.flags = prm->flags;
+  sp.pattern = prm->pattern;
+  sp.pat_n = prm->pat_n;
+  KCommon k = make_common(facets, facet_stride, n_facets, traces, trace_stride, n_traces,
+                          power, dropped, pair_bins, kTTSurf, n_seg);
+  const SurfaceConst sc = make_surface_const(sp, pattern_table, nullptr);
+  dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
+  const int pm = pattern_mode(prm->pattern);
+#define SS_INC(P, A) \
+  hipLaunchKernelGGL((incoherent_kernel<P, A>), grid, block, 0, st, k, sc)
+#define SS_INC_P(P) \
+  if (pm == 0) SS_INC(P, 0); else if (pm == 5) SS_INC(P, 5); else SS_INC(P, -1)
+  if (precision == SS_PREC_FP64) {
+    SS_INC_P(SS_PREC_FP64);
+  } else {
+    SS_INC_P(SS_PREC_MIXED);
+  }
+#undef SS_INC_P
+#undef SS_INC
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(SS_EHIP, std::string("incoherent launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
 }

@@ -416,6 +416,8 @@ struct PairOut {
   bool valid;      // path validity (refracted); always true for the surface
   bool redo;       // reduced-precision path unresolved: the float64 redo
                    // kernel owns this pair (it contributes nothing here)
+  bool power;      // re holds a power (incoherent): an out-of-window part
+                   // books re itself to dropped, not |re + j im|^2
   double pwr_invalid;  // |contrib|^2 when the path is invalid
 };
 
@@ -598,6 +600,92 @@ SS_HD void surface_pair_mixed(const SurfFacet32& f, const SurfFacet64* f64,
   }
   o.right = (-(dxf * (float)t.u[0] + dyf * (float)t.u[1] + dzf * (float)t.u[2])) > 0.0f ? 1 : 0;
   (void)f64;
+}
+
+// ---------------------------------------------------------------------------
+// Incoherent surface pair (kernels/incoherent.py:36-53): power
+// (A cos)^2 / r^4, times g^4 for a non-isotropic antenna at the departure
+// direction (c - p)/r; bin floor((2r/c - t0)/dt); side (c - p).u_ct > 0.
+// The part's re is the power (o.power): an out-of-window part books it to
+// dropped as it is.
+// ---------------------------------------------------------------------------
+struct IncohFacet {
+  double c[3], n[3], A;
+};
+
+template <int PAT = -1>
+SS_HD void incoherent_pair_fp64(const IncohFacet& f, const TraceRec& t, const SurfaceConst& sc,
+                                PairOut& o) {
+#pragma clang fp contract(off)
+  double dx = t.p[0] - f.c[0], dy = t.p[1] - f.c[1], dz = t.p[2] - f.c[2];
+  double r = sqrt((dx * dx + dy * dy) + dz * dz);
+  double cs = ((dx * f.n[0] + dy * f.n[1]) + dz * f.n[2]) / r;
+  double a = f.A * cs;
+  double pw = (a * a) / ((r * r) * (r * r));
+  if (PAT != 0 && sc.pat.kind != SS_PAT_ISOTROPIC) {
+    double dh[3] = {-dx / r, -dy / r, -dz / r};
+    double g = antenna_gain<double, PAT>(sc.pat, dh, t.pv0, t.pv1);
+    pw = pw * ((g * g) * (g * g));
+  }
+  double fb = floor((2.0 * r / sc.c - sc.t0) / sc.dt);
+  int b = (fb < -2147483647.0) ? (-2147483647) : (fb > 2147483646.0 ? 2147483646 : (int)fb);
+  o.raw_bin = b;
+  o.valid = true;
+  o.power = true;
+  o.pwr_invalid = 0.0;
+  o.nparts = 1;
+  o.re[0] = pw;
+  o.im[0] = 0.0;
+  o.bin[0] = b;
+  o.right = (((-dx) * t.u[0] + (-dy) * t.u[1]) + (-dz) * t.u[2]) > 0.0 ? 1 : 0;
+}
+
+// Mixed tier: the float64 range and bin of the mixed surface pair (float32
+// square root with one float64 correction, exact fallback within 1e-7 bin
+// of an edge, so bins equal the float64 tier's) and the float64 side,
+// float32 power and gain.
+template <int PAT = -1>
+SS_HD void incoherent_pair_mixed(const IncohFacet& f, const TraceRec& t, const SurfaceConst& sc,
+                                 PairOut& o) {
+  double dx = t.p[0] - f.c[0], dy = t.p[1] - f.c[1], dz = t.p[2] - f.c[2];
+  double r2 = dx * dx + dy * dy + dz * dz;
+  float r0 = f_sqrt((float)r2);
+  float ir0 = f_rcp(r0);
+  double res = fma(-(double)r0, (double)r0, r2);
+  float corr = (float)res * (0.5f * ir0);
+  double r = (double)r0 + (double)corr;
+  double pos = fma(r, sc.k_bin, -sc.t0_dt);
+  double fb = floor(pos);
+  double fr = pos - fb;
+  if (fr < kBinGuard || fr > 1.0 - kBinGuard) {
+    pos = exact_bin_pos(sqrt((dx * dx + dy * dy) + dz * dz), sc.c, sc.t0, sc.dt);
+    fb = floor(pos);
+  }
+  float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+  float cs = (dxf * (float)f.n[0] + dyf * (float)f.n[1] + dzf * (float)f.n[2]) * ir0;
+  float a = (float)f.A * cs;
+  float ir2 = ir0 * ir0;
+  float pw = (a * a) * (ir2 * ir2);
+  if (PAT != 0 && sc.pat.kind != SS_PAT_ISOTROPIC) {
+    float dh[3] = {-dxf * ir0, -dyf * ir0, -dzf * ir0};
+    float pv0[3] = {(float)t.pv0[0], (float)t.pv0[1], (float)t.pv0[2]};
+    float pv1[3] = {(float)t.pv1[0], (float)t.pv1[1], (float)t.pv1[2]};
+    float g = antenna_gain<float, PAT>(sc.pat, dh, pv0, pv1);
+    float g2 = g * g;
+    pw *= g2 * g2;
+  }
+  int b = (fb < -2147483647.0) ? (-2147483647) : (fb > 2147483646.0 ? 2147483646 : (int)fb);
+  o.raw_bin = b;
+  o.valid = true;
+  o.power = true;
+  o.pwr_invalid = 0.0;
+  o.nparts = 1;
+  o.re[0] = pw;
+  o.im[0] = 0.0;
+  o.bin[0] = b;
+  // float64 side: facets straight below the track sit at a cross-track
+  // offset near zero
+  o.right = (((-dx) * t.u[0] + (-dy) * t.u[1]) + (-dz) * t.u[2]) > 0.0 ? 1 : 0;
 }
 
 }  // namespace ss

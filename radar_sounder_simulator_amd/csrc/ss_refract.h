@@ -481,6 +481,7 @@ SS_HD void radiometry_fp64(const RefrFacet& f, const TraceRec& t, const RefrCons
     re = a * a * att_f * att_f;
     im = 0.0;
     pwr = re;
+    o.power = true;
   }
   double pos = (2.0 * opl / rc.c - rc.t0) / rc.dt;
   double fb = floor(pos);
@@ -698,6 +699,7 @@ SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
     re = a * a * att_f * att_f;
     im = 0.0f;
     pwr = re;
+    o.power = true;
   }
   int b = (fb < -2147483647.0) ? (-2147483647) : (fb > 2147483646.0 ? 2147483646 : (int)fb);
   o.nparts = 1;

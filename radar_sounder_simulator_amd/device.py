@@ -363,6 +363,39 @@ def surface_field(facets, traces, *, k, gamma, t0, dt, n_samples, c,
     return out, dropped, table
 
 
+def incoherent_power(facets, traces, *, t0, dt, n_samples, c, precision="mixed",
+                     split_sides=False, pattern=None, out=None, dropped=None,
+                     pair_bins=False, stream=None):
+    """Enqueue the incoherent kernel.  Returns (power (T, n_seg) float64,
+    dropped (T,) float64[, pair_bins (T, F) int32], keep-alive) device tensors."""
+    dev = facets.device
+    p = _lib.IncoherentParams()
+    p.t0, p.dt, p.c = float(t0), float(dt), float(c)
+    p.n_samples = int(n_samples)
+    p.flags = _lib.SS_SPLIT_SIDES if split_sides else 0
+    kid, pn, pa, pb, table = pattern_setup(pattern, dev)
+    p.pattern, p.pat_n, p.pat_a = kid, pn, pa
+    for i in range(4):
+        p.pat_b[i] = pb[i]
+    n_seg = (2 if split_sides else 1) * int(n_samples)
+    T = traces.n
+    if out is None:
+        out = torch.empty((T, n_seg), dtype=torch.float64, device=dev)
+    if dropped is None:
+        dropped = torch.empty((T,), dtype=torch.float64, device=dev)
+    pb_t = (torch.empty((T, facets.n), dtype=torch.int32, device=dev)
+            if pair_bins else None)
+    prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
+    rc = _lib.load().ss_incoherent_power(
+        prec, ctypes.byref(p), _ptr(facets.buf), facets.stride, facets.n,
+        _ptr(traces.buf), traces.stride, T, _ptr(table), _ptr(out), _ptr(dropped),
+        _ptr(pb_t), _stream_handle(stream, dev))
+    _lib.check(rc, "ss_incoherent_power")
+    if pair_bins:
+        return out, dropped, pb_t, table
+    return out, dropped, table
+
+
 def is_horizontal(f):
     """Facets ``f`` form one horizontal plane: every normal (0, 0, +nz) and
     every centre at one z (exact comparisons)."""

@@ -102,6 +102,41 @@ def coherent_cluttergram(positions, u_ct, centers, normals, areas, e1, e2, *,
     return field, drop
 
 
+def incoherent_cluttergram(positions, u_ct, centers, normals, areas, *, t0, dt,
+                           n_samples, c, split_sides=False, pattern=None,
+                           block_size=65536, precision="mixed", device=None,
+                           return_bins=False):
+    """Binned incoherent power for every trace (``kernels/incoherent.py:66-105``):
+    per facet (A cos)^2 / r^4 [x g^4], out-of-window power dropped.
+
+    Returns ``(power, dropped)``: power (T, n) or (T, n, 2) [left, right],
+    float32 (mixed, like the reference) / float64 (fp64); dropped (T,).
+    ``block_size`` is accepted and ignored (2-D facet tiles on the device)."""
+    if precision not in ("mixed", "fp64"):
+        raise ValueError("precision must be 'mixed' or 'fp64'")
+    _check_gpu()
+    n = len(centers)
+    pos = np.asarray(positions, np.float64)
+    z3 = np.zeros((n, 3))
+    facets = _dv.DeviceFacets(centers, normals, areas, z3, z3, device=device)
+    pv = None if pattern is None else pattern[1]
+    traces = _dv.DeviceTraces(pos, u_ct, None, pv, device=device)
+    res = _dv.incoherent_power(facets, traces, t0=t0, dt=dt, n_samples=n_samples, c=c,
+                               precision=precision, split_sides=split_sides,
+                               pattern=pattern, pair_bins=return_bins)
+    torch.cuda.synchronize(facets.device)
+    power = res[0].cpu().numpy()
+    drop = res[1].cpu().numpy()
+    if precision == "mixed":
+        power = power.astype(np.float32)
+        drop = drop.astype(np.float32)
+    if split_sides:
+        power = power.reshape(-1, 2, n_samples).transpose(0, 2, 1)
+    if return_bins:
+        return power, drop, facets.unpermute_pairs(res[2].cpu().numpy())
+    return power, drop
+
+
 def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
                           mode, t0, dt, n_samples, c, gamma=0.0, k0=None,
                           split_sides=False, pattern=None, roughness=None,

@@ -18,6 +18,11 @@ iterations) 378 + lookup/renormalise 20 + local plane (25) 813 + cos 1 +
 TE Fresnel 14 + transmission 3 + path sums 10 + cos 1 = 1240.
 Final leg + spreading (:500-516) 42; antenna at departure 14 + gain;
 coherent amplitude (:527-532) 27; phase + complex multiply 10; bin 5.
+
+Incoherent surface pair (kernels/incoherent.py:36-53): 30 -- d (3), r^2
+(5), sqrt (1), d.n (5), /r (1), A cos (1), its square (1), r^4 (2),
+divide (1), twtt (2), bin (3), side (5); a pattern adds the departure
+direction (3), the gain and g^4 (3).
 """
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X vector/matrix FP32, dense
@@ -25,6 +30,7 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X FP64, dense
 PEAK_HBM_GBS = 8000.0
 
 SURFACE_BASE = 59
+INCOHERENT_BASE = 30
 CROSSING = 1240
 FINAL_LEG = 42
 REFR_AMP = 27
@@ -58,3 +64,7 @@ def refracted_pair_flops(n_crossed=1, pattern="isotropic", taper=False,
     f += TAPER if taper else 0
     f += rough_flops(n_terms, area_only) if n_terms else 0
     return f
+
+
+def incoherent_pair_flops(pattern="isotropic"):
+    return INCOHERENT_BASE + ((6 + GAIN[pattern]) if pattern != "isotropic" else 0)

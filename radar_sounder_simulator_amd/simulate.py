@@ -1,17 +1,24 @@
-"""simulate(): the front door of the hot path on a planar scene.
+"""simulate(): the front door of the hot path.
 
-Follows the reference orchestrator (``simulate.py:172-412``) step for step,
-minus geodesy (scenes here are already in a planar local frame):
+Follows the reference orchestrator (``simulate.py:172-412``) step for step:
 
 * single interface: the surface coherent kernel, gamma = fresnel_normal of
   the first two media, roughness arguments with the wavenumber of the medium
-  above, grazing fix, antenna pattern, waveform (``:172-226``);
-* several interfaces: interface 0 through the surface kernel, interface j >= 1
-  through the refracted kernel -- sequential for j = 1, ``cfg.refraction``
-  for j > 1 with joint calls padded to power-of-two interface buckets --
-  per-layer gamma (or a per-facet gamma map), roughness, crossing roughness,
-  diffuse channel, then one waveform pass over the layer axis (linear) and
+  above, grazing fix, antenna pattern, waveform (``:172-226``); in
+  incoherent mode the power-summing kernel instead (``:186-193``);
+* several interfaces: interface 0 through the surface kernel (coherent or
+  incoherent), interface j >= 1 through the refracted kernel in the run's
+  mode -- sequential for j = 1, ``cfg.refraction`` for j > 1 with joint
+  calls padded to power-of-two interface buckets -- per-layer gamma (or a
+  per-facet gamma map), roughness, crossing roughness, diffuse channel
+  (coherent only), then one waveform pass over the layer axis (linear) and
   per-layer nadir delays (``:263-412``).
+
+Scenes: a ``PlanarScene`` (vertex lattices already in a local frame) or a
+``geodesy.ProjectedScene`` (EPSG:3413 DEMs and a lat/lon/h nav track):
+the latter is tessellated in the ENU frame centred on its DEM and its
+first return is also reported in lat/lon (``output.py:77-82``), as the
+reference does for its synthetic and DEM scenes.
 
 The output is a ``SimResult`` (``output.py``); interface kinds dem / flat /
 offset resolve as in ``layered.py:56-118`` (offsets shift along +z here).
@@ -22,8 +29,10 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import antenna as _ant
+from . import geodesy as _geo
 from .config import DemInterface, FlatInterface, OffsetInterface, SimConfig
-from .kernels import coherent_cluttergram, refracted_cluttergram, trace_diagnostics
+from .kernels import (coherent_cluttergram, incoherent_cluttergram, refracted_cluttergram,
+                      trace_diagnostics)
 from .output import SimResult
 from .physics import fresnel_normal
 from .roughness import n_terms_for, speckle_phasors, speckle_phasors_for_keys
@@ -78,8 +87,15 @@ def joint_pad_to(j, n_max):
     return min(b, n_max)
 
 
-def build_interfaces(scene, cfg):
-    """Facets per configured interface, top-down."""
+def _lattice_facets(scene, heights, frame):
+    if frame is not None:
+        return _geo.build_facets(heights, scene.transform, frame)
+    return grid_facets(heights, scene.spacing)
+
+
+def build_interfaces(scene, cfg, frame=None):
+    """Facets per configured interface, top-down (``frame``: the local frame
+    of a projected scene, else None)."""
     out, names = [], []
     for i, ic in enumerate(cfg.interfaces):
         names.append(ic.name or f"interface_{i}")
@@ -90,10 +106,10 @@ def build_interfaces(scene, cfg):
             if i >= len(scene.dems):
                 raise ValueError(f"interface {i} ({ic.name or 'unnamed'}) needs a DEM: the "
                                  "scene provides none")
-            out.append(grid_facets(scene.dems[i], scene.spacing))
+            out.append(_lattice_facets(scene, scene.dems[i], frame))
         elif isinstance(ic, FlatInterface):
             ref = np.asarray(scene.dems[0], np.float64)
-            out.append(grid_facets(np.full_like(ref, ic.elevation), scene.spacing))
+            out.append(_lattice_facets(scene, np.full_like(ref, ic.elevation), frame))
         else:
             out.append(None)
     for i, ic in enumerate(cfg.interfaces):
@@ -126,22 +142,26 @@ def nadir_twtt_layers(pos, interfaces, eps, c):
     return nadir
 
 
-def simulate(scene: PlanarScene, sim_config: SimConfig, device=None):
-    """Run the coherent hot path on ``scene``; returns a ``SimResult``."""
+def simulate(scene, sim_config: SimConfig, device=None):
+    """Run the hot path on ``scene`` (``PlanarScene`` or
+    ``geodesy.ProjectedScene``); returns a ``SimResult``."""
     cfg = sim_config
-    if cfg.mode != "coherent":
-        raise NotImplementedError("incoherent surface mode is not part of this library "
-                                  "(coherent facet kernel only)")
+    coherent = cfg.mode == "coherent"
     rc = cfg.radar
-    if rc.f0 is None:
+    if coherent and rc.f0 is None:
         raise ValueError("coherent mode requires radar.f0 to be set")
     if len(cfg.interfaces) > 1 and rc.waveform.interp_bins:
         raise ValueError("waveform.interp_bins is not yet supported for multilayer runs")
-    track = scene.track
-    ifaces, names = build_interfaces(scene, cfg)
+    frame = None
+    if isinstance(scene, _geo.ProjectedScene):
+        frame = _geo.LocalFrame.centered_on(scene)
+        track = _geo.nav_to_frame(scene.nav_llh, frame, scene.nav_roll)
+    else:
+        track = scene.track
+    ifaces, names = build_interfaces(scene, cfg, frame)
     eps = [m.eps_r for m in cfg.media]
     att = [m.attenuation_db_per_km for m in cfg.media]
-    k0 = 2.0 * np.pi / rc.wavelength
+    k0 = 2.0 * np.pi / rc.wavelength if coherent else None
     window = dict(t0=rc.t0, dt=rc.dt, n_samples=rc.n_samples, c=rc.c,
                   split_sides=cfg.split_sides, pattern=_pattern(rc, scene, track),
                   precision=cfg.precision, device=device)
@@ -149,51 +169,63 @@ def simulate(scene: PlanarScene, sim_config: SimConfig, device=None):
     gfx_kw = dict(taper_s=gfx.s_eff if gfx else None, d_phi_area=gfx is not None)
     sig_all = np.array([(ic.roughness.sigma_m if ic.roughness else 0.0)
                         for ic in cfg.interfaces])
-    unknown = (set(scene.gamma_maps) | set(scene.diffuse_maps)) - set(names)
+    gmaps = getattr(scene, "gamma_maps", None) or {}
+    dmaps = getattr(scene, "diffuse_maps", None) or {}
+    if (gmaps or dmaps) and not coherent:
+        raise ValueError("gamma_maps / diffuse_maps require coherent mode")
+    unknown = (set(gmaps) | set(dmaps)) - set(names)
     if unknown:
         raise ValueError(f"maps for unknown interfaces: {unknown}")
-    if names[0] in scene.diffuse_maps:
+    if names[0] in dmaps:
         raise ValueError("diffuse_maps on the top interface is not wired (the surface "
                          "uses the surface kernel)")
     outs, drops = [], []
     for j, target in enumerate(ifaces):
         gamma_j = fresnel_normal(eps[j], eps[j + 1])
-        if names[j] in scene.gamma_maps:
-            gamma_j = np.asarray(scene.gamma_maps[names[j]], np.float64).ravel()
-        rough = _roughness_args(cfg.interfaces[j], target, k0 * np.sqrt(eps[j]),
-                                (cfg.roughness_seed, j))
-        if j == 0:
+        if names[j] in gmaps:
+            gamma_j = np.asarray(gmaps[names[j]], np.float64).ravel()
+        rough = (_roughness_args(cfg.interfaces[j], target, k0 * np.sqrt(eps[j]),
+                                 (cfg.roughness_seed, j)) if coherent else None)
+        if j == 0 and coherent:
             out, drop = coherent_cluttergram(
                 track.positions, track.u_ct, target.centers, target.normals,
                 target.areas, target.e1, target.e2, k=k0, gamma=gamma_j,
                 interp_bins=rc.waveform.interp_bins, roughness=rough, **gfx_kw, **window)
+        elif j == 0:
+            out, drop = incoherent_cluttergram(
+                track.positions, track.u_ct, target.centers, target.normals,
+                target.areas, **window)
         else:
             refr = cfg.refraction if j > 1 else "sequential"
             n_max = len(ifaces) - 1
-            crossed_sig = sig_all[:j] if sig_all[:j].any() else None
+            crossed_sig = sig_all[:j] if coherent and sig_all[:j].any() else None
             dif = None
-            if names[j] in scene.diffuse_maps:
+            if names[j] in dmaps:
                 ph = (speckle_phasors_for_keys(target.phase_keys,
                                                (cfg.roughness_seed, 1000 + j))
                       if target.phase_keys is not None else
                       speckle_phasors(len(target), (cfg.roughness_seed, 1000 + j)))
-                dif = (np.asarray(scene.diffuse_maps[names[j]], np.float64).ravel(), ph,
+                dif = (np.asarray(dmaps[names[j]], np.float64).ravel(), ph,
                        float(cfg.diffuse_exponent))
             out, drop = refracted_cluttergram(
                 track.positions, track.u_ct, target, ifaces[:j], eps[:j + 1], att[:j + 1],
-                mode="coherent", gamma=gamma_j, k0=k0, refraction=refr,
+                mode=cfg.mode, gamma=gamma_j if coherent else 0.0, k0=k0, refraction=refr,
                 pad_to=joint_pad_to(j, n_max) if refr == "joint" else None,
-                roughness=rough, crossed_sigma=crossed_sig, diffuse=dif, **gfx_kw, **window)
+                roughness=rough, crossed_sigma=crossed_sig, diffuse=dif,
+                **(gfx_kw if coherent else {}), **window)
         outs.append(out)
         drops.append(drop)
     single = len(ifaces) == 1
-    field = outs[0] if single else np.stack(outs, axis=-1)
-    field = apply_waveform(field, rc, "coherent")
+    out = outs[0] if single else np.stack(outs, axis=-1)
+    out = apply_waveform(out, rc, cfg.mode)
     dropped = drops[0] if single else np.stack(drops, axis=-1)
     nadir, first_twtt, first_xyz = trace_diagnostics(
         track.positions, ifaces, eps, t0=rc.t0, dt=rc.dt, n_samples=rc.n_samples, c=rc.c,
         device=device)
-    return SimResult.build(field=field, dropped=dropped, nadir_twtt=nadir,
-                           t0=rc.t0, dt=rc.dt, positions=track.positions,
+    first_llh = frame.local_to_llh(first_xyz) if frame is not None else None
+    return SimResult.build(field=out if coherent else None,
+                           power=None if coherent else out, dropped=dropped,
+                           nadir_twtt=nadir, t0=rc.t0, dt=rc.dt, positions=track.positions,
                            layers=None if single else names, config=cfg,
-                           first_return_twtt=first_twtt, first_return_xyz=first_xyz)
+                           first_return_twtt=first_twtt, first_return_xyz=first_xyz,
+                           first_return_llh=first_llh)

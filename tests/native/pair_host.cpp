@@ -30,7 +30,7 @@ static void accumulate(const PairOut& o, long t, int n, int n_seg, bool split,
       field[idx] += o.re[p];
       field[idx + 1] += o.im[p];
     } else {
-      dropped[t] += o.re[p] * o.re[p] + o.im[p] * o.im[p];
+      dropped[t] += o.power ? o.re[p] : o.re[p] * o.re[p] + o.im[p] * o.im[p];
     }
   }
 }
@@ -73,7 +73,7 @@ extern "C" int host_surface(int prec, const ss_surface_params* prm, const double
     b.l1 = (float)a.l1; b.l2 = (float)a.l2;
     for (long t = 0; t < nt; ++t) {
       TraceRec r = load_trace(tr, ts, t);
-      PairOut o;
+      PairOut o{};
       if (prec == SS_PREC_FP64) surface_pair_fp64(a, r, sc, o);
       else surface_pair_mixed(b, nullptr, r, sc, o);
       accumulate(o, t, n, n_seg, split, field, dropped);
@@ -133,7 +133,7 @@ extern "C" int host_refracted(int prec, const ss_refracted_params* prm, const do
     a.dph_im = (prm->flags & SS_DIFFUSE) ? F[SS_FROW_DIF_IM * fs + f] : 0.0;
     for (long t = 0; t < nt; ++t) {
       TraceRec r = load_trace(tr, ts, t);
-      PairOut o;
+      PairOut o{};
       if (prec == SS_PREC_FP64) refracted_pair_fp64<SS_MAX_JOINT>(a, r, rc, o);
       else if (!refracted_pair_mixed<SS_MAX_JOINT>(a, r, rc, o)) {
         ++fb;

@@ -375,3 +375,49 @@ def test_device_resident_step_matches_batched_launch(sim, precision):
                     else:
                         assert np.all(rs.raw[j].numpy() == 7.0)   # not in this table
                     done += 1
+
+
+@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("n_samples,r0", [(600, 700.0), (80, 820.0)])
+def test_incoherent_kernel_vs_oracle(sim, split, n_samples, r0):
+    """Incoherent kernel (both tiers) against oracle/incoherent.py: bins
+    exact; power and dropped to 1e-12 (float64) / 1e-6 (mixed); the short
+    window cuts most pairs into dropped."""
+    from oracle import incoherent as oi
+    rng = np.random.default_rng(3)
+    f = grid_facets(20.0 * gaussian_surface(41, 25.0, 150.0, rng), 25.0)
+    tr = straight_track(7, 60.0, 800.0)
+    kw = dict(t0=2 * r0 / C, dt=1 / 48e6, n_samples=n_samples, c=C, split_sides=split)
+    ref, dref, bref = oi.incoherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals,
+                                                f.areas, return_bins=True, **kw)
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-6)):
+        with _lib.use(sim):
+            fac = dv.DeviceFacets(f.centers, f.normals, f.areas, f.e1, f.e2, device="cpu")
+            trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
+            out, drop, bins, _ = dv.incoherent_power(fac, trs, precision=prec, pair_bins=True,
+                                                     **kw)
+        o = out.numpy()
+        if split:
+            o = o.reshape(-1, 2, n_samples).transpose(0, 2, 1)
+        np.testing.assert_array_equal(fac.unpermute_pairs(bins.numpy()), bref)
+        assert np.linalg.norm(o - ref) <= tol * np.linalg.norm(ref)
+        np.testing.assert_allclose(drop.numpy(), dref, rtol=tol, atol=1e-300)
+
+
+def test_incoherent_refracted_dropped_is_power(sim):
+    """Regression: an out-of-window incoherent refracted pair books its power
+    (not its square) to dropped."""
+    surf, bed, tr = _bed()
+    kw = dict(mode="incoherent", t0=2 * 3200 / C, dt=1 / 48e6, n_samples=40, c=C,
+              gamma=0.0, k0=None)
+    with _lib.use(sim):
+        fac = dv.DeviceFacets(bed.centers, bed.normals, bed.areas, bed.e1, bed.e2, device="cpu")
+        ifc = dv.DeviceInterfaces([surf], device="cpu", z_platform=tr.positions[:, 2].min())
+        trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
+        out, drop, _ = dv.refracted_field(fac, trs, ifc, [1.0, 3.15], [0.0, 10.0],
+                                          precision="fp64", **kw)
+    ref, dref = om.refracted_cluttergram(tr.positions, tr.u_ct, bed, [surf], [1.0, 3.15],
+                                         [0.0, 10.0], **kw)
+    assert (dref > 0).all()
+    np.testing.assert_allclose(drop.numpy(), dref, rtol=1e-9)
+    np.testing.assert_allclose(out.numpy()[..., 0], ref, rtol=1e-9, atol=1e-300)
