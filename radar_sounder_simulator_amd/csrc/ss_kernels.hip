@@ -133,7 +133,7 @@ __host__ __device__ inline long redo_bytes(long n_items, int tt) {
 }
 
 // POWER: a real-valued output (n_traces, n_seg) float64 with only the re
-// slots of the window in use (incoherent kernel); else complex interleaved.
+// slots of the window in use (incoherent_kernel); else complex interleaved.
 template <typename ACC, int TT, bool POWER = false>
 __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int buf,
                                     long t, int base, int span, int n_samples,
@@ -166,9 +166,8 @@ __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int b
 // Trace loop shared by both kernel families.  PAIR(t, o, it) evaluates this
 // lane's facet against trace record t (the it-th of the tile) into o; a pair
 // left with o.redo set contributes nothing and writes no pair bin.  REDO:
-// such pairs are recorded under work item ``item`` of ``rd``.  POWER: the
-// pairs carry powers into a real output (flush_window).
-template <typename ACC, int TT, bool REDO = false, bool POWER = false, typename PairFn>
+// such pairs are recorded under work item ``item`` of ``rd``.
+template <typename ACC, int TT, bool REDO = false, typename PairFn>
 __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
                                   bool live, long f, int n_samples, bool split,
                                   PairFn pair, const Redo& rd = Redo{}, long item = 0) {
@@ -221,8 +220,8 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     }
     __syncthreads();
     if (it > 0)
-      flush_window<ACC, TT, POWER>(sh, k, buf ^ 1, t_base + it - 1, prev_base, prev_span,
-                                   n_samples, split);
+      flush_window<ACC, TT>(sh, k, buf ^ 1, t_base + it - 1, prev_base, prev_span,
+                            n_samples, split);
     int base = min(min(sh.wmin[buf][0], sh.wmin[buf][1]),
                    min(sh.wmin[buf][2], sh.wmin[buf][3]));
     int top = max(max(sh.wmax[buf][0], sh.wmax[buf][1]),
@@ -241,12 +240,7 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
           if (b >= 0 && b < n_samples) {
             int off = b - base;
             int side = split ? o.right : 0;
-            if constexpr (POWER) {
-              if (off < kWin)
-                atomicAdd(&sh.win[buf][side * kWin + off][0], (ACC)re);
-              else if (re != 0.0)
-                atomicAdd(&k.field[t * k.n_seg + (long)side * n_samples + b], re);
-            } else if (off < kWin) {
+            if (off < kWin) {
               atomicAdd(&sh.win[buf][side * kWin + off][0], (ACC)re);
               atomicAdd(&sh.win[buf][side * kWin + off][1], (ACC)im);
             } else {
@@ -267,8 +261,8 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
   }
   __syncthreads();
   if (nT > 0)
-    flush_window<ACC, TT, POWER>(sh, k, (nT - 1) & 1, t_base + nT - 1, prev_base, prev_span,
-                                 n_samples, split);
+    flush_window<ACC, TT>(sh, k, (nT - 1) & 1, t_base + nT - 1, prev_base, prev_span,
+                          n_samples, split);
   __syncthreads();
   if (tid < nT && sh.drop[tid] != 0.0) atomicAdd(&k.dropped[t_base + tid], sh.drop[tid]);
   if constexpr (REDO) {
@@ -293,7 +287,21 @@ __host__ __device__ inline PatternArgs make_pattern(int kind, int n, double a, c
   p.a = a;
   for (int i = 0; i < 4; ++i) p.b[i] = b[i];
   p.table = table;
+  p.ftab = nullptr;
+  p.fidx = nullptr;
+  p.fscale = 0.0f;
   return p;
+}
+
+// Stages the float32 lookup of a tabulated pattern of 2 .. kPatLds samples
+// (pattern_lds_entry) in ``ptab`` / ``pidx`` and points ``pa`` at it;
+// visible after the caller's next barrier.
+__device__ inline void stage_pattern(PatternArgs& pa, float* ptab, int* pidx) {
+  const int m = pa.n > kPatCells ? pa.n : kPatCells;
+  for (int i = threadIdx.x; i < m; i += kBlock) pattern_lds_entry(pa.table, pa.n, i, ptab, pidx);
+  pa.ftab = ptab;
+  pa.fidx = pidx;
+  pa.fscale = pattern_cell_scale(pa.table, pa.n);
 }
 
 __host__ __device__ inline SurfaceConst make_surface_const(const ss_surface_params& prm,
@@ -455,35 +463,137 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
 }
 
 // ------------------------------------------------------------- incoherent
-// Power-summing surface kernel (kernels/incoherent.py:31-64): the surface
-// kernel's tiles and trace loop with a real output; only the centre,
-// normal and area rows of a facet are read.
+// Power-summing surface kernel (kernels/incoherent.py:31-64).  Its pair is
+// ~46 flops, so per-trace work (bin-range reduction, two barriers, window
+// flush with one global atomic per occupied bin) and LDS atomics weigh
+// more than in the coherent kernels: each lane takes FPL Morton-consecutive
+// facets (FPL = 4: a compact 2x2 patch, often one bin), merges equal (bin,
+// side) runs in registers before its LDS atomic, and a workgroup covers
+// kBlock * FPL facets per trace.  The float32 tier gains from FPL = 4; the
+// float64 tier's registers (180 at FPL = 4, 2 waves/SIMD) make FPL = 1
+// faster there.  Only the centre, normal and area rows of a facet are read.
+template <int PREC>
+constexpr int incoherent_fpl() { return PREC == SS_PREC_MIXED ? 4 : 1; }
+
 template <int PREC, int PAT>
 __global__ __launch_bounds__(kBlock) void incoherent_kernel(KCommon k, SurfaceConst sc) {
+  constexpr int kIncFPL = incoherent_fpl<PREC>();
   using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
-  __shared__ Shared<ACC, kTTSurf> sh;
+  constexpr int TT = kTTSurf;
+  __shared__ Shared<ACC, TT> sh;
   const long w = work_id(k);
   if (w < 0) return;
-  const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
-  const bool live = f < k.nf;
-  IncohFacet fc;
-  {
-    const long fi = live ? f : 0;
-    const double* F = k.facets;
-    for (int j = 0; j < 3; ++j) {
-      fc.c[j] = F[(SS_FROW_CX + j) * k.fs + fi];
-      fc.n[j] = F[(SS_FROW_NX + j) * k.fs + fi];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long fbase = (w % k.n_ftiles) * (long)(kBlock * kIncFPL) + (long)tid * kIncFPL;
+  IncohFacet fc[kIncFPL];
+  bool live[kIncFPL];
+#pragma unroll
+  for (int j = 0; j < kIncFPL; ++j) {
+    const long f = fbase + j;
+    live[j] = f < k.nf;
+    const long fi = live[j] ? f : 0;
+    for (int q = 0; q < 3; ++q) {
+      fc[j].c[q] = k.facets[(SS_FROW_CX + q) * k.fs + fi];
+      fc[j].n[q] = k.facets[(SS_FROW_NX + q) * k.fs + fi];
     }
-    fc.A = F[SS_FROW_AREA * k.fs + fi];
+    fc[j].A = k.facets[SS_FROW_AREA * k.fs + fi];
   }
   const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
-  trace_loop<ACC, kTTSurf, false, true>(sh, k, w, live, f, sc.n_samples, split,
-                                        [&](const TraceRec& t, PairOut& o, int) {
-                                          if constexpr (PREC == SS_PREC_FP64)
-                                            incoherent_pair_fp64<PAT>(fc, t, sc, o);
-                                          else
-                                            incoherent_pair_mixed<PAT>(fc, t, sc, o);
-                                        });
+  const int n_samples = sc.n_samples;
+  const long t_base = (w / k.n_ftiles) * TT;
+  const int nT = (int)((k.nt - t_base) < TT ? (k.nt - t_base) : TT);
+  for (int i = tid; i < nT * SS_TROW_COUNT; i += kBlock) {
+    int tt = i / SS_TROW_COUNT, r = i - tt * SS_TROW_COUNT;
+    reinterpret_cast<double*>(&sh.tr[tt])[r] = k.traces[(long)r * k.ts + t_base + tt];
+  }
+  for (int i = tid; i < 2 * 2 * kWin; i += kBlock) sh.win[i / (2 * kWin)][i % (2 * kWin)][0] = (ACC)0;
+  if (tid < TT) sh.drop[tid] = 0.0;
+  SurfaceConst scl = sc;
+  if constexpr (PREC == SS_PREC_MIXED && PAT == SS_PAT_TABULATED) {
+    // float32 gain from the staged lookup (the host picks this
+    // instantiation for tables of at most kPatLds samples)
+    __shared__ float ptab[3 * kPatLds];
+    __shared__ int pidx[kPatCells];
+    stage_pattern(scl.pat, ptab, pidx);
+  }
+  __syncthreads();
+  int prev_base = 0, prev_span = 0;
+  for (int it = 0; it < nT; ++it) {
+    const int buf = it & 1;
+    const long t = t_base + it;
+    double pw[kIncFPL];
+    int bin[kIncFPL], side[kIncFPL];
+    int lmin = INT_MAX, lmax = INT_MIN;
+#pragma unroll
+    for (int j = 0; j < kIncFPL; ++j) {
+      PairOut o;
+      o.nparts = 0;
+      o.power = true;
+      if constexpr (PREC == SS_PREC_FP64)
+        incoherent_pair_fp64<PAT>(fc[j], sh.tr[it], sc, o);
+      else
+        incoherent_pair_mixed<PAT>(fc[j], sh.tr[it], scl, o);
+      pw[j] = live[j] ? o.re[0] : 0.0;
+      bin[j] = o.bin[0];
+      side[j] = split ? o.right : 0;
+      if (live[j] && bin[j] >= 0 && bin[j] < n_samples) {
+        lmin = min(lmin, bin[j]);
+        lmax = max(lmax, bin[j]);
+      }
+      if (k.pair_bins && live[j]) k.pair_bins[t * k.nf + fbase + j] = o.raw_bin;
+    }
+    lmin = wave_min_i(lmin);
+    lmax = wave_max_i(lmax);
+    if (lane == 0) {
+      sh.wmin[buf][wave] = lmin;
+      sh.wmax[buf][wave] = lmax;
+    }
+    __syncthreads();
+    if (it > 0)
+      flush_window<ACC, TT, true>(sh, k, buf ^ 1, t - 1, prev_base, prev_span, n_samples,
+                                  split);
+    const int base = min(min(sh.wmin[buf][0], sh.wmin[buf][1]),
+                         min(sh.wmin[buf][2], sh.wmin[buf][3]));
+    const int top = max(max(sh.wmax[buf][0], sh.wmax[buf][1]),
+                        max(sh.wmax[buf][2], sh.wmax[buf][3]));
+    // book: runs of equal (bin, side) merge in registers
+    double drop = 0.0, acc = 0.0;
+    int cb = 0, cs = 0;
+    bool open = false;
+#pragma unroll
+    for (int j = 0; j <= kIncFPL; ++j) {
+      const bool more = j < kIncFPL;
+      if (open && (!more || bin[j] != cb || side[j] != cs)) {
+        const int off = cb - base;
+        if (off < kWin)
+          atomicAdd(&sh.win[buf][cs * kWin + off][0], (ACC)acc);
+        else
+          atomicAdd(&k.field[t * k.n_seg + (long)cs * n_samples + cb], acc);
+        open = false;
+      }
+      if (!more) break;
+      if (pw[j] == 0.0) continue;
+      if (bin[j] < 0 || bin[j] >= n_samples) {
+        drop += pw[j];
+      } else if (open) {
+        acc += pw[j];
+      } else {
+        open = true;
+        cb = bin[j];
+        cs = side[j];
+        acc = pw[j];
+      }
+    }
+    if (drop != 0.0) atomicAdd(&sh.drop[it], drop);
+    prev_base = base;
+    prev_span = base <= top ? top - base + 1 : 0;
+  }
+  __syncthreads();
+  if (nT > 0)
+    flush_window<ACC, TT, true>(sh, k, (nT - 1) & 1, t_base + nT - 1, prev_base, prev_span,
+                                n_samples, split);
+  __syncthreads();
+  if (tid < nT && sh.drop[tid] != 0.0) atomicAdd(&k.dropped[t_base + tid], sh.drop[tid]);
 }
 
 // Per-trace reference-phase fold-back exp(-j ((2 k r_ref) mod 2 pi)), float64
@@ -1510,9 +1620,16 @@ int ss_incoherent_power(int precision, const ss_incoherent_params* prm, const do
   sp.pat_n = prm->pat_n;
   KCommon k = make_common(facets, facet_stride, n_facets, traces, trace_stride, n_traces,
                           power, dropped, pair_bins, kTTSurf, n_seg);
+  const int fpl = precision == SS_PREC_MIXED ? incoherent_fpl<SS_PREC_MIXED>()
+                                             : incoherent_fpl<SS_PREC_FP64>();
+  k.n_ftiles = (int)((n_facets + (long)kBlock * fpl - 1) / ((long)kBlock * fpl));
+  k.per_xcd = ((long)k.n_ftiles * k.n_ttiles + 7) / 8;
   const SurfaceConst sc = make_surface_const(sp, pattern_table, nullptr);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
-  const int pm = pattern_mode(prm->pattern);
+  // the tabulated instantiation stages its table in LDS (<= kPatLds samples)
+  const int pm = (prm->pattern == SS_PAT_TABULATED && prm->pat_n > kPatLds)
+                     ? -1
+                     : pattern_mode(prm->pattern);
 #define SS_INC(P, A) \
   hipLaunchKernelGGL((incoherent_kernel<P, A>), grid, block, 0, st, k, sc)
 #define SS_INC_P(P) \

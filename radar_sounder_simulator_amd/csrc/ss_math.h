@@ -270,7 +270,92 @@ struct PatternArgs {
   double a;            // n_elements / length_lam
   double b[4];         // spacing_lam, field_scale, q, enabled
   const double* table; // tabulated: theta[n] | gain[n]; tapered: tx[n] | rx[n]
+  // tabulated, staged lookup (pattern_lds_entry) in LDS, read by
+  // antenna_gain<float, SS_PAT_TABULATED, true> (the incoherent kernel's
+  // float32 tier); null elsewhere
+  const float* ftab;   // theta[n] | gain[n] | slope[n - 1], float
+  const int* fidx;     // kPatCells first-interval indices
+  float fscale;        // kPatCells / (theta[n-1] - theta[0])
 };
+
+// Float32 tabulated-gain lookup staged per workgroup (patterns of at most
+// kPatLds samples): the table rounded to float, the slope of every interval
+// evaluated as the interpolation evaluates it, and for kPatCells uniform
+// cells over [theta_0, theta_{n-1}] the interval holding each cell's lower
+// edge, so a lookup starts within a step of its interval.
+constexpr int kPatLds = 256;
+constexpr int kPatCells = 256;
+
+SS_HD float pattern_cell_scale(const double* table, int n) {
+  return (float)kPatCells / ((float)table[n - 1] - (float)table[0]);
+}
+
+// Entry i of the staged lookup (i < max(n, kPatCells)).
+SS_HD void pattern_lds_entry(const double* table, int n, int i, float* ftab, int* fidx) {
+  if (i < n) {
+    ftab[i] = (float)table[i];
+    ftab[n + i] = (float)table[n + i];
+    if (i < n - 1) {
+      const float x0 = (float)table[i], x1 = (float)table[i + 1];
+      const float y0 = (float)table[n + i], y1 = (float)table[n + i + 1];
+      ftab[2 * n + i] = (y1 - y0) / (x1 - x0);
+    }
+  }
+  if (i < kPatCells) {
+    const float xa = (float)table[0];
+    const float e = xa + (float)i / pattern_cell_scale(table, n);
+    int lo = 0, hi = n - 1;  // last interval whose start is <= e
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if ((float)table[mid] <= e)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    fidx[i] = lo;
+  }
+}
+
+// Linear interpolation in a staged tabulated pattern at theta = th: the
+// interval xf[lo] <= th < xf[lo + 1] (the one a bisection finds) from the
+// cell index and a step or two, then slope * (th - x_lo) + y_lo.
+SS_HD float tab_lookup_staged(const PatternArgs& pa, float th) {
+  const float* xf = pa.ftab;
+  const float* ff = pa.ftab + pa.n;
+  const float* sl = pa.ftab + 2 * pa.n;
+  const int n = pa.n;
+  const float xa = xf[0], xb = xf[n - 1];
+  int cell = (int)((fminf(fmaxf(th, xa), xb) - xa) * pa.fscale);
+  cell = cell < 0 ? 0 : (cell > kPatCells - 1 ? kPatCells - 1 : cell);
+  int lo = pa.fidx[cell];
+  // fixed steps (a cell holds at most two samples on a table no denser than
+  // kPatCells / 2 over its range), then the general walk, rarely entered
+  lo -= (lo > 0 && xf[lo] > th) ? 1 : 0;
+  lo += (lo < n - 2 && xf[lo + 1] <= th) ? 1 : 0;
+  lo += (lo < n - 2 && xf[lo + 1] <= th) ? 1 : 0;
+  while (lo < n - 2 && xf[lo + 1] <= th) ++lo;
+  while (lo > 0 && xf[lo] > th) --lo;
+  float g = sl[lo] * (th - xf[lo]) + ff[lo];
+  g = th < xb ? g : ff[n - 1];
+  return th > xa ? g : ff[0];  // (NaN: ff[0], as the bisection's first test)
+}
+
+// acos in float32 after Abramowitz & Stegun (1964) 4.4.46 (Hastings):
+// acos(x) = sqrt(1 - x) (a0 + a1 x + ... + a7 x^7) on [0, 1], |error| <=
+// 2e-8 rad, and acos(-x) = pi - acos(x).
+SS_HD float f_acos(float x) {
+  const float ax = fabsf(x);
+  float p = -0.0012624911f;
+  p = p * ax + 0.0066700901f;
+  p = p * ax - 0.0170881256f;
+  p = p * ax + 0.0308918810f;
+  p = p * ax - 0.0501743046f;
+  p = p * ax + 0.0889789874f;
+  p = p * ax - 0.2145988016f;
+  p = p * ax + 1.5707963050f;
+  const float r = f_sqrt(1.0f - ax) * p;
+  return x < 0.0f ? (float)kPi - r : r;
+}
 
 template <typename T>
 SS_HD T t_pow(T x, T y) { return pow(x, y); }
@@ -293,7 +378,9 @@ SS_HD T element_field(T c, T q, bool enabled) {
 
 // dh: departure unit vector; pv0/pv1: trace pattern vectors.
 // PAT: compile-time pattern kind (SS_PAT_*), or -1 to dispatch on pa.kind.
-template <typename T, int PAT = -1>
+// STAGED (float32, tabulated): the caller staged the table's lookup in
+// ``pa.ftab`` / ``pa.fidx``; theta from f_acos.
+template <typename T, int PAT = -1, bool STAGED = false>
 SS_HD T antenna_gain(const PatternArgs& pa, const T dh[3], const T pv0[3],
                      const T pv1[3]) {
   switch (PAT >= 0 ? PAT : pa.kind) {
@@ -351,6 +438,8 @@ SS_HD T antenna_gain(const PatternArgs& pa, const T dh[3], const T pv0[3],
     case SS_PAT_TABULATED: {
       T ct = (dh[0] * pv0[0] + dh[1] * pv0[1]) + dh[2] * pv0[2];
       ct = ct < (T)-1 ? (T)-1 : (ct > (T)1 ? (T)1 : ct);
+      if constexpr (STAGED && std::is_same<T, float>::value)
+        return tab_lookup_staged(pa, f_acos(ct));
       T th = t_acos<T>(ct);
       const double* xp = pa.table;
       const double* fp = pa.table + pa.n;
@@ -670,7 +759,8 @@ SS_HD void incoherent_pair_mixed(const IncohFacet& f, const TraceRec& t, const S
     float dh[3] = {-dxf * ir0, -dyf * ir0, -dzf * ir0};
     float pv0[3] = {(float)t.pv0[0], (float)t.pv0[1], (float)t.pv0[2]};
     float pv1[3] = {(float)t.pv1[0], (float)t.pv1[1], (float)t.pv1[2]};
-    float g = antenna_gain<float, PAT>(sc.pat, dh, pv0, pv1);
+    // the incoherent kernel stages a tabulated pattern (PAT == 5) in LDS
+    float g = antenna_gain<float, PAT, PAT == SS_PAT_TABULATED>(sc.pat, dh, pv0, pv1);
     float g2 = g * g;
     pw *= g2 * g2;
   }

@@ -174,3 +174,24 @@ def refracted(prec, positions, u_ct, target, crossed, eps_leg, att_leg, *, mode,
     if split_sides:
         res = res.reshape(T, 2, n_samples).transpose(0, 2, 1)
     return res, drop, fac.unpermute_pairs(bins), nfb.value
+
+
+def tab_lookup(table_theta, table_gain, th):
+    """The float32 tier's staged tabulated lookup at float32 angles ``th``."""
+    tab = np.ascontiguousarray(np.concatenate([table_theta, table_gain]), np.float64)
+    th = np.ascontiguousarray(th, np.float32)
+    out = np.empty_like(th)
+    h = lib()
+    h.host_tab_lookup.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_long, ctypes.c_void_p]
+    h.host_tab_lookup(_p(tab), len(table_theta), _p(th), len(th), _p(out))
+    return out
+
+
+def f_acos(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    h = lib()
+    h.host_f_acos.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]
+    h.host_f_acos(_p(x), len(x), _p(out))
+    return out

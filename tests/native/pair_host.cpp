@@ -10,6 +10,28 @@
 
 using namespace ss;
 
+// The incoherent kernel's float32 tier reads a tabulated pattern through
+// the lookup it stages in LDS (pattern_lds_entry); the harness builds the
+// same lookup in host memory.
+struct HostPatLookup {
+  std::vector<float> ftab;
+  std::vector<int> fidx;
+};
+
+static void stage_host(PatternArgs& pa, HostPatLookup& st) {
+  pa.ftab = nullptr;
+  pa.fidx = nullptr;
+  pa.fscale = 0.0f;
+  if (pa.kind != SS_PAT_TABULATED || pa.n < 2 || pa.n > kPatLds || !pa.table) return;
+  st.ftab.assign(3 * pa.n, 0.0f);
+  st.fidx.assign(kPatCells, 0);
+  const int m = pa.n > kPatCells ? pa.n : kPatCells;
+  for (int i = 0; i < m; ++i) pattern_lds_entry(pa.table, pa.n, i, st.ftab.data(), st.fidx.data());
+  pa.ftab = st.ftab.data();
+  pa.fidx = st.fidx.data();
+  pa.fscale = pattern_cell_scale(pa.table, pa.n);
+}
+
 static TraceRec load_trace(const double* tr, long ts, long t) {
   TraceRec r;
   double* d = reinterpret_cast<double*>(&r);
@@ -46,6 +68,8 @@ extern "C" int host_surface(int prec, const ss_surface_params* prm, const double
   sc.pat.kind = prm->pattern; sc.pat.n = prm->pat_n; sc.pat.a = prm->pat_a;
   for (int i = 0; i < 4; ++i) sc.pat.b[i] = prm->pat_b[i];
   sc.pat.table = table; sc.cst = cst;
+  sc.pat.ftab = nullptr;
+  sc.pat.fidx = nullptr;
   sc.k_over_pi = prm->k / kPi; sc.kfac = prm->k / kTwoPi;
   sc.k_bin = 2.0 / (prm->c * prm->dt); sc.t0_dt = prm->t0 / prm->dt; sc.cyc_per_m = prm->k / kPi;
   bool split = prm->flags & SS_SPLIT_SIDES;
@@ -107,6 +131,8 @@ extern "C" int host_refracted(int prec, const ss_refracted_params* prm, const do
   rc.pat.kind = prm->pattern; rc.pat.n = prm->pat_n; rc.pat.a = prm->pat_a;
   for (int i = 0; i < 4; ++i) rc.pat.b[i] = prm->pat_b[i];
   rc.pat.table = table; rc.cst = cst; rc.desc = desc; rc.tables = tables; rc.legs = legs;
+  rc.pat.ftab = nullptr;
+  rc.pat.fidx = nullptr;
   rc.k_bin = 2.0 / (prm->c * prm->dt); rc.t0_dt = prm->t0 / prm->dt; rc.cyc_per_m = prm->k0 / kPi;
   rc.joint = prm->refraction == 1; rc.k_pad = prm->k_pad;
   rc.n_newton = prm->joint_newton; rc.n_backtrack = prm->joint_backtrack;
@@ -145,4 +171,20 @@ extern "C" int host_refracted(int prec, const ss_refracted_params* prm, const do
   }
   if (n_fallback) *n_fallback = fb;
   return 0;
+}
+
+// The staged tabulated lookup at given angles, and the float32 acos.
+extern "C" void host_tab_lookup(const double* table, int n, const float* th, long m,
+                                float* out) {
+  PatternArgs pa{};
+  pa.kind = SS_PAT_TABULATED;
+  pa.n = n;
+  pa.table = table;
+  HostPatLookup plk;
+  stage_host(pa, plk);
+  for (long i = 0; i < m; ++i) out[i] = tab_lookup_staged(pa, th[i]);
+}
+
+extern "C" void host_f_acos(const float* x, long m, float* out) {
+  for (long i = 0; i < m; ++i) out[i] = f_acos(x[i]);
 }

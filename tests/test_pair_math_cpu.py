@@ -137,3 +137,58 @@ def test_refracted_pair_math(name):
         else:
             m = ref > ref.max() * 1e-3
             assert np.abs(10 * np.log10(out[m] / ref[m])).max() < 0.05
+
+
+def _interp_f32(xp, fp, th):
+    """np.interp restated in float32 arithmetic on float-rounded tables: the
+    bisection interval, then (y1 - y0) / (x1 - x0) * (th - x0) + y0."""
+    xf, ff = xp.astype(np.float32), fp.astype(np.float32)
+    out = np.empty_like(th)
+    for i, t in enumerate(th):
+        if not t > xf[0]:
+            out[i] = ff[0]
+            continue
+        if not t < xf[-1]:
+            out[i] = ff[-1]
+            continue
+        lo = int(np.searchsorted(xf, t, side="right")) - 1
+        s = np.float32(ff[lo + 1] - ff[lo]) / np.float32(xf[lo + 1] - xf[lo])
+        out[i] = np.float32(np.float32(s * np.float32(t - xf[lo])) + ff[lo])
+    return out
+
+
+@pytest.mark.parametrize("table", ["cos2", "uniform", "clustered", "two"])
+def test_staged_pattern_lookup_equals_bisection(table):
+    """The float32 tier's staged lookup (cell index + walk + staged slope)
+    picks the bisection's interval and value exactly, on the grid points,
+    just beside them and at random angles."""
+    rng = np.random.default_rng(5)
+    if table == "cos2":
+        a = A.cos_power_table(2.0)
+        xp, fp = np.deg2rad(np.asarray(a.theta_deg)), np.asarray(a.gain)
+    elif table == "uniform":
+        xp = np.linspace(0.0, np.pi, 256)
+        fp = rng.uniform(0, 1, 256)
+    elif table == "clustered":
+        xp = np.sort(np.concatenate([rng.uniform(0.0, 0.01, 100), rng.uniform(0.5, 3.0, 50)]))
+        fp = rng.uniform(0, 1, len(xp))
+    else:
+        xp, fp = np.array([0.2, 1.1]), np.array([1.0, 0.25])
+    xf = xp.astype(np.float32)
+    th = np.concatenate([xf, np.nextafter(xf, np.float32(-1)), np.nextafter(xf, np.float32(4)),
+                         rng.uniform(-0.1, np.pi + 0.1, 4000).astype(np.float32)])
+    got = H.tab_lookup(xp, fp, th)
+    want = _interp_f32(xp, fp, th)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_float_acos_within_published_bound():
+    """Abramowitz & Stegun 4.4.46: |acos error| <= 2e-8 rad, plus float32
+    rounding: of the polynomial, of pi (8.7e-8) and of pi - r (ulp(pi) / 2 =
+    1.2e-7) on x < 0 -- about 4e-7 rad in all near acos = pi; a gain slope of
+    at most 1 per radian (the cos^2 table) turns it into < 1e-6 of gain."""
+    x = np.concatenate([np.linspace(-1, 1, 20001), [-1.0, -0.0, 0.0, 1.0]]).astype(np.float32)
+    err = np.abs(H.f_acos(x).astype(np.float64) - np.arccos(x.astype(np.float64)))
+    assert err.max() < 5e-7, err.max()
+    pos = x >= 0
+    assert err[pos].max() < 2.5e-7, err[pos].max()
