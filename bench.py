@@ -244,10 +244,17 @@ def bench_incoherent(wl, dev, steps=10, cpu_traces=8):
     out["mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(res["mixed"] - ref) / np.linalg.norm(ref))
     f = roofline.incoherent_pair_flops("tabulated")
     ach = f * pairs / (out["mixed"]["kernel_ms"] * 1e-3) / 1e12
+    kname = "incoherent_kernel<1, 5>"
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(kname, {}).get("bytes")
     out["roofline"] = {"bound": "valu", "flops_per_pair": f, "achieved": ach,
                        "peak": roofline.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                       "frac": ach / roofline.PEAK_FP32_TFLOPS,
-                       "kernel": "incoherent_kernel<1, 5>"}
+                       "frac": ach / roofline.PEAK_FP32_TFLOPS, "traffic": traffic,
+                       "traffic_source": "profiles/incoherent_kernel_surface_512x512_256traces.md",
+                       "kernel": kname}
     out["speedup_vs_fp64"] = out["fp64"]["kernel_ms"] / out["mixed"]["kernel_ms"]
     from oracle import incoherent as oi
     t = time.time()
