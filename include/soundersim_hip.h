@@ -407,6 +407,28 @@ int ss_trace_diagnostics(const double* facets, int64_t facet_stride, int64_t n_f
                          int64_t* first_index, double* first_twtt, double* first_xyz,
                          void* stream);
 
+/* Along-track boxcar (processing.py:143-238: presum, unfocused_sar,
+ * multilook): out[k, c] = scale * sum_{j < window} in[k * stride + j, c] for
+ * k < n_out, on a row-major (rows, cols) float64 device array (a complex
+ * interleaved (rows, n) array is cols = 2 n).  presum: stride = window, scale
+ * 1 (coherent sum of the field); unfocused SAR: stride 1; multilook: stride =
+ * window, scale 1 / window on power.  (n_out - 1) * stride + window <= rows. */
+int ss_slow_time_boxcar(const double* in, int64_t rows, int64_t cols, int64_t window,
+                        int64_t stride, int64_t n_out, double scale, double* out,
+                        void* stream);
+
+/* Straight-track time-domain backprojection through air (processing.py:
+ * 270-348, focused_sar): field (n_traces, n_bins) complex128 interleaved,
+ * twtt (n_bins) the bin delays (twtt[0] and twtt[1] - twtt[0] give t0 and
+ * dt), s (n_traces) the non-decreasing along-track arc length, k the carrier
+ * wavenumber 2 pi / lambda, window 0 rectangular / 1 hann; out (n_traces,
+ * n_bins) complex128 interleaved (every element written).  SS_PREC_FP64
+ * follows the reference expressions; SS_PREC_MIXED keeps the migrated delay
+ * and phase cycles in float64 and interpolates and sums in float32. */
+int ss_focused_sar(int precision, const double* field, int64_t n_traces, int64_t n_bins,
+                   const double* twtt, const double* s, double c, double k, double aperture_m,
+                   int window, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,7 +47,8 @@ EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_sizeof_job", "ss_range_compress", "ss_compress_fft_size", "ss_pulse_spectrum",
             "ss_range_compress_fft", "ss_batch_table_bytes", "ss_batch_pack",
             "ss_batch_step_workspace_bytes", "ss_batch_step", "ss_facets_from_heights",
-            "ss_facets_workspace_bytes", "ss_trace_diagnostics", "ss_incoherent_power")
+            "ss_facets_workspace_bytes", "ss_trace_diagnostics", "ss_incoherent_power",
+            "ss_slow_time_boxcar", "ss_focused_sar")
 
 
 class SurfaceParams(ctypes.Structure):
@@ -179,6 +180,10 @@ def configure(lib):
     lib.ss_trace_diagnostics.argtypes = [dp, i64, i64, dp, dp, i64, i64, d, d, i64, d,
                                          dp, dp, dp, dp, dp, vp]
     lib.ss_trace_diagnostics.restype = ctypes.c_int
+    lib.ss_slow_time_boxcar.argtypes = [dp, i64, i64, i64, i64, i64, d, dp, vp]
+    lib.ss_slow_time_boxcar.restype = ctypes.c_int
+    lib.ss_focused_sar.argtypes = [i32, dp, i64, i64, dp, dp, d, d, d, i32, dp, vp]
+    lib.ss_focused_sar.restype = ctypes.c_int
     return lib
 
 

@@ -1268,6 +1268,149 @@ static KCommon make_common(const double* facets, int64_t fs, int64_t nf,
   return k;
 }
 
+// ----------------------------------------------- along-track processing
+// Boxcar along slow time (processing.py:143-238): out[k, c] = scale * sum_j
+// in[k * stride + j, c], j < window; a complex (n, 2) interleaved array is
+// 2 n real columns.  One thread per output column, rows by grid-stride.
+__global__ __launch_bounds__(kBlock) void boxcar_kernel(const double* __restrict__ in, long cols,
+                                                        long window, long stride, long n_out,
+                                                        double scale, double* __restrict__ out) {
+  const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= cols) return;
+  for (long k = blockIdx.y; k < n_out; k += gridDim.y) {
+    const double* p = in + k * stride * cols + c;
+    double acc = 0.0;
+    for (long j = 0; j < window; ++j) acc += p[j * cols];
+    out[k * cols + c] = acc * scale;
+  }
+}
+
+// Straight-track time-domain backprojection through air (processing.py:
+// 270-348).  Workgroup = one output trace i x kBlock bins.  The aperture
+// traces j with |s_j - s_i| <= half form one index range of the
+// non-decreasing arc length s (found by bisection, tested exactly per
+// trace); their offsets and window weights are staged in LDS a chunk at a
+// time.  Per (j, bin): dr = sqrt(r0^2 + d^2) - r0, the field read at the
+// migrated delay by linear interpolation (zero unless 0 <= i0 < nb - 1),
+// times w exp(+2 j k dr).  Float64 tier: the reference's expressions and
+// order, contraction off.  Mixed tier: float64 delay and phase cycles
+// (2 k dr / 2 pi), float32 sin/cos, interpolation and sum.
+constexpr int kFocusChunk = kBlock;
+
+__device__ inline long lower_index(const double* s, long n, double v) {
+  long lo = 0, hi = n;  // first index with s >= v
+  while (lo < hi) {
+    const long mid = (lo + hi) >> 1;
+    if (s[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void focus_kernel(const double* __restrict__ field, long T,
+                                                       long nb, const double* __restrict__ twtt,
+                                                       const double* __restrict__ s, double c,
+                                                       double k, double half, int hann,
+                                                       double* __restrict__ out) {
+  __shared__ double sd[kFocusChunk];
+  __shared__ double sw[kFocusChunk];
+  __shared__ long range[2];
+  const int tid = threadIdx.x;
+  const long b = (long)blockIdx.x * kBlock + tid;
+  const double t0 = twtt[0];
+  const double dt = twtt[1] - twtt[0];
+  const double cyc_per_m = 2.0 * k / kTwoPi;
+  for (long i = blockIdx.y; i < T; i += gridDim.y) {
+    const double si = s[i];
+    if (tid == 0) {
+      // one index of margin on each side: the exact test below decides
+      const long lo = lower_index(s, T, si - half) - 1;
+      const long hi = lower_index(s, T, si + half) + 2;  // past the last s <= si + half
+      range[0] = lo < 0 ? 0 : lo;
+      range[1] = hi > T ? T : hi;
+    }
+    __syncthreads();
+    const long j0 = range[0], j1 = range[1];
+    const double tw = b < nb ? twtt[b] : 0.0;
+    double r0;
+    {
+#pragma clang fp contract(off)
+      r0 = c * tw / 2.0;
+    }
+    const double r02 = r0 * r0;
+    double are = 0.0, aim = 0.0;
+    float fre = 0.0f, fim = 0.0f;
+    for (long jc = j0; jc < j1; jc += kFocusChunk) {
+      const int n = (int)((j1 - jc) < kFocusChunk ? (j1 - jc) : kFocusChunk);
+      __syncthreads();  // the previous chunk is consumed
+      if (tid < n) {
+        const double d = s[jc + tid] - si;
+        double w = -1.0;  // outside the aperture
+        if (fabs(d) <= half) {
+#pragma clang fp contract(off)
+          w = hann ? 0.5 * (1.0 + cos(kPi * d / half)) : 1.0;
+        }
+        sd[tid] = d;
+        sw[tid] = w;
+      }
+      __syncthreads();
+      if (b < nb) {
+        for (int q = 0; q < n; ++q) {
+          const double w = sw[q];
+          if (w < 0.0) continue;
+          const double d = sd[q];
+          const double* F = field + (jc + q) * nb * 2;
+          if constexpr (PREC == SS_PREC_FP64) {
+#pragma clang fp contract(off)
+            const double dr = sqrt(r02 + d * d) - r0;
+            const double tau = tw + 2.0 * dr / c;
+            const double x = (tau - t0) / dt;
+            const double fl = floor(x);
+            if (!(fl >= 0.0 && fl < (double)(nb - 1))) continue;
+            const long i0 = (long)fl;
+            const double fr = x - fl;
+            const double sre = F[2 * i0] * (1.0 - fr) + F[2 * i0 + 2] * fr;
+            const double sim = F[2 * i0 + 1] * (1.0 - fr) + F[2 * i0 + 3] * fr;
+            const double ph = (2.0 * k) * dr;
+            const double cp = cos(ph), sp = sin(ph);
+            const double wre = w * sre, wim = w * sim;
+            are = are + (wre * cp - wim * sp);
+            aim = aim + (wre * sp + wim * cp);
+          } else {
+            const double dr = sqrt(r02 + d * d) - r0;
+            const double x = (tw + 2.0 * dr / c - t0) / dt;
+            const double fl = floor(x);
+            if (!(fl >= 0.0 && fl < (double)(nb - 1))) continue;
+            const long i0 = (long)fl;
+            const float fr = (float)(x - fl);
+            const float sre = (float)F[2 * i0] * (1.0f - fr) + (float)F[2 * i0 + 2] * fr;
+            const float sim = (float)F[2 * i0 + 1] * (1.0f - fr) + (float)F[2 * i0 + 3] * fr;
+            const double cyc = dr * cyc_per_m;
+            const float ph = (float)(cyc - rint(cyc));
+            const float cp = f_cos_rev(ph), sp = f_sin_rev(ph);
+            const float wre = (float)w * sre, wim = (float)w * sim;
+            fre += wre * cp - wim * sp;
+            fim += wre * sp + wim * cp;
+          }
+        }
+      }
+    }
+    if (b < nb) {
+      if constexpr (PREC == SS_PREC_FP64) {
+        out[(i * nb + b) * 2] = are;
+        out[(i * nb + b) * 2 + 1] = aim;
+      } else {
+        out[(i * nb + b) * 2] = (double)fre;
+        out[(i * nb + b) * 2 + 1] = (double)fim;
+      }
+    }
+    __syncthreads();  // range[] is rewritten for the next trace
+  }
+}
+
 }  // namespace ss
 
 using namespace ss;
@@ -2179,6 +2322,61 @@ int ss_range_compress(const double* in, int64_t n_rows, int64_t n_samples, const
                      (long)n_samples, (const double2*)pulse, (long)m, (double2*)out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("compress launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_slow_time_boxcar(const double* in, int64_t rows, int64_t cols, int64_t window,
+                        int64_t stride, int64_t n_out, double scale, double* out,
+                        void* stream) {
+  if (rows < 0 || cols < 0 || n_out < 0) return fail(SS_EINVAL, "negative sizes");
+  if (window < 1 || stride < 1) return fail(SS_EINVAL, "window and stride must be >= 1");
+  if (n_out > 0 && (n_out - 1) * stride + window > rows)
+    return fail(SS_EINVAL, "the last window runs past the input rows");
+  if (n_out == 0 || cols == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  if (!in || !out) return fail(SS_EINVAL, "null pointer");
+  if (in == out) return fail(SS_EINVAL, "in and out must not alias");
+  const long gy = n_out < 65535 ? n_out : 65535;
+  dim3 grid((unsigned)((cols + kBlock - 1) / kBlock), (unsigned)gy);
+  hipLaunchKernelGGL(boxcar_kernel, grid, dim3(kBlock), 0, (hipStream_t)stream, in, (long)cols,
+                     (long)window, (long)stride, (long)n_out, scale, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("boxcar launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_focused_sar(int precision, const double* field, int64_t n_traces, int64_t n_bins,
+                   const double* twtt, const double* s, double c, double k, double aperture_m,
+                   int window, double* out, void* stream) {
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_MIXED)
+    return fail(SS_EINVAL, "precision must be SS_PREC_FP64 or SS_PREC_MIXED");
+  if (n_traces < 0 || n_bins < 0) return fail(SS_EINVAL, "negative sizes");
+  if (n_traces > 0 && n_bins < 2) return fail(SS_EINVAL, "need at least two bins");
+  if (window != 0 && window != 1) return fail(SS_EINVAL, "window must be 0 (none) or 1 (hann)");
+  if (!(aperture_m > 0.0) || !(c > 0.0)) return fail(SS_EINVAL, "aperture_m and c must be > 0");
+  if (n_traces == 0) {
+    g_last_error.clear();
+    return SS_OK;
+  }
+  if (!field || !twtt || !s || !out) return fail(SS_EINVAL, "null pointer");
+  if (field == out) return fail(SS_EINVAL, "field and out must not alias");
+  const long gy = n_traces < 65535 ? n_traces : 65535;
+  dim3 grid((unsigned)((n_bins + kBlock - 1) / kBlock), (unsigned)gy);
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == SS_PREC_FP64)
+    hipLaunchKernelGGL(focus_kernel<SS_PREC_FP64>, grid, dim3(kBlock), 0, st, field,
+                       (long)n_traces, (long)n_bins, twtt, s, c, k, aperture_m / 2.0, window,
+                       out);
+  else
+    hipLaunchKernelGGL(focus_kernel<SS_PREC_MIXED>, grid, dim3(kBlock), 0, st, field,
+                       (long)n_traces, (long)n_bins, twtt, s, c, k, aperture_m / 2.0, window,
+                       out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("focus launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
 }

@@ -796,3 +796,35 @@ def range_compress(field, pulse, m=None, out=None, stream=None, method="auto"):
                                        _stream_handle(stream, field.device))
             _lib.check(rc, "ss_range_compress")
     return out, ps
+
+
+def slow_time_boxcar(x, window, stride, n_out, scale=1.0, stream=None):
+    """``ss_slow_time_boxcar`` on a (rows, ...) float64 device tensor
+    (trailing dims flattened into columns); returns (n_out, ...) float64."""
+    x = x.contiguous()
+    rows = x.shape[0]
+    cols = x[0].numel() if rows else 0
+    out = torch.empty((n_out,) + tuple(x.shape[1:]), dtype=torch.float64, device=x.device)
+    rc = _lib.load().ss_slow_time_boxcar(_ptr(x), rows, cols, int(window), int(stride),
+                                         int(n_out), float(scale), _ptr(out),
+                                         _stream_handle(stream, x.device))
+    _lib.check(rc, "ss_slow_time_boxcar")
+    return out
+
+
+def focused_sar(field, twtt, s, *, c, k, aperture_m, window="hann", precision="fp64",
+                stream=None):
+    """``ss_focused_sar`` on a (T, nb, 2) float64 device field; twtt (nb,)
+    and s (T,) device tensors.  Returns (T, nb, 2) float64."""
+    if window not in ("hann", "none", "rect", None):
+        raise ValueError(f"unknown window {window!r} (use 'hann' or 'none')")
+    field = field.contiguous()
+    T, nb = field.shape[0], field.shape[1]
+    out = torch.empty_like(field)
+    prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
+    rc = _lib.load().ss_focused_sar(prec, _ptr(field), T, nb, _ptr(twtt.contiguous()),
+                                    _ptr(s.contiguous()), float(c), float(k),
+                                    float(aperture_m), 1 if window == "hann" else 0,
+                                    _ptr(out), _stream_handle(stream, field.device))
+    _lib.check(rc, "ss_focused_sar")
+    return out

@@ -421,3 +421,51 @@ def test_incoherent_refracted_dropped_is_power(sim):
     assert (dref > 0).all()
     np.testing.assert_allclose(drop.numpy(), dref, rtol=1e-9)
     np.testing.assert_allclose(out.numpy()[..., 0], ref, rtol=1e-9, atol=1e-300)
+
+
+@pytest.mark.parametrize("n,stride", [(4, 4), (4, 1), (1, 1), (9, 9)])
+def test_slow_time_boxcar_vs_oracle(sim, n, stride):
+    """presum / unfocused / multilook windows of ss_slow_time_boxcar against
+    oracle/processing.py's weight matrices (complex as 2 real columns)."""
+    from oracle import processing as opr
+    rng = np.random.default_rng(8)
+    T = 37
+    x = rng.standard_normal((T, 30)) + 1j * rng.standard_normal((T, 30))
+    W = opr.boxcar_weights(T, n, stride)
+    ref = opr.apply_weights(W, x)
+    with _lib.use(sim):
+        xt = torch.as_tensor(np.stack([x.real, x.imag], -1))
+        y = dv.slow_time_boxcar(xt, n, stride, W.shape[0]).numpy()
+        m = dv.slow_time_boxcar(torch.as_tensor(np.abs(x) ** 2), n, stride, W.shape[0],
+                                scale=1.0 / n).numpy()
+    np.testing.assert_allclose(y[..., 0] + 1j * y[..., 1], ref, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m, opr.apply_weights(W, np.abs(x) ** 2, average=True),
+                               rtol=1e-13)
+
+
+@pytest.mark.parametrize("window", ["hann", "none"])
+def test_focused_sar_kernel_vs_oracle(sim, window):
+    """Backprojection kernel, both tiers, against oracle/processing.py on a
+    point-target history plus noise: float64 to 1e-12, mixed to 1e-5."""
+    from oracle import processing as opr
+    rng = np.random.default_rng(9)
+    T, nb, dx, H = 41, 96, 2.0, 600.0
+    lam = C / 150e6
+    k = 2 * np.pi / lam
+    B = 15e6
+    dt = 1 / (4 * B)
+    t0 = 2 * H / C - 30 * dt
+    xs = (np.arange(T) - T // 2) * dx
+    R = np.sqrt(H * H + xs * xs)
+    twtt = t0 + dt * np.arange(nb)
+    F = np.sinc(B * (twtt[None, :] - 2 * R[:, None] / C)) * np.exp(-2j * k * R)[:, None]
+    F = F + 0.05 * (rng.standard_normal(F.shape) + 1j * rng.standard_normal(F.shape))
+    s = opr.along_track(np.column_stack([xs, np.zeros(T), np.full(T, H)]))
+    ref = opr.focused_sar(F, s, twtt, c=C, k=k, aperture_m=30.0, window=window)
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-5)):
+        with _lib.use(sim):
+            out = dv.focused_sar(torch.as_tensor(np.stack([F.real, F.imag], -1)),
+                                 torch.as_tensor(twtt), torch.as_tensor(s), c=C, k=k,
+                                 aperture_m=30.0, window=window, precision=prec).numpy()
+        o = out[..., 0] + 1j * out[..., 1]
+        assert np.linalg.norm(o - ref) <= tol * np.linalg.norm(ref), prec
