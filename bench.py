@@ -35,7 +35,11 @@ Two further sections ride on the same line:
                kernel times and the mixed tier's error per layer;
   ``incoherent`` the power-summing kernel on the surface layer of the same
                scene: both tiers, the mixed tier's error, roofline, and the
-               NumPy oracle timed on 8 traces.
+               NumPy oracle timed on 8 traces;
+  ``processing`` focused SAR (time-domain backprojection, 2 km hann
+               aperture) of the scene's compressed surface field, both tiers,
+               presum / unfocused boxcars, and the NumPy oracle on the
+               first 41 traces.
 """
 
 import argparse
@@ -272,6 +276,79 @@ def bench_incoherent(wl, dev, steps=10, cpu_traces=8):
     return out
 
 
+def bench_processing(field, wl, dev, aperture_m=2000.0, steps=5, cpu_traces=41):
+    """Focused SAR (backprojection) over the compressed surface field of the
+    surface+bed scene (256 traces at 25 m, 4096 bins, 60 MHz) with a 2 km
+    hann aperture (81 traces), both tiers; presum (8) and the unfocused
+    boxcar over the same 81 traces; the oracle's backprojection timed on the
+    first ``cpu_traces`` traces."""
+    import torch
+    from radar_sounder_simulator_amd import device as dv
+    from radar_sounder_simulator_amd import roofline
+    T, nb = field.shape[0], field.shape[1]
+    pos = wl.track.positions
+    seg = np.linalg.norm(np.diff(pos, axis=0), axis=1)
+    s = np.concatenate([[0.0], np.cumsum(seg)])
+    twtt = wl.t0 + np.arange(nb) * wl.dt
+    k = wl.k0
+    ft = torch.as_tensor(field, device=dev)
+    tw = torch.as_tensor(twtt, device=dev)
+    st_ = torch.as_tensor(s, device=dev)
+    n_ap = np.array([int(np.sum(np.abs(s - si) <= aperture_m / 2)) for si in s])
+    terms = int(n_ap.sum()) * nb
+    st = torch.cuda.current_stream()
+    out = {"workload": f"focused SAR of the surface field, {T} traces at 25 m, {nb} bins, "
+                       f"{aperture_m:.0f} m hann aperture", "terms_per_launch": terms}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        dv.focused_sar(ft, tw, st_, c=wl.c, k=k, aperture_m=aperture_m, precision=prec)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        ev[0].record(st)
+        for i in range(steps):
+            y = dv.focused_sar(ft, tw, st_, c=wl.c, k=k, aperture_m=aperture_m, precision=prec)
+            ev[i + 1].record(st)
+        torch.cuda.synchronize()
+        ms = float(np.mean([ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]))
+        res[prec] = y.cpu().numpy()
+        out[prec] = {"kernel_ms": ms, "terms_per_s": terms / (ms * 1e-3)}
+        log(f"bench: focused SAR {prec}: {ms:.3f} ms, {terms / (ms * 1e-3):.3e} terms/s")
+    a = res["mixed"][..., 0] + 1j * res["mixed"][..., 1]
+    b = res["fp64"][..., 0] + 1j * res["fp64"][..., 1]
+    out["mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    f = roofline.FOCUS_TERM_FLOPS
+    ach = f * terms / (out["fp64"]["kernel_ms"] * 1e-3) / 1e12
+    out["roofline"] = {"bound": "valu", "flops_per_term": f, "achieved": ach,
+                       "peak": roofline.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                       "frac": ach / roofline.PEAK_FP64_TFLOPS, "kernel": "focus_kernel<0>"}
+    for name, n, stride in (("presum_8", 8, 8), ("unfocused_81", 81, 1)):
+        n_out = T // n if stride == n else T - n + 1
+        dv.slow_time_boxcar(ft, n, stride, n_out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(steps):
+            dv.slow_time_boxcar(ft, n, stride, n_out)
+        e1.record(st)
+        torch.cuda.synchronize()
+        out[name] = {"kernel_ms": e0.elapsed_time(e1) / steps,
+                     "bytes_read": n_out * n * nb * 16, "bytes_written": n_out * nb * 16}
+    from oracle import processing as opr
+    m = cpu_traces
+    s_sl = s[:m]
+    sl_terms = int(sum(np.sum(np.abs(s_sl - si) <= aperture_m / 2) for si in s_sl)) * nb
+    t = time.time()
+    opr.focused_sar(a[:m].astype(np.complex128), s_sl, twtt, c=wl.c, k=k, aperture_m=aperture_m)
+    el = time.time() - t
+    out["cpu_baseline"] = {"value": sl_terms / el, "unit": "terms/s", "cores": 1,
+                           "kind": "port",
+                           "sample": f"oracle/processing.py focused_sar (NumPy float64) on the "
+                                     f"first {m} traces ({sl_terms} terms)"}
+    log(f"bench: focused SAR mixed vs fp64 rel {out['mixed_vs_fp64_rel_l2']:.2e}; "
+        f"cpu {out['cpu_baseline']['value']:.3e} terms/s")
+    return out
+
+
 def bench_serving(n_requests, lam, scale, dev, seed=7):
     """The request stream on the mixed and on the float64 tier."""
     import torch
@@ -320,6 +397,7 @@ def main():
                     help="traces of the multilayer scene's 1000 in its section")
     ap.add_argument("--no-multilayer", action="store_true")
     ap.add_argument("--no-incoherent", action="store_true")
+    ap.add_argument("--no-processing", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the coherent facet kernel shards by "
@@ -429,6 +507,9 @@ def main():
     if not args.no_incoherent:
         log("bench: incoherent kernel on the surface layer")
         line["incoherent"] = bench_incoherent(wl, dev)
+    if not args.no_processing:
+        log("bench: focused SAR of the surface field")
+        line["processing"] = bench_processing(out_mixed[0], wl, dev)
     del rs
     torch.cuda.empty_cache()
     if not args.no_batch:

@@ -19,6 +19,11 @@ TE Fresnel 14 + transmission 3 + path sums 10 + cos 1 = 1240.
 Final leg + spreading (:500-516) 42; antenna at departure 14 + gain;
 coherent amplitude (:527-532) 27; phase + complex multiply 10; bin 5.
 
+Focused SAR term (processing.py:318-331), one (output trace, aperture
+trace, bin): 30 -- d^2 + r0^2 (2), sqrt (1), dr (1), tau (3), x (2), floor
+and fraction (2), two complex linear interpolations (10), phase (1),
+cos and sin (2), window product (2), complex multiply-add (4).
+
 Incoherent surface pair (kernels/incoherent.py:36-53): 30 -- d (3), r^2
 (5), sqrt (1), d.n (5), /r (1), A cos (1), its square (1), r^4 (2),
 divide (1), twtt (2), bin (3), side (5); a pattern adds the departure
@@ -30,6 +35,7 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X FP64, dense
 PEAK_HBM_GBS = 8000.0
 
 SURFACE_BASE = 59
+FOCUS_TERM_FLOPS = 30
 INCOHERENT_BASE = 30
 CROSSING = 1240
 FINAL_LEG = 42
