@@ -214,24 +214,46 @@ SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
   if constexpr (std::is_same<T, float>::value && RM != 2) {
     // float32 tier, area-only ACFs: the hardware reciprocal / rsqrt for the
     // per-term 1/m and base^-1/2 (IEEE float divisions otherwise cost ~10
-    // instructions each, twice per term)
+    // instructions each, twice per term); the Poisson weight x^m e^-x / m!
+    // by the recurrence P_m = P_(m-1) x / m while e^-x is a normal float
+    // (x < 80: no exp, no lgamma load per term; relative drift <= m ulp),
+    // the log form otherwise; the constant factors 2 pi Lx Ly (pi Lx Ly)
+    // applied once to the sum.
     if (area_only) {
       const float ll = l * l;
       const bool expo = acf == SS_ACF_EXPONENTIAL;
-      for (int m = 1; m <= n_terms; ++m) {
-        const float mf = (float)m;
-        const float im = f_rcp(mf);
-        const float logp = mf * logx - (float)cst[SS_CONST_LGAMMA + m] - x;
-        float term;
-        if (expo) {
-          const float q = ll * (im * im);
-          const float rb = f_rsq((T)1 + ab2 * q);
-          term = t_exp<T>(logp) * ((T)kTwoPi * q * (rb * rb * rb)) * Lx * Ly;
-        } else {
-          term = t_exp<T>(logp - ab2 * ll * (0.25f * im)) * ((T)kPi * ll * im) * Lx * Ly;
+      auto series = [&](auto rec_tag) -> float {
+        constexpr bool REC = decltype(rec_tag)::value;
+        float P = REC ? f_exp(-x) : 0.0f;
+        float sum = 0.0f;
+        for (int m = 1; m <= n_terms; ++m) {
+          const float mf = (float)m;
+          const float im = f_rcp(mf);
+          if constexpr (REC)
+            P *= x * im;
+          else
+            P = f_exp(mf * logx - (float)cst[SS_CONST_LGAMMA + m] - x);
+          float term;
+          if (expo) {
+            const float q = ll * (im * im);
+            const float rb = f_rsq(1.0f + ab2 * q);
+            term = P * (q * (rb * rb * rb));
+          } else {
+            term = P * f_exp(-ab2 * ll * (0.25f * im)) * (ll * im);
+          }
+          sum += term;
+          // exponential ACF: term_{m+1} / term_m <= x / m (the Poisson
+          // weight gives x / (m + 1), q (1 + a q)^-1.5 at q = l^2 / m^2 at
+          // most (m + 1) / m), so from m >= 2 x + 2 on every further term
+          // is at most half the one before and the whole tail at most
+          // ``term``: stop once that is below 1e-6 of the sum (the budget
+          // n_terms is sized for the largest x of the scene)
+          if (expo && mf >= 2.0f * x + 2.0f && term <= 1e-6f * sum) break;
         }
-        acc += term;
-      }
+        return sum;
+      };
+      const float sum = x < 80.0f ? series(std::true_type{}) : series(std::false_type{});
+      acc = sum * ((expo ? (float)kTwoPi : (float)kPi) * Lx * Ly);
       return acc > (T)0 ? acc : (T)0;
     }
   }

@@ -26,6 +26,34 @@ constexpr int kLocalIters = 25;    // reference budget, local plane
 constexpr int kMeanIters32 = 12;   // mixed: float32 anchor iterations
 constexpr int kLocalIters32 = 10;  // mixed: float32 local iterations
 constexpr int kPolish64 = 2;       // mixed: float64 polish steps
+
+// Mixed tier, float64: 1/x and 1/sqrt(x) from the hardware estimates
+// refined by two Newton-Raphson steps (r <- r (2 - x r); y <- y (3 - x y^2)
+// / 2), accurate to about an ulp without the IEEE division's scale / fixup
+// sequence.  The host build divides.
+SS_HD double rcp64(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
+SS_HD double rsq64(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  return fma(y, e, y);
+#else
+  return 1.0 / sqrt(x);
+#endif
+}
 constexpr double kCellGuard = 2e-4;  // cells; float32 anchor error ~1e-5
 constexpr double kResLo = 1e-11, kResHi = 1e-7;  // ambiguous residual band
 // float32 Newton loops stop once a full step moves sigma by at most this
@@ -167,15 +195,15 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
     c1 = c1 > kTiny ? c1 : kTiny;
     double c2 = (1.0 - v) * (1.0 + v);
     c2 = c2 > kTiny ? c2 : kTiny;
-    double q1 = 1.0 / sqrt(c1), q2 = 1.0 / sqrt(c2);
+    double q1 = rsq64(c1), q2 = rsq64(c2);
     double F = (ar * sg * q1 + bd * v * q2) - Lh;
     double Fp = ar * (q1 * q1 * q1) + (bd * ratio) * (q2 * q2 * q2);
-    double sn = sg - F / (Fp > kTiny ? Fp : kTiny);
+    double sn = sg - F * rcp64(Fp > kTiny ? Fp : kTiny);
     sg = sn < 0.0 ? 0.0 : (sn > hi064 ? hi064 : sn);
   }
   double c1 = (1.0 - sg) * (1.0 + sg);
   c1 = c1 > kTiny ? c1 : kTiny;
-  double tr = ar * sg / sqrt(c1);
+  double tr = ar * sg * rsq64(c1);
   double tt = swap ? Lh - tr : tr;
   double t = tt < 0.0 ? 0.0 : (tt > Lh ? Lh : tt);
   double d2 = Lh - t;
@@ -183,7 +211,7 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
   r1 = r1 > kTiny ? r1 : kTiny;
   double r2 = sqrt(bh * bh + d2 * d2);
   r2 = r2 > kTiny ? r2 : kTiny;
-  double ir1 = 1.0 / r1, ir2 = 1.0 / r2;
+  double ir1 = rcp64(r1), ir2 = rcp64(r2);
   out.res = n1 * t * ir1 - n2 * d2 * ir2;
   out.s1 = s * r1;
   double st = s * t;

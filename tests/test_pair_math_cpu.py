@@ -31,7 +31,8 @@ def _err(out, ref):
 
 
 SURF = ["smooth", "taper", "rough_gauss", "rough_area", "rough_exp", "split", "interp",
-        "gamma_facet", "tabulated", "dipole", "finite_dipole", "array", "array_tapered"]
+        "gamma_facet", "tabulated", "dipole", "finite_dipole", "array", "array_tapered",
+        "rough_exp_deep", "rough_area_deep"]
 
 
 @pytest.mark.parametrize("name", SURF)
@@ -52,6 +53,10 @@ def test_surface_pair_math(name):
                  rough_area=dict(roughness=(0.3, 5.0, ph, 12), d_phi_area=True),
                  rough_exp=dict(roughness=(0.3, 5.0, ph, 12, "exponential"), d_phi_area=True,
                                 taper_s=0.05),
+                 # (sigma K)^2 ~ 100 > 80: the float32 tier's log-form series
+                 rough_exp_deep=dict(roughness=(4.0, 5.0, ph, 220, "exponential"),
+                                     d_phi_area=True, taper_s=0.05),
+                 rough_area_deep=dict(roughness=(4.0, 5.0, ph, 220), d_phi_area=True),
                  split=dict(split_sides=True), interp=dict(interp_bins=True),
                  gamma_facet=dict(gamma=np.linspace(-0.3, -0.2, len(f))))
     if name in ants:
