@@ -972,7 +972,7 @@ SS_HD void refracted_pair_fp64(const RefrFacet& f, const TraceRec& t, const Refr
 
 // Mixed tier: the joint solve itself stays float64 (its damped Newton needs
 // the float64 merit comparisons); radiometry runs in the mixed tier.
-template <int NMAX, int RM = -1, int PAT = -1, typename FT = RefrFacet>
+template <int NMAX, int RM = -1, int PAT = -1, typename FT = RefrFacet, bool STAGED = false>
 SS_HD bool refracted_pair_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
                                 PairOut& o) {
   PathSums ps;
@@ -984,7 +984,7 @@ SS_HD bool refracted_pair_mixed(const FT& f, const TraceRec& t, const RefrConst&
     else if (!seq_path_mixed(f, t, rc, ps))
       return false;
   }
-  return radiometry_mixed<RM, PAT, FT>(f, t, rc, ps, o);
+  return radiometry_mixed<RM, PAT, FT, STAGED>(f, t, rc, ps, o);
 }
 
 }  // namespace ss

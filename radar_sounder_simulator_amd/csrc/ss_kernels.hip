@@ -296,7 +296,7 @@ __host__ __device__ inline PatternArgs make_pattern(int kind, int n, double a, c
 // Stages the float32 lookup of a tabulated pattern of 2 .. kPatLds samples
 // (pattern_lds_entry) in ``ptab`` / ``pidx`` and points ``pa`` at it;
 // visible after the caller's next barrier.
-__device__ inline void stage_pattern(PatternArgs& pa, float* ptab, int* pidx) {
+__device__ inline void stage_pattern(PatternArgs& pa, float* ptab, short* pidx) {
   const int m = pa.n > kPatCells ? pa.n : kPatCells;
   for (int i = threadIdx.x; i < m; i += kBlock) pattern_lds_entry(pa.table, pa.n, i, ptab, pidx);
   pa.ftab = ptab;
@@ -455,9 +455,18 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
   } else {
     SurfFacet32 f32;
     to_f32(f64, f32);
+    // the tabulated instantiation reads its pattern through the lookup
+    // staged in LDS (the host picks it for tables of <= kPatLds samples)
+    constexpr bool ST = PAT == SS_PAT_TABULATED;
+    SurfaceConst scl = sc;
+    if constexpr (ST) {
+      __shared__ float ptab[3 * kPatLds];
+      __shared__ short pidx[kPatCells];
+      stage_pattern(scl.pat, ptab, pidx);
+    }
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o, int) {
-                               surface_pair_mixed<RM, PAT>(f32, nullptr, t, sc, o);
+                               surface_pair_mixed<RM, PAT, ST>(f32, nullptr, t, scl, o);
                              });
   }
 }
@@ -513,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void incoherent_kernel(KCommon k, SurfaceCo
     // float32 gain from the staged lookup (the host picks this
     // instantiation for tables of at most kPatLds samples)
     __shared__ float ptab[3 * kPatLds];
-    __shared__ int pidx[kPatCells];
+    __shared__ short pidx[kPatCells];
     stage_pattern(scl.pat, ptab, pidx);
   }
   __syncthreads();
@@ -642,10 +651,18 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
       load_refr_facet(k, fi, rc.flags, fr);
       to_f32(fr, f32);
     }
+    constexpr bool ST = PAT == SS_PAT_TABULATED;  // staged pattern lookup
+    RefrConst rcl = rc;
+    if constexpr (ST) {
+      __shared__ float ptab[3 * kPatLds];
+      __shared__ short pidx[kPatCells];
+      stage_pattern(rcl.pat, ptab, pidx);
+    }
     trace_loop<ACC, kTTRefr, true>(sh, k, w, live, f, rc.n_samples, split,
                                    [&](const TraceRec& t, PairOut& o, int) {
                                      if ((rc.flags & SS_REDO_ALL) ||
-                                         !refracted_pair_mixed<NMAX, RM, PAT>(f32, t, rc, o)) {
+                                         !refracted_pair_mixed<NMAX, RM, PAT, RefrFacet32, ST>(
+                                             f32, t, rcl, o)) {
                                        o.valid = false;
                                        o.pwr_invalid = 0.0;
                                        o.redo = true;
@@ -1236,6 +1253,13 @@ static int pattern_mode(int kind) {
   return -1;
 }
 
+// Single launches: their tabulated instantiations stage the table in LDS
+// (at most kPatLds samples); larger tables take the generic one.
+static int launch_pattern_mode(int kind, int n) {
+  if (kind == SS_PAT_TABULATED && n > kPatLds) return -1;
+  return pattern_mode(kind);
+}
+
 static int check_pattern(int kind, int n, const double* table) {
   if (kind < SS_PAT_ISOTROPIC || kind > SS_PAT_TABULATED)
     return fail(SS_EINVAL, "unknown antenna pattern kind");
@@ -1691,7 +1715,7 @@ int ss_surface_field(int precision, const ss_surface_params* prm, const double* 
                           field, dropped, pair_bins, kTTSurf, n_seg);
   const SurfaceConst sc = make_surface_const(*prm, pattern_table, constants);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
-  const int rm = rough_mode(prm->flags), pm = pattern_mode(prm->pattern);
+  const int rm = rough_mode(prm->flags), pm = launch_pattern_mode(prm->pattern, prm->pat_n);
 #define SS_SURF(P, R, A) \
   hipLaunchKernelGGL((surface_kernel<P, R, A>), grid, block, 0, st, k, sc)
 #define SS_SURF_PA(P, R) \
@@ -1769,10 +1793,7 @@ int ss_incoherent_power(int precision, const ss_incoherent_params* prm, const do
   k.per_xcd = ((long)k.n_ftiles * k.n_ttiles + 7) / 8;
   const SurfaceConst sc = make_surface_const(sp, pattern_table, nullptr);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
-  // the tabulated instantiation stages its table in LDS (<= kPatLds samples)
-  const int pm = (prm->pattern == SS_PAT_TABULATED && prm->pat_n > kPatLds)
-                     ? -1
-                     : pattern_mode(prm->pattern);
+  const int pm = launch_pattern_mode(prm->pattern, prm->pat_n);
 #define SS_INC(P, A) \
   hipLaunchKernelGGL((incoherent_kernel<P, A>), grid, block, 0, st, k, sc)
 #define SS_INC_P(P) \
@@ -1859,7 +1880,7 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
                                        constants);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
   const bool fp64 = precision == SS_PREC_FP64;
-  const int rm = rough_mode(prm->flags), pm = pattern_mode(prm->pattern);
+  const int rm = rough_mode(prm->flags), pm = launch_pattern_mode(prm->pattern, prm->pat_n);
   const long n_items = (long)k.n_ftiles * k.n_ttiles;
   const Redo rd = fp64 ? Redo{} : redo_view(workspace, n_items);
   // Redo pass: each block sweeps chunks of kBlock item flags.

@@ -241,14 +241,9 @@ SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
           } else {
             term = P * f_exp(-ab2 * ll * (0.25f * im)) * (ll * im);
           }
+          // (no early exit: the per-term test and the divergent break cost
+          // more than the tail terms it could skip)
           sum += term;
-          // exponential ACF: term_{m+1} / term_m <= x / m (the Poisson
-          // weight gives x / (m + 1), q (1 + a q)^-1.5 at q = l^2 / m^2 at
-          // most (m + 1) / m), so from m >= 2 x + 2 on every further term
-          // is at most half the one before and the whole tail at most
-          // ``term``: stop once that is below 1e-6 of the sum (the budget
-          // n_terms is sized for the largest x of the scene)
-          if (expo && mf >= 2.0f * x + 2.0f && term <= 1e-6f * sum) break;
         }
         return sum;
       };
@@ -296,7 +291,7 @@ struct PatternArgs {
   // antenna_gain<float, SS_PAT_TABULATED, true> (the incoherent kernel's
   // float32 tier); null elsewhere
   const float* ftab;   // theta[n] | gain[n] | slope[n - 1], float
-  const int* fidx;     // kPatCells first-interval indices
+  const short* fidx;   // kPatCells first-interval indices
   float fscale;        // kPatCells / (theta[n-1] - theta[0])
 };
 
@@ -304,16 +299,18 @@ struct PatternArgs {
 // kPatLds samples): the table rounded to float, the slope of every interval
 // evaluated as the interpolation evaluates it, and for kPatCells uniform
 // cells over [theta_0, theta_{n-1}] the interval holding each cell's lower
-// edge, so a lookup starts within a step of its interval.
+// edge, so a lookup starts within a step of its interval (finer cells than
+// table intervals: one step up or down at most on a table whose samples are
+// no closer than its range / kPatCells).
 constexpr int kPatLds = 256;
-constexpr int kPatCells = 256;
+constexpr int kPatCells = 1024;
 
 SS_HD float pattern_cell_scale(const double* table, int n) {
   return (float)kPatCells / ((float)table[n - 1] - (float)table[0]);
 }
 
 // Entry i of the staged lookup (i < max(n, kPatCells)).
-SS_HD void pattern_lds_entry(const double* table, int n, int i, float* ftab, int* fidx) {
+SS_HD void pattern_lds_entry(const double* table, int n, int i, float* ftab, short* fidx) {
   if (i < n) {
     ftab[i] = (float)table[i];
     ftab[n + i] = (float)table[n + i];
@@ -334,7 +331,7 @@ SS_HD void pattern_lds_entry(const double* table, int n, int i, float* ftab, int
       else
         hi = mid;
     }
-    fidx[i] = lo;
+    fidx[i] = (short)lo;
   }
 }
 
@@ -350,13 +347,17 @@ SS_HD float tab_lookup_staged(const PatternArgs& pa, float th) {
   int cell = (int)((fminf(fmaxf(th, xa), xb) - xa) * pa.fscale);
   cell = cell < 0 ? 0 : (cell > kPatCells - 1 ? kPatCells - 1 : cell);
   int lo = pa.fidx[cell];
-  // fixed steps (a cell holds at most two samples on a table no denser than
-  // kPatCells / 2 over its range), then the general walk, rarely entered
+  // one step either way (the cell index rounds), then the general walk for
+  // tables with several samples in a cell, rarely entered
   lo -= (lo > 0 && xf[lo] > th) ? 1 : 0;
   lo += (lo < n - 2 && xf[lo + 1] <= th) ? 1 : 0;
-  lo += (lo < n - 2 && xf[lo + 1] <= th) ? 1 : 0;
-  while (lo < n - 2 && xf[lo + 1] <= th) ++lo;
-  while (lo > 0 && xf[lo] > th) --lo;
+  if (lo < n - 2 && xf[lo + 1] <= th) {
+    do ++lo;
+    while (lo < n - 2 && xf[lo + 1] <= th);
+  } else if (lo > 0 && xf[lo] > th) {
+    do --lo;
+    while (lo > 0 && xf[lo] > th);
+  }
   float g = sl[lo] * (th - xf[lo]) + ff[lo];
   g = th < xb ? g : ff[n - 1];
   return th > xa ? g : ff[0];  // (NaN: ff[0], as the bisection's first test)
@@ -619,7 +620,7 @@ SS_HD void surface_pair_fp64(const SurfFacet64& f, const TraceRec& t,
 // ---------------------------------------------------------------------------
 constexpr double kBinGuard = 1e-7;   // bin units; fast-path error is ~1e-10
 
-template <int RM = -1, int PAT = -1>
+template <int RM = -1, int PAT = -1, bool STAGED = false>
 SS_HD void surface_pair_mixed(const SurfFacet32& f, const SurfFacet64* f64,
                               const TraceRec& t, const SurfaceConst& sc,
                               PairOut& o) {
@@ -685,7 +686,7 @@ SS_HD void surface_pair_mixed(const SurfFacet32& f, const SurfFacet64* f64,
     float dh[3] = {-rx, -ry, -rz};
     float pv0[3] = {(float)t.pv0[0], (float)t.pv0[1], (float)t.pv0[2]};
     float pv1[3] = {(float)t.pv1[0], (float)t.pv1[1], (float)t.pv1[2]};
-    float g = antenna_gain<float, PAT>(sc.pat, dh, pv0, pv1);
+    float g = antenna_gain<float, PAT, STAGED>(sc.pat, dh, pv0, pv1);
     float g2 = g * g;
     re *= g2;
     im *= g2;

@@ -628,7 +628,7 @@ SS_HD bool seq_path_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
 // Mixed tier, final leg + radiometry + bin.  Returns false when the bin
 // position lies within kBinGuard of an edge (the caller redoes the pair on
 // the float64 tier).
-template <int RM = -1, int PAT = -1, typename FT = RefrFacet>
+template <int RM = -1, int PAT = -1, typename FT = RefrFacet, bool STAGED = false>
 SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
                             const PathSums& ps, PairOut& o) {
   const int flags = rc.flags;
@@ -674,7 +674,7 @@ SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
     float dh[3] = {d0[0] * idm, d0[1] * idm, d0[2] * idm};
     float pv0[3] = {(float)t.pv0[0], (float)t.pv0[1], (float)t.pv0[2]};
     float pv1[3] = {(float)t.pv1[0], (float)t.pv1[1], (float)t.pv1[2]};
-    float g = antenna_gain<float, PAT>(rc.pat, dh, pv0, pv1);
+    float g = antenna_gain<float, PAT, STAGED>(rc.pat, dh, pv0, pv1);
     spread *= g * g;
   }
   float re, im, pwr;

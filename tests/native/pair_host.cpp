@@ -15,7 +15,7 @@ using namespace ss;
 // same lookup in host memory.
 struct HostPatLookup {
   std::vector<float> ftab;
-  std::vector<int> fidx;
+  std::vector<short> fidx;
 };
 
 static void stage_host(PatternArgs& pa, HostPatLookup& st) {
@@ -68,8 +68,8 @@ extern "C" int host_surface(int prec, const ss_surface_params* prm, const double
   sc.pat.kind = prm->pattern; sc.pat.n = prm->pat_n; sc.pat.a = prm->pat_a;
   for (int i = 0; i < 4; ++i) sc.pat.b[i] = prm->pat_b[i];
   sc.pat.table = table; sc.cst = cst;
-  sc.pat.ftab = nullptr;
-  sc.pat.fidx = nullptr;
+  HostPatLookup plk;
+  stage_host(sc.pat, plk);  // the float32 tier reads a staged tabulated table
   sc.k_over_pi = prm->k / kPi; sc.kfac = prm->k / kTwoPi;
   sc.k_bin = 2.0 / (prm->c * prm->dt); sc.t0_dt = prm->t0 / prm->dt; sc.cyc_per_m = prm->k / kPi;
   bool split = prm->flags & SS_SPLIT_SIDES;
@@ -99,6 +99,7 @@ extern "C" int host_surface(int prec, const ss_surface_params* prm, const double
       TraceRec r = load_trace(tr, ts, t);
       PairOut o{};
       if (prec == SS_PREC_FP64) surface_pair_fp64(a, r, sc, o);
+      else if (sc.pat.ftab) surface_pair_mixed<-1, -1, true>(b, nullptr, r, sc, o);
       else surface_pair_mixed(b, nullptr, r, sc, o);
       accumulate(o, t, n, n_seg, split, field, dropped);
       if (bins) bins[t * nf + f] = o.raw_bin;
@@ -131,8 +132,8 @@ extern "C" int host_refracted(int prec, const ss_refracted_params* prm, const do
   rc.pat.kind = prm->pattern; rc.pat.n = prm->pat_n; rc.pat.a = prm->pat_a;
   for (int i = 0; i < 4; ++i) rc.pat.b[i] = prm->pat_b[i];
   rc.pat.table = table; rc.cst = cst; rc.desc = desc; rc.tables = tables; rc.legs = legs;
-  rc.pat.ftab = nullptr;
-  rc.pat.fidx = nullptr;
+  HostPatLookup plk;
+  stage_host(rc.pat, plk);
   rc.k_bin = 2.0 / (prm->c * prm->dt); rc.t0_dt = prm->t0 / prm->dt; rc.cyc_per_m = prm->k0 / kPi;
   rc.joint = prm->refraction == 1; rc.k_pad = prm->k_pad;
   rc.n_newton = prm->joint_newton; rc.n_backtrack = prm->joint_backtrack;
@@ -161,7 +162,8 @@ extern "C" int host_refracted(int prec, const ss_refracted_params* prm, const do
       TraceRec r = load_trace(tr, ts, t);
       PairOut o{};
       if (prec == SS_PREC_FP64) refracted_pair_fp64<SS_MAX_JOINT>(a, r, rc, o);
-      else if (!refracted_pair_mixed<SS_MAX_JOINT>(a, r, rc, o)) {
+      else if (!(rc.pat.ftab ? refracted_pair_mixed<SS_MAX_JOINT, -1, -1, RefrFacet, true>(a, r, rc, o)
+                             : refracted_pair_mixed<SS_MAX_JOINT>(a, r, rc, o))) {
         ++fb;
         refracted_pair_fp64<SS_MAX_JOINT>(a, r, rc, o);
       }
