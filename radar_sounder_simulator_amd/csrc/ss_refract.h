@@ -152,14 +152,14 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
   double w[3] = {fq[0] - fp[0], fq[1] - fp[1], fq[2] - fp[2]};
   double L = sqrt((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
   double Lm = L > kTiny ? L : kTiny;
-  double iLm = 1.0 / Lm;
+  double iLm = rcp64(Lm);
   double u[3] = {w[0] * iLm, w[1] * iLm, w[2] * iLm};
   double sm = (a + b) + L;
   double s = sm > kTiny ? sm : kTiny;
-  double inv_s = 1.0 / s;
+  double inv_s = rcp64(s);
   double ah = a * inv_s, bh = b * inv_s, Lh = L * inv_s;
   bool swap = n1 > n2;
-  double ratio = (n1 < n2 ? n1 : n2) / (n1 > n2 ? n1 : n2);
+  double ratio = (n1 < n2 ? n1 : n2) * rcp64(n1 > n2 ? n1 : n2);
   double ar = swap ? bh : ah;
   double bd = swap ? ah : bh;
   // float32 Newton in sigma (same safeguarded iteration)
