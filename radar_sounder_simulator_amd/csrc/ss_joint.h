@@ -415,10 +415,39 @@ struct FlatStack {
   }
 };
 
+// Float64 division, reciprocal square root and square root of the
+// flat-stack passes: IEEE for the float64 tier (FAST = false), the
+// Newton-refined hardware estimates of rcp64 / rsq64 for the mixed tier's
+// float64 step and pass (FAST = true; sqrt keeps 0 -> 0 and x < 0 -> NaN).
+template <bool FAST>
+SS_HD double q_div(double a, double b) {
+  if constexpr (FAST)
+    return a * rcp64(b);
+  else
+    return a / b;
+}
+template <bool FAST>
+SS_HD double q_rsq(double x) {
+  if constexpr (FAST)
+    return rsq64(x);
+  else
+    return 1.0 / sqrt(x);
+}
+template <bool FAST>
+SS_HD double q_sqrt(double x) {
+  if constexpr (FAST) {
+    const double y = x * rsq64(x > 0.0 ? x : 1.0);
+    return x > 0.0 ? y : (x == 0.0 ? x : __builtin_nan(""));
+  } else {
+    return sqrt(x);
+  }
+}
+
 // The path of slowness v: F(v) and (jac) dF/dv in the plane basis B, the
 // surface crossing s, its climb length factor lam, c_1, n.u; returns
 // feasibility (every c real, lam > 0, the climbing ray meets the plane from
 // below).
+template <bool FAST = false>
 SS_HD bool flat_eval(const FlatStack& fs, const double p[3], const double q[3],
                      const double pt0[3], const double nr0[3], const double B[2][3],
                      const double v[2], bool jac, double F[2], double J[2][2], double s[3],
@@ -432,7 +461,7 @@ SS_HD bool flat_eval(const FlatStack& fs, const double p[3], const double q[3],
     for (int k = 2; k <= fs.N; ++k) {
       const double ck2 = nl[k] * nl[k] - vv;
       ok = ok && (ck2 > 0.0);
-      const double rk = 1.0 / sqrt(ck2);
+      const double rk = q_rsq<FAST>(ck2);
       const double dk = fs.d(k, q[2]);
       a += dk * rk;
       b += dk * (rk * rk * rk);
@@ -667,6 +696,7 @@ SS_HD void flat_newton64(const FlatStack& fs, const double p[3], const double q[
 // transmission cosines are c_k/n_k and c_{k+1}/n_{k+1}, so the TE
 // coefficient is (c_k - c_{k+1})/(c_k + c_{k+1}) (the reference's formula
 // with its clamp where c_k/n_k < C_MIN).
+template <bool FAST = false>
 SS_HD double flat_final(const FlatStack& fs, const double p[3], const double q[3],
                         const double pt0[3], const double nr0[3], const double v[2],
                         PathSums& ps) {
@@ -686,7 +716,7 @@ SS_HD double flat_final(const FlatStack& fs, const double p[3], const double q[3
     double gm;
     if (flat && ci_raw >= kCMin) {
       const double a0 = nl[i] * ci_raw, a1 = nl[i + 1] * ct_raw;   // c_i, c_{i+1}
-      gm = (a0 - a1) / (a0 + a1);
+      gm = q_div<FAST>(a0 - a1, a0 + a1);
     } else {
       const double e1 = epsl[i], e2 = epsl[i + 1];
       const double sin2 = (e1 / e2) * (1.0 - ci * ci);
@@ -708,25 +738,25 @@ SS_HD double flat_final(const FlatStack& fs, const double p[3], const double q[3
   // interfaces 1..N-1
   double a = 0.0, c_prev = c1, c_last = 0.0;
   for (int k = 2; k <= N; ++k) {
-    const double ck = sqrt(nl[k] * nl[k] - vv);
-    const double rk = 1.0 / ck;
+    const double ck = q_sqrt<FAST>(nl[k] * nl[k] - vv);
+    const double rk = q_div<FAST>(1.0, ck);
     const double dk = fs.d(k, q[2]);
     a += dk * rk;
     fin = fin && isfinite(rk);
     sides = sides && (dk > 0.0);
     evan = evan || (vm > nl[k] * (1.0 + 64.0 * kEps64));
     // interface k-1 (medium k-1 -> k)
-    cross(k - 1, c_prev / nl[k - 1], ck / nl[k], true);
+    cross(k - 1, q_div<FAST>(c_prev, nl[k - 1]), q_div<FAST>(ck, nl[k]), true);
     if (k < N) {
       // segment k: length |d_k| n_k / c_k; incidence cosine c_k / n_k
       const double ad = fabs(dk);
       const double len = ad * nl[k] * rk;
-      const double cik = ck / nl[k];
+      const double cik = q_div<FAST>(ck, nl[k]);
       const double ci = fmax(cik, kCMin);
       opl += nl[k] * len;
-      loss += len * (attl[k] / 1000.0);
+      loss += len * q_div<FAST>(attl[k], 1000.0);
       perp += ad * rk;
-      par += len / (nl[k] * ci * ci);
+      par += q_div<FAST>(len, nl[k] * ci * ci);
     }
     c_prev = ck;
     c_last = ck;
@@ -933,12 +963,12 @@ SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst
   if (!stuck) {
     // one float64 Newton step
     double F[2], J[2][2], s[3], lam, c1, nu;
-    if (!flat_eval(fs, p, q, pt0, nr0, B, v, true, F, J, s, lam, c1, nu)) return false;
+    if (!flat_eval<true>(fs, p, q, pt0, nr0, B, v, true, F, J, s, lam, c1, nu)) return false;
     const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
     v[0] -= (J[1][1] * F[0] - J[0][1] * F[1]) / det;
     v[1] -= (J[0][0] * F[1] - J[1][0] * F[0]) / det;
   }
-  const double resid = flat_final(fs, p, q, pt0, nr0, v, ps);
+  const double resid = flat_final<true>(fs, p, q, pt0, nr0, v, ps);
   return stuck ? !ps.valid : (ps.valid && resid <= 1e-12);
 }
 
