@@ -435,12 +435,10 @@ SS_HD double q_rsq(double x) {
 }
 template <bool FAST>
 SS_HD double q_sqrt(double x) {
-  if constexpr (FAST) {
-    const double y = x * rsq64(x > 0.0 ? x : 1.0);
-    return x > 0.0 ? y : (x == 0.0 ? x : __builtin_nan(""));
-  } else {
+  if constexpr (FAST)
+    return sqrt64(x);
+  else
     return sqrt(x);
-  }
 }
 
 // The path of slowness v: F(v) and (jac) dF/dv in the plane basis B, the

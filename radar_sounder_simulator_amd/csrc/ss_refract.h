@@ -54,6 +54,11 @@ SS_HD double rsq64(double x) {
   return 1.0 / sqrt(x);
 #endif
 }
+// sqrt as x rsq64(x), keeping sqrt(0) = 0 and sqrt(x < 0) = NaN.
+SS_HD double sqrt64(double x) {
+  const double y = x * rsq64(x > 0.0 ? x : 1.0);
+  return x > 0.0 ? y : (x == 0.0 ? x : __builtin_nan(""));
+}
 constexpr double kCellGuard = 2e-4;  // cells; float32 anchor error ~1e-5
 constexpr double kResLo = 1e-11, kResHi = 1e-7;  // ambiguous residual band
 // float32 Newton loops stop once a full step moves sigma by at most this
