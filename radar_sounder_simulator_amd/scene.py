@@ -4,9 +4,8 @@ Facets follow the reference's record (``scene.py:80-101``): per rectangular
 mean-plane facet a centre, a unit +z-oriented normal, the parallelogram area
 |e1 x e2| and the two mean edge vectors, plus the source grid cell and grid
 shape (the refracted kernel's crossing lookup uses both).  Coordinates are a
-planar local frame in metres (east, north, up); the reference's WGS84/ENU
-geodesy (pyproj) is not part of this library -- callers that have geodetic
-scenes convert them to the local frame first.
+local frame in metres (east, north, up): a planar lattice here, or the
+WGS84 ENU frame of a projected DEM built by ``geodesy.build_facets``.
 
 Corner math restates ``scene.py:158-176`` / ``compare/brute_force_layered.py
 :44-71``: from the (ny+1, nx+1) vertex lattice V, e1 = ((v01 + v11) - (v00 +
