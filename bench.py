@@ -474,14 +474,21 @@ def main():
     achieved = f_bed * pairs_l[1] / (ms_b * 1e-3) / 1e12
     kname = "refracted_kernel<1, 0, 1, 5>"      # mixed, sequential, area-only D_Phi, tabulated
     traffic = None
+    prof = {}
     tpath = os.path.join(HERE, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(kname, {}).get("bytes")
+            prof = json.load(f).get(kname, {})
+            traffic = prof.get("bytes")
     roof = {"bound": "valu", "achieved": achieved, "peak": roofline.PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / roofline.PEAK_FP32_TFLOPS,
             "traffic": traffic, "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE "
             "(calibrated by a 1 GiB copy), profiles/refracted_kernel_mixed_traffic_surface_bed.md",
+            # algorithmic flops count a transcendental as one and leave out the
+            # integer, compare, select and conversion instructions; the VALU
+            # issue counters of the same launch show how busy the vector ALUs are
+            "valu_busy": prof.get("valu_busy"), "valu_utilization": prof.get("valu_utilization"),
+            "valu_source": prof.get("valu_source"),
             "kernel": kname,
             "flops_per_pair": f_bed, "kernel_ms": ms_b,
             "surface_kernel": {"flops_per_pair": f_surf, "kernel_ms": ms_s,
