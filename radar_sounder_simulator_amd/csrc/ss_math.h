@@ -200,8 +200,11 @@ SS_HD T f_factor(T m, T a0, T edge, T l, const double* cst) {
 
 // D_Phi, clamped at 0 (exactly 0 when (sigma K)^2 == 0).
 // RM (compile-time roughness mode): 1 area-only terms, 2 finite-facet
-// Gaussian terms, -1 decided by ``area_only`` at run time.
-template <typename T, int RM = -1>
+// Gaussian terms, -1 decided by ``area_only`` at run time.  PAIRED: the
+// float32 area-only series shares one reciprocal between two terms (the
+// refracted kernels, whose occupancy it does not change; it costs the
+// surface kernel a wave per SIMD).
+template <typename T, int RM = -1, bool PAIRED = false>
 SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
               bool area_only_rt, int acf, const double* cst) {
   const bool area_only = RM == 1 ? true : (RM == 2 ? false : area_only_rt);
@@ -221,34 +224,55 @@ SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
     // applied once to the sum.
     if (area_only) {
       const float ll = l * l;
+      const float c = ab2 * ll;
       const bool expo = acf == SS_ACF_EXPONENTIAL;
+      // exponential term q rb^3 with q = l^2 / m^2, rb = (1 + ab2 q)^-1/2
+      // rewritten as l^2 m (m^2 + ab2 l^2)^-3/2 (m^2 exact in float32 up
+      // to m = 4096); the common factor l^2 of both ACFs applied once
       auto series = [&](auto rec_tag) -> float {
         constexpr bool REC = decltype(rec_tag)::value;
         float P = REC ? f_exp(-x) : 0.0f;
         float sum = 0.0f;
-        for (int m = 1; m <= n_terms; ++m) {
-          const float mf = (float)m;
-          const float im = f_rcp(mf);
+        auto add_term = [&](int m, float mf, float im) {
           if constexpr (REC)
             P *= x * im;
           else
             P = f_exp(mf * logx - (float)cst[SS_CONST_LGAMMA + m] - x);
           float term;
           if (expo) {
-            const float q = ll * (im * im);
-            const float rb = f_rsq(1.0f + ab2 * q);
-            term = P * (q * (rb * rb * rb));
+            const float r = f_rsq(mf * mf + c);
+            term = (P * mf) * (r * r * r);
           } else {
-            term = P * f_exp(-ab2 * ll * (0.25f * im)) * (ll * im);
+            term = P * f_exp(-c * (0.25f * im)) * im;
           }
-          // (no early exit: the per-term test and the divergent break cost
-          // more than the tail terms it could skip)
           sum += term;
+        };
+        // No early exit: the per-term test and the divergent break cost
+        // more than the tail terms it could skip.
+        float mf = 0.0f;
+        if constexpr (PAIRED) {
+          // terms in pairs sharing one reciprocal: 1/m = (m+1) / (m (m+1)),
+          // 1/(m+1) = m / (m (m+1)) (m (m+1) exact in float32 up to m =
+          // 4095); one more live pair of registers than the plain loop
+          int m = 1;
+          for (; m < n_terms; m += 2) {
+            const float a = mf + 1.0f, b = mf + 2.0f;
+            const float rab = f_rcp(a * b);
+            add_term(m, a, b * rab);
+            add_term(m + 1, b, a * rab);
+            mf = b;
+          }
+          if (m == n_terms) add_term(m, mf + 1.0f, f_rcp(mf + 1.0f));
+        } else {
+          for (int m = 1; m <= n_terms; ++m) {
+            mf += 1.0f;
+            add_term(m, mf, f_rcp(mf));
+          }
         }
         return sum;
       };
       const float sum = x < 80.0f ? series(std::true_type{}) : series(std::false_type{});
-      acc = sum * ((expo ? (float)kTwoPi : (float)kPi) * Lx * Ly);
+      acc = sum * ll * ((expo ? (float)kTwoPi : (float)kPi) * Lx * Ly);
       return acc > (T)0 ? acc : (T)0;
     }
   }
