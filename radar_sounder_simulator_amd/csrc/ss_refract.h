@@ -334,6 +334,9 @@ SS_HD void to_f32(const RefrFacet& a, RefrFacet32& b) {
   b.dph_im = (float)a.dph_im;
 }
 
+// FAST (the mixed tier): the normal's 1/|n| by the refined reciprocal
+// square root instead of an IEEE division of an IEEE square root.
+template <bool FAST = false>
 SS_HD void lookup_cell(const RefrConst& rc, int i, int row, int col, double pt[3],
                        double nr[3]) {
 #pragma clang fp contract(off)
@@ -345,7 +348,8 @@ SS_HD void lookup_cell(const RefrConst& rc, int i, int row, int col, double pt[3
   pt[1] = g[1];
   pt[2] = g[2];
   double n0 = g[3], n1 = g[4], n2 = g[5];
-  double inn = 1.0 / sqrt((n0 * n0 + n1 * n1) + n2 * n2);
+  const double nn = (n0 * n0 + n1 * n1) + n2 * n2;
+  double inn = FAST ? rsq64(nn) : 1.0 / sqrt(nn);
   nr[0] = n0 * inn;
   nr[1] = n1 * inn;
   nr[2] = n2 * inn;
@@ -577,7 +581,7 @@ SS_HD bool seq_path_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
     int row = clamp_round(rc0, (int)d[SS_IDESC_NY] - 1);
     int col = clamp_round(rc1, (int)d[SS_IDESC_NX] - 1);
     double pt[3], nr[3];
-    lookup_cell(rc, i, row, col, pt, nr);
+    lookup_cell<true>(rc, i, row, col, pt, nr);
     SnellMixed s;
     snell_mixed(cur, f.c, pt, nr, nl[i], nl[i + 1], s);
     if (s.ambiguous) return false;
