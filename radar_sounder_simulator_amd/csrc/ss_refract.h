@@ -247,7 +247,7 @@ SS_HD void snell_anchor32(const float p[3], const float q[3], const float nrm[3]
   float is = f_rcp(s);
   float ah = a * is, bh = b * is, Lh = L * is;
   bool swap = n1 > n2;
-  float ratio = fminf(n1, n2) / fmaxf(n1, n2);
+  float ratio = fminf(n1, n2) * f_rcp(fmaxf(n1, n2));
   float ar = swap ? bh : ah;
   float bd = swap ? ah : bh;
   float hi0 = 1.0f - 8.0f * 1.1920929e-7f;
