@@ -64,14 +64,29 @@ __device__ inline long uniform_l(long v) {
   return (long)(((unsigned long)hi << 32) | lo);
 }
 
-__device__ inline int wave_min_i(int v) {
-  for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+// Wave min / max, called with every lane of the wave active; the result is
+// wave-uniform.  On the device: four DPP steps make each 16-lane row uniform
+// (quad swaps [1,0,3,2] and [2,3,0,1], half-row mirror, row mirror), then the
+// four row values are read into scalars -- no LDS permute round trips.  The
+// CPU emulation of the kernels takes the butterfly of lane shuffles.
+template <bool MAX>
+__device__ inline int wave_ext_i(int v) {
+  auto f = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+#if defined(__HIP_DEVICE_COMPILE__)
+  v = f(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+  v = f(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+  v = f(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));
+  v = f(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));
+  const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+  const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+  return f(f(a, b), f(c, d));
+#else
+  for (int o = 32; o >= 1; o >>= 1) v = f(v, __shfl_xor(v, o, 64));
   return v;
+#endif
 }
-__device__ inline int wave_max_i(int v) {
-  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
-}
+__device__ inline int wave_min_i(int v) { return wave_ext_i<false>(v); }
+__device__ inline int wave_max_i(int v) { return wave_ext_i<true>(v); }
 
 struct KCommon {
   const double* facets;
