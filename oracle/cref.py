@@ -81,7 +81,9 @@ def _params(k, gamma, t0, dt, n, c, taper_s, roughness, d_phi_area, pattern, nth
 
 def coherent_cluttergram(positions, u_ct, centers, normals, areas, e1, e2, *, k, gamma,
                          t0, dt, n_samples, c, pattern=None, roughness=None, taper_s=None,
-                         d_phi_area=False, nthreads=1):
+                         d_phi_area=False, nthreads=1, return_bins=False):
+    """Surface field (T, n) complex128 and dropped (T,); ``return_bins``
+    adds the (T, F) int32 floor bins of every pair."""
     pos = _a(positions)
     T, F = len(pos), len(centers)
     p, ph, tab, pv = _params(k, gamma, t0, dt, n_samples, c, taper_s, roughness,
@@ -91,15 +93,21 @@ def coherent_cluttergram(positions, u_ct, centers, normals, areas, e1, e2, *, k,
     field = np.zeros((T, n_samples, 2))
     drop = np.zeros(T)
     C, N, A, E1, E2 = (_a(v) for v in (centers, normals, areas, e1, e2))
+    bins = np.empty((T, F), np.int32) if return_bins else None
     lib().oracle_surface(ctypes.byref(p), _ptr(pos), _ptr(pv), _ptr(rref), ctypes.c_long(T),
                          _ptr(C), _ptr(N), _ptr(A), _ptr(E1), _ptr(E2), _ptr(gf), _ptr(ph),
-                         ctypes.c_long(F), _ptr(field), _ptr(drop))
-    return field[..., 0] + 1j * field[..., 1], drop
+                         ctypes.c_long(F), _ptr(field), _ptr(drop), _ptr(bins))
+    out = field[..., 0] + 1j * field[..., 1]
+    return (out, drop, bins) if return_bins else (out, drop)
 
 
 def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *, t0, dt,
                           n_samples, c, gamma=0.0, k0=None, pattern=None, roughness=None,
-                          taper_s=None, d_phi_area=False, nthreads=1, mode="coherent"):
+                          taper_s=None, d_phi_area=False, nthreads=1, mode="coherent",
+                          return_bins=False):
+    """Sequential-chain refracted field (T, n) complex128 and dropped (T,);
+    ``return_bins`` adds the (T, F) int32 floor bins, INT32_MIN where the
+    path is invalid."""
     if mode != "coherent":
         raise ValueError("C oracle: coherent mode only")
     pos = _a(positions)
@@ -128,9 +136,11 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
     C, N, A, E1, E2 = (_a(v) for v in (target.centers, target.normals, target.areas,
                                        target.e1, target.e2))
     desc = _a(desc)
+    bins = np.empty((T, F), np.int32) if return_bins else None
     lib().oracle_refracted(ctypes.byref(p), _ptr(pos), _ptr(pv), ctypes.c_long(T), _ptr(C),
                            _ptr(N), _ptr(A), _ptr(E1), _ptr(E2), _ptr(gf), _ptr(ph),
                            ctypes.c_long(F), ctypes.c_int(len(crossed)), _ptr(desc),
                            _ptr(tables), _ptr(nl), _ptr(eps), _ptr(att), _ptr(field),
-                           _ptr(drop))
-    return field[..., 0] + 1j * field[..., 1], drop
+                           _ptr(drop), _ptr(bins))
+    out = field[..., 0] + 1j * field[..., 1]
+    return (out, drop, bins) if return_bins else (out, drop)

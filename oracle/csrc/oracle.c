@@ -11,8 +11,13 @@
  * chain), :500-583 (radiometry).  Feature subset: the full finite-facet
  * Gaussian D_Phi, array patterns, split sides and interp bins are served
  * by the NumPy oracle only.
+ *
+ * bins (nullable): (T, F) int32, the floor bin of every pair clamped to
+ * [-2147483647, 2147483646] as the kernels report it, or INT32_MIN for a
+ * refracted pair whose path is invalid.
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -20,6 +25,10 @@
 #define TWO_PI 6.28318530717958647692
 #define TINY 1e-30
 #define CMIN 1e-9
+
+static int32_t clamp_bin(double fb) {
+  return fb < -2147483647.0 ? -2147483647 : (fb > 2147483646.0 ? 2147483646 : (int32_t)fb);
+}
 
 static double sincn(double x) {
   double y = PI * (x == 0.0 ? 1.0e-20 : x);
@@ -82,7 +91,7 @@ typedef struct {
 int oracle_surface(const oc_params* P, const double* pos, const double* pv, const double* rref,
                    long T, const double* cen, const double* nrm, const double* area,
                    const double* e1, const double* e2, const double* gf, const double* ph,
-                   long F, double* field, double* dropped) {
+                   long F, double* field, double* dropped, int32_t* bins) {
   memset(field, 0, sizeof(double) * T * P->n * 2);
   memset(dropped, 0, sizeof(double) * T);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(P->nthreads)
@@ -130,6 +139,7 @@ int oracle_surface(const oc_params* P, const double* pos, const double* pv, cons
         im *= gg * gg;
       }
       double b = floor((2.0 * r / P->c - P->t0) / P->dt);
+      if (bins) bins[t * F + f] = clamp_bin(b);
       if (b >= 0.0 && b < (double)P->n) {
         h[2 * (long)b] += re;
         h[2 * (long)b + 1] += im;
@@ -205,7 +215,7 @@ int oracle_refracted(const oc_params* P, const double* pos, const double* pv, lo
                      const double* e2, const double* gf, const double* ph, long F,
                      int n_crossed, const double* desc, const double* tables,
                      const double* n_leg, const double* eps_leg, const double* att_leg,
-                     double* field, double* dropped) {
+                     double* field, double* dropped, int32_t* bins) {
   memset(field, 0, sizeof(double) * T * P->n * 2);
   memset(dropped, 0, sizeof(double) * T);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(P->nthreads)
@@ -300,6 +310,7 @@ int oracle_refracted(const oc_params* P, const double* pos, const double* pv, lo
       double phase = -2.0 * P->k * opl, sp = sin(phase), cp = cos(phase);
       double re = -(are * sp + aim * cp), im = are * cp - aim * sp;
       double b = floor((2.0 * opl / P->c - P->t0) / P->dt);
+      if (bins) bins[t * F + f] = valid ? clamp_bin(b) : INT32_MIN;
       if (valid && b >= 0.0 && b < (double)P->n) {
         h[2 * (long)b] += re;
         h[2 * (long)b + 1] += im;

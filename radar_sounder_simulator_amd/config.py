@@ -4,8 +4,13 @@
 Same model names, field names, defaults and ``ValueError`` cases: waveform
 chirp parameters, antenna pattern parameters, media/interface count, roughness
 and grazing-fix coherent-only rules, exponential ACF requiring the grazing fix,
-offset-interface references.  Extension: ``WaveformConfig.window`` also
-accepts ``"tukey"`` (with ``tukey_alpha``).
+offset-interface references.  Extensions: ``WaveformConfig.window`` also
+accepts ``"tukey"`` (with ``tukey_alpha``); ``SimConfig.precision`` picks the
+device tier ("mixed" or "fp64"); ``SimConfig.joint_form`` picks the joint
+solver: "reference" (default) the reference's 2N-unknown damped Newton,
+"flat_stack" its two-unknown form for stacks whose interfaces below the
+surface are horizontal planes (opt-in; it converges where the reference's
+budget can stop short).
 """
 
 from typing import Annotated, Literal, Optional, Union
@@ -183,6 +188,7 @@ class SimConfig(BaseModel):
     roughness_seed: int = 0
     diffuse_exponent: float = 1.0
     precision: Literal["mixed", "fp64"] = "mixed"
+    joint_form: Literal["reference", "flat_stack"] = "reference"
     radar: RadarConfig
     facets: FacetConfig = Field(default_factory=FacetConfig)
     media: list[Medium] = Field(default_factory=_default_media)

@@ -602,14 +602,14 @@ class DeviceInterfaces:
 def refracted_params(target, ifaces, eps_leg, att_leg, *, mode, t0, dt, n_samples, c,
                      gamma=0.0, k0=None, split_sides=False, pattern=None, roughness=None,
                      taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
-                     joint_newton=6, joint_backtrack=4, extra_flags=0, flat_stack="auto"):
+                     joint_newton=6, joint_backtrack=4, extra_flags=0, flat_stack=False):
     """(RefractedParams, legs tensor, pattern table or None) for the
     refracted kernel over the (possibly padded) stack ``ifaces``;
     ``eps_leg``/``att_leg`` are the unpadded per-leg media.  ``extra_flags``
     is ORed into the flags (e.g. SS_REDO_ALL in tests).  ``flat_stack``
-    (joint only): "auto" takes the flat-stack joint solve (refraction code
-    2) when every real interface below the surface is a horizontal plane,
-    False always the general one, True requires the flat one."""
+    (joint only): False (default) the reference's general joint solve,
+    "auto" the flat-stack form (refraction code 2) when every real interface
+    below the surface is a horizontal plane, True requires the flat one."""
     coherent = mode == "coherent"
     flags = _lib.SS_COHERENT if coherent else 0
     flags |= int(extra_flags)
@@ -665,7 +665,7 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
                     taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
                     joint_newton=6, joint_backtrack=4, out=None,
                     dropped=None, pair_bins=False, workspace=None, extra_flags=0,
-                    stream=None, flat_stack="auto"):
+                    stream=None, flat_stack=False):
     """Enqueue the refracted kernel (sequential chain, joint solve or its
     flat-stack form, see ``refracted_params``) and,
     for the mixed tier, its float64 redo launch.  ``workspace``: a uint8

@@ -145,15 +145,20 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
                           refraction="sequential", pad_to=None,
                           joint_newton=6, joint_backtrack=4, window_cull=True,
                           precision="mixed", device=None, return_bins=False,
-                          flat_stack="auto"):
+                          flat_stack=False):
     """Binned refracted-path contributions of one target interface.
 
     Same conventions as the reference (module docstring); returns
     ``(out, dropped)`` with out complex (coherent) or real power
-    (incoherent), (T, n[, 2]).  ``flat_stack`` (joint only): "auto" runs
-    the joint solve in its flat-stack form when every crossed interface
-    below the surface is a horizontal plane (then stacks up to 64 deep are
-    accepted), False always in its general form."""
+    (incoherent), (T, n[, 2]).  ``flat_stack`` (joint only, an opt-in
+    beyond the reference): False (default) runs the reference's joint solve
+    (2 unknowns per crossed interface, damped Newton with its budgets; the
+    stack, pads included, at most 32 deep); True or "auto" runs its
+    two-unknown flat-stack form when every crossed interface below the
+    surface is a horizontal plane (stacks up to 64 deep).  The flat-stack
+    form converges on deep stacks where the reference's 6 x 4 budget can
+    stop short, so a few far off-nadir lanes that the reference drops are
+    valid there."""
     coherent = mode == "coherent"
     if mode not in ("coherent", "incoherent"):
         raise ValueError(f"unknown mode {mode!r}")
@@ -184,7 +189,8 @@ def refracted_cluttergram(positions, u_ct, target, crossed, eps_leg, att_leg, *,
     joint = refraction == "joint"
     flat = joint and flat_stack is not False and all(_dv.is_horizontal(f) for f in crossed[1:])
     if joint and not flat and (pad_to or len(crossed)) > 32:
-        raise ValueError("joint refraction supports stacks of at most 32 interfaces")
+        raise ValueError("joint refraction supports stacks of at most 32 interfaces "
+                         "(flat_stack=True takes horizontal stacks up to 64 deep)")
     if precision not in ("mixed", "fp64"):
         raise ValueError("precision must be 'mixed' or 'fp64'")
     _check_gpu()

@@ -67,12 +67,19 @@ class SimResult:
         return cls(field, dropped, nad, t0 + dt * np.arange(n), np.asarray(positions),
                    list(layers) if layers is not None else None, attrs,
                    first_return_twtt, first_return_xyz,
-                   None if power is None else np.asarray(power, np.float32),
+                   None if power is None else _power_dtype(power),
                    first_return_llh)
 
     def sel_layer(self, name):
         j = self.layer.index(name)
         return (self.field if self.field is not None else self.power_data)[..., j]
+
+
+def _power_dtype(power):
+    """Incoherent power as float32 (the reference's dtype) unless it arrives
+    as float64 (the FP64 tier), which is kept."""
+    p = np.asarray(power)
+    return p if p.dtype == np.float64 else p.astype(np.float32)
 
 
 def combine(res, dim="layer"):

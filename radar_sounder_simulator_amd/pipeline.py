@@ -99,7 +99,7 @@ class ResidentScene:
                 precision=precision, split_sides=wl.split_sides, pattern=wl.pattern,
                 roughness=rough, taper_s=wl.taper_s, d_phi_area=wl.d_phi_area,
                 out=self.raw[j], dropped=self.dropped[j], workspace=self.workspace[j],
-                refraction=L.refraction, stream=stream)
+                refraction=L.refraction, flat_stack=L.flat_stack, stream=stream)
         self._keep.append(res[-1])
 
     def compress(self, stream=None):

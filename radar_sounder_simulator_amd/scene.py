@@ -17,6 +17,7 @@ FFT convolution of white noise with exp(-rho^2/a^2), a = l/sqrt(2),
 normalised by sqrt(sum g^2) so the periodic field has unit variance).
 """
 
+import warnings
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -71,6 +72,21 @@ def facets_from_vertices(V):
     cell = np.stack([ci.ravel(), cj.ravel()], axis=1)
     return Facets(centers, normals, mag, e1.reshape(-1, 3), e2.reshape(-1, 3),
                   cell, (ny, nx), cell.astype(np.int64).copy())
+
+
+def check_facet_size(facets, wavelength, min_range, beta=0.5):
+    """Fresnel-zone validity of the linear-phase approximation
+    (``scene.py:179-195`` of the reference): warns when the longest facet edge
+    exceeds beta sqrt(wavelength min_range) (half a Fresnel-zone radius by
+    default); returns the largest edge / limit ratio."""
+    edge = np.maximum(np.linalg.norm(facets.e1, axis=1), np.linalg.norm(facets.e2, axis=1))
+    limit = beta * np.sqrt(wavelength * min_range)
+    ratio = float(edge.max() / limit)
+    if ratio > 1.0:
+        warnings.warn(f"max facet edge {edge.max():.1f} m exceeds the Fresnel-zone limit "
+                      f"{limit:.1f} m (ratio {ratio:.2f}): the linear-phase approximation "
+                      "may be invalid; subdivide the scene")
+    return ratio
 
 
 def grid_facets(heights, spacing, origin=(0.0, 0.0)):

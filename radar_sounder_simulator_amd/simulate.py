@@ -24,6 +24,7 @@ The output is a ``SimResult`` (``output.py``); interface kinds dem / flat /
 offset resolve as in ``layered.py:56-118`` (offsets shift along +z here).
 """
 
+import warnings
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -36,7 +37,7 @@ from .kernels import (coherent_cluttergram, incoherent_cluttergram, refracted_cl
 from .output import SimResult
 from .physics import fresnel_normal
 from .roughness import n_terms_for, speckle_phasors, speckle_phasors_for_keys
-from .scene import Facets, grid_facets
+from .scene import Facets, check_facet_size, grid_facets
 from .waveform import apply_waveform
 
 
@@ -67,10 +68,45 @@ def _pattern(rc, scene, track):
     return _ant.pattern_args(ant, track.u_at, track.u_ct, roll)
 
 
+def warn_quantization_alias(rc):
+    """Warn when a chirped coherent run has the envelope-quantisation alias
+    in band (``simulate.py:71-95`` of the reference): binning quantises the
+    envelope delay to dt, which plants noise at the aliased carrier
+    f_a = |f0 - round(f0 dt)/dt|; the compressed pulse passes it when
+    f_a < B/2."""
+    wf = rc.waveform
+    if wf.kind != "chirp" or wf.interp_bins:
+        return
+    f_alias = abs(rc.f0 - round(rc.f0 * rc.dt) / rc.dt)
+    if f_alias < wf.bandwidth / 2.0:
+        warnings.warn(
+            f"chirped run with the envelope-quantization alias in band "
+            f"(|f0 - round(f0*dt)/dt| = {f_alias / 1e6:.1f} MHz < B/2 = "
+            f"{wf.bandwidth / 2e6:.1f} MHz): expect a nonphysical off-nadir floor; "
+            "set waveform.interp_bins=true or choose dt so the alias falls out of band")
+
+
+def _check_facet_sizes(ifaces, positions, lam, eps):
+    """Fresnel-zone facet-size check of every interface with the in-medium
+    wavelength above it (``simulate.py:210-213, 300-304``)."""
+    for j, f in enumerate(ifaces):
+        lam_j = lam / np.sqrt(eps[j])
+        min_range = max(float(positions[:, 2].min() - f.centers[:, 2].max()), lam_j)
+        check_facet_size(f, lam_j, min_range)
+
+
 def _roughness_args(iface, facets, k_local, seed):
     rc = iface.roughness
     if rc is None or rc.sigma_m == 0.0:
         return None
+    edge = max(float(np.linalg.norm(facets.e1, axis=1).max()),
+               float(np.linalg.norm(facets.e2, axis=1).max()))
+    if rc.corr_length_m > edge:
+        # simulate.py:157-162: roughness above the facet scale is facet tilt
+        warnings.warn(f"interface {iface.name or '?'}: roughness corr_length_m = "
+                      f"{rc.corr_length_m:g} m exceeds the facet size ({edge:.1f} m); "
+                      "roughness at scales above the facet is really facet tilt -- "
+                      "put it in the DEM")
     n_terms = n_terms_for((2.0 * k_local * rc.sigma_m) ** 2)
     ph = (speckle_phasors_for_keys(facets.phase_keys, seed)
           if facets.phase_keys is not None else speckle_phasors(len(facets), seed))
@@ -162,6 +198,9 @@ def simulate(scene, sim_config: SimConfig, device=None):
     eps = [m.eps_r for m in cfg.media]
     att = [m.attenuation_db_per_km for m in cfg.media]
     k0 = 2.0 * np.pi / rc.wavelength if coherent else None
+    if coherent:
+        warn_quantization_alias(rc)
+        _check_facet_sizes(ifaces, track.positions, rc.wavelength, eps)
     window = dict(t0=rc.t0, dt=rc.dt, n_samples=rc.n_samples, c=rc.c,
                   split_sides=cfg.split_sides, pattern=_pattern(rc, scene, track),
                   precision=cfg.precision, device=device)
@@ -211,6 +250,7 @@ def simulate(scene, sim_config: SimConfig, device=None):
                 track.positions, track.u_ct, target, ifaces[:j], eps[:j + 1], att[:j + 1],
                 mode=cfg.mode, gamma=gamma_j if coherent else 0.0, k0=k0, refraction=refr,
                 pad_to=joint_pad_to(j, n_max) if refr == "joint" else None,
+                flat_stack=refr == "joint" and cfg.joint_form == "flat_stack",
                 roughness=rough, crossed_sigma=crossed_sig, diffuse=dif,
                 **(gfx_kw if coherent else {}), **window)
         outs.append(out)

@@ -108,21 +108,29 @@ def apply_waveform_device(field, waveform, dt, stream=None):
 def apply_waveform(out, radar, mode):
     """Reference-signature ``apply_waveform(out, radar, mode)``
     (``waveform.py:141-160``): ``radar`` has ``.waveform`` and ``.dt``.
-    Coherent: complex field (T, n[, ...]) -> complex64 after the device
-    convolution.  Incoherent: |p|^2 envelope only when
-    ``waveform.incoherent_envelope`` (else unchanged)."""
+
+    Coherent: the complex field (T, n[, ...]) convolved along fast time on
+    the device in float64.  The result keeps the input's precision: a
+    complex64 field (the reference's kernel output, and the mixed tier's)
+    comes back complex64 as in the reference; a complex128 field (the FP64
+    tier's) comes back complex128, so a float64 run stays float64 end to end.
+    Incoherent: the |p|^2 power envelope only when
+    ``waveform.incoherent_envelope`` is set (else unchanged), clamped at 0,
+    float32 (float64 for a float64 input).  Delta waveforms return ``out``
+    itself."""
     wf = radar.waveform
     if wf.kind == "delta":
         return out
     arr = np.asarray(out)
     if mode != "coherent" and not wf.incoherent_envelope:
         return out
+    wide = arr.dtype in (np.complex128, np.float64)
     p, m = compressed_pulse(wf.bandwidth, wf.pulse_length, radar.dt, wf.window,
                             wf.construction, getattr(wf, "tukey_alpha", 0.2))
     if mode != "coherent":
         p = np.abs(p) ** 2
-    # move fast time to the last-but-one axis: rows = (T, trailing...)
-    x = np.moveaxis(np.asarray(arr, np.complex128), 1, -1)  # (T, ..., n)
+    # fast time (axis 1) to the last axis: rows = (T, trailing...)
+    x = np.moveaxis(np.asarray(arr, np.complex128), 1, -1)
     shp = x.shape
     flat = np.ascontiguousarray(x.reshape(-1, shp[-1]))
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -133,5 +141,6 @@ def apply_waveform(out, radar, mode):
     yc = (yh[..., 0] + 1j * yh[..., 1]).reshape(shp)
     yc = np.moveaxis(yc, -1, 1)
     if mode == "coherent":
-        return yc.astype(np.complex64)
-    return np.maximum(yc.real, 0.0).astype(np.float32)
+        return yc if wide else yc.astype(np.complex64)
+    y = np.maximum(yc.real, 0.0)
+    return y if wide else y.astype(np.float32)

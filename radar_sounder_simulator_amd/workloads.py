@@ -53,6 +53,7 @@ class Layer:
     refraction: str = "sequential"  # or "joint" (padded to pad_to planes)
     pad_to: int = None
     lattice: object = None          # scene.Lattice the facets tessellate, if any
+    flat_stack: bool = False        # joint: the two-unknown flat-stack form (opt-in)
 
 
 @dataclass
@@ -146,14 +147,17 @@ def _pad_bucket(j, n_max):
 
 
 def config3(nx=2048, ny=1024, n_traces=1000, seed=3, n_samples=8192, n_firn=20,
-            firn_depth=100.0, trace_spacing=30.0, horizons=(500.0, 1000.0, 1500.0)):
+            firn_depth=100.0, trace_spacing=30.0, horizons=(500.0, 1000.0, 1500.0),
+            flat_stack=True):
     """Multilayer scene: 2048 x 1024 facets at 30 m; firn rho(z) = 917 -
     567 exp(-z/30 m) in 20 horizontal sublayers with n = 1 + 0.845 rho; three
     horizontal horizons at 500/1000/1500 m (-60 dB); rough bed (sigma
     400 m, l 3 km); 1000 traces at 20 km, cos^2 pattern, 150 MHz centre,
     30 MHz chirp with a Tukey(0.2) pedestal, 8192 samples.  Layers: surface,
     the n_firn sublayer bases, the horizons, the bed (all targets); a layer
-    j >= 2 takes the joint solve over its crossed stack."""
+    j >= 2 takes the joint solve over its crossed stack, in its flat-stack
+    form when ``flat_stack`` (every crossed interface below the surface is
+    horizontal here), else the reference's general form."""
     rng_s = np.random.default_rng([seed, 0])
     rng_b = np.random.default_rng([seed, 1])
     dx = 30.0
@@ -194,13 +198,14 @@ def config3(nx=2048, ny=1024, n_traces=1000, seed=3, n_samples=8192, n_firn=20,
         layers.append(Layer(name, plane(z), gam, crossed=list(crossed), eps_leg=list(eps_leg),
                             att_leg=list(att_leg),
                             refraction="joint" if j > 1 else "sequential",
-                            pad_to=_pad_bucket(len(crossed), n_max) if j > 1 else None))
+                            pad_to=_pad_bucket(len(crossed), n_max) if j > 1 else None,
+                            flat_stack=flat_stack and j > 1))
         crossed.append(layers[-1].facets)
         eps_leg.append(e_below)
         att_leg.append(10.0)
     layers.append(Layer("bed", bed, -10.0 ** (-10.0 / 20.0), crossed=list(crossed),
                         eps_leg=list(eps_leg), att_leg=list(att_leg), refraction="joint",
-                        pad_to=_pad_bucket(len(crossed), n_max)))
+                        pad_to=_pad_bucket(len(crossed), n_max), flat_stack=flat_stack))
     dt = 1.0 / 60e6
     return Workload(
         name="cfg3_multilayer", track=track, layers=layers, k0=k0,

@@ -21,8 +21,10 @@ def build():
                     for h in ("ss_math.h", "ss_refract.h", "ss_joint.h")]
     if os.path.exists(SO) and all(os.path.getmtime(SO) >= os.path.getmtime(d) for d in deps):
         return SO
+    tmp = f"{SO}.{os.getpid()}.tmp"
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared",
-                    "-Wno-unknown-pragmas", "-o", SO, src], check=True)
+                    "-Wno-unknown-pragmas", "-o", tmp, src], check=True)
+    os.replace(tmp, SO)  # atomic: parallel test workers never load a partial file
     return SO
 
 

@@ -25,12 +25,14 @@ def build(asan=False):
                                         "ss_joint.h")]
     if os.path.exists(out) and all(os.path.getmtime(out) >= os.path.getmtime(d) for d in deps):
         return out
+    tmp = f"{out}.{os.getpid()}.tmp"
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-D__HIPCC__",
            "-I", os.path.join(HERE, "simt"), "-x", "c++", os.path.join(CSRC, "ss_kernels.hip"),
-           "-o", out, "-lpthread"]
+           "-o", tmp, "-lpthread"]
     if asan:
         cmd[1:1] = ["-fsanitize=address", "-fno-omit-frame-pointer"]
     subprocess.run(cmd, check=True)
+    os.replace(tmp, out)  # atomic: parallel test workers never load a partial file
     return out
 
 

@@ -25,6 +25,17 @@ MAX_DB = 0.05
 MAX_MRAD = 2.0
 
 
+# every antenna kind of antenna.py:204-258 besides isotropic and the two
+# cases above, on a track along +x (dipoles along track, arrays across it)
+ANTENNAS = {
+    "dipole": A.Antenna(kind="dipole"),
+    "finite_dipole": A.Antenna(kind="finite_dipole", length_lam=0.8),
+    "array_tapered": A.Antenna(kind="array_tapered", n_elements=4, element_directivity_db=6.0,
+                               tx_weights=[0.5, 1.0, 1.0, 0.5], rx_weights=[1.0, 0.8, 0.8, 1.0],
+                               axis="cross_track"),
+}
+
+
 def _err(out, ref, floor_db=30.0):
     pk = np.abs(ref).max(axis=-1, keepdims=True) if ref.ndim == 2 else np.abs(ref).max()
     m = np.abs(ref) > pk * 10 ** (-floor_db / 20)
@@ -58,11 +69,13 @@ def _surface_case(name):
                                          tr.u_at, tr.u_ct))
     elif name == "window_edge":
         kw.update(n_samples=40)   # cut the window: most facets dropped
+    elif name in ANTENNAS:
+        kw.update(pattern=A.pattern_args(ANTENNAS[name], tr.u_at, tr.u_ct))
     return wl, f, tr, kw
 
 
 SURFACE_CASES = ["smooth", "rough_exp", "rough_gauss", "split", "interp", "gamma_facet",
-                 "tabulated", "array", "window_edge"]
+                 "tabulated", "array", "window_edge", *ANTENNAS]
 
 
 @pytest.mark.parametrize("name", SURFACE_CASES)
@@ -101,7 +114,7 @@ def _bed_scene(seed=5):
 REFR_CASES = ["smooth", "incoherent", "rough_exp", "rough_gauss", "cross_sigma", "diffuse",
               "split", "tabulated", "gamma_facet", "two_crossings", "shadow", "joint2",
         "joint_pad", "joint3_sigma", "flat_stack", "flat_stack_pad", "flat_stack_deep",
-        "flat_stack_general"]
+        "flat_stack_general", *ANTENNAS]
 
 
 def _planes(zs, like):
@@ -133,6 +146,8 @@ def test_refracted_vs_oracle(name, precision):
         kw.update(split_sides=True)
     elif name == "tabulated":
         kw.update(pattern=A.pattern_args(A.cos_power_table(2.0), tr.u_at, tr.u_ct))
+    elif name in ANTENNAS:
+        kw.update(pattern=A.pattern_args(ANTENNAS[name], tr.u_at, tr.u_ct))
     elif name == "gamma_facet":
         kw.update(gamma=np.linspace(-0.3, 0.3, len(bed)))
     elif name == "two_crossings":
@@ -177,8 +192,7 @@ def test_refracted_vs_oracle(name, precision):
                                                      eps, att, return_bins=True,
                                                      flat_stack=flat, **kw)
     bref = np.where(vref, bref, K.BIN_INVALID)
-    if name == "flat_stack_general":
-        kw["flat_stack"] = False
+    kw["flat_stack"] = flat   # the flat-stack form is an opt-in; default is general
     out, drop, bins = K.refracted_cluttergram(tr.positions, tr.u_ct, target, crossed, eps,
                                               att, precision=precision, return_bins=True, **kw)
     if name.startswith("flat_stack"):
@@ -329,7 +343,7 @@ def test_multilayer_scene_vs_oracle(precision):
             tr.positions, tr.u_ct, f, L.crossed, L.eps_leg, L.att_leg, mode="coherent",
             t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c, gamma=L.gamma, k0=wl.k0,
             pattern=wl.pattern, refraction=L.refraction, pad_to=L.pad_to,
-            flat_stack=L.refraction == "joint")
+            flat_stack=L.flat_stack)
         o = raw[j][..., 0] + 1j * raw[j][..., 1]
         assert np.linalg.norm(ref) > 0, L.name
         if precision == "fp64":

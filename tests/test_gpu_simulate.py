@@ -48,7 +48,7 @@ def test_simulate_three_interfaces_vs_oracle():
             tr.positions, tr.u_ct, ifaces[j], ifaces[:j], eps[:j + 1], att[:j + 1],
             mode="coherent", gamma=fresnel_normal(eps[j], eps[j + 1]), k0=k0, refraction=refr,
             pad_to=sm.joint_pad_to(j, 2) if refr == "joint" else None,
-            flat_stack=refr == "joint", **win)[0])
+            **win)[0])
     assert res.layer == names == ["surface", "horizon", "bed"]
     for j in range(3):
         out = res.field[..., j]

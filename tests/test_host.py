@@ -139,3 +139,65 @@ def test_flop_model_and_workloads():
     assert wl.layers[1].gamma == pytest.approx(-10 ** -0.5)
     wl1 = workloads.config1(n=8, n_traces=2)
     assert wl1.layers[0].gamma == pytest.approx((1 - np.sqrt(3.15)) / (1 + np.sqrt(3.15)))
+
+
+def test_quantization_alias_warning():
+    """The chirped-run alias check (reference simulate.py:71-95): 195 MHz at
+    dt = 5 ns aliases to 5 MHz, inside B/2 = 7.5 MHz -> warns; 60 MHz at
+    48 MHz sampling aliases to 12 MHz, outside -> silent; interp_bins or a
+    delta waveform -> silent."""
+    import warnings as w
+    chirp = dict(kind="chirp", bandwidth=15e6, pulse_length=20e-6)
+    rc = cf.RadarConfig(dt=5e-9, n_samples=64, t0=0.0, f0=195e6,
+                        waveform=cf.WaveformConfig(**chirp))
+    with pytest.warns(UserWarning, match="alias in band"):
+        sm.warn_quantization_alias(rc)
+    quiet = [cf.RadarConfig(dt=1 / 48e6, n_samples=64, t0=0.0, f0=60e6,
+                            waveform=cf.WaveformConfig(**chirp)),
+             cf.RadarConfig(dt=5e-9, n_samples=64, t0=0.0, f0=195e6,
+                            waveform=cf.WaveformConfig(interp_bins=True, **chirp)),
+             cf.RadarConfig(dt=5e-9, n_samples=64, t0=0.0, f0=195e6)]
+    for q in quiet:
+        with w.catch_warnings():
+            w.simplefilter("error")
+            sm.warn_quantization_alias(q)
+
+
+def test_facet_size_and_corr_length_warnings():
+    """Fresnel-zone facet check (reference scene.py:179-195): limit
+    0.5 sqrt(lambda R); at 5 m and R = 1000 m the limit is 35.4 m, so 25 m
+    facets pass (ratio 0.707) and 100 m facets warn (ratio 2.83).  The
+    roughness check (simulate.py:157-162) warns when the correlation length
+    exceeds the facet edge."""
+    import warnings as w
+    from radar_sounder_simulator_amd.scene import check_facet_size
+    small = grid_facets(np.zeros((5, 5)), 25.0)
+    big = grid_facets(np.zeros((5, 5)), 100.0)
+    with w.catch_warnings():
+        w.simplefilter("error")
+        assert check_facet_size(small, 5.0, 1000.0) == pytest.approx(25.0 / (0.5 * np.sqrt(5e3)))
+    with pytest.warns(UserWarning, match="Fresnel-zone limit"):
+        assert check_facet_size(big, 5.0, 1000.0) == pytest.approx(100.0 / (0.5 * np.sqrt(5e3)))
+    small.phase_keys = None
+    rough = cf.DemInterface(name="s", roughness=cf.RoughnessConfig(sigma_m=0.1,
+                                                                   corr_length_m=40.0))
+    with pytest.warns(UserWarning, match="exceeds the facet size"):
+        args = sm._roughness_args(rough, small, 1.0, (0, 0))
+    assert args[3] == 10 and len(args[2]) == len(small)
+    fine = cf.DemInterface(name="s", roughness=cf.RoughnessConfig(sigma_m=0.1,
+                                                                  corr_length_m=10.0))
+    with w.catch_warnings():
+        w.simplefilter("error")
+        sm._roughness_args(fine, small, 1.0, (0, 0))
+
+
+def test_joint_form_default_is_reference():
+    """refraction='joint' runs the reference's solver unless the flat-stack
+    form is asked for (SimConfig.joint_form, flat_stack=...)."""
+    cfg = cf.SimConfig(mode="coherent", radar=_radar())
+    assert cfg.refraction == "joint" and cfg.joint_form == "reference"
+    import inspect
+    assert inspect.signature(K.refracted_cluttergram).parameters["flat_stack"].default is False
+    assert inspect.signature(dv.refracted_field).parameters["flat_stack"].default is False
+    assert all(not L.flat_stack for L in workloads.config3(nx=16, ny=8, n_traces=2,
+                                                           flat_stack=False).layers)

@@ -245,16 +245,19 @@ def test_c_oracle_matches_numpy_oracle():
     f = s.facets
     kw = dict(k=wl.k0, gamma=s.gamma, t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c,
               pattern=wl.pattern, roughness=s.roughness, taper_s=wl.taper_s, d_phi_area=True)
-    r1, d1 = oc.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals, f.areas,
-                                     f.e1, f.e2, **kw)
-    r2, d2 = cref.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals, f.areas,
-                                       f.e1, f.e2, nthreads=2, **kw)
+    r1, d1, b1, _ = oc.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals,
+                                            f.areas, f.e1, f.e2, return_bins=True, **kw)
+    r2, d2, b2 = cref.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals, f.areas,
+                                           f.e1, f.e2, nthreads=2, return_bins=True, **kw)
     assert np.linalg.norm(r1 - r2) / np.linalg.norm(r1) < 1e-13
+    np.testing.assert_array_equal(b2, b1)
     from oracle import multilayer as om
     kw = dict(t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c, gamma=b.gamma, k0=wl.k0,
               pattern=wl.pattern, roughness=b.roughness, taper_s=wl.taper_s, d_phi_area=True)
-    r1, _ = om.refracted_cluttergram(tr.positions, tr.u_ct, b.facets, b.crossed, b.eps_leg,
-                                     b.att_leg, mode="coherent", **kw)
-    r2, _ = cref.refracted_cluttergram(tr.positions, tr.u_ct, b.facets, b.crossed, b.eps_leg,
-                                       b.att_leg, nthreads=2, **kw)
+    r1, _, b1, v1 = om.refracted_cluttergram(tr.positions, tr.u_ct, b.facets, b.crossed,
+                                             b.eps_leg, b.att_leg, mode="coherent",
+                                             return_bins=True, **kw)
+    r2, _, b2 = cref.refracted_cluttergram(tr.positions, tr.u_ct, b.facets, b.crossed, b.eps_leg,
+                                           b.att_leg, nthreads=2, return_bins=True, **kw)
     assert np.linalg.norm(r1 - r2) / np.linalg.norm(r1) < 1e-13
+    np.testing.assert_array_equal(b2, np.where(v1, b1, np.iinfo(np.int32).min))

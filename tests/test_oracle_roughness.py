@@ -123,3 +123,48 @@ def test_lgamma_table_in_library_constants():
     np.testing.assert_allclose(c[1:33], a, rtol=0, atol=1e-13)
     for m in (0, 1, 5, 100, 300):
         assert c[33 + m] == pytest.approx(math.lgamma(m + 1.0), rel=1e-14)
+
+
+def test_splitmix64_known_answers():
+    """The splitmix64 generator of Steele, Lea & Flood (2014) seeded with 0
+    (Vigna's reference splitmix64.c) emits 0xE220A8397B1DCDAF,
+    0x6E789E6AA1B965F4, 0x06C45D188009454F: its n-th output is the finaliser
+    applied to the state (n - 1) * golden, golden = 0x9E3779B97F4A7C15."""
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    golden = 0x9E3779B97F4A7C15
+    states = [(i * golden) % (1 << 64) for i in range(3)]
+    assert [orr._splitmix_scalar(s) for s in states] == want
+    got = prod._splitmix(np.array(states, dtype=np.uint64))
+    assert [int(v) for v in got] == want
+
+
+def test_keyed_phasors_crop_and_order_invariant():
+    """Keyed phasors depend on the cell key alone: any crop or permutation of
+    a lattice gets the same phasor per cell, another seed different ones
+    (reference tests/test_roughness.py:171-183, on the product and on the
+    oracle)."""
+    rows, cols = np.meshgrid(np.arange(100, 300), np.arange(400, 500), indexing="ij")
+    keys = np.column_stack([rows.ravel(), cols.ravel()])
+    ph = prod.speckle_phasors_for_keys(keys, (3, 1))
+    take = np.random.default_rng(4).permutation(len(keys))[:3000]
+    np.testing.assert_array_equal(ph[take], prod.speckle_phasors_for_keys(keys[take], (3, 1)))
+    assert not np.array_equal(ph[take], prod.speckle_phasors_for_keys(keys[take], (3, 2)))
+    assert np.mean(np.abs(ph) ** 2) == pytest.approx(1.0, abs=0.03)
+    small = take[:200]
+    o = orr.speckle_phasors_for_keys(keys[small], (3, 1))
+    np.testing.assert_array_equal(ph[small], o.astype(np.complex64))
+    np.testing.assert_array_equal(orr.speckle_phasors_for_keys(keys[small][::-1], (3, 1)),
+                                  o[::-1])
+
+
+def test_keyed_phasors_are_circular_gaussian():
+    """Box-Muller on two independent uniforms gives a circular complex
+    Gaussian of unit variance: |ph|^2 = (X^2 + Y^2)/2 ~ Exp(1) and arg(ph) ~ U(-pi, pi) (Kolmogorov-Smirnov),
+    real and imaginary parts uncorrelated."""
+    from scipy import stats
+    rows, cols = np.meshgrid(np.arange(300), np.arange(300), indexing="ij")
+    keys = np.column_stack([rows.ravel(), cols.ravel()])
+    ph = prod.speckle_phasors_for_keys(keys, 11).astype(np.complex128)
+    assert stats.kstest(np.abs(ph) ** 2, "expon").pvalue > 1e-3
+    assert stats.kstest(np.angle(ph), "uniform", args=(-np.pi, 2 * np.pi)).pvalue > 1e-3
+    assert abs(np.corrcoef(ph.real, ph.imag)[0, 1]) < 0.02

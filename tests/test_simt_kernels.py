@@ -168,7 +168,7 @@ def test_flat_stack_joint_kernel_vs_oracle(sim, pad_to, rough):
         for extra in (0, _lib.SS_REDO_ALL):
             out, drop, bins, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision="mixed",
                                                     pair_bins=True, refraction="joint",
-                                                    extra_flags=extra, **kw)
+                                                    extra_flags=extra, flat_stack=True, **kw)
             o = out.numpy()
             o = o[..., 0] + 1j * o[..., 1]
             np.testing.assert_array_equal(fac.unpermute_pairs(bins.numpy()), outs[True][2])
@@ -201,7 +201,8 @@ def test_deep_flat_stack_kernel_vs_flat_oracle(sim):
                                   z_platform=tr.positions[:, 2].min())
         trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
         out, drop, bins, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision="fp64",
-                                                pair_bins=True, refraction="joint", **kw)
+                                                pair_bins=True, refraction="joint",
+                                                flat_stack=True, **kw)
     o = out.numpy()
     o = o[..., 0] + 1j * o[..., 1]
     ref, dref, bref, vref = om.refracted_cluttergram(tr.positions, tr.u_ct, bed, crossed, eps,
@@ -216,7 +217,8 @@ def test_deep_flat_stack_kernel_vs_flat_oracle(sim):
     # float64 redo and book their dropped power from the start path
     with _lib.use(sim):
         om_, dm, bm, _ = dv.refracted_field(fac, trs, ifc, eps, att, precision="mixed",
-                                            pair_bins=True, refraction="joint", **kw)
+                                            pair_bins=True, refraction="joint",
+                                            flat_stack=True, **kw)
     np.testing.assert_array_equal(fac.unpermute_pairs(bm.numpy()),
                                   np.where(vref, bref, _lib.BIN_INVALID))
     np.testing.assert_allclose(dm.numpy(), dref, rtol=1e-4, atol=1e-30)
@@ -242,7 +244,7 @@ def test_multilayer_scene_layers_vs_oracle(sim):
             tr.positions, tr.u_ct, f, L.crossed, L.eps_leg, L.att_leg, mode="coherent",
             t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c, gamma=L.gamma, k0=wl.k0,
             pattern=wl.pattern, refraction=L.refraction, pad_to=L.pad_to,
-            flat_stack=L.refraction == "joint")
+            flat_stack=L.flat_stack)
         o = raw[j][..., 0] + 1j * raw[j][..., 1]
         assert np.linalg.norm(ref) > 0, L.name
         assert np.linalg.norm(o - ref) <= 1e-9 * np.linalg.norm(ref), L.name
