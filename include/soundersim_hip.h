@@ -269,6 +269,27 @@ int ss_refracted_field(int precision, const ss_refracted_params* params,
                        int32_t* pair_bins, void* workspace, int64_t workspace_bytes,
                        void* stream);
 
+/* ss_refracted_field without a workspace argument, for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * mixed tier's redo flags and lane masks go to a library-owned device array
+ * of SS_NOALLOC_SCRATCH_BYTES per GPU (held from library load until the
+ * process ends); its address is resolved once per device on the first call
+ * made with that device current.  The call allocates nothing.  Inputs,
+ * outputs and semantics are those of ss_refracted_field; a call whose
+ * ss_refracted_workspace_bytes() exceeds SS_NOALLOC_SCRATCH_BYTES fails with
+ * SS_ERANGE before any work is enqueued.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (different streams or threads
+ * with no ordering between them) race; calls ordered on one stream do not. */
+#define SS_NOALLOC_SCRATCH_BYTES (32LL << 20)
+int ss_refracted_field_noalloc(int precision, const ss_refracted_params* params,
+                               const double* facets, int64_t facet_stride, int64_t n_facets,
+                               const double* traces, int64_t trace_stride, int64_t n_traces,
+                               const double* iface_desc, const double* tables,
+                               const double* legs, const double* pattern_table,
+                               const double* constants, double* out, double* dropped,
+                               int32_t* pair_bins, void* stream);
+
 /* Workspace bytes ss_refracted_field needs: 0 for SS_PREC_FP64 or an empty
  * call; for SS_PREC_MIXED 256 + one flag byte per (8-trace, 256-facet) work
  * item (rounded up to 8) + 256 bytes of lane masks per work item.  No

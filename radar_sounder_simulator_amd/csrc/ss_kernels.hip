@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "ss_joint.h"
@@ -1950,6 +1951,63 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   if (e != hipSuccess) return fail(SS_EHIP, std::string("refracted launch: ") + hipGetErrorString(e));
   g_last_error.clear();
   return SS_OK;
+}
+
+}  // extern "C"
+
+namespace ss {
+// Workspace of ss_refracted_field_noalloc: one array per GPU, part of the
+// code object, so it exists from library load on and is never allocated or
+// freed.  In host code ``scratch`` names a host-side stand-in; the device
+// address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[SS_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array.
+static int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SS_ERANGE, "device index beyond 64");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return SS_OK;
+}
+}  // namespace ss
+
+extern "C" {
+
+int ss_refracted_field_noalloc(int precision, const ss_refracted_params* prm,
+                               const double* facets, int64_t facet_stride, int64_t n_facets,
+                               const double* traces, int64_t trace_stride, int64_t n_traces,
+                               const double* iface_desc, const double* tables,
+                               const double* legs, const double* pattern_table,
+                               const double* constants, double* out, double* dropped,
+                               int32_t* pair_bins, void* stream) {
+  const int64_t need = ss_refracted_workspace_bytes(precision, n_facets, n_traces);
+  if (need > SS_NOALLOC_SCRATCH_BYTES)
+    return fail(SS_ERANGE, "ss_refracted_field_noalloc: the call needs " + std::to_string(need) +
+                               " workspace bytes, more than SS_NOALLOC_SCRATCH_BYTES (" +
+                               std::to_string(SS_NOALLOC_SCRATCH_BYTES) +
+                               "); use ss_refracted_field with a workspace");
+  void* ws = nullptr;
+  if (need > 0) {
+    const int r = scratch_address(&ws);
+    if (r) return r;
+  }
+  return ss_refracted_field(precision, prm, facets, facet_stride, n_facets, traces,
+                            trace_stride, n_traces, iface_desc, tables, legs, pattern_table,
+                            constants, out, dropped, pair_bins, ws,
+                            need > 0 ? SS_NOALLOC_SCRATCH_BYTES : 0, stream);
 }
 
 int64_t ss_refracted_workspace_bytes(int precision, int64_t n_facets, int64_t n_traces) {

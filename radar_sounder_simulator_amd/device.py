@@ -665,12 +665,14 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
                     taper_s=None, d_phi_area=False, n_exp=None, refraction="sequential",
                     joint_newton=6, joint_backtrack=4, out=None,
                     dropped=None, pair_bins=False, workspace=None, extra_flags=0,
-                    stream=None, flat_stack=False):
+                    stream=None, flat_stack=False, noalloc=False):
     """Enqueue the refracted kernel (sequential chain, joint solve or its
     flat-stack form, see ``refracted_params``) and,
     for the mixed tier, its float64 redo launch.  ``workspace``: a uint8
     device tensor of at least ``refracted_workspace_bytes`` (allocated here
-    when None).  Returns (out (T, n_seg, 2) float64, dropped (T,)
+    when None).  ``noalloc``: call ss_refracted_field_noalloc instead (the
+    library's own per-device scratch array holds the redo record; no
+    workspace).  Returns (out (T, n_seg, 2) float64, dropped (T,)
     [, pair_bins], keep-alive) device tensors."""
     dev = target.device
     p, legs_t, table = refracted_params(
@@ -689,16 +691,24 @@ def refracted_field(target, traces, ifaces, eps_leg, att_leg, *, mode, t0, dt,
             if pair_bins else None)
     cst = constants(dev)
     prec = _lib.SS_PREC_FP64 if precision == "fp64" else _lib.SS_PREC_MIXED
-    need = refracted_workspace_bytes(precision, target.n, T)
-    if workspace is None and need > 0:
-        workspace = torch.empty(need, dtype=torch.uint8, device=dev)
-    ws_bytes = 0 if workspace is None else workspace.numel()
-    rc = _lib.load().ss_refracted_field(
-        prec, ctypes.byref(p), _ptr(target.buf), target.stride, target.n,
-        _ptr(traces.buf), traces.stride, T, _ptr(ifaces.desc), _ptr(ifaces.tables),
-        _ptr(legs_t), _ptr(table), _ptr(cst), _ptr(out), _ptr(dropped), _ptr(pb_t),
-        _ptr(workspace), ws_bytes, _stream_handle(stream, dev))
-    _lib.check(rc, "ss_refracted_field")
+    if noalloc:
+        rc = _lib.load().ss_refracted_field_noalloc(
+            prec, ctypes.byref(p), _ptr(target.buf), target.stride, target.n,
+            _ptr(traces.buf), traces.stride, T, _ptr(ifaces.desc), _ptr(ifaces.tables),
+            _ptr(legs_t), _ptr(table), _ptr(cst), _ptr(out), _ptr(dropped), _ptr(pb_t),
+            _stream_handle(stream, dev))
+        _lib.check(rc, "ss_refracted_field_noalloc")
+    else:
+        need = refracted_workspace_bytes(precision, target.n, T)
+        if workspace is None and need > 0:
+            workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+        ws_bytes = 0 if workspace is None else workspace.numel()
+        rc = _lib.load().ss_refracted_field(
+            prec, ctypes.byref(p), _ptr(target.buf), target.stride, target.n,
+            _ptr(traces.buf), traces.stride, T, _ptr(ifaces.desc), _ptr(ifaces.tables),
+            _ptr(legs_t), _ptr(table), _ptr(cst), _ptr(out), _ptr(dropped), _ptr(pb_t),
+            _ptr(workspace), ws_bytes, _stream_handle(stream, dev))
+        _lib.check(rc, "ss_refracted_field")
     keep = (legs_t, table, workspace)
     if pair_bins:
         return out, dropped, pb_t, keep

@@ -39,6 +39,7 @@
 #define __shared__ static
 #define __launch_bounds__(...)
 #define HIP_SYMBOL(x) (&(x))
+#define __align__(n) alignas(n)
 
 struct dim3 {
   unsigned x, y, z;
@@ -263,6 +264,11 @@ inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char* hipGetErrorString(hipError_t) { return "simulated"; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+// a __device__ variable is a host variable here: its address is the symbol's
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+  *p = const_cast<void*>(sym);
+  return hipSuccess;
+}
 inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
   *v = sim::cus();
   return hipSuccess;

@@ -104,3 +104,17 @@ def test_workspace_sizes():
 def test_fill_constants():
     c = _lib.constants_host()
     assert c.shape == (_lib.CONST_COUNT,) and np.isfinite(c).all()
+
+
+def test_noalloc_rejects_calls_beyond_its_scratch():
+    """ss_refracted_field_noalloc refuses, before any device work, a call
+    whose redo record would exceed SS_NOALLOC_SCRATCH_BYTES."""
+    lib = _lib.load()
+    r = _lib.RefractedParams()
+    r.n_samples, r.dt, r.c, r.n_crossed = 64, 1e-8, 3e8, 1
+    n = 1 << 22       # 16384 facet tiles x 524288 trace tiles: far beyond 32 MiB
+    assert lib.ss_refracted_workspace_bytes(1, n, n) > _lib.NOALLOC_SCRATCH_BYTES
+    rc = lib.ss_refracted_field_noalloc(1, ctypes.byref(r), None, n, n, None, n, n, None, None,
+                                        None, None, None, None, None, None, None)
+    assert rc == -3
+    assert b"SS_NOALLOC_SCRATCH_BYTES" in lib.ss_last_error()

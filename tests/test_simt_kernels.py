@@ -471,3 +471,29 @@ def test_focused_sar_kernel_vs_oracle(sim, window):
                                  aperture_m=30.0, window=window, precision=prec).numpy()
         o = out[..., 0] + 1j * out[..., 1]
         assert np.linalg.norm(o - ref) <= tol * np.linalg.norm(ref), prec
+
+
+def test_noalloc_entry_matches_workspace_entry(sim):
+    """ss_refracted_field_noalloc (the library's scratch array) against
+    ss_refracted_field with a caller workspace, every pair through the redo
+    record, two calls in a row, the second smaller."""
+    kw = dict(mode="coherent", t0=2 * 2900 / C, dt=1 / 48e6, n_samples=512, c=C,
+              gamma=-0.3162, k0=2 * np.pi * 60e6 / C)
+    for n, seed in ((12, 5), (7, 6)):
+        surf, bed, tr = _bed(seed=seed, n=n)
+        res = {}
+        with _lib.use(sim):
+            fac = dv.DeviceFacets(bed.centers, bed.normals, bed.areas, bed.e1, bed.e2,
+                                  device="cpu")
+            ifc = dv.DeviceInterfaces([surf], device="cpu")
+            trs = dv.DeviceTraces(tr.positions, tr.u_ct, None, None, device="cpu")
+            for noalloc in (False, True):
+                out, drop, bins, _ = dv.refracted_field(
+                    fac, trs, ifc, [1.0, 3.15], [0.0, 10.0], precision="mixed",
+                    pair_bins=True, extra_flags=_lib.SS_REDO_ALL, noalloc=noalloc, **kw)
+                res[noalloc] = (out.numpy().copy(), drop.numpy().copy(), bins.numpy().copy())
+        np.testing.assert_array_equal(res[True][2], res[False][2])
+        assert np.linalg.norm(res[False][0]) > 0
+        np.testing.assert_allclose(res[True][0], res[False][0], rtol=0,
+                                   atol=1e-12 * np.abs(res[False][0]).max())
+        np.testing.assert_allclose(res[True][1], res[False][1], rtol=1e-12, atol=1e-30)
