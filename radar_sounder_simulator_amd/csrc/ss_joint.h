@@ -415,37 +415,10 @@ struct FlatStack {
   }
 };
 
-// Float64 division, reciprocal square root and square root of the
-// flat-stack passes: IEEE for the float64 tier (FAST = false), the
-// Newton-refined hardware estimates of rcp64 / rsq64 for the mixed tier's
-// float64 step and pass (FAST = true; sqrt keeps 0 -> 0 and x < 0 -> NaN).
-template <bool FAST>
-SS_HD double q_div(double a, double b) {
-  if constexpr (FAST)
-    return a * rcp64(b);
-  else
-    return a / b;
-}
-template <bool FAST>
-SS_HD double q_rsq(double x) {
-  if constexpr (FAST)
-    return rsq64(x);
-  else
-    return 1.0 / sqrt(x);
-}
-template <bool FAST>
-SS_HD double q_sqrt(double x) {
-  if constexpr (FAST)
-    return sqrt64(x);
-  else
-    return sqrt(x);
-}
-
 // The path of slowness v: F(v) and (jac) dF/dv in the plane basis B, the
 // surface crossing s, its climb length factor lam, c_1, n.u; returns
 // feasibility (every c real, lam > 0, the climbing ray meets the plane from
 // below).
-template <bool FAST = false>
 SS_HD bool flat_eval(const FlatStack& fs, const double p[3], const double q[3],
                      const double pt0[3], const double nr0[3], const double B[2][3],
                      const double v[2], bool jac, double F[2], double J[2][2], double s[3],
@@ -459,7 +432,7 @@ SS_HD bool flat_eval(const FlatStack& fs, const double p[3], const double q[3],
     for (int k = 2; k <= fs.N; ++k) {
       const double ck2 = nl[k] * nl[k] - vv;
       ok = ok && (ck2 > 0.0);
-      const double rk = q_rsq<FAST>(ck2);
+      const double rk = 1.0 / sqrt(ck2);
       const double dk = fs.d(k, q[2]);
       a += dk * rk;
       b += dk * (rk * rk * rk);
@@ -579,38 +552,101 @@ SS_HD void flat_start(const double p[3], const double s0[3], const double nr0[3]
   v[1] = nl[0] * i1 + m * nr0[1];
 }
 
+// Layer constants of a flat stack for the mixed tier, staged in LDS once per
+// workgroup (index k: real medium k, 0 the air above the surface; layers
+// 2..N-1 lie between horizontal planes, layer N ends at each pair's target).
+struct FlatLds {
+  double n2[SS_MAX_CROSSED + 1];   // n_k^2
+  double d[SS_MAX_CROSSED + 1];    // d_k = z_{k-1} - z_k, layers 2..N-1
+  double dn2[SS_MAX_CROSSED + 1];  // |d_k| n_k^2: the layer's optical path is dn2 / c_k
+  float n2f[SS_MAX_CROSSED + 1];
+  float df[SS_MAX_CROSSED + 1];    // d_k
+  float dnf[SS_MAX_CROSSED + 1];   // |d_k| n_k: segment length dnf / c_k
+  float dn2f[SS_MAX_CROSSED + 1];  // |d_k| n_k^2
+  float dabs[SS_MAX_CROSSED + 1];  // |d_k|
+  float attf[SS_MAX_CROSSED + 1];  // attenuation dB per m of medium k
+  float sigf[SS_MAX_CROSSED + 1];  // crossing roughness of interface k (k < N)
+  double z1, zlast;                // planes kp + 1 and kp + N - 1
+  double min_n2;                   // min n_k^2 over k = 2..N
+  double min_n_evan;               // min n_k (1 + 64 eps) over k = 2..N
+  float min_n2f;                   // min of n2f over k = 2..N
+  int dpos;                        // d_k > 0 for every k = 2..N-1
+};
+
+// Thread k of the workgroup fills entry k (k <= N); stage_flat_mins (one
+// thread, after a barrier) derives the stack-wide minima.
+SS_HD void stage_flat_layer(const RefrConst& rc, FlatLds& L, int k) {
+  const int kp = rc.k_pad, N = rc.n_crossed - rc.k_pad;
+  if (k > N) return;
+  const int NA = rc.n_crossed;
+  const double n = rc.legs[kp + k];
+  const double att = rc.legs[2 * (NA + 1) + kp + k];
+  double d = 0.0;
+  if (k >= 2 && k < N) d = flat_z(rc, kp + k - 1) - flat_z(rc, kp + k);
+  const double ad = fabs(d);
+  L.n2[k] = n * n;
+  L.d[k] = d;
+  L.dn2[k] = ad * (n * n);
+  L.n2f[k] = (float)(n * n);
+  L.df[k] = (float)d;
+  L.dnf[k] = (float)(ad * n);
+  L.dn2f[k] = (float)(ad * (n * n));
+  L.dabs[k] = (float)ad;
+  L.attf[k] = (float)(att / 1000.0);
+  L.sigf[k] = k < N ? (float)rc.desc[(kp + k) * SS_IDESC_COUNT + SS_IDESC_SIGC] : 0.0f;
+}
+
+SS_HD void stage_flat_mins(const RefrConst& rc, FlatLds& L) {
+  const int kp = rc.k_pad, N = rc.n_crossed - rc.k_pad;
+  double mn2 = 1e300, mne = 1e300;
+  float mnf = 3.0e38f;
+  int dpos = 1;
+  for (int k = 2; k <= N; ++k) {
+    mn2 = fmin(mn2, L.n2[k]);
+    mne = fmin(mne, rc.legs[kp + k] * (1.0 + 64.0 * kEps64));
+    mnf = fminf(mnf, L.n2f[k]);
+    if (k < N) dpos = dpos && (L.d[k] > 0.0);
+  }
+  L.min_n2 = mn2;
+  L.min_n_evan = mne;
+  L.min_n2f = mnf;
+  L.dpos = dpos;
+  L.z1 = N > 1 ? flat_z(rc, kp + 1) : 0.0;
+  L.zlast = N > 1 ? flat_z(rc, kp + N - 1) : 0.0;
+}
+
 // Float32 twin of flat_eval (F and J only) in coordinates relative to s0:
 // P, Q, O = p, q, pt0 minus s0; zt = first plane's z minus s0's; dN = the
-// last layer's height (z_{N-1} - q_z).
+// last layer's height (z_{N-1} - q_z); layer constants from L.
 // Returns 0 feasible, 1 clearly infeasible (some c^2 < -1e-4, n.u < -1e-4
 // or the plane more than 5 cm past the climbing ray's start: the float64
-// test agrees), 2 infeasible within those margins.
-SS_HD int flat_eval32(const FlatStack& fs, const float P[3], const float Q[3],
+// test agrees), 2 infeasible within those margins.  Every c_k^2 = n_k^2 -
+// |v|^2 is positive exactly when the smallest is (float subtraction is
+// monotonic), so the layer loop carries no tests.
+SS_HD int flat_eval32(const FlatStack& fs, const FlatLds& L, const float P[3], const float Q[3],
                       const float O[3], const float nr[3], const float B[2][3], float zt,
                       float dN, const float v[2], float F[2], float J[2][2]) {
-  const double* nl = fs.n;
   const float vv = v[0] * v[0] + v[1] * v[1];
   bool ok = true, clear = false;
   float a = 0.0f, b = 0.0f;
   float xt[3] = {Q[0], Q[1], Q[2]};
   if (fs.N > 1) {
-    for (int k = 2; k <= fs.N; ++k) {
-      const float nk = (float)nl[k];
-      const float ck2 = nk * nk - vv;
-      ok = ok && (ck2 > 0.0f);
-      clear = clear || (ck2 < -1e-4f);
-      const float rk = f_rsq(fmaxf(ck2, 1e-30f));
-      const float dk = k == fs.N ? dN
-                                 : (float)(flat_z(*fs.rc, fs.kp + k - 1) - flat_z(*fs.rc, fs.kp + k));
-      a += dk * rk;
-      b += dk * (rk * rk * rk);
+    const float cmin = L.min_n2f - vv;
+    ok = cmin > 0.0f;
+    clear = cmin < -1e-4f;
+    for (int k = 2; k < fs.N; ++k) {
+      const float rk = f_rsq(fmaxf(L.n2f[k] - vv, 1e-30f));
+      a += L.df[k] * rk;
+      b += L.df[k] * (rk * rk * rk);
     }
+    const float rN = f_rsq(fmaxf(L.n2f[fs.N] - vv, 1e-30f));
+    a += dN * rN;
+    b += dN * (rN * rN * rN);
     xt[0] = Q[0] - a * v[0];
     xt[1] = Q[1] - a * v[1];
     xt[2] = zt;
   }
-  const float n1 = (float)nl[1];
-  const float c12 = n1 * n1 - vv;
+  const float c12 = L.n2f[1] - vv;
   ok = ok && (c12 > 0.0f);
   clear = clear || (c12 < -1e-4f);
   const float c1 = f_sqrt(fmaxf(c12, 1e-30f));
@@ -625,10 +661,10 @@ SS_HD int flat_eval32(const FlatStack& fs, const float P[3], const float Q[3],
   float sv[3];
   for (int c = 0; c < 3; ++c) sv[c] = xt[c] + lam * u[c];
   const float ds[3] = {sv[0] - P[0], sv[1] - P[1], sv[2] - P[2]};
-  const float L = f_sqrt(fmaxf(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2], 1e-30f));
-  const float iL = f_rcp(L);
+  const float Ls = f_sqrt(fmaxf(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2], 1e-30f));
+  const float iL = f_rcp(Ls);
   const float dh[3] = {ds[0] * iL, ds[1] * iL, ds[2] * iL};
-  const float n0 = (float)nl[0];
+  const float n0 = (float)fs.n[0];
   const float g[3] = {n0 * dh[0] - v[0], n0 * dh[1] - v[1], n0 * dh[2] + c1};
   F[0] = B[0][0] * g[0] + B[0][1] * g[1] + B[0][2] * g[2];
   F[1] = B[1][0] * g[0] + B[1][1] * g[1] + B[1][2] * g[2];
@@ -694,7 +730,6 @@ SS_HD void flat_newton64(const FlatStack& fs, const double p[3], const double q[
 // transmission cosines are c_k/n_k and c_{k+1}/n_{k+1}, so the TE
 // coefficient is (c_k - c_{k+1})/(c_k + c_{k+1}) (the reference's formula
 // with its clamp where c_k/n_k < C_MIN).
-template <bool FAST = false>
 SS_HD double flat_final(const FlatStack& fs, const double p[3], const double q[3],
                         const double pt0[3], const double nr0[3], const double v[2],
                         PathSums& ps) {
@@ -714,7 +749,7 @@ SS_HD double flat_final(const FlatStack& fs, const double p[3], const double q[3
     double gm;
     if (flat && ci_raw >= kCMin) {
       const double a0 = nl[i] * ci_raw, a1 = nl[i + 1] * ct_raw;   // c_i, c_{i+1}
-      gm = q_div<FAST>(a0 - a1, a0 + a1);
+      gm = (a0 - a1) / (a0 + a1);
     } else {
       const double e1 = epsl[i], e2 = epsl[i + 1];
       const double sin2 = (e1 / e2) * (1.0 - ci * ci);
@@ -736,25 +771,25 @@ SS_HD double flat_final(const FlatStack& fs, const double p[3], const double q[3
   // interfaces 1..N-1
   double a = 0.0, c_prev = c1, c_last = 0.0;
   for (int k = 2; k <= N; ++k) {
-    const double ck = q_sqrt<FAST>(nl[k] * nl[k] - vv);
-    const double rk = q_div<FAST>(1.0, ck);
+    const double ck = sqrt(nl[k] * nl[k] - vv);
+    const double rk = 1.0 / ck;
     const double dk = fs.d(k, q[2]);
     a += dk * rk;
     fin = fin && isfinite(rk);
     sides = sides && (dk > 0.0);
     evan = evan || (vm > nl[k] * (1.0 + 64.0 * kEps64));
     // interface k-1 (medium k-1 -> k)
-    cross(k - 1, q_div<FAST>(c_prev, nl[k - 1]), q_div<FAST>(ck, nl[k]), true);
+    cross(k - 1, c_prev / nl[k - 1], ck / nl[k], true);
     if (k < N) {
       // segment k: length |d_k| n_k / c_k; incidence cosine c_k / n_k
       const double ad = fabs(dk);
       const double len = ad * nl[k] * rk;
-      const double cik = q_div<FAST>(ck, nl[k]);
+      const double cik = ck / nl[k];
       const double ci = fmax(cik, kCMin);
       opl += nl[k] * len;
-      loss += len * q_div<FAST>(attl[k], 1000.0);
+      loss += len * (attl[k] / 1000.0);
       perp += ad * rk;
-      par += q_div<FAST>(len, nl[k] * ci * ci);
+      par += len / (nl[k] * ci * ci);
     }
     c_prev = ck;
     c_last = ck;
@@ -853,28 +888,44 @@ SS_HD void flat_path(const double p[3], const double q[3], const RefrConst& rc,
 }
 
 // Mixed tier: float32 anchor, float32 Newton in coordinates relative to s0,
-// one float64 Newton step, the float64 polyline.  Returns false (float64
-// redo of the pair) unless the float32 phase took only full steps and the
-// float64 result is valid with a surface residual <= 1e-12 -- the cases
-// where the float64 solve reaches the same stationary point -- or the lane
-// is stuck: no first step is feasible, its full step and every halving the
-// float64 solve would try being clearly infeasible, so that solve stays at
-// the start (the lane has no valid path: the anchored surface plane passes
-// below the first horizontal plane) and the lane is booked from the start
-// path.  One float32 evaluation site and one final pass keep the registers
-// of the float64 tier's kernel.
+// one float64 Newton step, then the polyline.  Returns false (float64 redo
+// of the pair) unless the float32 phase took only full steps and the result
+// is valid with a surface residual <= 1e-12 -- the cases where the float64
+// solve reaches the same stationary point -- or the lane is stuck: no first
+// step is feasible, its full step and every halving the float64 solve would
+// try being clearly infeasible, so that solve stays at the start (the lane
+// has no valid path: the anchored surface plane passes below the first
+// horizontal plane) and the lane is booked from the start path.
+//
+// The float64 work is one layer loop at the float32 solution v: per layer
+// one refined rsq64 gives 1/c_k, which sums a = sum d_k / c_k (the horizontal
+// reach, hence F(v) and the crossings) and the layers' optical path
+// sum |d_k| n_k^2 / c_k.  The Newton step dv = -J^-1 F(v) takes the float32
+// Jacobian (its 1e-7 relative error leaves |F| ~ 1e-14 after the step, as the
+// float64 one would), and the sums follow the step to first order: da/dv =
+// b v, b = sum d_k / c_k^3, and likewise for the optical path and 1/c_N; the
+// second-order remainder is ~b |dv|^2 ~ 1e-11 m against a bin guard of
+// 1e-7 bins (2.5e-7 m at 60 MHz sampling).  Fresnel, crossing roughness,
+// attenuation and the ray-tube sums are float32 from the same loop (at v;
+// their change over dv is ~1e-7 relative).  The stack-wide tests (every
+// c_k real, no evanescence, every layer straddled) compare |v| against the
+// staged minima.
 SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst& rc,
-                           PathSums& ps) {
+                           const FlatLds& Ls, PathSums& ps) {
   FlatStack fs;
   fs.rc = &rc;
   fs.kp = rc.k_pad;
   fs.N = rc.n_crossed - rc.k_pad;
   fs.n = rc.legs + rc.k_pad;
+  const int N = fs.N, kp = fs.kp;
+  const double* nl = fs.n;
   double pt0[3], nr0[3], s0[3], B[2][3], v[2];
   flat_anchor(p, q, rc, fs, true, pt0, nr0, s0);
   plane_basis(nr0, B);
-  flat_start(p, s0, nr0, fs.n, v);
+  flat_start(p, s0, nr0, nl, v);
+  const double dN = N > 1 ? Ls.zlast - q[2] : 0.0;
   bool stuck = false;
+  float Jf[2][2] = {{1.0f, 0.0f}, {0.0f, 1.0f}};
   {
     float P[3], Q[3], O[3], nr[3], Bf[2][3];
     for (int c = 0; c < 3; ++c) {
@@ -885,17 +936,17 @@ SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst
       Bf[0][c] = (float)B[0][c];
       Bf[1][c] = (float)B[1][c];
     }
-    const float zt = fs.N > 1 ? (float)(flat_z(rc, fs.kp + 1) - s0[2]) : Q[2];
-    const float dN = fs.N > 1 ? (float)fs.d(fs.N, q[2]) : 0.0f;
+    const float zt = N > 1 ? (float)(Ls.z1 - s0[2]) : Q[2];
+    const float dNf = (float)dN;
     const float lm_min = ldexpf(1.0f, 1 - rc.n_backtrack);
     float vf[2] = {(float)v[0], (float)v[1]}, vt[2] = {vf[0], vf[1]};
-    float F[2] = {0.0f, 0.0f}, J[2][2] = {{1.0f, 0.0f}, {0.0f, 1.0f}}, st[2] = {0.0f, 0.0f};
+    float F[2] = {0.0f, 0.0f}, st[2] = {0.0f, 0.0f};
     float m0 = 0.0f, lm = 1.0f;
     int steps = 0;
     bool base = true, halving = false, ok = false, fail = false;
     for (int ev = 0; ev < 40; ++ev) {
       float Ft[2], Jt[2][2];
-      const int ft = flat_eval32(fs, P, Q, O, nr, Bf, zt, dN, vt, Ft, Jt);
+      const int ft = flat_eval32(fs, Ls, P, Q, O, nr, Bf, zt, dNf, vt, Ft, Jt);
       const float mt = Ft[0] * Ft[0] + Ft[1] * Ft[1];
       bool take = false;
       if (base) {
@@ -942,13 +993,13 @@ SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst
         F[0] = Ft[0];
         F[1] = Ft[1];
         for (int r = 0; r < 2; ++r)
-          for (int c = 0; c < 2; ++c) J[r][c] = Jt[r][c];
+          for (int c = 0; c < 2; ++c) Jf[r][c] = Jt[r][c];
       }
       m0 = F[0] * F[0] + F[1] * F[1];
       if (!(m0 > 1e-13f) || steps == 4) break;
-      const float idet = f_rcp(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-      st[0] = (J[1][1] * F[0] - J[0][1] * F[1]) * idet;
-      st[1] = (J[0][0] * F[1] - J[1][0] * F[0]) * idet;
+      const float idet = f_rcp(Jf[0][0] * Jf[1][1] - Jf[0][1] * Jf[1][0]);
+      st[0] = (Jf[1][1] * F[0] - Jf[0][1] * F[1]) * idet;
+      st[1] = (Jf[0][0] * F[1] - Jf[1][0] * F[0]) * idet;
       vt[0] = vf[0] - st[0];
       vt[1] = vf[1] - st[1];
     }
@@ -958,15 +1009,187 @@ SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst
       v[1] = vf[1];
     }
   }
-  if (!stuck) {
-    // one float64 Newton step
-    double F[2], J[2][2], s[3], lam, c1, nu;
-    if (!flat_eval<true>(fs, p, q, pt0, nr0, B, v, true, F, J, s, lam, c1, nu)) return false;
-    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    v[0] -= (J[1][1] * F[0] - J[0][1] * F[1]) / det;
-    v[1] -= (J[0][0] * F[1] - J[1][0] * F[0]) / det;
+  // the float64 layer loop at v, with the float32 radiometry sums
+  const double vv = v[0] * v[0] + v[1] * v[1];
+  const double c12 = Ls.n2[1] - vv;
+  const double c1 = sqrt64(c12);
+  const float c1f = (float)c1;
+  const bool rough = (rc.flags & SS_CROSS_ROUGH) != 0;
+  const float k0f = (float)rc.k0;
+  double a = 0.0, o = 0.0, rN = 0.0;
+  float b = 0.0f, o3 = 0.0f, tau2 = 1.0f, loss = 0.0f, perp = 0.0f, par = 0.0f;
+  float c_prev = c1f, c_lastf = c1f;
+  // interface i between cosines-times-index c_i -> c_{i+1} of two horizontal
+  // media: TE coefficient (c_i - c_{i+1}) / (c_i + c_{i+1}), crossing
+  // roughness exp(-2 (sigma k0 (c_i - c_{i+1}))^2)
+  auto cross_flat = [&](int i, float ca, float cb) {
+    const float gm = (ca - cb) * f_rcp(ca + cb);
+    tau2 *= 1.0f - gm * gm;
+    if (rough) {
+      const float sk = Ls.sigf[i] * (k0f * (ca - cb));
+      tau2 *= f_exp(-2.0f * (sk * sk));
+    }
+  };
+  bool lay_ok = true;
+  if (N > 1) {
+    lay_ok = vv < Ls.min_n2;
+    for (int k = 2; k < N; ++k) {
+      const double rk = rsq64(Ls.n2[k] - vv);
+      a = fma(Ls.d[k], rk, a);
+      o = fma(Ls.dn2[k], rk, o);
+      const float rf = (float)rk;
+      const float r3 = rf * rf * rf;
+      const float ck = f_rcp(rf);
+      b += Ls.df[k] * r3;
+      o3 += Ls.dn2f[k] * r3;
+      cross_flat(k - 1, c_prev, ck);
+      const float len = Ls.dnf[k] * rf;
+      loss += len * Ls.attf[k];
+      perp += Ls.dabs[k] * rf;
+      c_prev = ck;
+    }
+    rN = rsq64(Ls.n2[N] - vv);
+    a = fma(dN, rN, a);
+    const float rf = (float)rN;
+    b += (float)dN * (rf * rf * rf);
+    c_lastf = f_rcp(rf);
+    cross_flat(N - 1, c_prev, c_lastf);
   }
-  const double resid = flat_final<true>(fs, p, q, pt0, nr0, v, ps);
+  par = o3;  // sum |d_k| n_k^2 / c_k^3 = sum len_k / (n_k cos_k^2)
+  double u[3] = {-v[0], -v[1], c1};
+  double xt[3] = {q[0], q[1], q[2]};
+  if (N > 1) {
+    xt[0] = q[0] - a * v[0];
+    xt[1] = q[1] - a * v[1];
+    xt[2] = Ls.z1;
+  }
+  double nu = nr0[0] * u[0] + nr0[1] * u[1] + nr0[2] * u[2];
+  double lam = (nr0[0] * (pt0[0] - xt[0]) + nr0[1] * (pt0[1] - xt[1]) +
+                nr0[2] * (pt0[2] - xt[2])) * rcp64(nu);
+  double vn[2] = {v[0], v[1]}, c1n = c1;
+  if (!stuck) {
+    // feasibility at v, F(v), the step
+    if (!(lay_ok && c12 > 0.0 && nu > 0.0 && lam > 0.0)) return false;
+    double s[3];
+    for (int c = 0; c < 3; ++c) s[c] = xt[c] + lam * u[c];
+    const double ds[3] = {s[0] - p[0], s[1] - p[1], s[2] - p[2]};
+    const double iL = rsq64(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+    const double n0 = nl[0];
+    const double g[3] = {n0 * ds[0] * iL - v[0], n0 * ds[1] * iL - v[1], n0 * ds[2] * iL + c1};
+    const double F0 = B[0][0] * g[0] + B[0][1] * g[1] + B[0][2] * g[2];
+    const double F1 = B[1][0] * g[0] + B[1][1] * g[1] + B[1][2] * g[2];
+    const double j00 = Jf[0][0], j01 = Jf[0][1], j10 = Jf[1][0], j11 = Jf[1][1];
+    const double idet = 1.0 / (j00 * j11 - j01 * j10);
+    const double dv0 = -(j11 * F0 - j01 * F1) * idet;
+    const double dv1 = -(j00 * F1 - j10 * F0) * idet;
+    const double vdv = v[0] * dv0 + v[1] * dv1;
+    vn[0] = v[0] + dv0;
+    vn[1] = v[1] + dv1;
+    if (N > 1) {
+      a = fma((double)b, vdv, a);
+      o = fma((double)o3, vdv, o);
+      rN = fma(rN * rN * rN, vdv, rN);
+    }
+    const double vvn = vn[0] * vn[0] + vn[1] * vn[1];
+    c1n = sqrt64(Ls.n2[1] - vvn);
+    lay_ok = N > 1 ? vvn < Ls.min_n2 : true;
+    u[0] = -vn[0];
+    u[1] = -vn[1];
+    u[2] = c1n;
+    if (N > 1) {
+      xt[0] = q[0] - a * vn[0];
+      xt[1] = q[1] - a * vn[1];
+    }
+    nu = nr0[0] * u[0] + nr0[1] * u[1] + nr0[2] * u[2];
+    lam = (nr0[0] * (pt0[0] - xt[0]) + nr0[1] * (pt0[1] - xt[1]) +
+           nr0[2] * (pt0[2] - xt[2])) * rcp64(nu);
+  }
+  // the polyline at vn: surface crossing, pads, residual, sums
+  const double vvn = vn[0] * vn[0] + vn[1] * vn[1];
+  const double vm = sqrt64(vvn);
+  bool fin = lay_ok && isfinite(c1n) && isfinite(a) && isfinite(rN);
+  bool sides = N > 1 ? (Ls.dpos != 0 && dN > 0.0) : true;
+  bool evan = N > 1 && vm > Ls.min_n_evan;
+  double s[3];
+  for (int c = 0; c < 3; ++c) s[c] = xt[c] + lam * u[c];
+  fin = fin && isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]);
+  for (int i = 0; i < kp; ++i) {
+    const double zp = rc.desc[i * SS_IDESC_COUNT + SS_IDESC_MP + 2];
+    sides = sides && ((p[2] - zp) * (s[2] - zp) < 0.0);
+  }
+  if (kp > 0) {
+    const double zp = rc.desc[SS_IDESC_MP + 2];
+    const double f = (p[2] - zp) * rcp64(p[2] - s[2]);
+    for (int c = 0; c < 3; ++c) ps.x_first[c] = p[c] + f * (s[c] - p[c]);
+    ps.x_first[2] = zp;
+  } else {
+    for (int c = 0; c < 3; ++c) ps.x_first[c] = s[c];
+  }
+  const double ds[3] = {s[0] - p[0], s[1] - p[1], s[2] - p[2]};
+  const double sa2 = ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2];
+  const double isa = rsq64(sa2 > kTiny * kTiny ? sa2 : kTiny * kTiny);
+  const double sa = sa2 * isa;
+  const double din[3] = {ds[0] * isa, ds[1] * isa, ds[2] * isa};
+  const double in1 = rcp64(nl[1]);
+  const double dout[3] = {vn[0] * in1, vn[1] * in1, -c1n * in1};
+  const double ct1 = nr0[0] * din[0] + nr0[1] * din[1] + nr0[2] * din[2];
+  const double ct2 = nr0[0] * dout[0] + nr0[1] * dout[1] + nr0[2] * dout[2];
+  const double t1[3] = {din[0] - ct1 * nr0[0], din[1] - ct1 * nr0[1], din[2] - ct1 * nr0[2]};
+  const double st1 = sqrt64(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+  const double dG[3] = {nl[0] * din[0] - nl[1] * dout[0], nl[0] * din[1] - nl[1] * dout[1],
+                        nl[0] * din[2] - nl[1] * dout[2]};
+  const double nd = nr0[0] * dG[0] + nr0[1] * dG[1] + nr0[2] * dG[2];
+  const double rt[3] = {dG[0] - nd * nr0[0], dG[1] - nd * nr0[1], dG[2] - nd * nr0[2]};
+  const double resid = sqrt64(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+  fin = fin && isfinite(resid) && isfinite(sa);
+  sides = sides &&
+          ((nr0[0] * (p[0] - s[0]) + nr0[1] * (p[1] - s[1]) + nr0[2] * (p[2] - s[2])) *
+               (-lam * nu) < 0.0);
+  evan = evan || (nl[0] * st1 > nl[1] * (1.0 + 64.0 * kEps64));
+  // surface (medium 0 -> 1, general form) in float32
+  {
+    const float ci_raw = (float)fabs(ct1), ct_raw = (float)fabs(ct2);
+    const float ci = fmaxf(ci_raw, (float)kCMin);
+    const float e1 = (float)rc.legs[(rc.n_crossed + 1) + kp], e2 = (float)rc.legs[(rc.n_crossed + 1) + kp + 1];
+    const float n0f = (float)nl[0], n1f = (float)nl[1];
+    const float sin2 = (e1 * f_rcp(e2)) * (1.0f - ci * ci);
+    const float cc2 = f_sqrt(fmaxf(1.0f - sin2, 0.0f));
+    const float gm = (n0f * ci - n1f * cc2) * f_rcp(n0f * ci + n1f * cc2);
+    tau2 *= 1.0f - gm * gm;
+    if (rough) {
+      const float sk = Ls.sigf[0] * (k0f * (n0f * ci_raw - n1f * ct_raw));
+      tau2 *= f_exp(-2.0f * (sk * sk));
+    }
+    const float csurf = ci;
+    loss += (float)sa * Ls.attf[0];
+    perp += (float)sa * f_rcp(n0f);
+    par += (float)sa * f_rcp(n0f * csurf * csurf);
+    ps.c_first = csurf;
+  }
+  double opl = nl[0] * sa;
+  if (N > 1) {
+    // layer 1: the climb |lam| n_1; then the layers' optical path
+    const double alam = fabs(lam);
+    opl += Ls.n2[1] * alam + o;
+    const float len = (float)(alam * nl[1]);
+    const float ci = fmaxf(c1f * f_rcp((float)nl[1]), (float)kCMin);
+    loss += len * Ls.attf[1];
+    perp += (float)alam;
+    par += len * f_rcp((float)nl[1] * ci * ci);
+    ps.cur[0] = q[0] - dN * vn[0] * rN;
+    ps.cur[1] = q[1] - dN * vn[1] * rN;
+    ps.cur[2] = Ls.zlast;
+    ps.c_last = fmaxf(c_lastf * f_rcp((float)nl[N]), (float)kCMin);
+  } else {
+    for (int c = 0; c < 3; ++c) ps.cur[c] = s[c];
+    ps.c_last = fmax(fabs(ct2), kCMin);
+  }
+  ps.valid = (resid <= 1e-9) && fin && sides && !evan;
+  ps.opl = opl;
+  ps.loss_db = loss;
+  ps.sum_par = par;
+  ps.sum_perp = perp;
+  ps.tau2 = tau2;
   return stuck ? !ps.valid : (ps.valid && resid <= 1e-12);
 }
 
@@ -999,13 +1222,15 @@ SS_HD void refracted_pair_fp64(const RefrFacet& f, const TraceRec& t, const Refr
 }
 
 // Mixed tier: the joint solve itself stays float64 (its damped Newton needs
-// the float64 merit comparisons); radiometry runs in the mixed tier.
+// the float64 merit comparisons); radiometry runs in the mixed tier.  The
+// flat-stack form (NMAX < 0) reads its layer constants from ``fl`` (staged
+// in LDS by the caller).
 template <int NMAX, int RM = -1, int PAT = -1, typename FT = RefrFacet, bool STAGED = false>
 SS_HD bool refracted_pair_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
-                                PairOut& o) {
+                                PairOut& o, const FlatLds* fl = nullptr) {
   PathSums ps;
   if constexpr (NMAX < 0) {
-    if (!flat_path_mixed(t.p, f.c, rc, ps)) return false;
+    if (!flat_path_mixed(t.p, f.c, rc, *fl, ps)) return false;
   } else {
     if (NMAX > 0 && rc.joint)
       joint_sums<(NMAX > 0 ? NMAX : 1)>(f, t, rc, ps);

@@ -674,11 +674,21 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
       __shared__ short pidx[kPatCells];
       stage_pattern(rcl.pat, ptab, pidx);
     }
+    const FlatLds* flp = nullptr;
+    if constexpr (NMAX < 0) {
+      // flat stack: the layer constants once per workgroup
+      __shared__ FlatLds fl;
+      stage_flat_layer(rc, fl, threadIdx.x);
+      __syncthreads();
+      if (threadIdx.x == 0) stage_flat_mins(rc, fl);
+      __syncthreads();
+      flp = &fl;
+    }
     trace_loop<ACC, kTTRefr, true>(sh, k, w, live, f, rc.n_samples, split,
                                    [&](const TraceRec& t, PairOut& o, int) {
                                      if ((rc.flags & SS_REDO_ALL) ||
                                          !refracted_pair_mixed<NMAX, RM, PAT, RefrFacet32, ST>(
-                                             f32, t, rcl, o)) {
+                                             f32, t, rcl, o, flp)) {
                                        o.valid = false;
                                        o.pwr_invalid = 0.0;
                                        o.redo = true;
