@@ -175,9 +175,14 @@ typedef struct ss_incoherent_params {
 } ss_incoherent_params;
 
 /* One layer of one request in a batched launch (ss_batch_field).  kind 0:
- * surface layer (uses ``surface``), kind 1: refracted layer, sequential
- * chain (uses ``refracted`` and the interface pointers).  All pointers are
- * device pointers with the layouts above; outputs are zeroed by the call. */
+ * surface layer (uses ``surface``), kind 1: refracted layer (uses
+ * ``refracted`` and the interface pointers) with the solver that
+ * refracted.refraction names: 0 sequential chain, 1 joint solve (stack
+ * incl. pads at most SS_MAX_JOINT deep), 2 joint solve over a flat stack
+ * (as ss_refracted_field).  All pointers are device pointers with the
+ * layouts above; field and dropped must be ordinary (coarse-grained) device
+ * allocations such as hipMalloc or torch give: the kernels accumulate into
+ * them with hardware float64 atomics.  Outputs are zeroed by the call. */
 typedef struct ss_job {
   int32_t kind;
   int32_t precision_unused;   /* the batch precision is the call's          */
@@ -322,13 +327,19 @@ int64_t ss_batch_workspace_bytes(int precision, const ss_job* jobs_host, int n_j
 /* Device-resident batched step (capturable in a HIP graph).  A job table in
  * device memory -- SS_BATCH_TABLE_HEADER bytes holding the int32 job count,
  * then the ss_job records -- is read by the kernels themselves: outputs are
- * zeroed, both job kinds run as persistent launches, the float64 redo and
- * the fold-back follow, all with grids fixed at the call, so the launch
- * sequence can be captured once and replayed after each refresh of the
- * table (e.g. a copy from a pinned host table written by ss_batch_pack).
- * max_items bounds the refracted work items a table may hold (the redo
- * record's size); feature_class (from ss_batch_pack) must cover the jobs.
- * No counterpart in the reference; serves the request loop. */
+ * zeroed, each job class (surface, and the refracted jobs of each solver)
+ * runs as a persistent launch, the float64 redo and the fold-back follow,
+ * all with grids fixed at the call, so the launch sequence can be captured
+ * once and replayed after each refresh of the table (e.g. a copy from a
+ * pinned host table written by ss_batch_pack).  max_items bounds the
+ * refracted work items (items[1] of ss_batch_pack) a table may hold: it
+ * sizes the mixed tier's redo record.  A mixed step over a table with more
+ * refracted items cannot defer their unresolved pairs: it sets the
+ * workspace's overflow word (ss_batch_overflowed) and those pairs are
+ * missing from the outputs, so callers check items[1] <= max_items before
+ * the step (the request loop does) or read the word after it.
+ * feature_class (from ss_batch_pack) must cover the jobs.  No counterpart
+ * in the reference; serves the request loop. */
 #define SS_BATCH_TABLE_HEADER 256
 
 /* Bytes of a job table for up to max_jobs jobs (-1 if out of range). */
@@ -348,6 +359,12 @@ int64_t ss_batch_step_workspace_bytes(int precision, int64_t max_items, int trac
 int ss_batch_step(int precision, const void* table_device, void* workspace,
                   int64_t workspace_bytes, int64_t max_items, int trace_tile, int n_blocks,
                   int feature_class, void* stream);
+
+/* Whether the last batched launch on ``workspace`` (ss_batch_field or
+ * ss_batch_step, mixed tier) met more refracted work items than its redo
+ * record holds: *overflowed = 1 then (its result is incomplete), else 0.
+ * Waits for ``stream``.  No counterpart in the reference. */
+int ss_batch_overflowed(const void* workspace, int* overflowed, void* stream);
 
 /* sizeof(ss_job) as compiled into the library (for sizing jobs_device and
  * checking a binding's struct mirror).  No counterpart in the reference. */

@@ -48,7 +48,8 @@ EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_range_compress_fft", "ss_batch_table_bytes", "ss_batch_pack",
             "ss_batch_step_workspace_bytes", "ss_batch_step", "ss_facets_from_heights",
             "ss_facets_workspace_bytes", "ss_trace_diagnostics", "ss_incoherent_power",
-            "ss_slow_time_boxcar", "ss_focused_sar", "ss_refracted_field_noalloc")
+            "ss_slow_time_boxcar", "ss_focused_sar", "ss_refracted_field_noalloc",
+            "ss_batch_overflowed")
 NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
@@ -152,6 +153,8 @@ def configure(lib):
                                                i64, i64, dp, i64, i64, dp, dp, dp, dp,
                                                dp, dp, dp, dp, vp]
     lib.ss_refracted_field_noalloc.restype = ctypes.c_int
+    lib.ss_batch_overflowed.argtypes = [dp, ctypes.POINTER(ctypes.c_int), vp]
+    lib.ss_batch_overflowed.restype = ctypes.c_int
     lib.ss_refracted_workspace_bytes.argtypes = [i32, i64, i64]
     lib.ss_refracted_workspace_bytes.restype = ctypes.c_int64
     lib.ss_batch_field.argtypes = [i32, ctypes.POINTER(Job), i32, dp, dp, i64, i32, i32, vp]

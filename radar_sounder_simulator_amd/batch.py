@@ -9,9 +9,12 @@ layer of every scene becomes one ``ss_job``; one persistent launch drains all
 ``uneven_workloads`` draws the 64-scene mix: facets log-uniform in
 [1e4, 2e6] per interface, traces uniform in [50, 2000], 1-4 layers, centre
 frequency 30-330 MHz with 10-30 MHz bandwidth, per-scene seeds; subsurface
-layers use the sequential refraction chain (the kernel-level default of the
-reference, ``kernels/multilayer.py:599``).  ``scale`` shrinks facet and
-trace counts for quick runs (1.0 = the stated distribution).
+layers follow the reference's ``simulate()``: the sequential chain for the
+first one under the surface, the joint solve (``SimConfig.refraction``'s
+default) below it, padded to the reference's power-of-two buckets
+(``simulate.py:253-260, 357``; with at most 4 interfaces no pads arise).
+``scale`` shrinks facet and trace counts for quick runs (1.0 = the stated
+distribution).
 """
 
 import ctypes
@@ -25,7 +28,7 @@ from . import device as dv
 from .physics import C, fresnel_normal
 from .scene import Lattice, gaussian_surface, grid_facets, straight_track
 from .waveform import Waveform
-from .workloads import Layer, Workload
+from .workloads import Layer, Workload, _pad_bucket
 
 
 class BatchRunner:
@@ -187,9 +190,12 @@ def draw_scene(rng, srng, scale=1.0, max_facets=None):
         lat = Lattice(-depth + 20.0 * gaussian_surface(side + 1, dx, 1000.0, srng), dx)
         f = lat.facets()
         eps_below = eps_ice if j < L - 1 else 8.0
+        joint = j > 1
         layers.append(Layer(f"layer{j}", f, fresnel_normal(eps[-1], eps_below)
                             if j == L - 1 else -10 ** (-40 / 20), crossed=list(crossed),
-                            eps_leg=list(eps), att_leg=list(att), lattice=lat))
+                            eps_leg=list(eps), att_leg=list(att), lattice=lat,
+                            refraction="joint" if joint else "sequential",
+                            pad_to=_pad_bucket(j, L - 1) if joint else None))
         crossed.append(f)
         eps = eps + [eps_ice]
         att = att + [10.0]

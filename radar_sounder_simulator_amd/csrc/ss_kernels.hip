@@ -8,7 +8,7 @@
 //   1. evaluates its 256 pairs,
 //   2. finds the tile's bin range with a wavefront min/max reduction
 //      (cross-lane shuffles) and one LDS exchange between the 4 waves,
-//   3. accumulates the pairs into an LDS bin window (ds_add_f64 / ds_add_f32)
+//   3. accumulates the pairs into a float64 LDS bin window (ds_add_f64)
 //      -- a compact patch spans a few to a few tens of bins per trace --
 //   4. flushes the occupied window slots to HBM with one float64 atomic add
 //      per slot (the flush of trace k overlaps the accumulation of trace
@@ -112,6 +112,11 @@ __device__ inline long work_id(const KCommon& k) {
   return w < total ? w : -1;
 }
 
+// Per-workgroup LDS of the trace loop.  Both precision tiers accumulate
+// their window in float64: ds_add_f64 measured faster than ds_add_f32 on
+// gfx950 for these scatter patterns (surface kernel of the 512 x 512
+// surface+bed scene 1.19 -> 1.02 ms), and the sums of <= 256 float32 pair
+// contributions per bin carry no float32 rounding.
 template <typename ACC, int TT>
 struct Shared {
   TraceRec tr[TT];
@@ -123,7 +128,11 @@ struct Shared {
 };
 
 // Workspace header of the entry points that take a workspace.
-constexpr long kWsHeader = 256;  // [0], [1] work counters of a batched launch (per kind)
+// [0..3]: work counters of a batched launch (one per job class), byte
+// kWsOverflow: the redo record's overflow word.
+constexpr long kWsHeader = 256;
+constexpr long kWsOverflow = 64;
+constexpr int kJobCodes = 4;  // batched launch classes, see job_code
 
 // Redo record of one launch: ``flags`` holds one byte per work item (1 =
 // float64 lanes pending), ``masks`` TT x 4 lane masks per flagged item.
@@ -131,6 +140,7 @@ struct Redo {
   unsigned char* flags;
   unsigned long long* masks;
   long cap;  // work items the flags/masks can hold
+  unsigned* overflow;  // batched launches: set when an item lies beyond cap
 };
 
 __host__ __device__ inline long redo_flag_bytes(long n_items) { return (n_items + 7) & ~7L; }
@@ -141,6 +151,7 @@ __host__ __device__ inline Redo redo_view(void* ws, long n_items) {
   r.flags = reinterpret_cast<unsigned char*>(b + kWsHeader);
   r.masks = reinterpret_cast<unsigned long long*>(b + kWsHeader + redo_flag_bytes(n_items));
   r.cap = n_items;
+  r.overflow = reinterpret_cast<unsigned*>(b + kWsOverflow);
   return r;
 }
 
@@ -454,7 +465,7 @@ __device__ inline void load_refr_facet(const KCommon& k, long fi, int flags, Ref
 // picks the narrowest instantiation so unused features cost no registers.
 template <int PREC, int RM, int PAT>
 __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst sc) {
-  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   __shared__ Shared<ACC, kTTSurf> sh;
   const long w = work_id(k);
   if (w < 0) return;
@@ -503,7 +514,7 @@ constexpr int incoherent_fpl() { return PREC == SS_PREC_MIXED ? 4 : 1; }
 template <int PREC, int PAT>
 __global__ __launch_bounds__(kBlock) void incoherent_kernel(KCommon k, SurfaceConst sc) {
   constexpr int kIncFPL = incoherent_fpl<PREC>();
-  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   constexpr int TT = kTTSurf;
   __shared__ Shared<ACC, TT> sh;
   const long w = work_id(k);
@@ -700,7 +711,7 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
 
 template <int PREC, int NMAX, int RM, int PAT>
 __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc, Redo rd) {
-  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   __shared__ Shared<ACC, kTTRefr> sh;
   refracted_body<PREC, NMAX, RM, PAT, ACC>(sh, k, rc, rd);
 }
@@ -714,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst 
 template <int PREC, int RM, int PAT, int WAVES = (RM == 0 ? 3 : 2)>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void flat_kernel(
     KCommon k, RefrConst rc, Redo rd) {
-  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   __shared__ Shared<ACC, kTTRefr> sh;
   refracted_body<PREC, -1, RM, PAT, ACC>(sh, k, rc, rd);
 }
@@ -847,7 +858,8 @@ __global__ __launch_bounds__(kBlock) void redo_kernel(KCommon k, RefrConst rc, R
 }
 
 __global__ void reset_workspace_kernel(unsigned long long* hdr) {
-  if (threadIdx.x < 2) hdr[threadIdx.x] = 0ULL;
+  if (threadIdx.x < kJobCodes) hdr[threadIdx.x] = 0ULL;
+  if (threadIdx.x == 0) hdr[kWsOverflow / 8] = 0ULL;
 }
 
 // ------------------------------------------------------------ batched launch
@@ -918,19 +930,39 @@ __host__ __device__ inline long job_work(const ss_job& j, int tt) {
   return nft * ntt;
 }
 
+// Launch class of a job: 0 surface; refracted by solver: 1 sequential
+// chain, 2 joint solve, 3 joint solve over a flat stack.  Each class runs as
+// its own persistent launch (one pair function's registers each).
+__host__ __device__ inline int job_code(const ss_job& j) {
+  if (j.kind == 0) return 0;
+  const int r = j.refracted.refraction;
+  return (r >= 0 && r <= 2) ? 1 + r : 1;
+}
+
 // LDS prefix of per-job work counts: begins[i] = first work id of job i.
-// KIND >= 0 counts only the jobs of that kind (the others get no items).
-template <int TT, int KIND = -1>
-__device__ inline void job_prefix(const ss_job* jobs, int n_jobs, long* begins) {
+// CODE >= 0 counts only the jobs of that launch class (the others get no
+// items).  The redo record of a mixed launch holds the refracted classes
+// one after another: the items of class CODE start at the returned offset
+// (the items of the refracted classes below it).
+template <int TT, int CODE = -1>
+__device__ inline long job_prefix(const ss_job* jobs, int n_jobs, long* begins) {
   const int tid = threadIdx.x;
+  __shared__ long s_base;
   for (int i = tid; i < n_jobs; i += kBlock)
-    begins[i + 1] = (KIND < 0 || jobs[i].kind == KIND) ? job_work(jobs[i], TT) : 0;
+    begins[i + 1] = (CODE < 0 || job_code(jobs[i]) == CODE) ? job_work(jobs[i], TT) : 0;
   __syncthreads();
   if (tid == 0) {
     begins[0] = 0;
-    for (int i = 0; i < n_jobs; ++i) begins[i + 1] += begins[i];
+    long base = 0;
+    for (int i = 0; i < n_jobs; ++i) {
+      begins[i + 1] += begins[i];
+      const int c = job_code(jobs[i]);
+      if (CODE > 1 && c >= 1 && c < CODE) base += job_work(jobs[i], TT);
+    }
+    s_base = base;
   }
   __syncthreads();
+  return uniform_l(s_base);
 }
 
 // Job owning global work id gw (begins[lo] <= gw < begins[lo + 1]; the
@@ -970,28 +1002,37 @@ __device__ inline KCommon job_common(const ss_job& jb) {
 
 // Persistent workgroups drain (job, trace tile, facet tile) items from one
 // counter; every workgroup exits once the counter passes the total.  Mixed
-// refracted items flag their unresolved pairs under their global work id.
-// RM/PAT: the widest roughness mode / antenna kind of the job mix (as in
+// refracted items flag their unresolved pairs under their work id in the
+// redo record (each solver class in its own slice of it); an item beyond
+// the record's capacity sets the record's overflow word.  RM/PAT: the
+// widest roughness mode / antenna kind of the job mix (as in
 // surface_kernel), chosen on the host.  KIND: the launch runs the surface
-// (0) or the refracted (1) jobs of the table, so each instantiation carries
-// one pair function's registers.  The mixed instantiations are held to 3
-// waves per SIMD (168 VGPRs), where the single-launch kernels sit; the
-// job lookup would otherwise push them just past it.  The smooth float64
-// ones are held to 2 (the single-launch kernels' occupancy; a few spilled
-// dwords), the rough float64 ones keep 1 (2 would spill 100-200 dwords).
+// jobs (0) or the refracted jobs of one solver class (job_code: 1
+// sequential chain, 2 joint, 3 flat-stack joint) of the table, so each
+// instantiation carries one pair function's registers.  The mixed surface
+// and sequential instantiations are held to 3 waves per SIMD (168 VGPRs),
+// where the single-launch kernels sit; the job lookup would otherwise push
+// them just past it.  The smooth float64 ones are held to 2 (the
+// single-launch kernels' occupancy; a few spilled dwords), the rough
+// float64 ones keep 1 (2 would spill 100-200 dwords).  The joint classes
+// run the generic pair functions: the general joint solve at 1 wave (its
+// per-lane Newton state lives in private memory), the flat stack at 2
+// (mixed) / 1 (float64), as their single launches.
 template <int PREC, int TT, int RM, int PAT, int KIND,
-          int WAVES = (PREC == SS_PREC_MIXED ? 3 : (RM == 0 ? 2 : 1))>
+          int WAVES = (KIND >= 2 ? ((KIND == 3 && PREC == SS_PREC_MIXED) ? 2 : 1)
+                                 : (PREC == SS_PREC_MIXED ? 3 : (RM == 0 ? 2 : 1)))>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void batch_kernel(BatchArgs ba,
                                                       unsigned long long* counter, Redo rd) {
-  using ACC = typename std::conditional<PREC == SS_PREC_FP64, double, float>::type;
+  using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   __shared__ Shared<ACC, TT> sh;
   __shared__ long begins[kMaxJobs + 1];
   __shared__ long s_w;
   const int tid = threadIdx.x;
   const ss_job* __restrict__ jobs = ba.jobs;
   const int n_jobs = batch_n(ba);
-  job_prefix<TT, KIND>(jobs, n_jobs, begins);
+  const long redo_base = job_prefix<TT, KIND>(jobs, n_jobs, begins);
   const long total = uniform_l(begins[n_jobs]);
+  int staged = -1;  // job whose flat-stack layers are in LDS (KIND 3, mixed)
   for (;;) {
     if (tid == 0) s_w = (long)atomicAdd(counter, 1ULL);
     __syncthreads();
@@ -1023,6 +1064,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
                             });
       }
     } else {
+      // solver of the class: sequential chain, joint solve (stacks up to
+      // SS_MAX_JOINT), flat-stack joint solve
+      constexpr int NMAX = KIND == 2 ? SS_MAX_JOINT : (KIND == 3 ? -1 : 0);
       const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
                                            jb.pattern_table, jb.constants);
       const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
@@ -1032,25 +1076,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
         load_refr_facet(k, fi, rc.flags, fr);
         trace_loop<ACC, TT>(sh, k, w, live, f, rc.n_samples, split,
                             [&](const TraceRec& t, PairOut& o, int) {
-                              refracted_pair_fp64<0, RM, PAT>(fr, t, rc, o);
+                              refracted_pair_fp64<NMAX, RM, PAT>(fr, t, rc, o);
                             });
       } else {
+        const FlatLds* flp = nullptr;
+        if constexpr (KIND == 3) {
+          __shared__ FlatLds fl;
+          if (j != staged) {  // uniform: the work item's job
+            __syncthreads();
+            stage_flat_layer(rc, fl, tid);
+            __syncthreads();
+            if (tid == 0) stage_flat_mins(rc, fl);
+            __syncthreads();
+            staged = j;
+          }
+          flp = &fl;
+        }
         RefrFacet32 f32;
         {
           RefrFacet fr;
           load_refr_facet(k, fi, rc.flags, fr);
           to_f32(fr, f32);
         }
+        if (tid == 0 && redo_base + gw >= rd.cap && rd.overflow) *rd.overflow = 1u;
         trace_loop<ACC, TT, true>(sh, k, w, live, f, rc.n_samples, split,
                                   [&](const TraceRec& t, PairOut& o, int) {
                                     if ((rc.flags & SS_REDO_ALL) ||
-                                        !refracted_pair_mixed<0, RM, PAT>(f32, t, rc, o)) {
+                                        !refracted_pair_mixed<NMAX, RM, PAT>(f32, t, rc, o,
+                                                                             flp)) {
                                       o.valid = false;
                                       o.pwr_invalid = 0.0;
                                       o.redo = true;
                                     }
                                   },
-                                  rd, gw);
+                                  rd, redo_base + gw);
       }
     }
     // Reconverge before the next work id: without this barrier the compiler
@@ -1061,23 +1120,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
   }
 }
 
-// Float64 pass over the refracted work items a mixed batch_kernel launch
-// flagged (global work ids), masked to the recorded lanes.
-template <int TT>
+// Float64 pass over the refracted work items of one solver class (CODE)
+// that a mixed batch_kernel launch flagged (the class's slice of the redo
+// record), masked to the recorded lanes.
+template <int TT, int CODE>
 __global__ __launch_bounds__(kBlock) void batch_redo_kernel(BatchArgs ba, Redo rd) {
+  constexpr int NMAX = CODE == 2 ? SS_MAX_JOINT : (CODE == 3 ? -1 : 0);
   __shared__ Shared<double, TT> sh;
   __shared__ long begins[kMaxJobs + 1];
   __shared__ unsigned long long sh_mask[kBlock / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const ss_job* __restrict__ jobs = ba.jobs;
   const int n_jobs = batch_n(ba);
-  job_prefix<TT, 1>(jobs, n_jobs, begins);
+  const long base = job_prefix<TT, CODE>(jobs, n_jobs, begins);
   long total = uniform_l(begins[n_jobs]);
-  total = total < rd.cap ? total : rd.cap;
-  for_flagged_items(rd.flags, total, sh_mask, [&](long gw) {
+  const long room = rd.cap - base;
+  total = total < room ? total : (room > 0 ? room : 0);
+  for_flagged_items(rd.flags + base, total, sh_mask, [&](long gw) {
     const int j = uniform_i(find_job(begins, n_jobs, gw));
     const ss_job& jb = jobs[j];
-    if (uniform_i(jb.kind) != 1) return;
+    if (uniform_i(job_code(jb)) != CODE) return;
     const long w = gw - begins[j];
     const KCommon k = job_common<TT>(jb);
     const long f = (w % k.n_ftiles) * kBlock + tid;
@@ -1087,14 +1149,14 @@ __global__ __launch_bounds__(kBlock) void batch_redo_kernel(BatchArgs ba, Redo r
     const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
     RefrFacet fr;
     load_refr_facet(k, live ? f : 0, rc.flags, fr);
-    const unsigned long long* m = rd.masks + gw * (TT * 4);
+    const unsigned long long* m = rd.masks + (base + gw) * (TT * 4);
     trace_loop<double, TT>(sh, k, w, live, f, rc.n_samples, split,
                            [&](const TraceRec& t, PairOut& o, int it) {
                              if (((m[it * 4 + wave] >> lane) & 1ULL) == 0ULL) {
                                o.redo = true;
                                return;
                              }
-                             refracted_pair_fp64<0>(fr, t, rc, o);
+                             refracted_pair_fp64<NMAX>(fr, t, rc, o);
                            });
   });
 }
@@ -2058,10 +2120,20 @@ static int validate_job(const ss_job& j, int idx) {
     acf = r.acf;
     dt = r.dt;
     c = r.c;
-    if (r.refraction != 0)
-      return fail(SS_EINVAL, pre + "batched refracted jobs use the sequential chain");
+    if (r.refraction < 0 || r.refraction > 2)
+      return fail(SS_EINVAL, pre + "refraction must be 0 (sequential), 1 (joint) or 2 "
+                                   "(flat-stack joint)");
     if (r.n_crossed < 1 || r.n_crossed > SS_MAX_CROSSED)
       return fail(SS_ERANGE, pre + "n_crossed out of range [1, 64]");
+    if (r.refraction != 0) {
+      if (r.refraction == 1 && r.n_crossed > SS_MAX_JOINT)
+        return fail(SS_ERANGE, pre + "joint stack deeper than SS_MAX_JOINT (32)");
+      if (r.k_pad < 0 || r.k_pad >= r.n_crossed)
+        return fail(SS_EINVAL, pre + "k_pad must be in [0, n_crossed)");
+      if (r.joint_newton < 0 || r.joint_backtrack < 1 || r.joint_newton > 64 ||
+          r.joint_backtrack > 32)
+        return fail(SS_ERANGE, pre + "joint budgets out of range (newton 0..64, backtrack 1..32)");
+    }
     if (!j.iface_desc || !j.tables || !j.legs)
       return fail(SS_EINVAL, pre + "null interface data");
   }
@@ -2108,30 +2180,39 @@ static int device_cus() {
 static int default_blocks(int n_blocks) { return n_blocks > 0 ? n_blocks : 4 * device_cus(); }
 
 // The launches of one batched step on a job table already in device memory:
-// zero the outputs, reset the per-kind work counters, one persistent launch
-// per job kind (grid[kind] workgroups; 0 = skip), the float64 redo sweep
-// (mixed) and the fold-back of float64 surface jobs.  Nothing here depends
-// on the table's contents on the host side when grids are fixed, so the
-// sequence can be captured once and replayed with new tables.
+// zero the outputs, reset the per-class work counters and the overflow
+// word, one persistent launch per job class (grid[code] workgroups; 0 =
+// skip), the float64 redo sweep of each refracted class (mixed; rblocks[code]
+// workgroups) and the fold-back of float64 surface jobs.  Nothing here
+// depends on the table's contents on the host side when grids are fixed,
+// so the sequence can be captured once and replayed with new tables.  The
+// surface and sequential classes take the feature class's instantiation;
+// the joint classes the generic one.
 static void launch_batch(bool fp64, const BatchArgs& ba, void* workspace, const Redo& rd,
-                         int tt, const unsigned grid[2], unsigned rblocks, int cls,
-                         bool fold, hipStream_t st) {
+                         int tt, const unsigned grid[kJobCodes],
+                         const unsigned rblocks[kJobCodes], int cls, bool fold,
+                         hipStream_t st) {
   unsigned long long* hdr = static_cast<unsigned long long*>(workspace);
   const dim3 block(kBlock);
   const unsigned cus = (unsigned)device_cus();
   hipLaunchKernelGGL(batch_zero_kernel, dim3(cus, 8), dim3(256), 0, st, ba);
   hipLaunchKernelGGL(reset_workspace_kernel, dim3(1), dim3(64), 0, st, hdr);
-  for (int kind = 0; kind < 2; ++kind) {
-    if (grid[kind] == 0) continue;
-    const dim3 g(grid[kind]);
-    unsigned long long* ctr = hdr + kind;
+  for (int code = 0; code < kJobCodes; ++code) {
+    if (grid[code] == 0) continue;
+    const dim3 g(grid[code]);
+    const dim3 rg(rblocks[code]);
+    unsigned long long* ctr = hdr + code;
 #define SS_BATCH_K(P, T, R, A)                                                                \
-  if (kind == 0)                                                                              \
+  if (code == 0)                                                                              \
     hipLaunchKernelGGL((batch_kernel<P, T, R, A, 0>), g, block, 0, st, ba, ctr, rd);          \
   else                                                                                        \
     hipLaunchKernelGGL((batch_kernel<P, T, R, A, 1>), g, block, 0, st, ba, ctr, rd)
 #define SS_BATCH_C(P, T)                                                                      \
-  if (cls == 0) { SS_BATCH_K(P, T, 0, 0); }                                                   \
+  if (code == 2) {                                                                            \
+    hipLaunchKernelGGL((batch_kernel<P, T, -1, -1, 2>), g, block, 0, st, ba, ctr, rd);        \
+  } else if (code == 3) {                                                                     \
+    hipLaunchKernelGGL((batch_kernel<P, T, -1, -1, 3>), g, block, 0, st, ba, ctr, rd);        \
+  } else if (cls == 0) { SS_BATCH_K(P, T, 0, 0); }                                            \
   else if (cls == 1) { SS_BATCH_K(P, T, 1, 5); }                                              \
   else { SS_BATCH_K(P, T, -1, -1); }
 #define SS_LAUNCH_BATCH(T)                                                                    \
@@ -2139,8 +2220,14 @@ static void launch_batch(bool fp64, const BatchArgs& ba, void* workspace, const 
     SS_BATCH_C(SS_PREC_FP64, T)                                                               \
   } else {                                                                                    \
     SS_BATCH_C(SS_PREC_MIXED, T)                                                              \
-    if (kind == 1 && rblocks > 0)                                                             \
-      hipLaunchKernelGGL((batch_redo_kernel<T>), dim3(rblocks), block, 0, st, ba, rd);        \
+    if (rblocks[code] > 0) {                                                                  \
+      if (code == 1)                                                                          \
+        hipLaunchKernelGGL((batch_redo_kernel<T, 1>), rg, block, 0, st, ba, rd);              \
+      else if (code == 2)                                                                     \
+        hipLaunchKernelGGL((batch_redo_kernel<T, 2>), rg, block, 0, st, ba, rd);              \
+      else if (code == 3)                                                                     \
+        hipLaunchKernelGGL((batch_redo_kernel<T, 3>), rg, block, 0, st, ba, rd);              \
+    }                                                                                         \
   }
     if (tt == 4) {
       SS_LAUNCH_BATCH(4);
@@ -2155,6 +2242,11 @@ static void launch_batch(bool fp64, const BatchArgs& ba, void* workspace, const 
   }
   if (fp64 && fold)
     hipLaunchKernelGGL(batch_fold_kernel, dim3(cus > 4 ? cus / 4 : 1, 16), dim3(256), 0, st, ba);
+}
+
+static unsigned redo_grid(long items) {
+  const long chunks = (items + kBlock - 1) / kBlock;
+  return (unsigned)(chunks < 1024 ? chunks : 1024);
 }
 
 int64_t ss_batch_workspace_bytes(int precision, const ss_job* jobs_host, int n_jobs,
@@ -2186,10 +2278,10 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   }
   if (workspace_bytes < ss_batch_workspace_bytes(precision, jobs_host, n_jobs, trace_tile))
     return fail(SS_EINVAL, "workspace smaller than ss_batch_workspace_bytes()");
-  long items[2] = {0, 0};
+  long items[kJobCodes] = {0, 0, 0, 0};
   bool any_surface = false;
   for (int i = 0; i < n_jobs; ++i) {
-    items[jobs_host[i].kind] += job_work(jobs_host[i], trace_tile);
+    items[job_code(jobs_host[i])] += job_work(jobs_host[i], trace_tile);
     any_surface |= jobs_host[i].kind == 0;
   }
   hipStream_t st = (hipStream_t)stream;
@@ -2198,15 +2290,16 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   if (e != hipSuccess) return fail(SS_EHIP, std::string("job upload: ") + hipGetErrorString(e));
   const bool fp64 = precision == SS_PREC_FP64;
   const int nb = default_blocks(n_blocks);
-  unsigned grid[2];
-  for (int k = 0; k < 2; ++k) grid[k] = (unsigned)(items[k] < nb ? items[k] : nb);
-  // the redo flags and masks index the refracted launch's work ids
-  const Redo rd = fp64 ? Redo{} : redo_view(workspace, items[1]);
-  const long rchunks = fp64 ? 0 : (items[1] + kBlock - 1) / kBlock;
+  unsigned grid[kJobCodes], rblocks[kJobCodes] = {0, 0, 0, 0};
+  for (int k = 0; k < kJobCodes; ++k) {
+    grid[k] = (unsigned)(items[k] < nb ? items[k] : nb);
+    if (!fp64 && k > 0) rblocks[k] = redo_grid(items[k]);
+  }
+  // the redo flags and masks hold the refracted classes' work ids in turn
+  const Redo rd = fp64 ? Redo{} : redo_view(workspace, items[1] + items[2] + items[3]);
   const BatchArgs ba{(const ss_job*)jobs_device, nullptr, n_jobs};
-  launch_batch(fp64, ba, workspace, rd, trace_tile, grid,
-               (unsigned)(rchunks < 1024 ? rchunks : 1024), batch_class(jobs_host, n_jobs),
-               any_surface, st);
+  launch_batch(fp64, ba, workspace, rd, trace_tile, grid, rblocks,
+               batch_class(jobs_host, n_jobs), any_surface, st);
   e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
   g_last_error.clear();
@@ -2274,17 +2367,33 @@ int ss_batch_step(int precision, const void* table_device, void* workspace,
     return fail(SS_EINVAL, "workspace smaller than ss_batch_step_workspace_bytes()");
   const bool fp64 = precision == SS_PREC_FP64;
   const int nb = default_blocks(n_blocks);
-  const unsigned grid[2] = {(unsigned)nb, (unsigned)nb};
+  const unsigned grid[kJobCodes] = {(unsigned)nb, (unsigned)nb, (unsigned)nb, (unsigned)nb};
+  unsigned rblocks[kJobCodes] = {0, 0, 0, 0};
+  if (!fp64)
+    for (int k = 1; k < kJobCodes; ++k) rblocks[k] = redo_grid(max_items);
   const Redo rd = fp64 ? Redo{} : redo_view(workspace, max_items);
-  const long rchunks = fp64 ? 0 : (max_items + kBlock - 1) / kBlock;
   const char* t = static_cast<const char*>(table_device);
   const BatchArgs ba{reinterpret_cast<const ss_job*>(t + SS_BATCH_TABLE_HEADER),
                      reinterpret_cast<const int*>(t), 0};
-  launch_batch(fp64, ba, workspace, rd, trace_tile, grid,
-               (unsigned)(rchunks < 1024 ? rchunks : 1024), feature_class, true,
+  launch_batch(fp64, ba, workspace, rd, trace_tile, grid, rblocks, feature_class, true,
                (hipStream_t)stream);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SS_EHIP, std::string("batch step launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_batch_overflowed(const void* workspace, int* overflowed, void* stream) {
+  if (!workspace || !overflowed) return fail(SS_EINVAL, "null pointer");
+  if ((reinterpret_cast<uintptr_t>(workspace) & 7) != 0)
+    return fail(SS_EINVAL, "workspace must be 8-byte aligned");
+  unsigned word = 0;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(&word, static_cast<const char*>(workspace) + kWsOverflow,
+                                sizeof(word), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail(SS_EHIP, std::string("overflow read: ") + hipGetErrorString(e));
+  *overflowed = word != 0u;
   g_last_error.clear();
   return SS_OK;
 }

@@ -118,7 +118,8 @@ class ResidentScene:
         return self.out
 
     def jobs(self, precision, out=None, dropped=None):
-        """``ss_job`` records (one per layer; sequential refraction) writing
+        """``ss_job`` records (one per layer, each refracted layer with its own
+        solver: sequential chain, joint solve or its flat-stack form) writing
         into this scene's ``raw``/``dropped`` buffers (or the given ones), and
         the tensors that must outlive the launch."""
         from . import _lib
@@ -128,8 +129,6 @@ class ResidentScene:
         jobs, keep = [], []
         cst = dv.constants(self.device)
         for j, (L, fac, ifc, rough) in enumerate(self.layers):
-            if L.refraction != "sequential":
-                raise ValueError(f"layer {L.name}: batched jobs take the sequential chain only")
             jb = _lib.Job()
             jb.facet_stride, jb.n_facets = fac.stride, fac.n
             jb.trace_stride, jb.n_traces = self.traces.stride, self.traces.n
@@ -149,7 +148,8 @@ class ResidentScene:
                     fac, ifc, L.eps_leg, L.att_leg, mode="coherent", t0=wl.t0, dt=wl.dt,
                     n_samples=wl.n_samples, c=wl.c, gamma=g, k0=wl.k0,
                     split_sides=wl.split_sides, pattern=wl.pattern, roughness=rough,
-                    taper_s=wl.taper_s, d_phi_area=wl.d_phi_area)
+                    taper_s=wl.taper_s, d_phi_area=wl.d_phi_area, refraction=L.refraction,
+                    flat_stack=L.flat_stack)
                 jb.kind = 1
                 jb.refracted = p
                 jb.iface_desc, jb.tables = ifc.desc.data_ptr(), ifc.tables.data_ptr()

@@ -88,6 +88,12 @@ class DeviceScene:
     def __init__(self, layers, device):
         self.layers = []
         self.center_mean = None
+        for L in layers:
+            if L.crossed and L.pad_to and L.pad_to > len(L.crossed):
+                # pads sit between the surface and each request's platform:
+                # a resident stack shared by requests carries none
+                raise ValueError(f"layer {L.name}: padded joint stacks are not served "
+                                 "(pad_to > crossed interfaces)")
         lat_of = {id(L.facets): L.lattice for L in layers if L.lattice is not None}
         if (all(L.lattice is not None for L in layers)
                 and all(id(c) in lat_of for L in layers for c in L.crossed)):
@@ -218,7 +224,8 @@ class ActiveRequest:
                     fac, ifc, L.eps_leg, L.att_leg, mode="coherent", t0=wl.t0, dt=wl.dt,
                     n_samples=wl.n_samples, c=wl.c, gamma=g, k0=wl.k0,
                     split_sides=wl.split_sides, pattern=wl.pattern, roughness=rough,
-                    taper_s=wl.taper_s, d_phi_area=wl.d_phi_area)
+                    taper_s=wl.taper_s, d_phi_area=wl.d_phi_area, refraction=L.refraction,
+                    flat_stack=L.flat_stack)
                 jb.kind = 1
                 jb.refracted = p
                 jb.iface_desc, jb.tables = ifc.desc.data_ptr(), ifc.tables.data_ptr()

@@ -118,3 +118,24 @@ def test_captured_step_replays_match_eager(precision):
                 else:
                     assert np.all(got == -1.0)
                 done += 1
+
+
+@pytest.mark.parametrize("flat", [False, True])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_joint_classes_match_single(flat, precision):
+    """Joint-solver jobs (the general joint solve, and its flat-stack form)
+    in the persistent launch against one launch per layer, every trace tile,
+    beside a sequential scene in the same batch."""
+    print(f"case batch/joint/flat{int(flat)}/{precision}", flush=True)
+    wls = [workloads.config3(nx=32, ny=16, n_traces=11, n_firn=4, horizons=(500.0,),
+                             flat_stack=flat),
+           workloads.config2(n=17, n_traces=21, n_samples=4096, seed=5)]
+    scenes = [ResidentScene(w, device="cuda") for w in wls]
+    ref = _single(scenes, precision)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for tt in (4, 8, 16):
+        got = _batched(scenes, precision, tt)
+        for (o, d), (ro, rd) in zip(got, ref):
+            assert all(np.linalg.norm(ro[j]) > 0 for j in range(ro.shape[0]))
+            assert _rel(o, ro) <= tol, tt
+            np.testing.assert_allclose(d, rd, rtol=tol, atol=1e-30)

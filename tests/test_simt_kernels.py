@@ -497,3 +497,55 @@ def test_noalloc_entry_matches_workspace_entry(sim):
         np.testing.assert_allclose(res[True][0], res[False][0], rtol=0,
                                    atol=1e-12 * np.abs(res[False][0]).max())
         np.testing.assert_allclose(res[True][1], res[False][1], rtol=1e-12, atol=1e-30)
+
+
+@pytest.mark.parametrize("flat", [False, True])
+def test_batched_joint_jobs_match_single_launches(sim, flat):
+    """Batched jobs of the joint solver classes (general joint solve, and its
+    flat-stack form) against one launch per layer: float64 to rounding, the
+    mixed tier with every refracted pair through the float64 redo to
+    rounding of the float64 single launches, and the plain mixed tier to its
+    summation order."""
+    wls = [W.config3(nx=16, ny=8, n_traces=3, n_firn=3, horizons=(500.0,), flat_stack=flat),
+           W.config2(n=9, n_traces=5, n_samples=4096, seed=4)]
+    assert any(L.refraction == "joint" for L in wls[0].layers)
+    ref64 = _run_single(sim, wls, "fp64")
+    for o, r in zip(_run_batch(sim, wls, "fp64", 8), ref64):
+        assert all(np.linalg.norm(r[j]) > 0 for j in range(r.shape[0]))
+        assert _rel(o, r) <= 1e-12
+    for o, r in zip(_run_batch(sim, wls, "mixed", 8, redo_all=True), ref64):
+        assert _rel(o[1:], r[1:]) <= 1e-12
+    refm = _run_single(sim, wls, "mixed")
+    for o, r in zip(_run_batch(sim, wls, "mixed", 4), refm):
+        assert _rel(o, r) <= 1e-6
+
+
+def test_batch_step_overflow_word(sim):
+    """A mixed step whose refracted items exceed the redo record sets the
+    workspace's overflow word (ss_batch_overflowed); a step within it
+    clears it."""
+    import ctypes
+    wls = _scenes()
+    with _lib.use(sim):
+        scenes = [ResidentScene(w, device="cpu") for w in wls]
+        jobs, keep = [], []
+        for rs in scenes:
+            j, k = rs.jobs("mixed")
+            jobs += j
+            keep += k
+        _, refr, _ = B.pack_info(jobs, 8)
+        lib = _lib.load()
+        flag = ctypes.c_int(-1)
+        for cap, want in ((max(1, refr // 2), 1), (refr, 0)):
+            sg = B.StepGraph("cpu", "mixed", 8, feature_class=1, max_jobs=8, max_items=cap,
+                             n_blocks=3, capture=False)
+            # bypass launch()'s own capacity check: pack and step directly
+            arr = (_lib.Job * len(jobs))(*jobs)
+            rc = lib.ss_batch_pack(arr, len(jobs), 8, ctypes.c_void_p(sg.table_host.data_ptr()),
+                                   sg.table_host.numel(), None, None)
+            _lib.check(rc, "ss_batch_pack")
+            sg._enqueue()
+            rc = lib.ss_batch_overflowed(ctypes.c_void_p(sg.workspace.data_ptr()),
+                                         ctypes.byref(flag), None)
+            _lib.check(rc, "ss_batch_overflowed")
+            assert flag.value == want, (cap, refr)
