@@ -113,8 +113,14 @@ def bench_batched(scale, dev, seed=4):
     from radar_sounder_simulator_amd import autotune
     from radar_sounder_simulator_amd.batch import BatchRunner, uneven_workloads
     from radar_sounder_simulator_amd.pipeline import ResidentScene
+    t = time.time()
     wls = uneven_workloads(64, seed=seed, scale=scale)
-    scenes = [ResidentScene(w, device=dev) for w in wls]
+    log(f"bench: batched mix drawn in {time.time() - t:.1f} s")
+    scenes = []
+    for i, w in enumerate(wls):
+        scenes.append(ResidentScene(w, device=dev))
+        if i % 16 == 15:
+            log(f"bench: batched mix: {i + 1} scenes resident ({time.time() - t:.1f} s)")
     pairs = sum(sum(rs.pairs_per_layer()) for rs in scenes)
     st = torch.cuda.current_stream()
     out = {"scenes": 64, "scale": scale, "pairs": pairs,
@@ -127,7 +133,9 @@ def bench_batched(scale, dev, seed=4):
             j, k = rs.jobs(prec)
             jobs += j
             keep += k
+        t = time.time()
         tt, nb = tuner.batch_shape(runner, jobs, prec)
+        log(f"bench: batched {prec}: tile {tt}, {nb} blocks (tuned in {time.time() - t:.1f} s)")
 
         def batched():
             runner.launch(jobs, prec, trace_tile=tt, n_blocks=nb)
@@ -150,20 +158,49 @@ def bench_batched(scale, dev, seed=4):
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1)
             res[name] = {"ms": ms, "pairs_per_s": pairs / (ms * 1e-3)}
+            log(f"bench: batched {prec} {name}: {ms:.1f} ms")
         res["speedup"] = res["sequential"]["ms"] / res["batched"]["ms"]
         out[prec] = res
+        torch.cuda.empty_cache()
         log(f"bench: batched {prec}: {res['batched']['ms']:.1f} ms vs sequential "
             f"{res['sequential']['ms']:.1f} ms (tile {tt}, {nb} blocks)")
+    # the comparison the batched configuration names: the mixed persistent
+    # launch against the same scenes as one plain float64 launch per layer
+    out["mixed_batched_vs_fp64_sequential"] = (out["fp64"]["sequential"]["ms"] /
+                                               out["mixed"]["batched"]["ms"])
+    out["joint_layers"] = sum(1 for w in wls for L in w.layers if L.refraction == "joint")
+    log(f"bench: batched mixed vs sequential fp64: {out['mixed_batched_vs_fp64_sequential']:.2f}x")
     return out
 
 
-def bench_multilayer(n_traces, dev, steps=2):
+def errors_db_mrad_dev(mixed, ref, floor_db=30.0):
+    """errors_db_mrad on device tensors (..., n, 2) float64."""
+    import torch
+    a = torch.view_as_complex(mixed.contiguous())
+    b = torch.view_as_complex(ref.contiguous())
+    ab = b.abs()
+    pk = ab.amax(dim=-1, keepdim=True)
+    m = ab > pk * 10 ** (-floor_db / 20.0)
+    r = a[m] / b[m]
+    db = (20 * torch.log10(r.abs())).abs().max().item() if r.numel() else 0.0
+    mrad = r.angle().abs().max().item() * 1e3 if r.numel() else 0.0
+    rel = ((a - b).abs().pow(2).sum().sqrt() / ab.pow(2).sum().sqrt()).item()
+    return float(db), float(mrad), float(rel)
+
+
+def bench_multilayer(n_traces, dev, sub_traces=8):
     """The multilayer scene (2048 x 1024 facets per interface, 25 target
     interfaces: rough surface, 20 firn sublayers, 3 horizons, rough bed;
     8192 samples, 150/30 MHz Tukey chirp) over ``n_traces`` of its 1000
-    traces: mixed and float64 passes, per-layer kernel times and the mixed
-    tier's error per layer against the float64 tier."""
+    traces, the joint layers in their flat-stack form (the opt-in the scene
+    is built for): one mixed and one float64 pass, per-layer kernel times and
+    the mixed tier's error per layer against the float64 tier.  Beside it,
+    the reference's general joint solve on the bed (24 crossed interfaces)
+    and a mid-stack firn layer over ``sub_traces`` traces and all facets:
+    its rate in both tiers, and how many lanes its validity masks and bins
+    differ from the flat-stack form's (float64 tier of both)."""
     import torch
+    from radar_sounder_simulator_amd import device as dv
     from radar_sounder_simulator_amd import workloads
     from radar_sounder_simulator_amd.pipeline import ResidentScene
     t = time.time()
@@ -175,40 +212,85 @@ def bench_multilayer(n_traces, dev, steps=2):
     st = torch.cuda.current_stream()
     names = [L.name for L in wl.layers]
     out = {"workload": "multilayer, 2048x1024 facets at 30 m x 25 interfaces, "
-                       f"{n_traces} of 1000 traces at 20 km, 8192 samples, 150/30 MHz",
+                       f"{n_traces} of 1000 traces at 20 km, 8192 samples, 150/30 MHz, "
+                       "joint layers in the flat-stack form",
            "traces": n_traces, "layers": len(names), "pairs_per_step": pairs}
-    fields = {}
+    kept = None
     for prec in ("mixed", "fp64"):
-        rs.run(prec)
+        for j in range(3):             # load each kernel once before timing
+            rs.run_layer(j, prec)
         torch.cuda.synchronize()
-        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 2)]
-              for _ in range(steps)]
-        for s in range(steps):
-            e = ev[s]
-            e[0].record(st)
-            for j in range(len(names)):
-                rs.run_layer(j, prec)
-                e[j + 1].record(st)
-            rs.compress()
-            e[-1].record(st)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 2)]
+        e[0].record(st)
+        for j in range(len(names)):
+            rs.run_layer(j, prec)
+            e[j + 1].record(st)
+        rs.compress()
+        e[-1].record(st)
         torch.cuda.synchronize()
-        ms = np.mean([e[0].elapsed_time(e[-1]) for e in ev])
-        per = [float(np.mean([e[j].elapsed_time(e[j + 1]) for e in ev]))
-               for j in range(len(names))]
-        fields[prec] = rs.raw.cpu().numpy()
+        ms = e[0].elapsed_time(e[-1])
+        per = [e[j].elapsed_time(e[j + 1]) for j in range(len(names))]
         out[prec] = {"ms_per_step": ms, "pairs_per_s": pairs / (ms * 1e-3),
                      "layer_ms": dict(zip(names, per))}
+        if prec == "mixed":
+            kept = rs.raw.clone()
         log(f"bench: multilayer {prec}: {ms:.1f} ms/step, {pairs / (ms * 1e-3):.3e} pairs/s")
     errs = {}
     for j, n in enumerate(names):
-        db, mrad, rel = errors_db_mrad(fields["mixed"][j:j + 1], fields["fp64"][j:j + 1])
+        db, mrad, rel = errors_db_mrad_dev(kept[j], rs.raw[j])
         errs[n] = {"max_db": db, "max_mrad": mrad, "rel_l2": rel}
+    del kept
     out["mixed_vs_fp64_per_layer"] = errs
     out["max_db"] = max(e["max_db"] for e in errs.values())
     out["max_mrad"] = max(e["max_mrad"] for e in errs.values())
     out["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
     log(f"bench: multilayer mixed vs fp64 worst layer {out['max_db']:.2e} dB "
-        f"{out['max_mrad']:.3f} mrad")
+        f"{out['max_mrad']:.3f} mrad, speedup {out['speedup_vs_fp64']:.2f}x")
+    # the reference's general joint solve on a trace subset, all facets
+    tr = wl.track
+    idx = np.linspace(0, n_traces - 1, sub_traces).round().astype(int)
+    trs = dv.DeviceTraces(tr.positions[idx], tr.u_ct[idx], None, wl.pattern[1][idx],
+                          device=dev)
+    gen = {"traces": sub_traces, "layers": {}}
+    for name in ("firn12", "bed"):
+        j = names.index(name)
+        L, fac, ifc, rough = rs.layers[j]
+        kw = dict(mode="coherent", t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c,
+                  gamma=L.gamma, k0=wl.k0, pattern=wl.pattern, refraction="joint")
+        res, bins = {}, {}
+        for form, flat in (("general", False), ("flat", True)):
+            for prec in ("fp64", "mixed"):
+                dv.refracted_field(fac, trs, ifc, L.eps_leg, L.att_leg, precision=prec,
+                                   flat_stack=flat, **kw)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                r = dv.refracted_field(fac, trs, ifc, L.eps_leg, L.att_leg, precision=prec,
+                                       flat_stack=flat, pair_bins=prec == "fp64", **kw)
+                e1.record(st)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1)
+                res[(form, prec)] = {"kernel_ms": ms, "pairs_per_s": fac.n * sub_traces / (ms * 1e-3)}
+                if prec == "fp64":
+                    bins[form] = r[2]
+        bg, bf = bins["general"], bins["flat"]
+        inv = -2147483648
+        vg, vf = bg != inv, bf != inv
+        rec = {"crossed": len(L.crossed), "pad_to": L.pad_to, "pairs": fac.n * sub_traces,
+               "general": {p: res[("general", p)] for p in ("fp64", "mixed")},
+               "flat": {p: res[("flat", p)] for p in ("fp64", "mixed")},
+               "validity_differs": int((vg != vf).sum().item()),
+               "valid_general_only": int((vg & ~vf).sum().item()),
+               "valid_flat_only": int((vf & ~vg).sum().item()),
+               "bins_differ_both_valid": int(((bg != bf) & vg & vf).sum().item()),
+               "invalid_both": int((~vg & ~vf).sum().item())}
+        rec["general_mixed_speedup_vs_fp64"] = (res[("general", "fp64")]["kernel_ms"] /
+                                                res[("general", "mixed")]["kernel_ms"])
+        gen["layers"][name] = rec
+        log(f"bench: multilayer {name} general joint: fp64 {res[('general', 'fp64')]['kernel_ms']:.1f} "
+            f"ms, mixed {res[('general', 'mixed')]['kernel_ms']:.1f} ms over {sub_traces} traces; "
+            f"validity differs on {rec['validity_differs']} of {rec['pairs']} lanes")
+    out["reference_joint_form"] = gen
     del rs
     return out
 
@@ -350,11 +432,14 @@ def bench_processing(field, wl, dev, aperture_m=2000.0, steps=5, cpu_traces=41):
 
 
 def bench_serving(n_requests, lam, scale, dev, seed=7):
-    """The request stream on the mixed and on the float64 tier."""
+    """The request stream at ``lam`` requests/s on the mixed and on the
+    float64 tier."""
     import torch
     from radar_sounder_simulator_amd import serve as S
+    t = time.time()
     resident, reqs = S.request_stream(n_requests, lam=lam, seed=seed, scale=scale,
                                       n_resident=8, reuse=0.5)
+    log(f"bench: serving stream drawn in {time.time() - t:.1f} s")
     out = {"requests": n_requests, "lambda_per_s": lam, "scale": scale,
            "resident_scenes": 8, "reuse": 0.5,
            "pairs": int(sum(r.workload.pairs() for r in reqs))}
@@ -387,13 +472,13 @@ def main():
     ap.add_argument("--cpu-traces", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--batch-scale", type=float, default=0.25)
-    ap.add_argument("--serve-scale", type=float, default=0.5)
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--lam", type=float, default=8.0)
+    ap.add_argument("--batch-scale", type=float, default=1.0)
+    ap.add_argument("--serve-scale", type=float, default=1.0)
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--lam", default="2,20", help="arrival rates (requests/s), comma-separated")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--ml-traces", type=int, default=64,
+    ap.add_argument("--ml-traces", type=int, default=1000,
                     help="traces of the multilayer scene's 1000 in its section")
     ap.add_argument("--no-multilayer", action="store_true")
     ap.add_argument("--no-incoherent", action="store_true")
@@ -524,8 +609,12 @@ def main():
         line["batched"] = bench_batched(args.batch_scale, dev)
         torch.cuda.empty_cache()
     if not args.no_serve:
-        log(f"bench: serving stream ({args.serve_requests} requests at {args.lam}/s)")
-        line["serving"] = bench_serving(args.serve_requests, args.lam, args.serve_scale, dev)
+        line["serving"] = {}
+        for lam in (float(x) for x in str(args.lam).split(",")):
+            log(f"bench: serving stream ({args.serve_requests} requests at {lam}/s)")
+            line["serving"][f"lambda_{lam:g}"] = bench_serving(args.serve_requests, lam,
+                                                               args.serve_scale, dev)
+            torch.cuda.empty_cache()
     if not args.no_multilayer:
         torch.cuda.empty_cache()
         log(f"bench: multilayer scene ({args.ml_traces} traces)")

@@ -4,8 +4,10 @@
 workgroup: 4, 8 or 16 -- larger tiles reuse each facet's registers over more
 traces but leave fewer work items to balance uneven jobs) and the number of
 persistent workgroups (a multiple of the CU count).  ``tune_batch`` times
-every candidate on the actual job list (HIP events around each launch on
-the launch stream, best of ``reps`` after a warm-up) and returns the fastest;
+every candidate on a proxy of the job list -- the same jobs cut to their
+first ``probe_traces`` traces (every facet tile, every job kind; a few
+percent of the work of a large mix) -- with HIP events around each launch on
+the launch stream, best of ``reps`` after a warm-up, and returns the fastest;
 ``Tuner`` caches the choice in a JSON file keyed by device, precision and a
 coarse signature of the job mix (log2 buckets of total pairs and of the
 mean facets per job, layer kinds), so a serving loop tunes each kind of mix
@@ -26,13 +28,15 @@ BLOCKS_PER_CU = (2, 4, 8)
 
 def mix_signature(jobs, precision):
     """Coarse key of a job list: precision, log2 bucket of total pairs and of
-    mean facets per job, and the refracted-job fraction (quarters)."""
+    mean facets per job, the refracted-job and joint-solver fractions
+    (quarters)."""
     n = max(len(jobs), 1)
     pairs = sum(j.n_facets * j.n_traces for j in jobs)
     facets = sum(j.n_facets for j in jobs) / n
     refr = sum(1 for j in jobs if j.kind == 1) / n
+    joint = sum(1 for j in jobs if j.kind == 1 and j.refracted.refraction != 0) / n
     return (f"{precision}|p{int(math.log2(max(pairs, 1)))}|f{int(math.log2(max(facets, 1)))}"
-            f"|r{int(round(refr * 4))}")
+            f"|r{int(round(refr * 4))}|j{int(round(joint * 4))}")
 
 
 def _cus(device):
@@ -41,11 +45,25 @@ def _cus(device):
     return torch.cuda.get_device_properties(device).multi_processor_count
 
 
-def tune_batch(runner, jobs, precision, reps=2, candidates=None):
-    """Time each (trace_tile, n_blocks) on ``jobs`` with ``runner`` (a
-    BatchRunner); returns (best (tt, n_blocks), {candidate: ms})."""
+def proxy_jobs(jobs, probe_traces=32):
+    """Copies of ``jobs`` cut to their first ``probe_traces`` traces (outputs
+    to the same rows), the tuning proxy of a large mix."""
+    out = []
+    for j in jobs:
+        c = _lib.Job.from_buffer_copy(j)
+        c.n_traces = min(c.n_traces, probe_traces)
+        out.append(c)
+    return out
+
+
+def tune_batch(runner, jobs, precision, reps=2, candidates=None, probe_traces=32):
+    """Time each (trace_tile, n_blocks) on the proxy of ``jobs`` with
+    ``runner`` (a BatchRunner); returns (best (tt, n_blocks), {candidate:
+    ms}).  ``probe_traces=None`` times the jobs themselves."""
     dev = runner.device
     cus = _cus(dev)
+    if probe_traces:
+        jobs = proxy_jobs(jobs, probe_traces)
     if candidates is None:
         candidates = [(tt, k * cus) for tt in TRACE_TILES for k in BLOCKS_PER_CU]
     times = {}

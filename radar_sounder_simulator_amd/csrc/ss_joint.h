@@ -87,6 +87,20 @@ struct JointWork {
   double f[NMAX][2];
 };
 
+// Loops over the stack's interfaces (i < N) and segments (k <= N) with the
+// compile-time bound NMAX: shallow instantiations (NMAX <= 4) unroll them,
+// so the per-lane state lives in registers instead of private memory.
+template <int NMAX>
+struct JUnroll {
+  static constexpr int kU = NMAX <= 4 ? NMAX + 1 : 1;
+};
+#define SS_JLOOP(i)                                  \
+  _Pragma("unroll JUnroll<NMAX>::kU") for (int i = 0; i < NMAX; ++i) if (i < N)
+#define SS_JLOOP_SEG(k)                              \
+  _Pragma("unroll JUnroll<NMAX>::kU") for (int k = 0; k <= NMAX; ++k) if (k <= N)
+#define SS_JLOOP_BACK(i)                             \
+  _Pragma("unroll JUnroll<NMAX>::kU") for (int i = NMAX - 1; i >= 0; --i) if (i < N)
+
 // merit |F|^2 of offsets ab (+ lam * delta) and optionally F itself.
 template <int NMAX>
 SS_HD double joint_geom(JointWork<NMAX>& w, int N, const double ps[3], const double qs[3],
@@ -95,7 +109,7 @@ SS_HD double joint_geom(JointWork<NMAX>& w, int N, const double ps[3], const dou
   double prev[3] = {ps[0], ps[1], ps[2]};
   double gprev[3] = {0, 0, 0};
   double merit = 0.0;
-  for (int k = 0; k <= N; ++k) {
+  SS_JLOOP_SEG(k) {
     double nxt[3];
     if (k < N) {
       double a0 = w.ab[k][0] + (with_delta ? lam * w.delta[k][0] : 0.0);
@@ -150,7 +164,7 @@ SS_HD bool joint_crossings(const double p[3], const double q[3], int N,
   JointWork<NMAX> w;
   // initial path: sequential chain over the local planes (25 iterations)
   double cur[3] = {p[0], p[1], p[2]};
-  for (int i = 0; i < N; ++i) {
+  SS_JLOOP(i) {
     Snell64 sn;
     snell_fp64(cur, q, pts[i], nrs[i], nseg[i], nseg[i + 1], kLocalIters, sn);
     x[i][0] = sn.x[0];
@@ -166,7 +180,7 @@ SS_HD bool joint_crossings(const double p[3], const double q[3], int N,
   double inv = 1.0 / scale;
   double ps[3] = {p[0] * inv, p[1] * inv, p[2] * inv};
   double qs[3] = {q[0] * inv, q[1] * inv, q[2] * inv};
-  for (int i = 0; i < N; ++i) {
+  SS_JLOOP(i) {
     for (int c = 0; c < 3; ++c) {
       w.o[i][c] = pts[i][c] * inv;
       w.nr[i][c] = nrs[i][c];
@@ -181,10 +195,13 @@ SS_HD bool joint_crossings(const double p[3], const double q[3], int N,
   double sk[NMAX + 1], dhk[NMAX + 1][3];
   for (int it = 0; it < n_newton; ++it) {
     double m0 = joint_geom<NMAX>(w, N, ps, qs, nseg, 0.0, false, true, sk, dhk);
+    // converged to the float64 floor: the remaining steps of the reference's
+    // fixed budget move the offsets by rounding only (as flat_newton64)
+    if (!(m0 > 1e-30)) break;
     // blocks
     double wprev[6];
     mat3_sym_outer(dhk[0], nseg[0] / sk[0], wprev);
-    for (int i = 0; i < N; ++i) {
+    SS_JLOOP(i) {
       double wn[6], ws[6];
       mat3_sym_outer(dhk[i + 1], nseg[i + 1] / sk[i + 1], wn);
       for (int e = 0; e < 6; ++e) ws[e] = wprev[e] + wn[e];
@@ -200,7 +217,7 @@ SS_HD bool joint_crossings(const double p[3], const double q[3], int N,
       for (int e = 0; e < 6; ++e) wprev[e] = wn[e];
     }
     // block Thomas: forward elimination
-    for (int i = 0; i < N; ++i) {
+    SS_JLOOP(i) {
       double M[2][2] = {{w.A[i][0][0], w.A[i][0][1]}, {w.A[i][1][0], w.A[i][1][1]}};
       double yv[2] = {-w.f[i][0], -w.f[i][1]};
       if (i > 0) {
@@ -222,7 +239,7 @@ SS_HD bool joint_crossings(const double p[3], const double q[3], int N,
     }
     // back substitution
     double dn[2] = {0.0, 0.0};
-    for (int i = N - 1; i >= 0; --i) {
+    SS_JLOOP_BACK(i) {
       double r0 = w.y[i][0], r1 = w.y[i][1];
       if (i < N - 1) {
         r0 -= w.Cu[i][0][0] * dn[0] + w.Cu[i][0][1] * dn[1];
@@ -243,13 +260,13 @@ SS_HD bool joint_crossings(const double p[3], const double q[3], int N,
       }
       lam *= 0.5;
     }
-    for (int i = 0; i < N; ++i) {
+    SS_JLOOP(i) {
       w.ab[i][0] += sel * w.delta[i][0];
       w.ab[i][1] += sel * w.delta[i][1];
     }
   }
   // final crossings (unscaled) and the honest polyline geometry
-  for (int i = 0; i < N; ++i)
+  SS_JLOOP(i)
     for (int c = 0; c < 3; ++c)
       x[i][c] = (w.o[i][c] + (w.B[i][0][c] * w.ab[i][0] + w.B[i][1][c] * w.ab[i][1])) * scale;
   double prev[3] = {p[0], p[1], p[2]};
@@ -257,7 +274,7 @@ SS_HD bool joint_crossings(const double p[3], const double q[3], int N,
   double resid = 0.0;
   bool sides = true, evan = false, fin = true;
   double sdir[NMAX + 1][3];
-  for (int k = 0; k <= N; ++k) {
+  SS_JLOOP_SEG(k) {
     const double* nxt = k < N ? x[k] : q;
     double d[3] = {nxt[0] - prev[0], nxt[1] - prev[1], nxt[2] - prev[2]};
     double sl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -272,7 +289,7 @@ SS_HD bool joint_crossings(const double p[3], const double q[3], int N,
     prev[2] = nxt[2];
   }
   const double eps64 = kEps64;
-  for (int i = 0; i < N; ++i) {
+  SS_JLOOP(i) {
     const double* n = w.nr[i];
     for (int c = 0; c < 3; ++c) {
       din[c] = sdir[i][c];
@@ -318,7 +335,7 @@ SS_HD void joint_path(const double p[3], const double q[3], const RefrConst& rc,
   const double* attl = rc.legs + 2 * (N + 1);
   double pts[NMAX][3], nrs[NMAX][3];
   double c0[3] = {p[0], p[1], p[2]};
-  for (int i = 0; i < N; ++i) {
+  SS_JLOOP(i) {
     const double* d = rc.desc + i * SS_IDESC_COUNT;
     double mp[3] = {d[SS_IDESC_MP], d[SS_IDESC_MP + 1], d[SS_IDESC_MP + 2]};
     double mn[3] = {d[SS_IDESC_MN], d[SS_IDESC_MN + 1], d[SS_IDESC_MN + 2]};
@@ -347,7 +364,7 @@ SS_HD void joint_path(const double p[3], const double q[3], const RefrConst& rc,
   double c_surf = fmax(cos(th1[k_pad]), kCMin);
   tau2 = 1.0;
   opl = loss_db = sum_par = sum_perp = 0.0;
-  for (int i = 0; i < N; ++i) {
+  SS_JLOOP(i) {
     double ci = fmax(cos(th1[i]), kCMin);
     double e1 = epsl[i], e2 = epsl[i + 1];
     double n1 = sqrt(e1), n2 = sqrt(e2);
@@ -374,6 +391,477 @@ SS_HD void joint_path(const double p[3], const double q[3], const RefrConst& rc,
   x_first[0] = x[0][0];
   x_first[1] = x[0][1];
   x_first[2] = x[0][2];
+}
+
+// Mixed tier of the joint path.  The crossings solve the same stationary
+// problem as joint_path; only its root enters the result (the reference's
+// damped Newton, started from the sequential chain over the local planes,
+// reaches it within its 6-step budget whenever full Newton steps converge),
+// so the search runs in float32 and one float64 step lands on the root:
+//   pass 1 -- the mean-plane chain in float32 (coordinates relative to each
+//     mean-plane point), the cell lookups from it; when any crossing lies
+//     within kCellGuard of a rounding boundary the whole chain is taken in
+//     float64 as the reference takes it;
+//   initial path -- the sequential chain over the local planes, float32,
+//     placed on the planes in float64;
+//   Newton -- float32 on in-plane corrections to the initial path (segment
+//     vectors = the initial path's, differenced in float64, plus the
+//     corrections: thin layers keep their accuracy), coordinates relative
+//     to q scaled by |p - q|, the
+//     reference's block-tridiagonal system (Levenberg term included) and
+//     step halving; a search where no step lowers |F|^2 above the float32
+//     floor, or a damped one that took more than the reference's budget
+//     less two steps, hands the pair to the float64 redo;
+//   one float64 step -- F(ab) in float64, solved with the float32 block
+//     factorisation (its 1e-7 relative error leaves |F| ~1e-14);
+//   the polyline in float64 (validity, optical path) and the sums in float32.
+// Returns false (float64 redo) unless the largest Snell residual after the
+// float64 step is <= 1e-10 (a tenth of the reference's validity tolerance):
+// then the float64 solve ends on the same root and the validity tests act
+// on the same polyline.
+template <int NMAX>
+SS_HD bool joint_path_mixed(const double p[3], const double q[3], const RefrConst& rc,
+                            PathSums& ps) {
+  const int N = rc.n_crossed;
+  const int k_pad = rc.k_pad;
+  const double* nl = rc.legs;
+  const double* epsl = rc.legs + (N + 1);
+  const double* attl = rc.legs + 2 * (N + 1);
+  double pts[NMAX][3], nrs[NMAX][3];
+  // pass 1: mean-plane chain and cell lookups
+  int row[NMAX], col[NMAX];
+  bool exact = false;
+  {
+    double c0[3] = {p[0], p[1], p[2]};
+    SS_JLOOP(i) {
+      const double* d = rc.desc + i * SS_IDESC_COUNT;
+      const double* cf = d + SS_IDESC_COEF;
+      const double mp[3] = {d[SS_IDESC_MP], d[SS_IDESC_MP + 1], d[SS_IDESC_MP + 2]};
+      float pr[3] = {(float)(c0[0] - mp[0]), (float)(c0[1] - mp[1]), (float)(c0[2] - mp[2])};
+      float qr[3] = {(float)(q[0] - mp[0]), (float)(q[1] - mp[1]), (float)(q[2] - mp[2])};
+      float mn[3] = {(float)d[SS_IDESC_MN], (float)d[SS_IDESC_MN + 1],
+                     (float)d[SS_IDESC_MN + 2]};
+      float xr[3];
+      snell_anchor32(pr, qr, mn, (float)nl[i], (float)nl[i + 1], xr);
+      const double x[3] = {(double)xr[0] + mp[0], (double)xr[1] + mp[1],
+                           (double)xr[2] + mp[2]};
+      const double rc0 = (x[0] * cf[0] + x[1] * cf[2]) + cf[4];
+      const double rc1 = (x[0] * cf[1] + x[1] * cf[3]) + cf[5];
+      const int ny = (int)d[SS_IDESC_NY], nx = (int)d[SS_IDESC_NX];
+      if (ny > 1 || nx > 1) {
+        const double f0 = rc0 - floor(rc0), f1 = rc1 - floor(rc1);
+        exact = exact || fabs(f0 - 0.5) < kCellGuard || fabs(f1 - 0.5) < kCellGuard;
+      }
+      row[i] = clamp_round(rc0, ny - 1);
+      col[i] = clamp_round(rc1, nx - 1);
+      c0[0] = x[0];
+      c0[1] = x[1];
+      c0[2] = x[2];
+    }
+  }
+  if (exact) {
+    // near a lookup rounding boundary: the reference's float64 chain
+    double c0[3] = {p[0], p[1], p[2]};
+    SS_JLOOP(i) {
+      const double* d = rc.desc + i * SS_IDESC_COUNT;
+      const double mp[3] = {d[SS_IDESC_MP], d[SS_IDESC_MP + 1], d[SS_IDESC_MP + 2]};
+      const double mn[3] = {d[SS_IDESC_MN], d[SS_IDESC_MN + 1], d[SS_IDESC_MN + 2]};
+      Snell64 sn;
+      snell_fp64(c0, q, mp, mn, nl[i], nl[i + 1], kMeanIters, sn);
+      c0[0] = sn.x[0];
+      c0[1] = sn.x[1];
+      c0[2] = sn.x[2];
+      const double* cf = d + SS_IDESC_COEF;
+      double rc0, rc1;
+      {
+#pragma clang fp contract(off)
+        rc0 = (sn.x[0] * cf[0] + sn.x[1] * cf[2]) + cf[4];
+        rc1 = (sn.x[0] * cf[1] + sn.x[1] * cf[3]) + cf[5];
+      }
+      row[i] = clamp_round(rc0, (int)d[SS_IDESC_NY] - 1);
+      col[i] = clamp_round(rc1, (int)d[SS_IDESC_NX] - 1);
+    }
+  }
+  SS_JLOOP(i) {
+    const double* d = rc.desc + i * SS_IDESC_COUNT;
+    const int nx = (int)d[SS_IDESC_NX];
+    const double* g = rc.tables + ((long)d[SS_IDESC_TABOFF] + (long)row[i] * nx + col[i]) * 6;
+    pts[i][0] = g[0];
+    pts[i][1] = g[1];
+    pts[i][2] = g[2];
+    const double inn = rsq64(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+    nrs[i][0] = g[3] * inn;
+    nrs[i][1] = g[4] * inn;
+    nrs[i][2] = g[5] * inn;
+  }
+  // frame: relative to q, scaled by |p - q|
+  const double dpq[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+  double scale = sqrt(dpq[0] * dpq[0] + dpq[1] * dpq[1] + dpq[2] * dpq[2]);
+  scale = scale > kTiny ? scale : kTiny;
+  const double inv = rcp64(scale);
+  const double ps64[3] = {dpq[0] * inv, dpq[1] * inv, dpq[2] * inv};
+  double B64[NMAX][2][3], x0[NMAX][3];
+  float Bf[NMAX][2][3], ab[NMAX][2];
+  float nsf[NMAX + 1];
+  SS_JLOOP_SEG(k) { nsf[k] = (float)nl[k]; }
+  // initial path: the chain over the local planes in float32 (relative to
+  // each plane's point), placed on the planes in float64
+  {
+    float cur[3] = {(float)ps64[0], (float)ps64[1], (float)ps64[2]};
+    SS_JLOOP(i) {
+      plane_basis(nrs[i], B64[i]);
+      const float ni[3] = {(float)nrs[i][0], (float)nrs[i][1], (float)nrs[i][2]};
+      double o64[3];
+      float o[3];
+      for (int c = 0; c < 3; ++c) {
+        o64[c] = (pts[i][c] - q[c]) * inv;
+        o[c] = (float)o64[c];
+        Bf[i][0][c] = (float)B64[i][0][c];
+        Bf[i][1][c] = (float)B64[i][1][c];
+      }
+      const float pr[3] = {cur[0] - o[0], cur[1] - o[1], cur[2] - o[2]};
+      const float qr[3] = {-o[0], -o[1], -o[2]};
+      float xr[3];
+      snell_anchor32(pr, qr, ni, nsf[i], nsf[i + 1], xr);
+      const float a0 = Bf[i][0][0] * xr[0] + Bf[i][0][1] * xr[1] + Bf[i][0][2] * xr[2];
+      const float a1 = Bf[i][1][0] * xr[0] + Bf[i][1][1] * xr[1] + Bf[i][1][2] * xr[2];
+      for (int c = 0; c < 3; ++c) x0[i][c] = o64[c] + (B64[i][0][c] * a0 + B64[i][1][c] * a1);
+      cur[0] = (float)x0[i][0];
+      cur[1] = (float)x0[i][1];
+      cur[2] = (float)x0[i][2];
+      ab[i][0] = 0.0f;  // the unknowns: in-plane corrections to x0
+      ab[i][1] = 0.0f;
+    }
+  }
+  // segment vectors of the initial path, differenced in float64 (thin
+  // layers: a few metres against a path of tens of kilometres)
+  float D[NMAX + 1][3];
+  SS_JLOOP_SEG(k) {
+    for (int c = 0; c < 3; ++c) {
+      const double hi = k < N ? x0[k][c] : 0.0;
+      const double lo = k > 0 ? x0[k - 1][c] : ps64[c];
+      D[k][c] = (float)(hi - lo);
+    }
+  }
+  // float32 Newton: F_i = B_i^T (n_i d_i - n_{i+1} d_{i+1}) on the polyline
+  float f[NMAX][2], dinv[NMAX][2][2], Cu[NMAX][2][2], delta[NMAX][2];
+  auto geom32 = [&](float lam, bool keep, float sk[NMAX + 1], float dk[NMAX + 1][3]) {
+    float gprev[3] = {0.0f, 0.0f, 0.0f};
+    float prevo[3] = {0.0f, 0.0f, 0.0f};  // in-plane correction of the last crossing
+    float merit = 0.0f;
+    SS_JLOOP_SEG(k) {
+      float off[3] = {0.0f, 0.0f, 0.0f};
+      if (k < N) {
+        const float a0 = ab[k][0] + lam * delta[k][0];
+        const float a1 = ab[k][1] + lam * delta[k][1];
+        for (int c = 0; c < 3; ++c) off[c] = Bf[k][0][c] * a0 + Bf[k][1][c] * a1;
+      }
+      const float d[3] = {D[k][0] + (off[0] - prevo[0]), D[k][1] + (off[1] - prevo[1]),
+                          D[k][2] + (off[2] - prevo[2])};
+      const float s2 = fmaxf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], 1e-36f);
+      const float is = f_rsq(s2);
+      const float dh[3] = {d[0] * is, d[1] * is, d[2] * is};
+      if (keep) {
+        sk[k] = s2 * is;
+        dk[k][0] = dh[0];
+        dk[k][1] = dh[1];
+        dk[k][2] = dh[2];
+      }
+      const float g[3] = {nsf[k] * dh[0], nsf[k] * dh[1], nsf[k] * dh[2]};
+      if (k > 0) {
+        const int i = k - 1;
+        const float dg[3] = {gprev[0] - g[0], gprev[1] - g[1], gprev[2] - g[2]};
+        const float f0 = Bf[i][0][0] * dg[0] + Bf[i][0][1] * dg[1] + Bf[i][0][2] * dg[2];
+        const float f1 = Bf[i][1][0] * dg[0] + Bf[i][1][1] * dg[1] + Bf[i][1][2] * dg[2];
+        merit += f0 * f0 + f1 * f1;
+        if (keep) {
+          f[i][0] = f0;
+          f[i][1] = f1;
+        }
+      }
+      gprev[0] = g[0];
+      gprev[1] = g[1];
+      gprev[2] = g[2];
+      prevo[0] = off[0];
+      prevo[1] = off[1];
+      prevo[2] = off[2];
+    }
+    return merit;
+  };
+  SS_JLOOP(i) {
+    delta[i][0] = 0.0f;
+    delta[i][1] = 0.0f;
+  }
+  const float mu = 1.8189894035458565e-12f;
+  bool factored = false;
+  int steps = 0;
+  bool damped = false;
+  for (int it = 0; it < rc.n_newton + 2; ++it) {
+    float sk[NMAX + 1], dk[NMAX + 1][3];
+    const float m0 = geom32(0.0f, true, sk, dk);
+    if (!(m0 == m0)) return false;
+    // blocks and block Thomas (forward elimination)
+    float wprev[6];
+    {
+      const float w0 = nsf[0] * f_rcp(sk[0]);
+      wprev[0] = w0 * (1.0f - dk[0][0] * dk[0][0]);
+      wprev[1] = w0 * (0.0f - dk[0][0] * dk[0][1]);
+      wprev[2] = w0 * (0.0f - dk[0][0] * dk[0][2]);
+      wprev[3] = w0 * (1.0f - dk[0][1] * dk[0][1]);
+      wprev[4] = w0 * (0.0f - dk[0][1] * dk[0][2]);
+      wprev[5] = w0 * (1.0f - dk[0][2] * dk[0][2]);
+    }
+    float yv[NMAX][2];
+    SS_JLOOP(i) {
+      float wn[6];
+      const float w1 = nsf[i + 1] * f_rcp(sk[i + 1]);
+      const float* dd = dk[i + 1];
+      wn[0] = w1 * (1.0f - dd[0] * dd[0]);
+      wn[1] = w1 * (0.0f - dd[0] * dd[1]);
+      wn[2] = w1 * (0.0f - dd[0] * dd[2]);
+      wn[3] = w1 * (1.0f - dd[1] * dd[1]);
+      wn[4] = w1 * (0.0f - dd[1] * dd[2]);
+      wn[5] = w1 * (1.0f - dd[2] * dd[2]);
+      float A[2][2];
+      for (int c = 0; c < 2; ++c) {
+        const float* bj = Bf[i][c];
+        const float mv[3] = {(wprev[0] + wn[0]) * bj[0] + (wprev[1] + wn[1]) * bj[1] +
+                                 (wprev[2] + wn[2]) * bj[2],
+                             (wprev[1] + wn[1]) * bj[0] + (wprev[3] + wn[3]) * bj[1] +
+                                 (wprev[4] + wn[4]) * bj[2],
+                             (wprev[2] + wn[2]) * bj[0] + (wprev[4] + wn[4]) * bj[1] +
+                                 (wprev[5] + wn[5]) * bj[2]};
+        for (int r = 0; r < 2; ++r)
+          A[r][c] = Bf[i][r][0] * mv[0] + Bf[i][r][1] * mv[1] + Bf[i][r][2] * mv[2];
+      }
+      A[0][0] += mu;
+      A[1][1] += mu;
+      if (i < N - 1) {
+        for (int c = 0; c < 2; ++c) {
+          const float* bj = Bf[i + 1][c];
+          const float mv[3] = {wn[0] * bj[0] + wn[1] * bj[1] + wn[2] * bj[2],
+                               wn[1] * bj[0] + wn[3] * bj[1] + wn[4] * bj[2],
+                               wn[2] * bj[0] + wn[4] * bj[1] + wn[5] * bj[2]};
+          for (int r = 0; r < 2; ++r)
+            Cu[i][r][c] = -(Bf[i][r][0] * mv[0] + Bf[i][r][1] * mv[1] + Bf[i][r][2] * mv[2]);
+        }
+      }
+      for (int e = 0; e < 6; ++e) wprev[e] = wn[e];
+      float y0 = -f[i][0], y1 = -f[i][1];
+      if (i > 0) {
+        float cpt[2][2];
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b)
+            cpt[a][b] = Cu[i - 1][0][a] * dinv[i - 1][0][b] + Cu[i - 1][1][a] * dinv[i - 1][1][b];
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b)
+            A[a][b] -= cpt[a][0] * Cu[i - 1][0][b] + cpt[a][1] * Cu[i - 1][1][b];
+        y0 -= cpt[0][0] * yv[i - 1][0] + cpt[0][1] * yv[i - 1][1];
+        y1 -= cpt[1][0] * yv[i - 1][0] + cpt[1][1] * yv[i - 1][1];
+      }
+      const float idet = f_rcp(A[0][0] * A[1][1] - A[0][1] * A[1][0]);
+      dinv[i][0][0] = A[1][1] * idet;
+      dinv[i][0][1] = -A[0][1] * idet;
+      dinv[i][1][0] = -A[1][0] * idet;
+      dinv[i][1][1] = A[0][0] * idet;
+      yv[i][0] = y0;
+      yv[i][1] = y1;
+    }
+    factored = true;
+    if (!(m0 > 1e-13f)) break;  // at the float32 floor, factorised here
+    float dn[2] = {0.0f, 0.0f};
+    SS_JLOOP_BACK(i) {
+      float r0 = yv[i][0], r1 = yv[i][1];
+      if (i < N - 1) {
+        r0 -= Cu[i][0][0] * dn[0] + Cu[i][0][1] * dn[1];
+        r1 -= Cu[i][1][0] * dn[0] + Cu[i][1][1] * dn[1];
+      }
+      delta[i][0] = dinv[i][0][0] * r0 + dinv[i][0][1] * r1;
+      delta[i][1] = dinv[i][1][0] * r0 + dinv[i][1][1] * r1;
+      dn[0] = delta[i][0];
+      dn[1] = delta[i][1];
+    }
+    // the reference's step halving: the first of 1, 1/2, ... lowering |F|^2
+    float lam = 1.0f, sel = 0.0f;
+    for (int b = 0; b < rc.n_backtrack; ++b, lam *= 0.5f) {
+      if (geom32(lam, false, nullptr, nullptr) < m0) {
+        sel = lam;
+        break;
+      }
+    }
+    if (sel == 0.0f) {
+      if (m0 < 1e-10f) break;  // rounding-limited: the float64 step finishes
+      return false;            // no step lowers |F|^2: the float64 solve decides
+    }
+    damped = damped || sel < 1.0f;
+    ++steps;
+    SS_JLOOP(i) {
+      ab[i][0] += sel * delta[i][0];
+      ab[i][1] += sel * delta[i][1];
+      delta[i][0] = 0.0f;
+      delta[i][1] = 0.0f;
+    }
+  }
+  if (!factored) return false;
+  // a damped search that needed most of the reference's step budget may
+  // leave the float64 solve unconverged where this one converged
+  if (damped && steps > rc.n_newton - 2) return false;
+  // one float64 step: F(ab) in float64, the float32 factorisation
+  double ab64[NMAX][2];
+  SS_JLOOP(i) {
+    ab64[i][0] = ab[i][0];
+    ab64[i][1] = ab[i][1];
+  }
+  {
+    double f64[NMAX][2];
+    double prev[3] = {ps64[0], ps64[1], ps64[2]};
+    double gprev[3] = {0.0, 0.0, 0.0};
+    SS_JLOOP_SEG(k) {
+      double nxt[3] = {0.0, 0.0, 0.0};
+      if (k < N)
+        for (int c = 0; c < 3; ++c)
+          nxt[c] = x0[k][c] + (B64[k][0][c] * ab64[k][0] + B64[k][1][c] * ab64[k][1]);
+      const double d[3] = {nxt[0] - prev[0], nxt[1] - prev[1], nxt[2] - prev[2]};
+      const double is = rsq64(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const double g[3] = {nl[k] * d[0] * is, nl[k] * d[1] * is, nl[k] * d[2] * is};
+      if (k > 0) {
+        const int i = k - 1;
+        const double dg[3] = {gprev[0] - g[0], gprev[1] - g[1], gprev[2] - g[2]};
+        f64[i][0] = B64[i][0][0] * dg[0] + B64[i][0][1] * dg[1] + B64[i][0][2] * dg[2];
+        f64[i][1] = B64[i][1][0] * dg[0] + B64[i][1][1] * dg[1] + B64[i][1][2] * dg[2];
+      }
+      gprev[0] = g[0];
+      gprev[1] = g[1];
+      gprev[2] = g[2];
+      prev[0] = nxt[0];
+      prev[1] = nxt[1];
+      prev[2] = nxt[2];
+    }
+    double y64[NMAX][2];
+    SS_JLOOP(i) {
+      double y0 = -f64[i][0], y1 = -f64[i][1];
+      if (i > 0) {
+        double cpt[2][2];
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b)
+            cpt[a][b] = (double)Cu[i - 1][0][a] * dinv[i - 1][0][b] +
+                        (double)Cu[i - 1][1][a] * dinv[i - 1][1][b];
+        y0 -= cpt[0][0] * y64[i - 1][0] + cpt[0][1] * y64[i - 1][1];
+        y1 -= cpt[1][0] * y64[i - 1][0] + cpt[1][1] * y64[i - 1][1];
+      }
+      y64[i][0] = y0;
+      y64[i][1] = y1;
+    }
+    double dn[2] = {0.0, 0.0};
+    SS_JLOOP_BACK(i) {
+      double r0 = y64[i][0], r1 = y64[i][1];
+      if (i < N - 1) {
+        r0 -= (double)Cu[i][0][0] * dn[0] + (double)Cu[i][0][1] * dn[1];
+        r1 -= (double)Cu[i][1][0] * dn[0] + (double)Cu[i][1][1] * dn[1];
+      }
+      dn[0] = (double)dinv[i][0][0] * r0 + (double)dinv[i][0][1] * r1;
+      dn[1] = (double)dinv[i][1][0] * r0 + (double)dinv[i][1][1] * r1;
+      ab64[i][0] += dn[0];
+      ab64[i][1] += dn[1];
+    }
+  }
+  // the polyline in float64
+  double x[NMAX][3];
+  SS_JLOOP(i) {
+    for (int c = 0; c < 3; ++c)
+      x[i][c] = q[c] + (x0[i][c] + (B64[i][0][c] * ab64[i][0] + B64[i][1][c] * ab64[i][1])) *
+                           scale;
+  }
+  double s[NMAX + 1], sdir[NMAX + 1][3];
+  bool fin = true;
+  {
+    double prev[3] = {p[0], p[1], p[2]};
+    SS_JLOOP_SEG(k) {
+      const double* nxt = k < N ? x[k] : q;
+      const double d[3] = {nxt[0] - prev[0], nxt[1] - prev[1], nxt[2] - prev[2]};
+      const double s2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      const double is = rsq64(s2 > kTiny * kTiny ? s2 : kTiny * kTiny);
+      s[k] = s2 * is;
+      sdir[k][0] = d[0] * is;
+      sdir[k][1] = d[1] * is;
+      sdir[k][2] = d[2] * is;
+      fin = fin && isfinite(s[k]);
+      prev[0] = nxt[0];
+      prev[1] = nxt[1];
+      prev[2] = nxt[2];
+    }
+  }
+  double resid = 0.0;
+  bool sides = true, evan = false;
+  float ci_raw[NMAX], ct_raw[NMAX];
+  SS_JLOOP(i) {
+    const double* n = nrs[i];
+    const double* din = sdir[i];
+    const double* dout = sdir[i + 1];
+    const double ct1 = n[0] * din[0] + n[1] * din[1] + n[2] * din[2];
+    const double ct2 = n[0] * dout[0] + n[1] * dout[1] + n[2] * dout[2];
+    const double t1[3] = {din[0] - ct1 * n[0], din[1] - ct1 * n[1], din[2] - ct1 * n[2]};
+    const double st1 = sqrt64(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+    ci_raw[i] = (float)fabs(ct1);
+    ct_raw[i] = (float)fabs(ct2);
+    const double dG[3] = {nl[i] * din[0] - nl[i + 1] * dout[0],
+                          nl[i] * din[1] - nl[i + 1] * dout[1],
+                          nl[i] * din[2] - nl[i + 1] * dout[2]};
+    const double nd = n[0] * dG[0] + n[1] * dG[1] + n[2] * dG[2];
+    const double rt[3] = {dG[0] - nd * n[0], dG[1] - nd * n[1], dG[2] - nd * n[2]};
+    const double rn = sqrt64(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+    resid = rn > resid ? rn : resid;
+    fin = fin && isfinite(rn) && isfinite(x[i][0]) && isfinite(x[i][1]) && isfinite(x[i][2]);
+    const double* pa = i == 0 ? p : x[i - 1];
+    const double* pb = i == N - 1 ? q : x[i + 1];
+    const double hp = n[0] * (pa[0] - x[i][0]) + n[1] * (pa[1] - x[i][1]) + n[2] * (pa[2] - x[i][2]);
+    const double hn = n[0] * (pb[0] - x[i][0]) + n[1] * (pb[1] - x[i][1]) + n[2] * (pb[2] - x[i][2]);
+    sides = sides && (hp * hn < 0.0);
+    evan = evan || (nl[i] * st1 > nl[i + 1] * (1.0 + 64.0 * kEps64));
+  }
+  // converged to the root (the reference's own residual there is ~1e-15):
+  // the validity tests below are the reference's on the same polyline
+  if (!(resid <= 1e-10)) return false;
+  // sums (float32) and the optical path (float64)
+  const float c_surf = fmaxf(ci_raw[k_pad], (float)kCMin);
+  float tau2 = 1.0f, loss = 0.0f, perp = 0.0f, par = 0.0f;
+  double opl = 0.0;
+  const bool rough = (rc.flags & SS_CROSS_ROUGH) != 0;
+  SS_JLOOP(i) {
+    const float ci = fmaxf(ci_raw[i], (float)kCMin);
+    const float e1 = (float)epsl[i], e2 = (float)epsl[i + 1];
+    const float n1 = (float)nl[i], n2 = (float)nl[i + 1];
+    const float sin2 = (e1 * f_rcp(e2)) * (1.0f - ci * ci);
+    const float c2 = f_sqrt(fmaxf(1.0f - sin2, 0.0f));
+    const float g = (n1 * ci - n2 * c2) * f_rcp(n1 * ci + n2 * c2);
+    tau2 *= 1.0f - g * g;
+    if (rough) {
+      const float kt = (float)rc.k0 * (n1 * ci_raw[i] - n2 * ct_raw[i]);
+      const float sk = (float)rc.desc[i * SS_IDESC_COUNT + SS_IDESC_SIGC] * kt;
+      tau2 *= f_exp(-2.0f * (sk * sk));
+    }
+    const float cp = i < k_pad ? c_surf : ci;
+    const float sf = (float)s[i];
+    opl += nl[i] * s[i];
+    loss += sf * (float)(attl[i] / 1000.0);
+    perp += sf * f_rcp(n1);
+    par += sf * f_rcp(n1 * cp * cp);
+  }
+  ps.valid = fin && sides && !evan && resid <= 1e-9;
+  ps.opl = opl;
+  ps.loss_db = loss;
+  ps.sum_par = par;
+  ps.sum_perp = perp;
+  ps.tau2 = tau2;
+  ps.c_first = c_surf;
+  ps.c_last = fmaxf(ct_raw[N - 1], (float)kCMin);
+  ps.cur[0] = x[N - 1][0];
+  ps.cur[1] = x[N - 1][1];
+  ps.cur[2] = x[N - 1][2];
+  ps.x_first[0] = x[0][0];
+  ps.x_first[1] = x[0][1];
+  ps.x_first[2] = x[0][2];
+  return true;
 }
 
 // ------------------------------------------------------------ flat stack
@@ -1232,10 +1720,11 @@ SS_HD bool refracted_pair_mixed(const FT& f, const TraceRec& t, const RefrConst&
   if constexpr (NMAX < 0) {
     if (!flat_path_mixed(t.p, f.c, rc, *fl, ps)) return false;
   } else {
-    if (NMAX > 0 && rc.joint)
-      joint_sums<(NMAX > 0 ? NMAX : 1)>(f, t, rc, ps);
-    else if (!seq_path_mixed(f, t, rc, ps))
+    if (NMAX > 0 && rc.joint) {
+      if (!joint_path_mixed<(NMAX > 0 ? NMAX : 1)>(t.p, f.c, rc, ps)) return false;
+    } else if (!seq_path_mixed(f, t, rc, ps)) {
       return false;
+    }
   }
   return radiometry_mixed<RM, PAT, FT, STAGED>(f, t, rc, ps, o);
 }

@@ -930,6 +930,12 @@ __host__ __device__ inline long job_work(const ss_job& j, int tt) {
   return nft * ntt;
 }
 
+// A joint stack (pads included) shallow enough for the register-resident
+// joint instantiation.
+__host__ __device__ inline bool rc_shallow(const ss_refracted_params& r) {
+  return r.n_crossed <= 4;
+}
+
 // Launch class of a job: 0 surface; refracted by solver: 1 sequential
 // chain, 2 joint solve, 3 joint solve over a flat stack.  Each class runs as
 // its own persistent launch (one pair function's registers each).
@@ -1065,8 +1071,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
       }
     } else {
       // solver of the class: sequential chain, joint solve (stacks up to
-      // SS_MAX_JOINT), flat-stack joint solve
+      // SS_MAX_JOINT; shallow stacks of at most 4 take the register-resident
+      // instantiation: the branch is uniform, one job per work item),
+      // flat-stack joint solve
       constexpr int NMAX = KIND == 2 ? SS_MAX_JOINT : (KIND == 3 ? -1 : 0);
+      const bool shallow = KIND == 2 && rc_shallow(jb.refracted);
       const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
                                            jb.pattern_table, jb.constants);
       const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
@@ -1076,7 +1085,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
         load_refr_facet(k, fi, rc.flags, fr);
         trace_loop<ACC, TT>(sh, k, w, live, f, rc.n_samples, split,
                             [&](const TraceRec& t, PairOut& o, int) {
-                              refracted_pair_fp64<NMAX, RM, PAT>(fr, t, rc, o);
+                              if (KIND == 2 && shallow)
+                                refracted_pair_fp64<4, RM, PAT>(fr, t, rc, o);
+                              else
+                                refracted_pair_fp64<NMAX, RM, PAT>(fr, t, rc, o);
                             });
       } else {
         const FlatLds* flp = nullptr;
@@ -1101,9 +1113,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
         if (tid == 0 && redo_base + gw >= rd.cap && rd.overflow) *rd.overflow = 1u;
         trace_loop<ACC, TT, true>(sh, k, w, live, f, rc.n_samples, split,
                                   [&](const TraceRec& t, PairOut& o, int) {
-                                    if ((rc.flags & SS_REDO_ALL) ||
-                                        !refracted_pair_mixed<NMAX, RM, PAT>(f32, t, rc, o,
-                                                                             flp)) {
+                                    const bool ok =
+                                        (KIND == 2 && shallow)
+                                            ? refracted_pair_mixed<4, RM, PAT>(f32, t, rc, o)
+                                            : refracted_pair_mixed<NMAX, RM, PAT>(f32, t, rc, o,
+                                                                                  flp);
+                                    if ((rc.flags & SS_REDO_ALL) || !ok) {
                                       o.valid = false;
                                       o.pwr_invalid = 0.0;
                                       o.redo = true;
@@ -1156,7 +1171,10 @@ __global__ __launch_bounds__(kBlock) void batch_redo_kernel(BatchArgs ba, Redo r
                                o.redo = true;
                                return;
                              }
-                             refracted_pair_fp64<NMAX>(fr, t, rc, o);
+                             if (CODE == 2 && rc_shallow(jb.refracted))
+                               refracted_pair_fp64<4>(fr, t, rc, o);
+                             else
+                               refracted_pair_fp64<NMAX>(fr, t, rc, o);
                            });
   });
 }
@@ -2009,6 +2027,8 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
     } else {
       SS_REFR_PA(2);
     }
+  } else if (prm->n_crossed <= 4) {
+    SS_BOTH(4, -1, -1);
   } else if (prm->n_crossed <= 8) {
     SS_BOTH(8, -1, -1);
   } else {
