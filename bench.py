@@ -107,8 +107,11 @@ def cpu_baseline(wl, n_traces):
                       f"restatement, OpenMP {nth} threads on {cpu}; {el:.1f} s"}
 
 
-def bench_batched(scale, dev, seed=4):
-    """One persistent launch over the 64-scene mix vs one launch per layer."""
+def bench_batched(scale, dev, seed=4, fp64_batched=False):
+    """One persistent launch over the 64-scene mix vs one launch per layer, on
+    the mixed tier; the float64 tier as one plain launch per layer (the
+    comparison the batched mix is quoted on), and its persistent launch only
+    when ``fp64_batched``."""
     import torch
     from radar_sounder_simulator_amd import autotune
     from radar_sounder_simulator_amd.batch import BatchRunner, uneven_workloads
@@ -134,8 +137,11 @@ def bench_batched(scale, dev, seed=4):
             jobs += j
             keep += k
         t = time.time()
-        tt, nb = tuner.batch_shape(runner, jobs, prec)
-        log(f"bench: batched {prec}: tile {tt}, {nb} blocks (tuned in {time.time() - t:.1f} s)")
+        if prec == "mixed" or fp64_batched:
+            tt, nb = tuner.batch_shape(runner, jobs, prec)
+            log(f"bench: batched {prec}: tile {tt}, {nb} blocks (tuned in {time.time() - t:.1f} s)")
+        else:
+            tt = nb = None
 
         def batched():
             runner.launch(jobs, prec, trace_tile=tt, n_blocks=nb)
@@ -147,7 +153,10 @@ def bench_batched(scale, dev, seed=4):
                 rs.run(prec)
 
         res = {"trace_tile": tt, "n_blocks": nb}
-        for name, fn in (("batched", batched), ("sequential", sequential)):
+        runs = (("batched", batched), ("sequential", sequential))
+        if prec == "fp64" and not fp64_batched:
+            runs = runs[1:]
+        for name, fn in runs:
             fn()
             torch.cuda.synchronize()
             e0 = torch.cuda.Event(enable_timing=True)
@@ -159,11 +168,12 @@ def bench_batched(scale, dev, seed=4):
             ms = e0.elapsed_time(e1)
             res[name] = {"ms": ms, "pairs_per_s": pairs / (ms * 1e-3)}
             log(f"bench: batched {prec} {name}: {ms:.1f} ms")
-        res["speedup"] = res["sequential"]["ms"] / res["batched"]["ms"]
+        if "batched" in res:
+            res["speedup"] = res["sequential"]["ms"] / res["batched"]["ms"]
+            log(f"bench: batched {prec}: {res['batched']['ms']:.1f} ms vs sequential "
+                f"{res['sequential']['ms']:.1f} ms (tile {tt}, {nb} blocks)")
         out[prec] = res
         torch.cuda.empty_cache()
-        log(f"bench: batched {prec}: {res['batched']['ms']:.1f} ms vs sequential "
-            f"{res['sequential']['ms']:.1f} ms (tile {tt}, {nb} blocks)")
     # the comparison the batched configuration names: the mixed persistent
     # launch against the same scenes as one plain float64 launch per layer
     out["mixed_batched_vs_fp64_sequential"] = (out["fp64"]["sequential"]["ms"] /
@@ -252,7 +262,7 @@ def bench_multilayer(n_traces, dev, sub_traces=8):
     trs = dv.DeviceTraces(tr.positions[idx], tr.u_ct[idx], None, wl.pattern[1][idx],
                           device=dev)
     gen = {"traces": sub_traces, "layers": {}}
-    for name in ("firn12", "bed"):
+    for name in ("firn5", "firn12", "bed"):
         j = names.index(name)
         L, fac, ifc, rough = rs.layers[j]
         kw = dict(mode="coherent", t0=wl.t0, dt=wl.dt, n_samples=wl.n_samples, c=wl.c,
@@ -431,9 +441,9 @@ def bench_processing(field, wl, dev, aperture_m=2000.0, steps=5, cpu_traces=41):
     return out
 
 
-def bench_serving(n_requests, lam, scale, dev, seed=7):
-    """The request stream at ``lam`` requests/s on the mixed and on the
-    float64 tier."""
+def bench_serving(n_requests, lam, scale, dev, seed=7, fp64_requests=0):
+    """The request stream at ``lam`` requests/s on the mixed tier, and its
+    first ``fp64_requests`` requests on the float64 tier (none when 0)."""
     import torch
     from radar_sounder_simulator_amd import serve as S
     t = time.time()
@@ -443,14 +453,15 @@ def bench_serving(n_requests, lam, scale, dev, seed=7):
     out = {"requests": n_requests, "lambda_per_s": lam, "scale": scale,
            "resident_scenes": 8, "reuse": 0.5,
            "pairs": int(sum(r.workload.pairs() for r in reqs))}
-    for prec in ("mixed", "fp64"):
+    runs = [("mixed", reqs)] + ([("fp64", reqs[:fp64_requests])] if fp64_requests else [])
+    for prec, rq in runs:
         srv = S.Server(device=dev, precision=prec, step_pairs=4e8)
         for sid, layers in resident.items():
             srv.add_scene(sid, layers)
-        res = srv.serve(reqs, realtime=True, keep_fields=False)
+        res = srv.serve(rq, realtime=True, keep_fields=False)
         torch.cuda.synchronize()
         st = S.latency_stats(res)
-        st.update({"steps": srv.steps, "graph_steps": srv.graph_steps,
+        st.update({"requests": len(rq), "steps": srv.steps, "graph_steps": srv.graph_steps,
                    "peak_hbm_bytes": srv.peak, "budget_bytes": srv.budget})
         out[prec] = st
         log(f"bench: serving {prec}: p50 {st['p50_s'] * 1e3:.1f} ms p99 "
@@ -476,6 +487,10 @@ def main():
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--lam", default="2,20", help="arrival rates (requests/s), comma-separated")
+    ap.add_argument("--serve-fp64-requests", type=int, default=32,
+                    help="requests of the highest-rate stream also served on the float64 tier")
+    ap.add_argument("--fp64-batched", action="store_true",
+                    help="also time the float64 tier's persistent launch over the mix")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--ml-traces", type=int, default=1000,
@@ -606,14 +621,18 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batch:
         log(f"bench: batched uneven mix (scale {args.batch_scale})")
-        line["batched"] = bench_batched(args.batch_scale, dev)
+        line["batched"] = bench_batched(args.batch_scale, dev, fp64_batched=args.fp64_batched)
         torch.cuda.empty_cache()
     if not args.no_serve:
         line["serving"] = {}
-        for lam in (float(x) for x in str(args.lam).split(",")):
-            log(f"bench: serving stream ({args.serve_requests} requests at {lam}/s)")
+        lams = [float(x) for x in str(args.lam).split(",")]
+        for lam in lams:
+            n64 = args.serve_fp64_requests if lam == max(lams) else 0
+            log(f"bench: serving stream ({args.serve_requests} requests at {lam}/s, "
+                f"{n64} of them also on the float64 tier)")
             line["serving"][f"lambda_{lam:g}"] = bench_serving(args.serve_requests, lam,
-                                                               args.serve_scale, dev)
+                                                               args.serve_scale, dev,
+                                                               fp64_requests=n64)
             torch.cuda.empty_cache()
     if not args.no_multilayer:
         torch.cuda.empty_cache()

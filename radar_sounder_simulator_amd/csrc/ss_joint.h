@@ -1709,16 +1709,27 @@ SS_HD void refracted_pair_fp64(const RefrFacet& f, const TraceRec& t, const Refr
   radiometry_fp64<RM, PAT>(f, t, rc, ps, o);
 }
 
-// Mixed tier: the joint solve itself stays float64 (its damped Newton needs
-// the float64 merit comparisons); radiometry runs in the mixed tier.  The
-// flat-stack form (NMAX < 0) reads its layer constants from ``fl`` (staged
-// in LDS by the caller).
+// Deepest stack the mixed tier's joint solve takes; deeper stacks keep the
+// float64 joint solve (their per-lane arrays live in scratch memory, where
+// the float32 search's extra arrays cost more than the float64 arithmetic
+// it saves).
+constexpr int kMixedJointMax = 8;
+
+// Mixed tier: crossings by the float32-searched joint solve (stacks up to
+// kMixedJointMax deep), the float64 joint solve (deeper stacks), the
+// flat-stack form (NMAX < 0, layer constants read from ``fl``, staged in LDS
+// by the caller) or the sequential chain; radiometry in the mixed tier.
 template <int NMAX, int RM = -1, int PAT = -1, typename FT = RefrFacet, bool STAGED = false>
 SS_HD bool refracted_pair_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
                                 PairOut& o, const FlatLds* fl = nullptr) {
   PathSums ps;
   if constexpr (NMAX < 0) {
     if (!flat_path_mixed(t.p, f.c, rc, *fl, ps)) return false;
+  } else if constexpr (NMAX > kMixedJointMax) {
+    if (rc.joint)
+      joint_sums<NMAX>(f, t, rc, ps);
+    else if (!seq_path_mixed(f, t, rc, ps))
+      return false;
   } else {
     if (NMAX > 0 && rc.joint) {
       if (!joint_path_mixed<(NMAX > 0 ? NMAX : 1)>(t.p, f.c, rc, ps)) return false;
