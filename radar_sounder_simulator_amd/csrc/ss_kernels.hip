@@ -331,6 +331,12 @@ __device__ inline void stage_pattern(PatternArgs& pa, float* ptab, short* pidx) 
   pa.fscale = pattern_cell_scale(pa.table, pa.n);
 }
 
+// Stages the float32 D_Phi term constants (term_tab_entry, kTermTab entries)
+// in ``tab``; visible after the caller's next barrier.
+__device__ inline void stage_term_tab(float* tab) {
+  for (int m = threadIdx.x; m < kTermTab; m += kBlock) term_tab_entry(m, tab);
+}
+
 __host__ __device__ inline SurfaceConst make_surface_const(const ss_surface_params& prm,
                                                            const double* table,
                                                            const double* cst) {
@@ -491,6 +497,8 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
       __shared__ short pidx[kPatCells];
       stage_pattern(scl.pat, ptab, pidx);
     }
+    // (no staged D_Phi term table here: its 2.4 KB would cost the kernel
+    // its fourth workgroup per CU)
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o, int) {
                                surface_pair_mixed<RM, PAT, ST>(f32, nullptr, t, scl, o);
@@ -685,6 +693,11 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
       __shared__ short pidx[kPatCells];
       stage_pattern(rcl.pat, ptab, pidx);
     }
+    if constexpr (RM != 0) {
+      __shared__ float mtab[2 * kTermTab];
+      stage_term_tab(mtab);
+      rcl.mtab = mtab;
+    }
     const FlatLds* flp = nullptr;
     if constexpr (NMAX < 0) {
       // flat stack: the layer constants once per workgroup
@@ -709,8 +722,13 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
   }
 }
 
-template <int PREC, int NMAX, int RM, int PAT>
-__global__ __launch_bounds__(kBlock) void refracted_kernel(KCommon k, RefrConst rc, Redo rd) {
+// The mixed sequential-chain instantiations are held to 3 waves per SIMD
+// (their LDS already allows 3 workgroups per CU; left alone the compiler
+// settles just past 168 VGPRs and the kernel runs at 2 waves per SIMD).
+template <int PREC, int NMAX, int RM, int PAT,
+          int WAVES = (PREC == SS_PREC_MIXED && NMAX == 0) ? 3 : 1>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void refracted_kernel(
+    KCommon k, RefrConst rc, Redo rd) {
   using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   __shared__ Shared<ACC, kTTRefr> sh;
   refracted_body<PREC, NMAX, RM, PAT, ACC>(sh, k, rc, rd);
@@ -1039,6 +1057,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
   const long redo_base = job_prefix<TT, KIND>(jobs, n_jobs, begins);
   const long total = uniform_l(begins[n_jobs]);
   int staged = -1;  // job whose flat-stack layers are in LDS (KIND 3, mixed)
+  float* mt = nullptr;
+  if constexpr (PREC == SS_PREC_MIXED && RM != 0) {
+    __shared__ float mtab[2 * kTermTab];  // visible after the loop's first barrier
+    stage_term_tab(mtab);
+    mt = mtab;
+  }
   for (;;) {
     if (tid == 0) s_w = (long)atomicAdd(counter, 1ULL);
     __syncthreads();
@@ -1052,7 +1076,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
     const long f = (w % k.n_ftiles) * kBlock + tid;
     const bool live = f < k.nf;
     if constexpr (KIND == 0) {
-      const SurfaceConst sc = make_surface_const(jb.surface, jb.pattern_table, jb.constants);
+      SurfaceConst sc = make_surface_const(jb.surface, jb.pattern_table, jb.constants);
+      sc.mtab = mt;
       const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
       SurfFacet64 f64;
       load_surface_facet(k, live ? f : 0, sc.flags, f64);
@@ -1076,8 +1101,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
       // flat-stack joint solve
       constexpr int NMAX = KIND == 2 ? SS_MAX_JOINT : (KIND == 3 ? -1 : 0);
       const bool shallow = KIND == 2 && rc_shallow(jb.refracted);
-      const RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
-                                           jb.pattern_table, jb.constants);
+      RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
+                                     jb.pattern_table, jb.constants);
+      rc.mtab = mt;
       const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
       const long fi = live ? f : 0;
       if constexpr (PREC == SS_PREC_FP64) {

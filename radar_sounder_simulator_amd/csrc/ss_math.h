@@ -198,84 +198,82 @@ SS_HD T f_factor(T m, T a0, T edge, T l, const double* cst) {
   return (T)1 - ey2 * t_cos<T>(ang) + (T)kSqrtPi * (t1 - t2);
 }
 
+// Per-term constants of the float32 area-only D_Phi series, staged per
+// workgroup in LDS: entry m (m = 1 .. SS_MAX_TERMS) holds 1/m and m^2 as
+// floats, read at a wave-uniform index (one broadcast LDS read per term).
+constexpr int kTermTab = SS_MAX_TERMS + 1;
+SS_HD void term_tab_entry(int m, float* tab) {
+  tab[2 * m] = m > 0 ? (float)(1.0 / (double)m) : 0.0f;
+  tab[2 * m + 1] = (float)((double)m * (double)m);
+}
+
 // D_Phi, clamped at 0 (exactly 0 when (sigma K)^2 == 0).
 // RM (compile-time roughness mode): 1 area-only terms, 2 finite-facet
-// Gaussian terms, -1 decided by ``area_only`` at run time.  PAIRED: the
-// float32 area-only series shares one reciprocal between two terms (the
-// refracted kernels, whose occupancy it does not change; it costs the
-// surface kernel a wave per SIMD).
-template <typename T, int RM = -1, bool PAIRED = false>
+// Gaussian terms, -1 decided by ``area_only`` at run time.  ``tab``: the
+// staged term constants (term_tab_entry) or null (computed per term).
+template <typename T, int RM = -1>
 SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
-              bool area_only_rt, int acf, const double* cst) {
+              bool area_only_rt, int acf, const double* cst, const float* tab = nullptr) {
   const bool area_only = RM == 1 ? true : (RM == 2 ? false : area_only_rt);
   T sk = sigma * K;
   T x = sk * sk;
   if (!(x > (T)0)) return (T)0;
-  T logx = t_log<T>(x);
   T acc = (T)0;
   T ab2 = A0 * A0 + B0 * B0;
   if constexpr (std::is_same<T, float>::value && RM != 2) {
-    // float32 tier, area-only ACFs: the hardware reciprocal / rsqrt for the
-    // per-term 1/m and base^-1/2 (IEEE float divisions otherwise cost ~10
-    // instructions each, twice per term); the Poisson weight x^m e^-x / m!
-    // by the recurrence P_m = P_(m-1) x / m while e^-x is a normal float
-    // (x < 80: no exp, no lgamma load per term; relative drift <= m ulp),
-    // the log form otherwise; the constant factors 2 pi Lx Ly (pi Lx Ly)
-    // applied once to the sum.
-    if (area_only) {
+    // float32 tier, area-only ACFs, e^-x a normal float (x < 80): the sum
+    // over the Poisson weights P_m = e^-x x^m / m! in Horner form, from the
+    // last term down, so no term needs its own weight, exp or lgamma:
+    //   exponential, term P_m m (m^2 + c)^-3/2 with m P_m = x P_(m-1):
+    //     x e^-x sum_(j<N) x^j/j! r_(j+1)^3,  G <- r_m^3 + (x/m) G
+    //   Gaussian, term P_m e^(-c/4m) / m:
+    //     e^-x sum_(m<=N) x^m/m! g_m,          G <- (x/m) (g_m + G)
+    // with c = (A0^2 + B0^2) l^2 and the common factor l^2 applied once.
+    // All terms are positive and G stays below e^x (r <= 1, g <= 1): no
+    // cancellation, no overflow.  The log form (below) otherwise.
+    if (area_only && x < 80.0f) {
       const float ll = l * l;
       const float c = ab2 * ll;
       const bool expo = acf == SS_ACF_EXPONENTIAL;
-      // exponential term q rb^3 with q = l^2 / m^2, rb = (1 + ab2 q)^-1/2
-      // rewritten as l^2 m (m^2 + ab2 l^2)^-3/2 (m^2 exact in float32 up
-      // to m = 4096); the common factor l^2 of both ACFs applied once
-      auto series = [&](auto rec_tag) -> float {
-        constexpr bool REC = decltype(rec_tag)::value;
-        float P = REC ? f_exp(-x) : 0.0f;
-        float sum = 0.0f;
-        auto add_term = [&](int m, float mf, float im) {
-          if constexpr (REC)
-            P *= x * im;
-          else
-            P = f_exp(mf * logx - (float)cst[SS_CONST_LGAMMA + m] - x);
-          float term;
-          if (expo) {
-            const float r = f_rsq(mf * mf + c);
-            term = (P * mf) * (r * r * r);
+      const float cg = -0.25f * kLog2E * c;
+      // one loop per (table, ACF) pair: both are uniform, and a branch on
+      // either inside the loop costs more than the term
+      auto horner = [&](auto tab_tag, auto expo_tag) -> float {
+        constexpr bool TAB = decltype(tab_tag)::value;
+        constexpr bool EXPO = decltype(expo_tag)::value;
+        float G = 0.0f;
+        float mf = (float)n_terms;
+#pragma unroll 2
+        for (int m = n_terms; m >= 1; --m) {
+          float im, m2;
+          if constexpr (TAB) {
+            im = tab[2 * m];
+            m2 = tab[2 * m + 1];
           } else {
-            term = P * f_exp(-c * (0.25f * im)) * im;
+            im = f_rcp(mf);
+            m2 = mf * mf;
+            mf -= 1.0f;
           }
-          sum += term;
-        };
-        // No early exit: the per-term test and the divergent break cost
-        // more than the tail terms it could skip.
-        float mf = 0.0f;
-        if constexpr (PAIRED) {
-          // terms in pairs sharing one reciprocal: 1/m = (m+1) / (m (m+1)),
-          // 1/(m+1) = m / (m (m+1)) (m (m+1) exact in float32 up to m =
-          // 4095); one more live pair of registers than the plain loop
-          int m = 1;
-          for (; m < n_terms; m += 2) {
-            const float a = mf + 1.0f, b = mf + 2.0f;
-            const float rab = f_rcp(a * b);
-            add_term(m, a, b * rab);
-            add_term(m + 1, b, a * rab);
-            mf = b;
-          }
-          if (m == n_terms) add_term(m, mf + 1.0f, f_rcp(mf + 1.0f));
-        } else {
-          for (int m = 1; m <= n_terms; ++m) {
-            mf += 1.0f;
-            add_term(m, mf, f_rcp(mf));
+          const float xm = x * im;
+          if constexpr (EXPO) {
+            const float r = f_rsq(m2 + c);
+            G = xm * G + r * (r * r);
+          } else {
+            G = xm * (im * f_exp2(cg * im) + G);
           }
         }
-        return sum;
+        return G;
       };
-      const float sum = x < 80.0f ? series(std::true_type{}) : series(std::false_type{});
+      using TT_ = std::true_type;
+      using FF_ = std::false_type;
+      const float G = tab ? (expo ? horner(TT_{}, TT_{}) : horner(TT_{}, FF_{}))
+                          : (expo ? horner(FF_{}, TT_{}) : horner(FF_{}, FF_{}));
+      const float sum = (expo ? x : 1.0f) * f_exp(-x) * G;
       acc = sum * ll * ((expo ? (float)kTwoPi : (float)kPi) * Lx * Ly);
       return acc > (T)0 ? acc : (T)0;
     }
   }
+  T logx = t_log<T>(x);
   if constexpr (std::is_same<T, double>::value && RM != 2) {
     // float64 tier, area-only exponential ACF: the Poisson weights by the
     // recurrence P_m = P_(m-1) x / m while e^-x is a normal double (x <
@@ -546,6 +544,7 @@ struct SurfaceConst {
   int n_samples, flags, n_terms, acf;
   PatternArgs pat;
   const double* cst;
+  const float* mtab = nullptr;   // staged D_Phi term constants (LDS) or null
   // derived (host-filled)
   double k_over_pi, kfac;        // k/pi, k/(2 pi)
   double k_bin, t0_dt;           // 2/(c dt), t0/dt   (mixed fast bin)
@@ -714,7 +713,7 @@ SS_HD void surface_pair_mixed(const SurfFacet32& f, const SurfFacet64* f64,
     if (f.A > 0.0f)
       dp = d_phi<float, RM>((float)sc.sigma, (float)sc.lc, kk, k2 * d1 * f_rcp(f.l1),
                         k2 * d2 * f_rcp(f.l2), f.l1, f.l2, sc.n_terms,
-                        (flags & SS_D_PHI_AREA) != 0, sc.acf, sc.cst);
+                        (flags & SS_D_PHI_AREA) != 0, sc.acf, sc.cst, sc.mtab);
     float amp_i = f.A > 0.0f ? kg * cs * ir2 * f_sqrt(dp) : 0.0f;
     float sk = (float)sc.sigma * kk;
     float matt = f_exp(-0.5f * sk * sk);

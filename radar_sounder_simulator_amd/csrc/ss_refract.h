@@ -25,7 +25,7 @@ constexpr int kMeanIters = 10;     // reference budget, mean-plane anchor
 constexpr int kLocalIters = 25;    // reference budget, local plane
 constexpr int kMeanIters32 = 12;   // mixed: float32 anchor iterations
 constexpr int kLocalIters32 = 10;  // mixed: float32 local iterations
-constexpr int kPolish64 = 2;       // mixed: float64 polish steps
+constexpr int kPolish64 = 1;       // mixed: float64 polish steps
 
 // Mixed tier, float64: 1/x and 1/sqrt(x) from the hardware estimates
 // refined by two Newton-Raphson steps (r <- r (2 - x r); y <- y (3 - x y^2)
@@ -155,7 +155,7 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
   double fp[3] = {p[0] - h1 * nrm[0], p[1] - h1 * nrm[1], p[2] - h1 * nrm[2]};
   double fq[3] = {q[0] - h2 * nrm[0], q[1] - h2 * nrm[1], q[2] - h2 * nrm[2]};
   double w[3] = {fq[0] - fp[0], fq[1] - fp[1], fq[2] - fp[2]};
-  double L = sqrt((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
+  double L = sqrt64((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
   double Lm = L > kTiny ? L : kTiny;
   double iLm = rcp64(Lm);
   double u[3] = {w[0] * iLm, w[1] * iLm, w[2] * iLm};
@@ -212,11 +212,13 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
   double tt = swap ? Lh - tr : tr;
   double t = tt < 0.0 ? 0.0 : (tt > Lh ? Lh : tt);
   double d2 = Lh - t;
-  double r1 = sqrt(ah * ah + t * t);
-  r1 = r1 > kTiny ? r1 : kTiny;
-  double r2 = sqrt(bh * bh + d2 * d2);
-  r2 = r2 > kTiny ? r2 : kTiny;
-  double ir1 = rcp64(r1), ir2 = rcp64(r2);
+  // r = sqrt(q) and 1/r from one refined reciprocal square root
+  const double q1s = ah * ah + t * t, q2s = bh * bh + d2 * d2;
+  const double kTiny2 = kTiny * kTiny;
+  double ir1 = rsq64(q1s > kTiny2 ? q1s : kTiny2);
+  double ir2 = rsq64(q2s > kTiny2 ? q2s : kTiny2);
+  double r1 = q1s > kTiny2 ? q1s * ir1 : kTiny;
+  double r2 = q2s > kTiny2 ? q2s * ir2 : kTiny;
   out.res = n1 * t * ir1 - n2 * d2 * ir2;
   out.s1 = s * r1;
   double st = s * t;
@@ -292,6 +294,7 @@ struct RefrConst {
   const double* desc;    // (n_crossed, SS_IDESC_COUNT)
   const double* tables;  // (rows, 6)
   const double* legs;    // (3, n_crossed + 1): n, eps, att
+  const float* mtab = nullptr;  // staged D_Phi term constants (LDS) or null
   int joint, k_pad, n_newton, n_backtrack;  // joint solver (pads first)
   // derived
   double k_bin, t0_dt, cyc_per_m;
@@ -652,7 +655,7 @@ SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
   const float c_first = (float)ps.c_first, c_last = (float)ps.c_last;
   const int J = rc.n_crossed;
   double dd[3] = {cur[0] - f.c[0], cur[1] - f.c[1], cur[2] - f.c[2]};
-  double s_j = sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+  double s_j = sqrt64(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
   double nj = nl[J];
   opl = opl + nj * s_j;
   // bin position (fast) -- redo the pair on the float64 tier near an edge
@@ -710,9 +713,9 @@ SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
       are = amp * f_exp(-0.5f * sk * sk);
       if (f.A > 0.0) {
         float l1 = (float)f.l1, l2 = (float)f.l2;
-        float dp = d_phi<float, RM, true>((float)rc.sigma, (float)rc.lc, kk, 2.0f * kj * d1 * f_rcp(l1),
+        float dp = d_phi<float, RM>((float)rc.sigma, (float)rc.lc, kk, 2.0f * kj * d1 * f_rcp(l1),
                                 2.0f * kj * d2 * f_rcp(l2), l1, l2, rc.n_terms,
-                                (flags & SS_D_PHI_AREA) != 0, rc.acf, rc.cst);
+                                (flags & SS_D_PHI_AREA) != 0, rc.acf, rc.cst, rc.mtab);
         float ai = base * f_sqrt(dp);
         are += ai * (float)f.ph_re;
         aim += ai * (float)f.ph_im;
