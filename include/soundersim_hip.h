@@ -426,6 +426,37 @@ int ss_facets_from_heights(const double* heights, int64_t ny, int64_t nx, double
  * for an empty lattice. */
 int64_t ss_facets_workspace_bytes(int64_t ny, int64_t nx);
 
+/* Seeded Gaussian-ACF height lattice built on the device (replaces
+ * compare/haynes.py:94-112, gaussian_surface, the reference's generator of
+ * synthetic DEMs): heights (rows, cols) float64 device, row-major,
+ *   heights = mean + sigma * (w (*) g) / sqrt(sum g^2),
+ * the circular convolution of white noise w with g(rho) = exp(-rho^2/a^2),
+ * a = corr_len / sqrt(2), on minimal-image distances at `spacing` (unit
+ * variance, correlation exp(-rho^2/corr_len^2)).  The noise is keyed, not
+ * drawn from a stream: w[i, j] = sqrt(-2 log u1) cos(2 pi u2), u1 and u2
+ * from the splitmix64 keying of roughness.py:207-233 with key (i, j) and
+ * stream base key_base (the host derives it from a seed with numpy
+ * SeedSequence(seed).generate_state(1)).  The convolution runs as two
+ * direct separable circular passes in float64 (the reference multiplies
+ * FFTs; results agree to rounding).  workspace: caller memory of
+ * ss_gaussian_heights_workspace_bytes(rows, cols) bytes, 8-byte aligned.
+ * rows, cols <= 4080 (SS_ERANGE beyond). */
+int ss_gaussian_heights(uint64_t key_base, int64_t rows, int64_t cols, double spacing,
+                        double corr_len, double sigma, double mean, double* heights,
+                        void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Workspace bytes of ss_gaussian_heights (the row pass's output), or -1. */
+int64_t ss_gaussian_heights_workspace_bytes(int64_t rows, int64_t cols);
+
+/* Keyed speckle phasors of an (ny, nx)-cell lattice written into facet rows
+ * SS_FROW_PH_RE / SS_FROW_PH_IM at the ss_facets_from_heights positions
+ * (replaces roughness.py:207-233, speckle_phasors_for_keys, for the keys
+ * (key_row0 + r, key_col0 + c) of cell (r, c)): (e1 + i e2) / sqrt 2 by
+ * splitmix64 keying and Box-Muller, rounded to complex64 as the reference
+ * returns it, stored as float64.  key_base as in ss_gaussian_heights. */
+int ss_keyed_phasors(uint64_t key_base, int64_t ny, int64_t nx, int64_t key_row0,
+                     int64_t key_col0, double* facets, int64_t facet_stride, void* stream);
+
 /* Per-trace diagnostics over one interface's facets, float64: the facet
  * whose centre is horizontally nearest the platform (nadir_index, its centre
  * z in nadir_z) and the earliest exact arrival 2|p - centre|/c inside

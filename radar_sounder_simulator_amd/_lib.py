@@ -49,7 +49,8 @@ EXPORTED = ("ss_version", "ss_last_error", "ss_device_count",
             "ss_batch_step_workspace_bytes", "ss_batch_step", "ss_facets_from_heights",
             "ss_facets_workspace_bytes", "ss_trace_diagnostics", "ss_incoherent_power",
             "ss_slow_time_boxcar", "ss_focused_sar", "ss_refracted_field_noalloc",
-            "ss_batch_overflowed")
+            "ss_batch_overflowed", "ss_gaussian_heights",
+            "ss_gaussian_heights_workspace_bytes", "ss_keyed_phasors")
 NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
@@ -192,6 +193,13 @@ def configure(lib):
     lib.ss_slow_time_boxcar.restype = ctypes.c_int
     lib.ss_focused_sar.argtypes = [i32, dp, i64, i64, dp, dp, d, d, d, i32, dp, vp]
     lib.ss_focused_sar.restype = ctypes.c_int
+    u64 = ctypes.c_uint64
+    lib.ss_gaussian_heights.argtypes = [u64, i64, i64, d, d, d, d, dp, dp, i64, vp]
+    lib.ss_gaussian_heights.restype = ctypes.c_int
+    lib.ss_gaussian_heights_workspace_bytes.argtypes = [i64, i64]
+    lib.ss_gaussian_heights_workspace_bytes.restype = ctypes.c_int64
+    lib.ss_keyed_phasors.argtypes = [u64, i64, i64, i64, i64, dp, i64, vp]
+    lib.ss_keyed_phasors.restype = ctypes.c_int
     return lib
 
 

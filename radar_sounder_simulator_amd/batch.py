@@ -26,7 +26,7 @@ import torch
 from . import _lib
 from . import device as dv
 from .physics import C, fresnel_normal
-from .scene import Lattice, gaussian_surface, grid_facets, straight_track
+from .scene import LatticeFacets, SeededLattice, grid_facets, straight_track
 from .waveform import Waveform
 from .workloads import Layer, Workload, _pad_bucket
 
@@ -166,19 +166,20 @@ class StepGraph:
             self.event.record()
 
 
-def draw_scene(rng, srng, scale=1.0, max_facets=None):
+def draw_scene(rng, key, scale=1.0, max_facets=None):
     """One scene of the uneven mix: (layers, deepest mean depth).  Facets
     log-uniform in [1e4, 2e6] x scale per interface, 1-4 interfaces: a
     surface and internal layers 150-800 m apart (eps 3.15, 10 dB/km ice;
-    internal layers -40 dB, the last one a bed over eps 8)."""
+    internal layers -40 dB, the last one a bed over eps 8).  Interface j is
+    the seeded lattice of seed (*key, j)."""
     F = math.exp(rng.uniform(math.log(1e4), math.log(2e6))) * scale
     if max_facets:
         F = min(F, max_facets)
     L = int(rng.integers(1, 5))
     side = max(8, int(round(math.sqrt(F))))
     dx = 25.0
-    lat = Lattice(2.0 * gaussian_surface(side + 1, dx, 200.0, srng), dx)
-    surf = lat.facets()
+    lat = SeededLattice(side, side, dx, sigma=2.0, corr_len=200.0, seed=(*key, 0))
+    surf = LatticeFacets(lat)
     eps_ice = 3.15
     layers = [Layer("surface", surf, fresnel_normal(1.0, eps_ice), lattice=lat)]
     crossed = [surf]
@@ -187,8 +188,9 @@ def draw_scene(rng, srng, scale=1.0, max_facets=None):
     depth = 0.0
     for j in range(1, L):
         depth += rng.uniform(150.0, 800.0)
-        lat = Lattice(-depth + 20.0 * gaussian_surface(side + 1, dx, 1000.0, srng), dx)
-        f = lat.facets()
+        lat = SeededLattice(side, side, dx, sigma=20.0, corr_len=1000.0, seed=(*key, j),
+                            mean=-depth)
+        f = LatticeFacets(lat)
         eps_below = eps_ice if j < L - 1 else 8.0
         joint = j > 1
         layers.append(Layer(f"layer{j}", f, fresnel_normal(eps[-1], eps_below)
@@ -228,6 +230,6 @@ def uneven_workloads(n_scenes=64, seed=4, scale=1.0, n_samples=None, max_facets=
     rng = np.random.default_rng(seed)
     out = []
     for s in range(n_scenes):
-        layers, depth = draw_scene(rng, np.random.default_rng([seed, s]), scale, max_facets)
+        layers, depth = draw_scene(rng, (seed, s), scale, max_facets)
         out.append(draw_instrument(rng, f"scene{s}", layers, depth, scale, n_samples))
     return out

@@ -1673,6 +1673,125 @@ __global__ __launch_bounds__(kBlock) void facet_stats_kernel(const double* __res
   if ((int)threadIdx.x < kStatCount) stats[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// ------------------------------------------------ seeded scene generators
+// Keyed standard normals (roughness.py:207-233): lattice key (kr, kc) and a
+// 64-bit stream base give keyed = base ^ kr ^ (kc * K), two splitmix64
+// finaliser outputs (keyed, keyed ^ S2) -> u1, u2 in (0, 1) with 53 bits,
+// then Box-Muller: radius sqrt(-2 log u1), angle 2 pi u2.  The speckle
+// phasor is radius e^{i angle} / sqrt 2 rounded to complex64; the height
+// noise of the Gaussian-ACF generator is its float64 real part x sqrt 2,
+// radius cos(angle).  Integer steps are exact, so host and device agree to
+// the ulps of log / cos / sin.
+constexpr unsigned long long kKeyMul = 0xD2B74407B1CE6E93ull;
+constexpr unsigned long long kStream2 = 0xCA5A826395121157ull;
+
+__host__ __device__ inline unsigned long long splitmix_fin(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ inline void keyed_polar(unsigned long long base, long kr, long kc, double& rad,
+                                   double& ang) {
+#pragma clang fp contract(off)
+  const unsigned long long keyed =
+      base ^ (unsigned long long)kr ^ ((unsigned long long)kc * kKeyMul);
+  const double u1 = ((double)(splitmix_fin(keyed) >> 11) + 0.5) * 0x1p-53;
+  const double u2 = ((double)(splitmix_fin(keyed ^ kStream2) >> 11) + 0.5) * 0x1p-53;
+  rad = sqrt(-2.0 * log(u1));
+  ang = kTwoPi * u2;
+}
+
+// Row pass of the Gaussian-ACF height generator (compare/haynes.py:94-112):
+// the circular convolution of keyed white noise with exp(-rho^2/a^2) is
+// separable, exp(-(dy^2 + dx^2)/a^2) = gy(dy) gx(dx), so the reference's
+// FFT product becomes a direct circular convolution along x (this kernel)
+// then along y (gauss_cols_kernel).  One workgroup per lattice row: the
+// row's noise and the taps gx[d], d = 0..nx1-1 (minimal-image distance),
+// are staged in LDS; out[j] = sum_k w[k] gx[(j - k) mod nx1], split at k = j
+// so no index wraps.
+__global__ __launch_bounds__(kBlock) void gauss_rows_kernel(unsigned long long base, long nx1,
+                                                            double spacing, double a2,
+                                                            double* __restrict__ tmp) {
+  HIP_DYNAMIC_SHARED(double, lds_g)
+  double* w = lds_g;
+  double* g = lds_g + nx1;
+  const long row = blockIdx.x;
+  for (long j = threadIdx.x; j < nx1; j += kBlock) {
+    double rad, ang;
+    keyed_polar(base, row, j, rad, ang);
+    w[j] = rad * cos(ang);
+    const long d = j < nx1 - j ? j : nx1 - j;
+    const double c = (double)d * spacing;
+    g[j] = exp(-(c * c) / a2);
+  }
+  __syncthreads();
+  for (long j = threadIdx.x; j < nx1; j += kBlock) {
+    double acc = 0.0;
+    for (long k = 0; k <= j; ++k) acc = fma(w[k], g[j - k], acc);
+    for (long k = j + 1; k < nx1; ++k) acc = fma(w[k], g[j - k + nx1], acc);
+    tmp[row * nx1 + j] = acc;
+  }
+}
+
+// Column pass: lanes are consecutive columns (coalesced loads), each lane
+// keeps kGaussRows consecutive outputs in registers; the y taps are staged
+// in LDS extended periodically to 2 ny1 + kGaussRows entries so that
+// i0 + r - k + ny1 never wraps.  heights = mean + scale * sum.
+constexpr int kGaussRows = 32;
+
+__global__ __launch_bounds__(kBlock) void gauss_cols_kernel(const double* __restrict__ tmp,
+                                                            long ny1, long nx1, double spacing,
+                                                            double a2, double scale, double mean,
+                                                            double* __restrict__ heights) {
+  HIP_DYNAMIC_SHARED(double, lds_g)
+  const long n_tap = 2 * ny1 + kGaussRows;
+  for (long t = threadIdx.x; t < n_tap; t += kBlock) {
+    const long m = t % ny1;
+    const long d = m < ny1 - m ? m : ny1 - m;
+    const double c = (double)d * spacing;
+    lds_g[t] = exp(-(c * c) / a2);
+  }
+  __syncthreads();
+  const long j = (long)blockIdx.x * kBlock + threadIdx.x;
+  const long i0 = (long)blockIdx.y * kGaussRows;
+  if (j >= nx1) return;  // no barrier below
+  double acc[kGaussRows];
+#pragma unroll
+  for (int r = 0; r < kGaussRows; ++r) acc[r] = 0.0;
+  for (long k = 0; k < ny1; ++k) {
+    const double v = tmp[k * nx1 + j];
+    const double* gk = lds_g + (i0 - k + ny1);  // gk[r] = gy[(i0 + r - k) mod ny1]
+#pragma unroll
+    for (int r = 0; r < kGaussRows; ++r) acc[r] = fma(v, gk[r], acc[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < kGaussRows; ++r)
+    if (i0 + r < ny1) heights[(i0 + r) * nx1 + j] = mean + scale * acc[r];
+}
+
+// Keyed speckle phasors of a lattice's cells (roughness.py:207-233 with
+// keys (kr0 + r, kc0 + c)), written to facet rows SS_FROW_PH_RE/IM at the
+// cells' tile-order positions as float64 copies of the complex64 values.
+__global__ __launch_bounds__(kBlock) void keyed_phasor_kernel(unsigned long long base, long ny,
+                                                              long nx, long kr0, long kc0,
+                                                              double* __restrict__ facets,
+                                                              long fs) {
+#pragma clang fp contract(off)
+  const long cell = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (cell >= ny * nx) return;
+  const long r = cell / nx, c = cell - r * nx;
+  double rad, ang;
+  keyed_polar(base, kr0 + r, kc0 + c, rad, ang);
+  const double s2 = sqrt(2.0);
+  const float re = (float)((rad * cos(ang)) / s2);
+  const float im = (float)((rad * sin(ang)) / s2);
+  const long pos = tile_position(r, c, ny, nx);
+  facets[SS_FROW_PH_RE * fs + pos] = (double)re;
+  facets[SS_FROW_PH_IM * fs + pos] = (double)im;
+}
+
 // ------------------------------------------------------ trace diagnostics
 // Per trace, over one interface's facets (output.py:69-82,
 // simulate.py:398-412): the horizontally nearest facet centre (nadir) and
@@ -2448,6 +2567,79 @@ int64_t ss_facets_workspace_bytes(int64_t ny, int64_t nx) {
   if (ny < 1 || nx < 1) return -1;
   const int64_t blocks = (ny * nx + kBlock - 1) / kBlock;
   return blocks * kStatCount * (int64_t)sizeof(double);
+}
+
+// lattice vertices per side: the column pass stages 2 rows + 32 taps in
+// 64 KiB of LDS, the row pass 2 cols values
+constexpr int64_t kGaussMaxSide = 4080;
+
+int64_t ss_gaussian_heights_workspace_bytes(int64_t rows, int64_t cols) {
+  if (rows < 1 || cols < 1) return -1;
+  return rows * cols * (int64_t)sizeof(double);
+}
+
+int ss_gaussian_heights(uint64_t key_base, int64_t rows, int64_t cols, double spacing,
+                        double corr_len, double sigma, double mean, double* heights,
+                        void* workspace, int64_t workspace_bytes, void* stream) {
+  if (rows < 1 || cols < 1) return fail(SS_EINVAL, "need at least one vertex per side");
+  if (rows > kGaussMaxSide || cols > kGaussMaxSide)
+    return fail(SS_ERANGE, "gaussian heights: at most 4080 vertices per side");
+  if (!(spacing > 0.0) || !(corr_len > 0.0))
+    return fail(SS_EINVAL, "spacing and corr_len must be > 0");
+  if (!std::isfinite(sigma) || !std::isfinite(mean))
+    return fail(SS_EINVAL, "sigma and mean must be finite");
+  if (!heights || !workspace) return fail(SS_EINVAL, "null pointer");
+  if (workspace_bytes < ss_gaussian_heights_workspace_bytes(rows, cols))
+    return fail(SS_EINVAL, "workspace smaller than ss_gaussian_heights_workspace_bytes()");
+  if ((reinterpret_cast<uintptr_t>(workspace) & 7) != 0)
+    return fail(SS_EINVAL, "workspace must be 8-byte aligned");
+  // normalisation 1 / sqrt(sum g^2) of the 2-D kernel = product of the
+  // separable 1-D sums (minimal-image distances, as the passes' taps)
+  const double a = corr_len / std::sqrt(2.0), a2 = a * a;
+  auto sum_sq = [&](int64_t n) {
+    double s = 0.0;
+    for (int64_t d0 = 0; d0 < n; ++d0) {
+      const int64_t d = d0 < n - d0 ? d0 : n - d0;
+      const double c = (double)d * spacing;
+      const double g = std::exp(-(c * c) / a2);
+      s += g * g;
+    }
+    return s;
+  };
+  const double scale = sigma / std::sqrt(sum_sq(rows) * sum_sq(cols));
+  hipStream_t st = (hipStream_t)stream;
+  double* tmp = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(gauss_rows_kernel, dim3((unsigned)rows), dim3(kBlock),
+                     (size_t)(2 * cols) * sizeof(double), st, (unsigned long long)key_base,
+                     (long)cols, spacing, a2, tmp);
+  const dim3 grid((unsigned)((cols + kBlock - 1) / kBlock),
+                  (unsigned)((rows + kGaussRows - 1) / kGaussRows));
+  hipLaunchKernelGGL(gauss_cols_kernel, grid, dim3(kBlock),
+                     (size_t)(2 * rows + kGaussRows) * sizeof(double), st, tmp, (long)rows,
+                     (long)cols, spacing, a2, scale, mean, heights);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(SS_EHIP, std::string("gaussian heights launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
+}
+
+int ss_keyed_phasors(uint64_t key_base, int64_t ny, int64_t nx, int64_t key_row0,
+                     int64_t key_col0, double* facets, int64_t facet_stride, void* stream) {
+  if (ny < 1 || nx < 1) return fail(SS_EINVAL, "need at least one cell (ny, nx >= 1)");
+  if (ny > (1L << 20) || nx > (1L << 20) || ny * nx > (1L << 31))
+    return fail(SS_ERANGE, "lattice too large (ny * nx <= 2^31)");
+  if (!facets) return fail(SS_EINVAL, "null pointer");
+  if (facet_stride < ny * nx) return fail(SS_EINVAL, "facet_stride smaller than ny * nx");
+  const long blocks = (long)((ny * nx + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(keyed_phasor_kernel, dim3((unsigned)blocks), dim3(kBlock), 0,
+                     (hipStream_t)stream, (unsigned long long)key_base, (long)ny, (long)nx,
+                     (long)key_row0, (long)key_col0, facets, (long)facet_stride);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(SS_EHIP, std::string("keyed phasors launch: ") + hipGetErrorString(e));
+  g_last_error.clear();
+  return SS_OK;
 }
 
 int ss_facets_from_heights(const double* heights, int64_t ny, int64_t nx, double spacing,

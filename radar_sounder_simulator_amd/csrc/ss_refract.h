@@ -25,7 +25,8 @@ constexpr int kMeanIters = 10;     // reference budget, mean-plane anchor
 constexpr int kLocalIters = 25;    // reference budget, local plane
 constexpr int kMeanIters32 = 12;   // mixed: float32 anchor iterations
 constexpr int kLocalIters32 = 10;  // mixed: float32 local iterations
-constexpr int kPolish64 = 1;       // mixed: float64 polish steps
+constexpr int kPolishMax = 3;      // mixed: float64 polish steps at most (one unless
+                                   // the residual is ambiguous after it)
 
 // Mixed tier, float64: 1/x and 1/sqrt(x) from the hardware estimates
 // refined by two Newton-Raphson steps (r <- r (2 - x r); y <- y (3 - x y^2)
@@ -191,35 +192,44 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
     sig = nx;
     if (done) break;  // settled to a few ulp: the float64 polish takes it from here
   }
-  // float64 polish (plain Newton, clamped into [0, hi0))
+  // float64 polish (plain Newton, clamped into [0, hi0)): one step, and
+  // more (at most kPolishMax) while the Snell residual sits in the
+  // ambiguous band.  The residual's sensitivity to sigma grows as the short
+  // leg shrinks (1/bh: a target a few metres below a 20 km air leg), where
+  // one step from the float32 root leaves it at 1e-11..1e-10.
   double sg = (double)sig;
   const double hi064 = 1.0 - 8.0 * kEps64;
-  for (int it = 0; it < kPolish64; ++it) {
-    double v = ratio * sg;
+  const double kTiny2 = kTiny * kTiny;
+  double t, d2, ir1, ir2, r1, res;
+  for (int it = 0;; ++it) {
+    {
+      double v = ratio * sg;
+      double c1 = (1.0 - sg) * (1.0 + sg);
+      c1 = c1 > kTiny ? c1 : kTiny;
+      double c2 = (1.0 - v) * (1.0 + v);
+      c2 = c2 > kTiny ? c2 : kTiny;
+      double q1 = rsq64(c1), q2 = rsq64(c2);
+      double F = (ar * sg * q1 + bd * v * q2) - Lh;
+      double Fp = ar * (q1 * q1 * q1) + (bd * ratio) * (q2 * q2 * q2);
+      double sn = sg - F * rcp64(Fp > kTiny ? Fp : kTiny);
+      sg = sn < 0.0 ? 0.0 : (sn > hi064 ? hi064 : sn);
+    }
     double c1 = (1.0 - sg) * (1.0 + sg);
     c1 = c1 > kTiny ? c1 : kTiny;
-    double c2 = (1.0 - v) * (1.0 + v);
-    c2 = c2 > kTiny ? c2 : kTiny;
-    double q1 = rsq64(c1), q2 = rsq64(c2);
-    double F = (ar * sg * q1 + bd * v * q2) - Lh;
-    double Fp = ar * (q1 * q1 * q1) + (bd * ratio) * (q2 * q2 * q2);
-    double sn = sg - F * rcp64(Fp > kTiny ? Fp : kTiny);
-    sg = sn < 0.0 ? 0.0 : (sn > hi064 ? hi064 : sn);
+    double tr = ar * sg * rsq64(c1);
+    double tt = swap ? Lh - tr : tr;
+    t = tt < 0.0 ? 0.0 : (tt > Lh ? Lh : tt);
+    d2 = Lh - t;
+    // r = sqrt(q) and 1/r from one refined reciprocal square root
+    const double q1s = ah * ah + t * t, q2s = bh * bh + d2 * d2;
+    ir1 = rsq64(q1s > kTiny2 ? q1s : kTiny2);
+    ir2 = rsq64(q2s > kTiny2 ? q2s : kTiny2);
+    r1 = q1s > kTiny2 ? q1s * ir1 : kTiny;
+    res = n1 * t * ir1 - n2 * d2 * ir2;
+    const double ar_ = fabs(res);
+    if (it + 1 >= kPolishMax || !(ar_ > kResLo && ar_ < kResHi)) break;
   }
-  double c1 = (1.0 - sg) * (1.0 + sg);
-  c1 = c1 > kTiny ? c1 : kTiny;
-  double tr = ar * sg * rsq64(c1);
-  double tt = swap ? Lh - tr : tr;
-  double t = tt < 0.0 ? 0.0 : (tt > Lh ? Lh : tt);
-  double d2 = Lh - t;
-  // r = sqrt(q) and 1/r from one refined reciprocal square root
-  const double q1s = ah * ah + t * t, q2s = bh * bh + d2 * d2;
-  const double kTiny2 = kTiny * kTiny;
-  double ir1 = rsq64(q1s > kTiny2 ? q1s : kTiny2);
-  double ir2 = rsq64(q2s > kTiny2 ? q2s : kTiny2);
-  double r1 = q1s > kTiny2 ? q1s * ir1 : kTiny;
-  double r2 = q2s > kTiny2 ? q2s * ir2 : kTiny;
-  out.res = n1 * t * ir1 - n2 * d2 * ir2;
+  out.res = res;
   out.s1 = s * r1;
   double st = s * t;
   out.x[0] = fp[0] + st * u[0];

@@ -24,6 +24,8 @@ import torch
 
 from . import _lib
 from ._lib import FROW, IDESC, TROW
+from .roughness import KeyedPhasors, key_base
+from .scene import SeededLattice
 
 
 def _dev(device):
@@ -156,6 +158,15 @@ class BuiltLattice:
         self.ny, self.nx = ny, nx
         self.center_mean = stats[7:10] / (ny * nx)
 
+    def z_max(self):
+        """The highest facet centre (the reference places joint pads above
+        it), reduced on the device; a lattice with host heights gives the
+        host value."""
+        if not isinstance(self.lattice, SeededLattice) or self.lattice._heights is not None:
+            h = self.lattice.heights
+            return float(((((h[:-1, :-1] + h[:-1, 1:]) + h[1:, :-1]) + h[1:, 1:]) / 4.0).max())
+        return float(self.fac.buf[2, :self.fac.n].max())
+
     def desc_row(self):
         """Descriptor row (TABOFF/SIGC unset): grid shape, the exact affine
         (x, y, 1) -> (row, col) map of the regular grid (what the reference's
@@ -178,6 +189,35 @@ class BuiltLattice:
         d[IDESC["MP"]:IDESC["MP"] + 3] = mp
         d[IDESC["MN"]:IDESC["MN"] + 3] = mn
         return d
+
+
+def lattice_heights(lat, device=None, stream=None):
+    """(ny+1, nx+1) float64 heights of ``lat`` in HBM: a ``SeededLattice``
+    drawn on the device (``ss_gaussian_heights``; a horizontal one filled),
+    any other lattice uploaded."""
+    dev = _dev(device)
+    if not isinstance(lat, SeededLattice):
+        return torch.as_tensor(np.ascontiguousarray(lat.heights, np.float64), device=dev)
+    rows, cols = lat.ny + 1, lat.nx + 1
+    if lat.sigma == 0.0:
+        h = torch.empty((rows, cols), dtype=torch.float64, device=dev)
+        if stream is None:
+            return h.fill_(lat.mean)
+        with torch.cuda.stream(stream):
+            return h.fill_(lat.mean)
+    lib = _lib.load()
+    h = torch.empty((rows, cols), dtype=torch.float64, device=dev)
+    ws = torch.empty(int(lib.ss_gaussian_heights_workspace_bytes(rows, cols)),
+                     dtype=torch.uint8, device=dev)
+    rc = lib.ss_gaussian_heights(int(key_base(lat.seed)), rows, cols, lat.spacing,
+                                 lat.corr_len, lat.sigma, lat.mean, _ptr(h), _ptr(ws),
+                                 ws.numel(), _stream_handle(stream, dev))
+    _lib.check(rc, "ss_gaussian_heights")
+    if dev.type == "cuda":
+        # the workspace is freed on return: keep the caching allocator from
+        # handing it to another stream before the launch has read it
+        ws.record_stream(stream if stream is not None else torch.cuda.current_stream(dev))
+    return h
 
 
 def build_lattices(lattices, crossed, device=None, gammas=None, phasors=None,
@@ -206,7 +246,7 @@ def build_lattices(lattices, crossed, device=None, gammas=None, phasors=None,
     for k, lat in uniq.items():
         ny, nx = lat.grid_shape
         n = ny * nx
-        h = torch.as_tensor(np.ascontiguousarray(lat.heights, np.float64), device=dev)
+        h = lattice_heights(lat, dev, stream)
         buf = torch.zeros((FROW["COUNT"], n), dtype=torch.float64, device=dev)
         stats = torch.zeros(16, dtype=torch.float64, device=dev)
         ws = torch.empty(max(int(lib.ss_facets_workspace_bytes(ny, nx)), 8),
@@ -227,16 +267,25 @@ def build_lattices(lattices, crossed, device=None, gammas=None, phasors=None,
         if fac.has_gamma:
             buf[13] = torch.as_tensor(np.asarray(gammas[k], np.float64)[order], device=dev)
         if fac.has_phasors:
-            ph = np.asarray(phasors[k])[order]
-            buf[14] = torch.as_tensor(np.ascontiguousarray(ph.real), device=dev)
-            buf[15] = torch.as_tensor(np.ascontiguousarray(ph.imag), device=dev)
+            ph = phasors[k]
+            if isinstance(ph, KeyedPhasors):
+                if ph.grid_shape != (ny, nx):
+                    raise ValueError("keyed phasors of another grid shape")
+                rc = lib.ss_keyed_phasors(int(key_base(ph.seed)), ny, nx, ph.key_origin[0],
+                                          ph.key_origin[1], _ptr(buf), n, st)
+                _lib.check(rc, "ss_keyed_phasors")
+            else:
+                ph = np.asarray(ph)[order]
+                buf[14] = torch.as_tensor(np.ascontiguousarray(ph.real), device=dev)
+                buf[15] = torch.as_tensor(np.ascontiguousarray(ph.imag), device=dev)
         pending.append((k, lat, fac, stats, h, ws))
     for k, lat, fac, stats, h, ws in pending:
         s_host = stats[:10].cpu().numpy()        # waits for the builds (stream order)
         out[k] = BuiltLattice(lat, fac, s_host, offs.get(k), hz[k])
+        out[k].heights = h
         if k in crossed and hz[k]:
             ny, nx = lat.grid_shape
-            z = float(lat.heights.flat[0])
+            z = lat.level()
             x = (0.5 - nx / 2.0) * lat.spacing + lat.origin[0]
             y = (0.5 - ny / 2.0) * lat.spacing + lat.origin[1]
             table[offs[k]] = torch.tensor([x, y, z, 0.0, 0.0, 1.0], dtype=torch.float64)
@@ -495,9 +544,8 @@ class DeviceInterfaces:
             n_real = len(crossed)
             if n_real < 1 or n_real > _lib.MAX_CROSSED:
                 raise ValueError(f"need 1..{_lib.MAX_CROSSED} crossed interfaces, got {n_real}")
-            h = crossed[0].heights
             # the surface's highest facet centre (the reference pads from it)
-            z_top = float(((((h[:-1, :-1] + h[:-1, 1:]) + h[1:, :-1]) + h[1:, 1:]) / 4.0).max())
+            z_top = built[id(crossed[0])].z_max()
             pads = _pad_planes_z(z_top, n_real, pad_to, z_platform)
             k = len(pads)
             desc = np.zeros((n_real + k, IDESC["COUNT"]), np.float64)

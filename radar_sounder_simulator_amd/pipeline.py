@@ -4,7 +4,9 @@ evaluated end to end (every target layer + range compression) on a stream.
 This is the simulate() hot path of the reference (``simulate.py:263-395``:
 surface through the coherent kernel, each deeper interface through the
 refracted kernel, stacked per layer, then ``apply_waveform``) without the
-host-side scene construction, with all buffers allocated once.
+host-side scene construction, with all buffers allocated once.  A workload
+whose every layer is a lattice is built in HBM from its heights (or, for
+seeded lattices, from the seeds alone: ``device.build_lattices``).
 """
 
 import numpy as np
@@ -23,21 +25,29 @@ class ResidentScene:
         self.device = dv._dev(device)
         tr = wl.track
         pv = None if wl.pattern is None else wl.pattern[1]
-        surf = wl.layers[0].facets
-        self.traces = dv.DeviceTraces(tr.positions, tr.u_ct,
-                                      _reference_range(tr.positions, surf.centers),
-                                      pv, device=self.device)
         self.layers = []
-        # every layer's crossed stack over one shared lookup table
+        self.built = None
         refr = [L for L in wl.layers if L.crossed]
-        stacks = iter(dv.DeviceInterfaces.stacks(
-            [L.crossed for L in refr], None, [L.pad_to for L in refr],
-            tr.positions[:, 2].min(), self.device) if refr else [])
-        for L in wl.layers:
-            ph = None if L.roughness is None else L.roughness[2]
-            g = np.asarray(L.gamma, np.float64)
-            fac = dv.DeviceFacets.from_facets(L.facets, gamma=g if g.ndim else None,
-                                              phasors=ph, device=self.device)
+        lat_of = {id(L.facets): L.lattice for L in wl.layers if L.lattice is not None}
+        if (all(L.lattice is not None for L in wl.layers)
+                and all(id(c) in lat_of for L in refr for c in L.crossed)):
+            facs, stacks, center_mean = self._build_on_device(wl, lat_of, refr)
+            d = tr.positions - center_mean
+            r_ref = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+        else:
+            # host-built facets; every layer's crossed stack over one shared table
+            facs = []
+            for L in wl.layers:
+                ph = None if L.roughness is None else L.roughness[2]
+                g = np.asarray(L.gamma, np.float64)
+                facs.append(dv.DeviceFacets.from_facets(L.facets, gamma=g if g.ndim else None,
+                                                        phasors=ph, device=self.device))
+            stacks = iter(dv.DeviceInterfaces.stacks(
+                [L.crossed for L in refr], None, [L.pad_to for L in refr],
+                tr.positions[:, 2].min(), self.device) if refr else [])
+            r_ref = _reference_range(tr.positions, wl.layers[0].facets.centers)
+        self.traces = dv.DeviceTraces(tr.positions, tr.u_ct, r_ref, pv, device=self.device)
+        for L, fac in zip(wl.layers, facs):
             ifc = next(stacks) if L.crossed else None
             rough = (None if L.roughness is None else
                      (L.roughness[0], L.roughness[1], L.roughness[3],
@@ -63,6 +73,33 @@ class ResidentScene:
                 max(1, dv.refracted_workspace_bytes("mixed", fac.n, T)),
                 dtype=torch.uint8, device=self.device)
             for _, fac, ifc, _ in self.layers]
+
+    def _build_on_device(self, wl, lat_of, refr):
+        """Every layer a lattice: heights drawn (seeded lattices) or uploaded,
+        tessellated, tabulated and keyed phasors drawn in HBM
+        (``device.build_lattices``); the crossed stacks, pads included, over
+        the one shared table.  Returns (facets per layer, stack iterator,
+        surface centre mean)."""
+        tr = wl.track
+        lats = [L.lattice for L in wl.layers]
+        crossed = {id(lat_of[id(c)]) for L in refr for c in L.crossed}
+        gammas, phasors = {}, {}
+        for L in wl.layers:
+            g = np.asarray(L.gamma, np.float64)
+            if g.ndim:
+                gammas[id(L.lattice)] = g
+            if L.roughness is not None:
+                phasors[id(L.lattice)] = L.roughness[2]
+        n_pad = sum(max(0, (L.pad_to or 0) - len(L.crossed)) for L in refr)
+        built, table = dv.build_lattices(lats, crossed, device=self.device, gammas=gammas,
+                                         phasors=phasors, n_pad_rows=n_pad)
+        stacks = iter(dv.DeviceInterfaces.lattice_stacks(
+            [[lat_of[id(c)] for c in L.crossed] for L in refr], built, table,
+            pad_row0=table.shape[0] - n_pad, pad_tos=[L.pad_to for L in refr],
+            z_platform=tr.positions[:, 2].min(), device=self.device) if refr else [])
+        self.built = built
+        return ([built[id(L.lattice)].fac for L in wl.layers], stacks,
+                built[id(lats[0])].center_mean)
 
     def pairs_per_layer(self):
         T = self.traces.n

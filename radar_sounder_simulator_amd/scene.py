@@ -14,13 +14,17 @@ corners, normal = (e1 x e2)/|e1 x e2| flipped to +z.
 
 The Gaussian-ACF height field restates ``compare/haynes.py:94-112`` (circular
 FFT convolution of white noise with exp(-rho^2/a^2), a = l/sqrt(2),
-normalised by sqrt(sum g^2) so the periodic field has unit variance).
+normalised by sqrt(sum g^2) so the periodic field has unit variance), with
+white noise from a numpy Generator (as the reference) or keyed by vertex
+(``SeededLattice``, drawn on the device by ``ss_gaussian_heights``).
 """
 
 import warnings
 from dataclasses import dataclass, field
 
 import numpy as np
+
+from .roughness import keyed_normals
 
 
 @dataclass
@@ -125,11 +129,95 @@ class Lattice:
     def facets(self):
         return grid_facets(self.heights, self.spacing, self.origin)
 
+    def level(self):
+        """Height of a horizontal lattice."""
+        return float(self.heights.flat[0])
 
-def gaussian_surface(n, dx, corr_len, rng, m=None):
-    """(n, m) zero-mean unit-variance Gaussian-ACF field, C = exp(-rho^2/l^2)
-    (``compare/haynes.py:94-112``, extended to rectangular grids)."""
-    m = n if m is None else m
+
+class SeededLattice(Lattice):
+    """A (ny+1, nx+1) height lattice given by its generator instead of its
+    heights: mean + sigma x the seeded Gaussian-ACF field of correlation
+    length ``corr_len`` (``gaussian_surface_keyed``; sigma 0 is a horizontal
+    plane at ``mean``).  The device draws it from these few numbers
+    (``ss_gaussian_heights``); ``heights`` materialises the host
+    restatement (FFT filter, equal to the device's direct convolution to
+    rounding) on first use, or holds heights read back from the device
+    (``device.lattice_heights``)."""
+
+    def __init__(self, ny, nx, spacing, sigma=0.0, corr_len=1.0, seed=0, mean=0.0,
+                 origin=(0.0, 0.0)):
+        self.ny, self.nx = int(ny), int(nx)
+        self.spacing = float(spacing)
+        self.sigma, self.corr_len, self.seed = float(sigma), float(corr_len), seed
+        self.mean = float(mean)
+        self.origin = tuple(origin)
+        self._heights = None
+
+    @property
+    def heights(self):
+        if self._heights is None:
+            if self.sigma == 0.0:
+                self._heights = np.full((self.ny + 1, self.nx + 1), self.mean)
+            else:
+                self._heights = self.mean + self.sigma * gaussian_surface_keyed(
+                    self.ny + 1, self.spacing, self.corr_len, self.seed, m=self.nx + 1)
+        return self._heights
+
+    @heights.setter
+    def heights(self, h):
+        h = np.asarray(h, np.float64)
+        if h.shape != (self.ny + 1, self.nx + 1):
+            raise ValueError(f"heights must have shape {(self.ny + 1, self.nx + 1)}")
+        self._heights = h
+
+    @property
+    def grid_shape(self):
+        return (self.ny, self.nx)
+
+    def __len__(self):
+        return self.ny * self.nx
+
+    def horizontal(self):
+        return self.sigma == 0.0
+
+    def level(self):
+        return self.mean
+
+    def __repr__(self):
+        return (f"SeededLattice(ny={self.ny}, nx={self.nx}, spacing={self.spacing}, "
+                f"sigma={self.sigma}, corr_len={self.corr_len}, seed={self.seed!r}, "
+                f"mean={self.mean}, origin={self.origin})")
+
+    __eq__ = object.__eq__
+    __hash__ = object.__hash__
+
+
+class LatticeFacets:
+    """The facets of a lattice, tessellated on the host only when an
+    attribute is read (``len`` needs none): a layer of a device-built scene
+    carries one, so the host never builds facets it does not use."""
+
+    def __init__(self, lattice):
+        self.lattice = lattice
+        self._f = None
+
+    def __len__(self):
+        return len(self.lattice)
+
+    def __getattr__(self, name):
+        if name.startswith("__"):
+            raise AttributeError(name)
+        if self._f is None:
+            self._f = self.lattice.facets()
+        return getattr(self._f, name)
+
+
+def gaussian_filter(w, dx, corr_len):
+    """The Gaussian-ACF filter of ``compare/haynes.py:94-112`` applied to a
+    white-noise array ``w`` (n, m): circular FFT convolution with
+    exp(-rho^2/a^2), a = l/sqrt(2), on minimal-image distances, divided by
+    sqrt(sum g^2)."""
+    n, m = w.shape
     cy = np.arange(n, dtype=np.float64)
     cy = np.minimum(cy, n - cy) * dx
     cx = np.arange(m, dtype=np.float64)
@@ -137,9 +225,25 @@ def gaussian_surface(n, dx, corr_len, rng, m=None):
     r2 = cy[:, None] ** 2 + cx[None, :] ** 2
     a = corr_len / np.sqrt(2.0)
     g = np.exp(-r2 / (a * a))
-    w = rng.standard_normal((n, m))
     z = np.fft.irfft2(np.fft.rfft2(w) * np.fft.rfft2(g), s=(n, m))
     return z / np.sqrt((g * g).sum())
+
+
+def gaussian_surface(n, dx, corr_len, rng, m=None):
+    """(n, m) zero-mean unit-variance Gaussian-ACF field, C = exp(-rho^2/l^2)
+    (``compare/haynes.py:94-112``, extended to rectangular grids): white
+    noise from ``rng.standard_normal``."""
+    m = n if m is None else m
+    return gaussian_filter(rng.standard_normal((n, m)), dx, corr_len)
+
+
+def gaussian_surface_keyed(n, dx, corr_len, seed, m=None):
+    """As ``gaussian_surface`` with keyed white noise (``roughness.
+    keyed_normals``): the realisation is a function of ``seed`` and the
+    vertex indices alone, which is what ``ss_gaussian_heights`` draws on the
+    device."""
+    m = n if m is None else m
+    return gaussian_filter(keyed_normals(n, m, seed), dx, corr_len)
 
 
 def straight_track(n_traces, spacing, altitude, start=None, heading=(1.0, 0.0),

@@ -580,7 +580,7 @@ def request_stream(n_requests, lam, seed=7, scale=1.0, n_resident=8, reuse=0.5,
     resident = {}
     depths = {}
     for s in range(n_resident):
-        layers, depth = draw_scene(rng, np.random.default_rng([seed, 1000 + s]), scale,
+        layers, depth = draw_scene(rng, (seed, 1000 + s), scale,
                                    max_facets)
         resident[f"resident{s}"] = layers
         depths[f"resident{s}"] = depth
@@ -593,7 +593,7 @@ def request_stream(n_requests, lam, seed=7, scale=1.0, n_resident=8, reuse=0.5,
             layers, depth = resident[sid], depths[sid]
         else:
             sid = f"new{i}"
-            layers, depth = draw_scene(rng, np.random.default_rng([seed, i]), scale, max_facets)
+            layers, depth = draw_scene(rng, (seed, i), scale, max_facets)
         wl = draw_instrument(rng, sid, layers, depth, scale, n_samples,
                              cross_offset=float(rng.uniform(-200.0, 200.0)))
         reqs.append(Request(req_id=i, scene_id=sid, workload=wl, arrival=t))

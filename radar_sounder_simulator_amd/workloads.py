@@ -2,7 +2,9 @@
 
 Every workload is a planar scene (metres, x east / y north / z up) with a
 straight level track along +x centred over the scene, built from seeded
-Gaussian-ACF height lattices (``scene.gaussian_surface``).  Frozen choices
+Gaussian-ACF height lattices (``scene.gaussian_surface``; the multilayer
+scene and the uneven mix from keyed ones, ``scene.SeededLattice``, which
+the device draws from their seeds).  Frozen choices
 the configuration text leaves open are recorded here:
 
 * sampling: 48 MHz (dt = 1/48 MHz) wherever the rate is not given; window
@@ -28,14 +30,15 @@ the configuration text leaves open are recorded here:
   power-of-two stacks as the reference's simulate() does.
 """
 
-from dataclasses import dataclass, field, replace
+from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import antenna as _ant
 from .physics import C, fresnel_normal
 from .roughness import n_terms_for, speckle_phasors
-from .scene import eps_kovacs, gaussian_surface, grid_facets, straight_track
+from .scene import (LatticeFacets, SeededLattice, eps_kovacs, gaussian_surface, grid_facets,
+                    straight_track)
 from .waveform import Waveform
 
 
@@ -158,19 +161,20 @@ def config3(nx=2048, ny=1024, n_traces=1000, seed=3, n_samples=8192, n_firn=20,
     j >= 2 takes the joint solve over its crossed stack, in its flat-stack
     form when ``flat_stack`` (every crossed interface below the surface is
     horizontal here), else the reference's general form."""
-    rng_s = np.random.default_rng([seed, 0])
-    rng_b = np.random.default_rng([seed, 1])
     dx = 30.0
-    hs = 1.0 * gaussian_surface(ny + 1, dx, 300.0, rng_s, m=nx + 1)
-    hb = -3000.0 + 400.0 * gaussian_surface(ny + 1, dx, 3000.0, rng_b, m=nx + 1)
-    surf = grid_facets(hs, dx)
-    bed = grid_facets(hb, dx)
-    flat0 = grid_facets(np.zeros((ny + 1, nx + 1)), dx)
+    # seeded lattices: the device draws and tessellates them (ResidentScene);
+    # the host builds facets only where an oracle reads them
+    surf_lat = SeededLattice(ny, nx, dx, sigma=1.0, corr_len=300.0, seed=(seed, 0))
+    bed_lat = SeededLattice(ny, nx, dx, sigma=400.0, corr_len=3000.0, seed=(seed, 1),
+                            mean=-3000.0)
+    surf, bed = LatticeFacets(surf_lat), LatticeFacets(bed_lat)
+    planes = {}
 
     def plane(z):
-        c = flat0.centers.copy()
-        c[:, 2] = z
-        return replace(flat0, centers=c)
+        lat = SeededLattice(ny, nx, dx, mean=z)
+        f = LatticeFacets(lat)
+        planes[id(f)] = lat
+        return f
 
     tops = np.arange(n_firn) * (firn_depth / n_firn)
     rho = 917.0 - 567.0 * np.exp(-tops / 30.0)
@@ -182,7 +186,7 @@ def config3(nx=2048, ny=1024, n_traces=1000, seed=3, n_samples=8192, n_firn=20,
     k0 = 2.0 * np.pi * f0 / C
     ant = _ant.cos_power_table(2.0)
     pattern = _ant.pattern_args(ant, track.u_at, track.u_ct)
-    layers = [Layer("surface", surf, fresnel_normal(1.0, eps[1]))]
+    layers = [Layer("surface", surf, fresnel_normal(1.0, eps[1]), lattice=surf_lat)]
     crossed = [surf]
     eps_leg, att_leg = [1.0, eps[1]], [0.0, 10.0]
     n_media = len(eps) - 1                     # media below the air
@@ -195,8 +199,9 @@ def config3(nx=2048, ny=1024, n_traces=1000, seed=3, n_samples=8192, n_firn=20,
         else:
             name, gam = f"horizon{int(horizons[i - n_firn])}", -10.0 ** (-60.0 / 20.0)
         j = len(layers)
-        layers.append(Layer(name, plane(z), gam, crossed=list(crossed), eps_leg=list(eps_leg),
-                            att_leg=list(att_leg),
+        f = plane(z)
+        layers.append(Layer(name, f, gam, crossed=list(crossed), eps_leg=list(eps_leg),
+                            att_leg=list(att_leg), lattice=planes[id(f)],
                             refraction="joint" if j > 1 else "sequential",
                             pad_to=_pad_bucket(len(crossed), n_max) if j > 1 else None,
                             flat_stack=flat_stack and j > 1))
@@ -205,6 +210,7 @@ def config3(nx=2048, ny=1024, n_traces=1000, seed=3, n_samples=8192, n_firn=20,
         att_leg.append(10.0)
     layers.append(Layer("bed", bed, -10.0 ** (-10.0 / 20.0), crossed=list(crossed),
                         eps_leg=list(eps_leg), att_leg=list(att_leg), refraction="joint",
+                        lattice=bed_lat,
                         pad_to=_pad_bucket(len(crossed), n_max), flat_stack=flat_stack))
     dt = 1.0 / 60e6
     return Workload(

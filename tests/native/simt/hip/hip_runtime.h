@@ -133,12 +133,17 @@ inline std::mutex& launch_mutex() {
   return m;
 }
 
+// Dynamic LDS of the running launch (HIP_DYNAMIC_SHARED): one buffer, the
+// workgroups run one after another.
+inline std::vector<double> g_dyn_shared;
+
 template <class F, class... A>
-void launch(F f, dim3 grid, dim3 block, A... args) {
+void launch(size_t shm, F f, dim3 grid, dim3 block, A... args) {
   // one launch at a time: launches from several host threads (a serving
   // loop's upload worker and its step loop) share the grid globals below,
   // where the device runs them on separate streams
   std::lock_guard<std::mutex> launch_lock(launch_mutex());
+  g_dyn_shared.assign((shm + sizeof(double) - 1) / sizeof(double), 0.0);
   const int nt = (int)(block.x * block.y * block.z);
   const long nblocks = (long)grid.x * grid.y;
   g_grid = grid;
@@ -280,4 +285,6 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
 // launches run to completion on the calling thread: a stream is always idle
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 
-#define hipLaunchKernelGGL(k, g, b, shm, st, ...) sim::launch(k, dim3(g), dim3(b), ##__VA_ARGS__)
+#define hipLaunchKernelGGL(k, g, b, shm, st, ...) \
+  sim::launch((size_t)(shm), k, dim3(g), dim3(b), ##__VA_ARGS__)
+#define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(sim::g_dyn_shared.data());

@@ -67,3 +67,95 @@ def test_refracted_field_through_device_built_scene(precision):
     od, oh = od.cpu().numpy(), oh.cpu().numpy()
     tol = 1e-12 if precision == "fp64" else 1e-6
     assert np.linalg.norm(od - oh) <= tol * np.linalg.norm(oh)
+
+
+@pytest.mark.parametrize("shape,l,sigma", [((1024, 2048), 3000.0, 400.0), ((700, 513), 200.0, 2.0)])
+def test_gaussian_heights_vs_oracle_filter(shape, l, sigma):
+    """ss_gaussian_heights at the multilayer scene's size (1025 x 2049
+    vertices, l = 100 cells) and an uneven-mix size against the FFT filter of
+    the same keyed noise (host restatement, itself pinned to the oracle on
+    small grids): agreement to rounding, 1e-11 sigma."""
+    from radar_sounder_simulator_amd.scene import SeededLattice
+    ny, nx = shape
+    print(f"case lattice/gaussian_heights/{ny}x{nx}", flush=True)
+    lat = SeededLattice(ny, nx, 30.0, sigma=sigma, corr_len=l, seed=(3, 1), mean=-3000.0)
+    h = dv.lattice_heights(lat, device="cuda")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    dv.lattice_heights(lat, device="cuda")
+    e1.record()
+    torch.cuda.synchronize()
+    got = h.cpu().numpy()
+    want = lat.heights
+    err = np.abs(got - want).max()
+    print(f"  device {e0.elapsed_time(e1):.2f} ms, max |device - host| {err:.3e} m", flush=True)
+    assert err <= 1e-11 * sigma
+    z = (got + 3000.0) / sigma
+    assert abs(z.std() - 1.0) < 0.5 and abs(z.mean()) < 1.0
+
+
+def test_keyed_phasors_vs_host():
+    """ss_keyed_phasors over a 1024 x 2048 lattice (key origin offset)
+    against the host keyed generator: complex64 values within one float32
+    ulp (the float64 log / cos / sin may differ by an ulp before rounding)."""
+    from radar_sounder_simulator_amd.roughness import KeyedPhasors
+    from radar_sounder_simulator_amd.scene import SeededLattice
+    print("case lattice/keyed_phasors", flush=True)
+    lat = SeededLattice(1024, 2048, 30.0, sigma=1.0, corr_len=300.0, seed=(3, 0))
+    kp = KeyedPhasors((1024, 2048), (3, 7), key_origin=(-512, 100))
+    built, _ = dv.build_lattices([lat], set(), device="cuda", phasors={id(lat): kp})
+    torch.cuda.synchronize()
+    b = built[id(lat)]
+    buf = b.fac.buf[14:16].cpu().numpy().astype(np.float32)
+    ph = np.asarray(kp)[b.fac.order]
+    for got, want in ((buf[0], ph.real), (buf[1], ph.imag)):
+        ulp = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+        print(f"  values off by one ulp: {int((ulp == 1).sum())} of {ulp.size}", flush=True)
+        assert ulp.max() <= 1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_seeded_multilayer_scene_built_on_device_vs_oracle(precision):
+    """The multilayer workload at reduced size built in HBM from its seeds
+    (surface and bed heights drawn on the device, firn and horizon planes
+    filled, everything tessellated and tabulated there): layers through the
+    resident pipeline against the oracle on the device's own heights read
+    back (float64 1e-9, mixed 0.05 dB / 2 mrad)."""
+    from oracle import coherent as oc
+    from oracle import multilayer as om
+    from radar_sounder_simulator_amd import workloads as W
+    from radar_sounder_simulator_amd.pipeline import ResidentScene
+    print(f"case lattice/seeded_multilayer/{precision}", flush=True)
+    wl = W.config3(nx=64, ny=32, n_traces=3, n_samples=4096)
+    rs = ResidentScene(wl, device="cuda")
+    assert rs.built is not None
+    for j in range(len(rs.layers)):
+        rs.run_layer(j, precision)
+    torch.cuda.synchronize()
+    raw = rs.raw.cpu().numpy()
+    for L in wl.layers:                      # host facets on the device's heights
+        L.lattice.heights = rs.built[id(L.lattice)].heights.cpu().numpy()
+    tr = wl.track
+    for j in (0, 1, 2, 13, 24):
+        L = wl.layers[j]
+        f = L.facets
+        if j == 0:
+            ref = oc.coherent_cluttergram(tr.positions, tr.u_ct, f.centers, f.normals, f.areas,
+                                          f.e1, f.e2, k=wl.k0, gamma=L.gamma, t0=wl.t0, dt=wl.dt,
+                                          n_samples=wl.n_samples, c=wl.c, pattern=wl.pattern)[0]
+        else:
+            ref = om.refracted_cluttergram(tr.positions, tr.u_ct, f, L.crossed, L.eps_leg,
+                                           L.att_leg, mode="coherent", t0=wl.t0, dt=wl.dt,
+                                           n_samples=wl.n_samples, c=wl.c, gamma=L.gamma,
+                                           k0=wl.k0, pattern=wl.pattern, refraction=L.refraction,
+                                           pad_to=L.pad_to, flat_stack=L.flat_stack)[0]
+        o = raw[j][..., 0] + 1j * raw[j][..., 1]
+        assert np.linalg.norm(ref) > 0, L.name
+        if precision == "fp64":
+            assert np.linalg.norm(o - ref) <= 1e-9 * np.linalg.norm(ref), L.name
+        else:
+            pk = np.abs(ref).max(axis=-1, keepdims=True)
+            m = np.abs(ref) > pk * 10 ** (-30 / 20)
+            assert np.abs(20 * np.log10(np.abs(o[m]) / np.abs(ref[m]))).max() <= 0.05, L.name
+            assert np.abs(np.angle(o[m] / ref[m])).max() * 1e3 <= 2.0, L.name
