@@ -18,6 +18,8 @@ reference's hot-path algorithm (soundersim, englacial/radar-sounder-simulator):
 * ``antenna``    -- ``antenna.py:138-258`` (in-kernel gain functions);
 * ``physics``    -- ``physics.py:16-67, 109-150``;
 * ``waveform``   -- ``waveform.py:76-160``;
+* ``scene``      -- ``compare/haynes.py:94-112`` (Gaussian-ACF height field)
+  and the keyed white noise the device generator draws;
 * ``brute``      -- referees ``compare/brute_force.py``, ``compare/
   brute_force_layered.py`` (used to PIN the oracle, not as the oracle);
 * ``csrc/oracle.c`` -- the same surface and sequential-refracted math in C
@@ -32,5 +34,5 @@ bit for bit.
 Pinning: every function here is checked by ``tests/test_oracle_*.py`` against
 closed forms, brute-force referees, independent library routines (SciPy
 ``wofz``, mpmath, ``np.convolve``) or invariants -- see the per-function
-docstrings.  Parity status of each function is listed in DESIGN.md.
+docstrings and each module's header, which names its pins.
 """

@@ -54,6 +54,9 @@ SS_HD void inv2(const double m[2][2], double o[2][2]) {
   o[1][1] = m[0][0] / det;
 }
 
+// FAST (mixed tiers): u normalised by the refined reciprocal square root
+// (any orthonormal basis of the plane serves the residual projection).
+template <bool FAST = false>
 SS_HD void plane_basis(const double n[3], double b[2][3]) {
   double a[3] = {0.0, 0.0, 0.0};
   if (fabs(n[2]) < 0.9)
@@ -62,9 +65,15 @@ SS_HD void plane_basis(const double n[3], double b[2][3]) {
     a[0] = 1.0;
   double u[3] = {a[1] * n[2] - a[2] * n[1], a[2] * n[0] - a[0] * n[2],
                  a[0] * n[1] - a[1] * n[0]};
-  double un = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-  un = un > kTiny ? un : kTiny;
-  for (int k = 0; k < 3; ++k) u[k] /= un;
+  if constexpr (FAST) {
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double iu = uu > kTiny * kTiny ? rsq64(uu) : 1.0 / kTiny;
+    for (int k = 0; k < 3; ++k) u[k] *= iu;
+  } else {
+    double un = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    un = un > kTiny ? un : kTiny;
+    for (int k = 0; k < 3; ++k) u[k] /= un;
+  }
   b[0][0] = u[0];
   b[0][1] = u[1];
   b[0][2] = u[2];
@@ -509,7 +518,7 @@ SS_HD bool joint_path_mixed(const double p[3], const double q[3], const RefrCons
   {
     float cur[3] = {(float)ps64[0], (float)ps64[1], (float)ps64[2]};
     SS_JLOOP(i) {
-      plane_basis(nrs[i], B64[i]);
+      plane_basis<true>(nrs[i], B64[i]);
       const float ni[3] = {(float)nrs[i][0], (float)nrs[i][1], (float)nrs[i][2]};
       double o64[3];
       float o[3];
@@ -980,7 +989,7 @@ SS_HD void flat_anchor(const double p[3], const double q[3], const RefrConst& rc
   double c0[3] = {p[0], p[1], p[2]};
   if (kp > 0) {
     const double zp = rc.desc[(kp - 1) * SS_IDESC_COUNT + SS_IDESC_MP + 2];
-    const double f = (p[2] - zp) / (p[2] - q[2]);
+    const double f = mixed ? (p[2] - zp) * rcp64(p[2] - q[2]) : (p[2] - zp) / (p[2] - q[2]);
     for (int c = 0; c < 3; ++c) c0[c] = p[c] + f * (q[c] - p[c]);
     c0[2] = zp;
   }
@@ -1017,24 +1026,44 @@ SS_HD void flat_anchor(const double p[3], const double q[3], const RefrConst& rc
   int col = clamp_round(rc1, (int)d0[SS_IDESC_NX] - 1);
   const double* g = rc.tables + ((long)d0[SS_IDESC_TABOFF] + (long)row * (int)d0[SS_IDESC_NX] +
                                  col) * 6;
-  double nn = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
-  for (int c = 0; c < 3; ++c) {
-    pt0[c] = g[c];
-    nr0[c] = g[3 + c] / nn;
+  if (mixed) {
+    // the refined reciprocal square root, as the mixed sequential chain
+    const double inn = rsq64(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+    for (int c = 0; c < 3; ++c) {
+      pt0[c] = g[c];
+      nr0[c] = g[3 + c] * inn;
+    }
+  } else {
+    double nn = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+    for (int c = 0; c < 3; ++c) {
+      pt0[c] = g[c];
+      nr0[c] = g[3 + c] / nn;
+    }
   }
   double h = (x[0] - pt0[0]) * nr0[0] + (x[1] - pt0[1]) * nr0[1] + (x[2] - pt0[2]) * nr0[2];
   for (int c = 0; c < 3; ++c) s0[c] = x[c] - h * nr0[c];
 }
 
 // Start slowness: the ray p -> s0 refracted n0 -> n1 at the plane nr0.
+// FAST (mixed tier): refined reciprocal / reciprocal square root estimates
+// for the divisions and square roots (a start point only).
+template <bool FAST = false>
 SS_HD void flat_start(const double p[3], const double s0[3], const double nr0[3],
                       const double* nl, double v[2]) {
   const double ds[3] = {s0[0] - p[0], s0[1] - p[1], s0[2] - p[2]};
-  const double L = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
-  const double i0 = ds[0] / L, i1 = ds[1] / L, i2 = ds[2] / L;
+  double i0, i1, i2, r, cost;
+  if constexpr (FAST) {
+    const double iL = rsq64(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+    i0 = ds[0] * iL, i1 = ds[1] * iL, i2 = ds[2] * iL;
+    r = nl[0] * rcp64(nl[1]);
+  } else {
+    const double L = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+    i0 = ds[0] / L, i1 = ds[1] / L, i2 = ds[2] / L;
+    r = nl[0] / nl[1];
+  }
   const double cosi = -(nr0[0] * i0 + nr0[1] * i1 + nr0[2] * i2);
-  const double r = nl[0] / nl[1];
-  const double cost = sqrt(1.0 - r * r * (1.0 - cosi * cosi));
+  const double ct2 = 1.0 - r * r * (1.0 - cosi * cosi);
+  cost = FAST ? sqrt64(ct2) : sqrt(ct2);
   const double m = nl[0] * cosi - nl[1] * cost;
   v[0] = nl[0] * i0 + m * nr0[0];
   v[1] = nl[0] * i1 + m * nr0[1];
@@ -1409,8 +1438,8 @@ SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst
   const double* nl = fs.n;
   double pt0[3], nr0[3], s0[3], B[2][3], v[2];
   flat_anchor(p, q, rc, fs, true, pt0, nr0, s0);
-  plane_basis(nr0, B);
-  flat_start(p, s0, nr0, nl, v);
+  plane_basis<true>(nr0, B);
+  flat_start<true>(p, s0, nr0, nl, v);
   const double dN = N > 1 ? Ls.zlast - q[2] : 0.0;
   bool stuck = false;
   float Jf[2][2] = {{1.0f, 0.0f}, {0.0f, 1.0f}};
@@ -1522,7 +1551,7 @@ SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst
   if (N > 1) {
     lay_ok = vv < Ls.min_n2;
     for (int k = 2; k < N; ++k) {
-      const double rk = rsq64(Ls.n2[k] - vv);
+      const double rk = rsq64_1(Ls.n2[k] - vv);
       a = fma(Ls.d[k], rk, a);
       o = fma(Ls.dn2[k], rk, o);
       const float rf = (float)rk;
@@ -1567,7 +1596,7 @@ SS_HD bool flat_path_mixed(const double p[3], const double q[3], const RefrConst
     const double F0 = B[0][0] * g[0] + B[0][1] * g[1] + B[0][2] * g[2];
     const double F1 = B[1][0] * g[0] + B[1][1] * g[1] + B[1][2] * g[2];
     const double j00 = Jf[0][0], j01 = Jf[0][1], j10 = Jf[1][0], j11 = Jf[1][1];
-    const double idet = 1.0 / (j00 * j11 - j01 * j10);
+    const double idet = rcp64(j00 * j11 - j01 * j10);
     const double dv0 = -(j11 * F0 - j01 * F1) * idet;
     const double dv1 = -(j00 * F1 - j10 * F0) * idet;
     const double vdv = v[0] * dv0 + v[1] * dv1;

@@ -55,6 +55,20 @@ SS_HD double rsq64(double x) {
   return 1.0 / sqrt(x);
 #endif
 }
+// 1/sqrt(x) from the hardware estimate (about 2^-23 relative) refined by
+// ONE Newton-Raphson step: about 2e-14 relative, for sums whose float64
+// result needs far less than an ulp (the flat-stack layer loop: optical
+// path and horizontal reach to 1e-10 m over kilometres).  The host build
+// starts from a float32 estimate so the CPU tests see the same error class.
+SS_HD double rsq64_1(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rsq(x);
+#else
+  double y = (double)(1.0f / sqrtf((float)x));
+#endif
+  const double e = fma(-(0.5 * x) * y, y, 0.5);
+  return fma(y, e, y);
+}
 // sqrt as x rsq64(x), keeping sqrt(0) = 0 and sqrt(x < 0) = NaN.
 SS_HD double sqrt64(double x) {
   const double y = x * rsq64(x > 0.0 ? x : 1.0);
