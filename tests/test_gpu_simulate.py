@@ -61,3 +61,53 @@ def test_simulate_three_interfaces_vs_oracle():
     fi, ftw, fxyz = oo.first_return(tr.positions, f.centers, cfg.radar.t0, cfg.radar.dt, 512, C)
     np.testing.assert_array_equal(res.first_return_twtt, ftw)
     np.testing.assert_array_equal(res.first_return_xyz, fxyz)
+
+
+def test_apply_waveform_reference_signature():
+    """The reference-signature ``apply_waveform(out, radar, mode)``
+    (waveform.py:141-160) on the device against the oracle's FFT
+    convolution: complex64 in -> complex64 out, complex128 in -> complex128
+    out (rel L2 1e-12), a trailing axis (split sides) convolved per index,
+    the incoherent |p|^2 envelope (clamped at 0, float32) only when enabled,
+    delta returning its input."""
+    from types import SimpleNamespace
+
+    from oracle import waveform as ow
+    from radar_sounder_simulator_amd.config import WaveformConfig
+    from radar_sounder_simulator_amd.waveform import apply_waveform
+
+    print("case waveform/apply_waveform", flush=True)
+    dt = 1.0 / 48e6
+    rng = np.random.default_rng(3)
+    wf = WaveformConfig(kind="chirp", bandwidth=15e6, pulse_length=20e-6)
+    radar = SimpleNamespace(waveform=wf, dt=dt)
+    f128 = rng.standard_normal((5, 1024)) + 1j * rng.standard_normal((5, 1024))
+    ref = ow.apply_waveform(f128, 15e6, 20e-6, dt)
+    out = apply_waveform(f128, radar, "coherent")
+    assert out.dtype == np.complex128 and out.shape == f128.shape
+    assert np.linalg.norm(out - ref) <= 1e-12 * np.linalg.norm(ref)
+    out32 = apply_waveform(f128.astype(np.complex64), radar, "coherent")
+    assert out32.dtype == np.complex64
+    ref32 = ow.apply_waveform(f128.astype(np.complex64), 15e6, 20e-6, dt)
+    assert np.linalg.norm(out32 - ref32) <= 1e-6 * np.linalg.norm(ref32)
+    # (T, n, 2): fast time is axis 1, the trailing axis convolved per index
+    f3 = np.stack([f128, 2.0 * f128[::-1]], axis=-1)
+    out3 = apply_waveform(f3, radar, "coherent")
+    assert out3.shape == f3.shape
+    for s in range(2):
+        r = ow.apply_waveform(f3[..., s], 15e6, 20e-6, dt)
+        assert np.linalg.norm(out3[..., s] - r) <= 1e-12 * np.linalg.norm(r)
+    # incoherent: unchanged unless the envelope is enabled
+    pw = np.abs(f128).astype(np.float32) ** 2
+    assert apply_waveform(pw, radar, "incoherent") is pw
+    radar_e = SimpleNamespace(waveform=WaveformConfig(kind="chirp", bandwidth=15e6,
+                                                      pulse_length=20e-6,
+                                                      incoherent_envelope=True), dt=dt)
+    env = apply_waveform(pw, radar_e, "incoherent")
+    p, m = ow.compressed_pulse(15e6, 20e-6, dt)
+    want = np.maximum(ow.convolve_fast_time(pw.astype(np.float64), np.abs(p) ** 2, m).real, 0.0)
+    assert env.dtype == np.float32 and env.min() >= 0.0
+    np.testing.assert_allclose(env, want.astype(np.float32), rtol=1e-5, atol=1e-6 * want.max())
+    # delta: the input itself
+    radar_d = SimpleNamespace(waveform=WaveformConfig(kind="delta"), dt=dt)
+    assert apply_waveform(f128, radar_d, "coherent") is f128
