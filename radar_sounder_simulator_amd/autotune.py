@@ -3,7 +3,8 @@
 ``ss_batch_field`` has two shape parameters: the trace tile (traces per
 workgroup: 4, 8 or 16 -- larger tiles reuse each facet's registers over more
 traces but leave fewer work items to balance uneven jobs) and the number of
-persistent workgroups (a multiple of the CU count).  ``tune_batch`` times
+persistent workgroups (2, 3, 4, 6 or 8 per CU: the pair kernels run 1 to 4
+waves per SIMD, so a CU holds 1 to 4 of their workgroups at once).  ``tune_batch`` times
 every candidate on a proxy of the job list -- the same jobs cut to their
 first ``probe_traces`` traces (every facet tile, every job kind; a few
 percent of the work of a large mix) -- with HIP events around each launch on
@@ -23,7 +24,7 @@ import torch
 from . import _lib
 
 TRACE_TILES = (4, 8, 16)
-BLOCKS_PER_CU = (2, 4, 8)
+BLOCKS_PER_CU = (2, 3, 4, 6, 8)
 
 
 def mix_signature(jobs, precision):
