@@ -27,7 +27,9 @@ stream's host side; the kernels are the batched ones of ``batch.py``.
 """
 
 import collections
+import contextlib
 import math
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -283,6 +285,46 @@ class ActiveRequest:
         return jb
 
 
+class CaptureGate:
+    """Keeps graph capture and the upload workers' device work apart.
+
+    A stream capture in global mode (torch's default) is invalidated by an
+    allocation, a synchronous copy or a synchronisation that ANY thread of
+    the process makes while it runs, so a scene upload on a worker thread
+    overlapping the capture of a step breaks the capture.  Workers enter
+    ``work()`` around their device calls (any number at once); ``capture()``
+    waits until none is inside and holds new ones off until it ends."""
+
+    def __init__(self):
+        self._cv = threading.Condition()
+        self._workers = 0
+        self._capturing = False
+
+    @contextlib.contextmanager
+    def work(self):
+        with self._cv:
+            self._cv.wait_for(lambda: not self._capturing)
+            self._workers += 1
+        try:
+            yield
+        finally:
+            with self._cv:
+                self._workers -= 1
+                self._cv.notify_all()
+
+    @contextlib.contextmanager
+    def capture(self):
+        with self._cv:
+            self._cv.wait_for(lambda: self._workers == 0 and not self._capturing)
+            self._capturing = True
+        try:
+            yield
+        finally:
+            with self._cv:
+                self._capturing = False
+                self._cv.notify_all()
+
+
 class Server:
     """Continuous-batching request loop on one device (see module docstring).
 
@@ -313,6 +355,7 @@ class Server:
         self.scene_bytes = {}
         self.uploads = {}                           # scene_id -> (future, reserved bytes)
         self.pool = ThreadPoolExecutor(max_workers=upload_workers)
+        self.gate = CaptureGate()
         self.upload_stream = torch.cuda.Stream(self.device) if self.cuda else None
         # captured steps: two StepGraphs per feature class, used alternately
         # so a step's pinned table is never rewritten while in flight
@@ -366,7 +409,7 @@ class Server:
         """Worker thread: prepare and upload a scene on the upload stream."""
         if not self.cuda:
             return DeviceScene(layers, self.device), None
-        with torch.cuda.stream(self.upload_stream):
+        with self.gate.work(), torch.cuda.stream(self.upload_stream):
             sc = DeviceScene(layers, self.device)
             ev = torch.cuda.Event()
             ev.record(self.upload_stream)
@@ -453,8 +496,9 @@ class Server:
         return True
 
     def _capture(self, cls):
-        return StepGraph(self.device, self.precision, self.tt, cls, max_jobs=self.max_jobs,
-                         max_items=self.max_items, n_blocks=self.n_blocks)
+        with self.gate.capture():
+            return StepGraph(self.device, self.precision, self.tt, cls, max_jobs=self.max_jobs,
+                             max_items=self.max_items, n_blocks=self.n_blocks)
 
     def _finish_launch(self, a):
         """Queue range compression and the copy to pinned host memory after

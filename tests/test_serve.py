@@ -84,3 +84,38 @@ def test_captured_step_capacity_is_bounded(sim):
         assert srv.max_items == 1 << 22
         ws = _lib.load().ss_batch_step_workspace_bytes(_lib.SS_PREC_MIXED, srv.max_items, 8)
         assert ws <= 1100 << 20
+
+
+def test_capture_gate_excludes_workers():
+    """A capture waits for workers inside ``work()`` and holds new ones off
+    until it ends; workers run together otherwise."""
+    import threading
+    import time
+
+    gate = S.CaptureGate()
+    log, lock = [], threading.Lock()
+
+    def worker(tag, hold):
+        with gate.work():
+            with lock:
+                log.append(("in", tag))
+            time.sleep(hold)
+            with lock:
+                log.append(("out", tag))
+
+    a = threading.Thread(target=worker, args=("a", 0.3))
+    b = threading.Thread(target=worker, args=("b", 0.3))
+    a.start(); b.start()
+    time.sleep(0.1)
+    with gate.capture():
+        with lock:
+            assert ("out", "a") in log and ("out", "b") in log   # both left first
+            log.append(("capture", None))
+        c = threading.Thread(target=worker, args=("c", 0.0))
+        c.start()
+        time.sleep(0.2)
+        with lock:
+            assert ("in", "c") not in log                        # held off meanwhile
+    c.join(); a.join(); b.join()
+    assert log.index(("in", "c")) > log.index(("capture", None))
+    assert log.index(("in", "b")) < log.index(("out", "a"))      # a and b overlapped
