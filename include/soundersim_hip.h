@@ -296,8 +296,8 @@ int ss_refracted_field_noalloc(int precision, const ss_refracted_params* params,
                                int32_t* pair_bins, void* stream);
 
 /* Workspace bytes ss_refracted_field needs: 0 for SS_PREC_FP64 or an empty
- * call; for SS_PREC_MIXED 256 + one flag byte per (8-trace, 256-facet) work
- * item (rounded up to 8) + 256 bytes of lane masks per work item.  No
+ * call; for SS_PREC_MIXED 256 + one flag byte per (16-trace, 256-facet) work
+ * item (rounded up to 8) + 512 bytes of lane masks per work item.  No
  * counterpart in the reference (its kernels run one precision). */
 int64_t ss_refracted_workspace_bytes(int precision, int64_t n_facets, int64_t n_traces);
 

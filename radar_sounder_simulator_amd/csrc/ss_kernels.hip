@@ -46,7 +46,7 @@ namespace ss {
 constexpr int kBlock = 256;
 constexpr int kWin = 512;      // LDS slots per side per trace
 constexpr int kTTSurf = 16;    // traces per workgroup, surface
-constexpr int kTTRefr = 8;     // traces per workgroup, refracted
+constexpr int kTTRefr = 16;    // traces per workgroup, refracted
 
 thread_local std::string g_last_error;
 
@@ -321,11 +321,13 @@ __host__ __device__ inline PatternArgs make_pattern(int kind, int n, double a, c
 }
 
 // Stages the float32 lookup of a tabulated pattern of 2 .. kPatLds samples
-// (pattern_lds_entry) in ``ptab`` / ``pidx`` and points ``pa`` at it;
-// visible after the caller's next barrier.
+// (pattern_lds_value, then pattern_lds_index over the staged values) in
+// ``ptab`` / ``pidx`` and points ``pa`` at it; visible after the caller's
+// next barrier.  Called by every thread of the workgroup (one barrier).
 __device__ inline void stage_pattern(PatternArgs& pa, float* ptab, short* pidx) {
-  const int m = pa.n > kPatCells ? pa.n : kPatCells;
-  for (int i = threadIdx.x; i < m; i += kBlock) pattern_lds_entry(pa.table, pa.n, i, ptab, pidx);
+  for (int i = threadIdx.x; i < pa.n; i += kBlock) pattern_lds_value(pa.table, pa.n, i, ptab);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kPatCells; i += kBlock) pattern_lds_index(ptab, pa.n, i, pidx);
   pa.ftab = ptab;
   pa.fidx = pidx;
   pa.fscale = pattern_cell_scale(pa.table, pa.n);
@@ -394,6 +396,7 @@ __host__ __device__ inline RefrConst make_refr_const(const ss_refracted_params& 
   rc.k_bin = 2.0 / (prm.c * prm.dt);
   rc.t0_dt = prm.t0 / prm.dt;
   rc.cyc_per_m = prm.k0 / kPi;
+  rc.fast_opl_tol = (float)(kFastOplFrac * kBinGuard / rc.k_bin);
   return rc;
 }
 

@@ -328,7 +328,7 @@ struct PatternArgs {
   double a;            // n_elements / length_lam
   double b[4];         // spacing_lam, field_scale, q, enabled
   const double* table; // tabulated: theta[n] | gain[n]; tapered: tx[n] | rx[n]
-  // tabulated, staged lookup (pattern_lds_entry) in LDS, read by
+  // tabulated, staged lookup (pattern_lds_value / _index) in LDS, read by
   // antenna_gain<float, SS_PAT_TABULATED, true> (the incoherent kernel's
   // float32 tier); null elsewhere
   const float* ftab;   // theta[n] | gain[n] | slope[n - 1], float
@@ -351,7 +351,14 @@ SS_HD float pattern_cell_scale(const double* table, int n) {
 }
 
 // Entry i of the staged lookup (i < max(n, kPatCells)).
-SS_HD void pattern_lds_entry(const double* table, int n, int i, float* ftab, short* fidx) {
+// Staged lookup of a tabulated pattern, in two passes over the workgroup
+// (a barrier between them): pattern_lds_value(i) for i < n writes the
+// float32 theta, gain and interval slope of sample i; pattern_lds_index(i)
+// for i < kPatCells then writes the cell index -- the last interval whose
+// start is <= the cell's start -- by a bisection over the staged float32
+// thetas (the same comparisons as on (float) table values, read from LDS
+// instead of a chain of dependent global loads).
+SS_HD void pattern_lds_value(const double* table, int n, int i, float* ftab) {
   if (i < n) {
     ftab[i] = (float)table[i];
     ftab[n + i] = (float)table[n + i];
@@ -361,13 +368,15 @@ SS_HD void pattern_lds_entry(const double* table, int n, int i, float* ftab, sho
       ftab[2 * n + i] = (y1 - y0) / (x1 - x0);
     }
   }
+}
+SS_HD void pattern_lds_index(const float* ftab, int n, int i, short* fidx) {
   if (i < kPatCells) {
-    const float xa = (float)table[0];
-    const float e = xa + (float)i / pattern_cell_scale(table, n);
+    const float xa = ftab[0];
+    const float e = xa + (float)i / ((float)kPatCells / (ftab[n - 1] - xa));
     int lo = 0, hi = n - 1;  // last interval whose start is <= e
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
-      if ((float)table[mid] <= e)
+      if (ftab[mid] <= e)
         lo = mid;
       else
         hi = mid;

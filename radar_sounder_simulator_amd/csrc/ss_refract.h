@@ -79,6 +79,16 @@ constexpr double kResLo = 1e-11, kResHi = 1e-7;  // ambiguous residual band
 // float32 Newton loops stop once a full step moves sigma by at most this
 // relative amount (4 ulp): settled, or alternating between neighbours
 constexpr float kF32Settle = 4.8e-7f;
+// Mixed local-plane solve, polish-free crossing: bound of the settled
+// float32 crossing's in-plane error relative to the in-plane span L (about
+// 1.5x the rounding estimate), and the sigma = sin(theta) below which it
+// applies (grazing incidence always takes the float64 polish).
+constexpr float kPosRelErr = 2e-6f;
+constexpr float kSigFast = 0.995f;
+// ... with its optical-path error at most this fraction of the bin guard
+// and its legs' direction error at most kDirRelErr (radians)
+constexpr double kFastOplFrac = 0.25;
+constexpr float kDirRelErr = 1e-5f;
 
 struct Snell64 {
   double x[3];
@@ -163,7 +173,8 @@ struct SnellMixed {
 };
 
 SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
-                       const double nrm[3], double n1, double n2, SnellMixed& out) {
+                       const double nrm[3], double n1, double n2, float opl_tol,
+                       SnellMixed& out) {
   double h1 = ((p[0] - o[0]) * nrm[0] + (p[1] - o[1]) * nrm[1]) + (p[2] - o[2]) * nrm[2];
   double h2 = ((q[0] - o[0]) * nrm[0] + (q[1] - o[1]) * nrm[1]) + (q[2] - o[2]) * nrm[2];
   double a = fabs(h1), b = fabs(h2);
@@ -189,6 +200,7 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
   float sig = Lhf * f_rsq(fmaxf(Lhf * Lhf + ab * ab, 1e-30f));
   sig = fminf(sig, hi0);
   float lo = 0.0f, hi = hi0;
+  bool settled = false;
   for (int it = 0; it < kLocalIters32; ++it) {
     float v = rf * sig;
     float c1 = fmaxf((1.0f - sig) * (1.0f + sig), 1e-30f);
@@ -204,7 +216,58 @@ SS_HD void snell_mixed(const double p[3], const double q[3], const double o[3],
     const float nx = ok ? sn : 0.5f * (lo + hi);
     const bool done = ok && fabsf(nx - sig) <= kF32Settle * sig;
     sig = nx;
-    if (done) break;  // settled to a few ulp: the float64 polish takes it from here
+    if (done) {  // settled to a few ulp by a Newton step
+      settled = true;
+      break;
+    }
+  }
+  if (settled && sig < kSigFast) {
+    // Crossing straight from the float32 root, no float64 Newton polish.
+    // The optical path n1 s1 + n2 s2 is stationary in the crossing point on
+    // the plane (Fermat), so an in-plane error d of the crossing changes it
+    // by at most d^2 (n1 / a + n2 / b) / 2 (a, b: the end points' heights
+    // over the plane, a <= s1, b <= s2).  At the settled root the residual
+    // F is float32 rounding noise of its terms (each <= Lh) and of the
+    // float32 inputs, well under 1e-6 Lh; since dt / dsigma = ar (1 -
+    // sigma^2)^(-3/2) <= F', the crossing parameter t is off by at most
+    // |F|, and t's own rounding adds a few ulp of Lh: d <= kPosRelErr L.
+    // The legs' directions move by at most d / a and d / b (first order:
+    // incidence and refraction cosines, departure direction), held to
+    // kDirRelErr.  Where both bounds hold -- the path error within opl_tol,
+    // the direction error within kDirRelErr -- the crossing is
+    // placed on the plane in float64 (fp + s t u), s1 is formed in float64
+    // for it, and the residual is taken as converged: the float32 iteration
+    // settled by a Newton step on an interior root from the reference's
+    // starting point, where the float64 iteration converges quadratically
+    // far inside the reference's 25-step budget.  Elsewhere (near grazing,
+    // short legs of high curvature, bisection endings) the float64 polish
+    // below runs.
+    const float c1f = fmaxf((1.0f - sig) * (1.0f + sig), 1e-30f);
+    const float trf = arf * sig * f_rsq(c1f);
+    const float ttf = swap ? Lhf - trf : trf;
+    const float tf = fminf(fmaxf(ttf, 0.0f), Lhf);
+    const float af = (float)a, bf = (float)b;
+    const float dpos = kPosRelErr * (float)L;
+    const float curv = ((float)n1 * bf + (float)n2 * af) * f_rcp(fmaxf(af * bf, 1e-30f));
+    if (0.5f * curv * dpos * dpos <= opl_tol && dpos <= kDirRelErr * fminf(af, bf)) {
+      const double t = (double)tf;
+      const double kTiny2 = kTiny * kTiny;
+      const double q1s = ah * ah + t * t;
+      const double ir1 = rsq64(q1s > kTiny2 ? q1s : kTiny2);
+      const double r1 = q1s > kTiny2 ? q1s * ir1 : kTiny;
+      out.res = 0.0;
+      out.s1 = s * r1;
+      const double st = s * t;
+      out.x[0] = fp[0] + st * u[0];
+      out.x[1] = fp[1] + st * u[1];
+      out.x[2] = fp[2] + st * u[2];
+      out.c_in = (float)(ah * ir1);
+      const float bhf = (float)bh, d2f = Lhf - tf;
+      out.c_out = bhf * f_rsq(fmaxf(bhf * bhf + d2f * d2f, 1e-30f));
+      out.valid = h1 * h2 < 0.0;
+      out.ambiguous = false;
+      return;
+    }
   }
   // float64 polish (plain Newton, clamped into [0, hi0)): one step, and
   // more (at most kPolishMax) while the Snell residual sits in the
@@ -322,6 +385,7 @@ struct RefrConst {
   int joint, k_pad, n_newton, n_backtrack;  // joint solver (pads first)
   // derived
   double k_bin, t0_dt, cyc_per_m;
+  float fast_opl_tol;  // optical-path error (m) allowed to a polish-free crossing
 };
 
 // Path sums handed from a crossing solver to the radiometry stage.
@@ -610,7 +674,7 @@ SS_HD bool seq_path_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
     double pt[3], nr[3];
     lookup_cell<true>(rc, i, row, col, pt, nr);
     SnellMixed s;
-    snell_mixed(cur, f.c, pt, nr, nl[i], nl[i + 1], s);
+    snell_mixed(cur, f.c, pt, nr, nl[i], nl[i + 1], rc.fast_opl_tol, s);
     if (s.ambiguous) return false;
     valid = valid && s.valid;
     float c_inc = fmaxf(s.c_in, (float)kCMin);

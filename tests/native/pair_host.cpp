@@ -11,7 +11,7 @@
 using namespace ss;
 
 // The incoherent kernel's float32 tier reads a tabulated pattern through
-// the lookup it stages in LDS (pattern_lds_entry); the harness builds the
+// the lookup it stages in LDS (pattern_lds_value / _index); the harness builds the
 // same lookup in host memory.
 struct HostPatLookup {
   std::vector<float> ftab;
@@ -25,8 +25,8 @@ static void stage_host(PatternArgs& pa, HostPatLookup& st) {
   if (pa.kind != SS_PAT_TABULATED || pa.n < 2 || pa.n > kPatLds || !pa.table) return;
   st.ftab.assign(3 * pa.n, 0.0f);
   st.fidx.assign(kPatCells, 0);
-  const int m = pa.n > kPatCells ? pa.n : kPatCells;
-  for (int i = 0; i < m; ++i) pattern_lds_entry(pa.table, pa.n, i, st.ftab.data(), st.fidx.data());
+  for (int i = 0; i < pa.n; ++i) pattern_lds_value(pa.table, pa.n, i, st.ftab.data());
+  for (int i = 0; i < kPatCells; ++i) pattern_lds_index(st.ftab.data(), pa.n, i, st.fidx.data());
   pa.ftab = st.ftab.data();
   pa.fidx = st.fidx.data();
   pa.fscale = pattern_cell_scale(pa.table, pa.n);

@@ -87,11 +87,11 @@ def test_invalid_arguments_rejected_on_host():
 
 def test_workspace_sizes():
     lib = _lib.load()
-    # fp64 needs none; mixed: 256-byte header + one flag byte per (8-trace,
-    # 256-facet) work item rounded up to 8 + 4 * 8 * 8 mask bytes per item
-    # -- 1000 facets x 20 traces = 4 x 3 items
+    # fp64 needs none; mixed: 256-byte header + one flag byte per (16-trace,
+    # 256-facet) work item rounded up to 8 + 4 * 8 * 16 mask bytes per item
+    # -- 1000 facets x 20 traces = 4 x 2 items
     assert lib.ss_refracted_workspace_bytes(0, 1000, 20) == 0
-    assert lib.ss_refracted_workspace_bytes(1, 1000, 20) == 256 + 16 + 12 * 256
+    assert lib.ss_refracted_workspace_bytes(1, 1000, 20) == 256 + 8 + 8 * 512
     assert lib.ss_refracted_workspace_bytes(1, 0, 20) == 0
     j = _lib.Job()
     j.kind, j.n_facets, j.n_traces = 1, 1000, 20
