@@ -44,8 +44,8 @@
 namespace ss {
 
 constexpr int kBlock = 256;
-constexpr int kWin = 512;      // LDS slots per side per trace
-constexpr int kTTSurf = 16;    // traces per workgroup, surface
+constexpr int kWin = 256;      // LDS slots per side per trace
+constexpr int kTTSurf = 32;    // traces per workgroup, surface
 constexpr int kTTRefr = 16;    // traces per workgroup, refracted
 
 thread_local std::string g_last_error;
