@@ -472,7 +472,12 @@ __device__ inline void load_refr_facet(const KCommon& k, long fi, int flags, Ref
 // RM: roughness mode (0 none, 1 area-only D_Phi, 2 finite-facet Gaussian,
 // -1 any); PAT: antenna kind (0 isotropic, 5 tabulated, -1 any).  The host
 // picks the narrowest instantiation so unused features cost no registers.
-template <int PREC, int RM, int PAT>
+// FS (mixed tier): 0 every flag at run time; FS_SURF_PLAIN no split sides,
+// interpolated bins or per-facet gamma (compile-time), taper and roughness
+// at run time.
+constexpr int FS_SURF_PLAIN = 1;
+constexpr int kSurfPlainRuntimeFlags = SS_TAPER | SS_ROUGH | SS_D_PHI_AREA;
+template <int PREC, int RM, int PAT, int FS = 0>
 __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst sc) {
   using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   __shared__ Shared<ACC, kTTSurf> sh;
@@ -482,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
   const bool live = f < k.nf;
   SurfFacet64 f64;
   load_surface_facet(k, live ? f : 0, sc.flags, f64);
-  const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
+  const bool split = FS == FS_SURF_PLAIN ? false : (sc.flags & SS_SPLIT_SIDES) != 0;
   if constexpr (PREC == SS_PREC_FP64) {
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o, int) {
@@ -495,13 +500,12 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst
     // staged in LDS (the host picks it for tables of <= kPatLds samples)
     constexpr bool ST = PAT == SS_PAT_TABULATED;
     SurfaceConst scl = sc;
+    if constexpr (FS == FS_SURF_PLAIN) scl.flags = sc.flags & kSurfPlainRuntimeFlags;
     if constexpr (ST) {
       __shared__ float ptab[3 * kPatLds];
       __shared__ short pidx[kPatCells];
       stage_pattern(scl.pat, ptab, pidx);
     }
-    // (no staged D_Phi term table here: its 2.4 KB would cost the kernel
-    // its fourth workgroup per CU)
     trace_loop<ACC, kTTSurf>(sh, k, w, live, f, sc.n_samples, split,
                              [&](const TraceRec& t, PairOut& o, int) {
                                surface_pair_mixed<RM, PAT, ST>(f32, nullptr, t, scl, o);
@@ -666,7 +670,24 @@ __global__ void fold_kernel(double* field, const double* traces, long ts, long n
 // NMAX = 0: sequential chain only; NMAX > 0: joint solve for stacks <= NMAX;
 // NMAX < 0: flat-stack joint solve.
 // The mixed instantiations leave unresolved pairs to redo_kernel (queue rd).
-template <int PREC, int NMAX, int RM, int PAT, typename ACC>
+// FS: flag profile.  0: every flag read at run time.  FS_PLAIN (mixed
+// tier): the launch is coherent with no split sides, diffuse term,
+// per-facet gamma or forced redo -- those flags are compile-time constants,
+// and so is the crossing count (one) of a sequential chain, so their tests
+// and the chain's crossing loop fold away (the reference likewise compiles
+// one variant per combination of static flags); taper, roughness and
+// crossing roughness stay run-time.  The host picks it when the launch
+// matches.
+constexpr int FS_PLAIN = 1;
+constexpr int kPlainRuntimeFlags = SS_TAPER | SS_CROSS_ROUGH | SS_ROUGH | SS_D_PHI_AREA;
+__host__ __device__ inline bool plain_flags(const ss_refracted_params& p) {
+  return (p.flags & SS_COHERENT) && !(p.flags & ~(kPlainRuntimeFlags | SS_COHERENT));
+}
+__host__ __device__ inline bool plain_profile(const ss_refracted_params& p) {
+  return plain_flags(p) && p.n_crossed == 1 && p.refraction == 0;
+}
+
+template <int PREC, int NMAX, int RM, int PAT, typename ACC, int FS = 0>
 __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k,
                                       const RefrConst& rc, const Redo& rd) {
   const long w = work_id(k);
@@ -674,7 +695,7 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
   const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
   const bool live = f < k.nf;
   const long fi = live ? f : 0;
-  const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
+  const bool split = FS == FS_PLAIN ? false : (rc.flags & SS_SPLIT_SIDES) != 0;
   if constexpr (PREC == SS_PREC_FP64) {
     RefrFacet fr;
     load_refr_facet(k, fi, rc.flags, fr);
@@ -691,6 +712,10 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
     }
     constexpr bool ST = PAT == SS_PAT_TABULATED;  // staged pattern lookup
     RefrConst rcl = rc;
+    if constexpr (FS == FS_PLAIN) {
+      rcl.flags = (rc.flags & kPlainRuntimeFlags) | SS_COHERENT;
+      if constexpr (NMAX == 0) rcl.n_crossed = 1;
+    }
     if constexpr (ST) {
       __shared__ float ptab[3 * kPatLds];
       __shared__ short pidx[kPatCells];
@@ -713,7 +738,7 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
     }
     trace_loop<ACC, kTTRefr, true>(sh, k, w, live, f, rc.n_samples, split,
                                    [&](const TraceRec& t, PairOut& o, int) {
-                                     if ((rc.flags & SS_REDO_ALL) ||
+                                     if ((rcl.flags & SS_REDO_ALL) ||
                                          !refracted_pair_mixed<NMAX, RM, PAT, RefrFacet32, ST>(
                                              f32, t, rcl, o, flp)) {
                                        o.valid = false;
@@ -726,29 +751,33 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
 }
 
 // The mixed sequential-chain instantiations are held to 3 waves per SIMD
-// (their LDS already allows 3 workgroups per CU; left alone the compiler
-// settles just past 168 VGPRs and the kernel runs at 2 waves per SIMD).
-template <int PREC, int NMAX, int RM, int PAT,
-          int WAVES = (PREC == SS_PREC_MIXED && NMAX == 0) ? 3 : 1>
+// (left alone the compiler settles just past 168 VGPRs and the kernel runs
+// at 2 waves per SIMD), the FS_PLAIN ones to 4 (127 VGPRs with the folded
+// flags, no spills; the 512 x 512 bed 2.86 -> 2.51 ms against 3 waves).
+template <int PREC, int NMAX, int RM, int PAT, int FS = 0,
+          int WAVES = (PREC == SS_PREC_MIXED && NMAX == 0) ? (FS == FS_PLAIN ? 4 : 3) : 1>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void refracted_kernel(
     KCommon k, RefrConst rc, Redo rd) {
   using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   __shared__ Shared<ACC, kTTRefr> sh;
-  refracted_body<PREC, NMAX, RM, PAT, ACC>(sh, k, rc, rd);
+  refracted_body<PREC, NMAX, RM, PAT, ACC, FS>(sh, k, rc, rd);
 }
 
 // The mixed tier of the flat-stack joint solve (refracted_body with
 // NMAX < 0; the float64 tier is refracted_kernel<FP64, -1, ...>).  The
 // smooth instantiations are held to 3 waves per SIMD like the sequential
 // mixed kernel (the float32 Newton loop, the float64 step and the float64
-// pass would otherwise settle just past 168 VGPRs), the generic (rough)
-// ones to 2.
-template <int PREC, int RM, int PAT, int WAVES = (RM == 0 ? 3 : 2)>
+// pass would otherwise settle just past 168 VGPRs), the smooth FS_PLAIN
+// ones to 4 (128 VGPRs and 19 spilled to scratch, yet faster on deep
+// stacks: the 24-interface bed of the multilayer scene 13.9 -> 12.5 ms per
+// 64 traces), the generic (rough) ones to 2.
+template <int PREC, int RM, int PAT, int FS = 0,
+          int WAVES = (RM == 0 ? (FS == FS_PLAIN ? 4 : 3) : 2)>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void flat_kernel(
     KCommon k, RefrConst rc, Redo rd) {
   using ACC = double;  // float64 LDS windows in both tiers (see Shared)
   __shared__ Shared<ACC, kTTRefr> sh;
-  refracted_body<PREC, -1, RM, PAT, ACC>(sh, k, rc, rd);
+  refracted_body<PREC, -1, RM, PAT, ACC, FS>(sh, k, rc, rd);
 }
 
 // Runs ITEM(w) for every flagged work item w < n_items: each block takes
@@ -1970,8 +1999,11 @@ int ss_surface_field(int precision, const ss_surface_params* prm, const double* 
   const SurfaceConst sc = make_surface_const(*prm, pattern_table, constants);
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
   const int rm = rough_mode(prm->flags), pm = launch_pattern_mode(prm->pattern, prm->pat_n);
-#define SS_SURF(P, R, A) \
-  hipLaunchKernelGGL((surface_kernel<P, R, A>), grid, block, 0, st, k, sc)
+#define SS_SURF(P, R, A)                                                                      \
+  if (P == SS_PREC_MIXED && !(prm->flags & ~kSurfPlainRuntimeFlags))                          \
+    hipLaunchKernelGGL((surface_kernel<P, R, A, FS_SURF_PLAIN>), grid, block, 0, st, k, sc);  \
+  else                                                                                        \
+    hipLaunchKernelGGL((surface_kernel<P, R, A>), grid, block, 0, st, k, sc)
 #define SS_SURF_PA(P, R) \
   if (pm == 0) SS_SURF(P, R, 0); else if (pm == 5) SS_SURF(P, R, 5); else SS_SURF(P, R, -1)
 #define SS_SURF_P(P) \
@@ -2135,6 +2167,7 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   dim3 grid((unsigned)(8 * k.per_xcd)), block(kBlock);
   const bool fp64 = precision == SS_PREC_FP64;
   const int rm = rough_mode(prm->flags), pm = launch_pattern_mode(prm->pattern, prm->pat_n);
+  const bool plain = plain_profile(*prm);
   const long n_items = (long)k.n_ftiles * k.n_ttiles;
   const Redo rd = fp64 ? Redo{} : redo_view(workspace, n_items);
   // Redo pass: each block sweeps chunks of kBlock item flags.
@@ -2147,14 +2180,26 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
 #define SS_BOTH(N, R, A) \
   if (fp64) { SS_REFR(SS_PREC_FP64, N, R, A); } \
   else { SS_REFR(SS_PREC_MIXED, N, R, A); SS_REDO(N, R, A); }
+#define SS_SEQ(R, A)                                                                          \
+  if (!fp64 && plain) {                                                                       \
+    hipLaunchKernelGGL((refracted_kernel<SS_PREC_MIXED, 0, R, A, FS_PLAIN>), grid, block, 0,  \
+                       st, k, rc, rd);                                                        \
+    SS_REDO(0, R, A);                                                                         \
+  } else {                                                                                    \
+    SS_BOTH(0, R, A);                                                                         \
+  }
 #define SS_REFR_PA(R) \
-  if (pm == 0) { SS_BOTH(0, R, 0); } else if (pm == 5) { SS_BOTH(0, R, 5); } \
-  else { SS_BOTH(0, R, -1); }
+  if (pm == 0) { SS_SEQ(R, 0); } else if (pm == 5) { SS_SEQ(R, 5); } \
+  else { SS_SEQ(R, -1); }
 #define SS_FLAT(R, A)                                                                         \
   if (fp64) {                                                                                 \
     SS_REFR(SS_PREC_FP64, -1, R, A);                                                          \
   } else {                                                                                    \
-    hipLaunchKernelGGL((flat_kernel<SS_PREC_MIXED, R, A>), grid, block, 0, st, k, rc, rd);    \
+    if (plain_flags(*prm))                                                                    \
+      hipLaunchKernelGGL((flat_kernel<SS_PREC_MIXED, R, A, FS_PLAIN>), grid, block, 0, st, k,   \
+                         rc, rd);                                                             \
+    else                                                                                      \
+      hipLaunchKernelGGL((flat_kernel<SS_PREC_MIXED, R, A>), grid, block, 0, st, k, rc, rd);  \
     SS_REDO(-1, R, A);                                                                        \
   }
   if (prm->refraction == 2) {
@@ -2184,6 +2229,7 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
   }
 #undef SS_FLAT
 #undef SS_REFR_PA
+#undef SS_SEQ
 #undef SS_BOTH
 #undef SS_REDO
 #undef SS_REFR
