@@ -116,11 +116,12 @@ __device__ inline long work_id(const KCommon& k) {
 // their window in float64: ds_add_f64 measured faster than ds_add_f32 on
 // gfx950 for these scatter patterns (surface kernel of the 512 x 512
 // surface+bed scene 1.19 -> 1.02 ms), and the sums of <= 256 float32 pair
-// contributions per bin carry no float32 rounding.
-template <typename ACC, int TT>
+// contributions per bin carry no float32 rounding.  SIDES: 2 when the
+// launch may split sides (both windows), 1 when it cannot.
+template <typename ACC, int TT, int SIDES = 2>
 struct Shared {
   TraceRec tr[TT];
-  ACC win[2][2 * kWin][2];
+  ACC win[2][SIDES * kWin][2];
   int wmin[2][4];
   int wmax[2][4];
   double drop[TT];
@@ -161,13 +162,13 @@ __host__ __device__ inline long redo_bytes(long n_items, int tt) {
 
 // POWER: a real-valued output (n_traces, n_seg) float64 with only the re
 // slots of the window in use (incoherent_kernel); else complex interleaved.
-template <typename ACC, int TT, bool POWER = false>
-__device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int buf,
+template <typename ACC, int TT, bool POWER = false, int SIDES = 2>
+__device__ inline void flush_window(Shared<ACC, TT, SIDES>& sh, const KCommon& k, int buf,
                                     long t, int base, int span, int n_samples,
                                     bool split) {
   if (span <= 0) return;
   int w = span < kWin ? span : kWin;
-  int nside = split ? 2 : 1;
+  int nside = (SIDES == 2 && split) ? 2 : 1;
   for (int s = threadIdx.x; s < nside * w; s += kBlock) {
     int side = s / w, slot = s - side * w;
     ACC* v = sh.win[buf][side * kWin + slot];
@@ -194,8 +195,8 @@ __device__ inline void flush_window(Shared<ACC, TT>& sh, const KCommon& k, int b
 // lane's facet against trace record t (the it-th of the tile) into o; a pair
 // left with o.redo set contributes nothing and writes no pair bin.  REDO:
 // such pairs are recorded under work item ``item`` of ``rd``.
-template <typename ACC, int TT, bool REDO = false, typename PairFn>
-__device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
+template <typename ACC, int TT, bool REDO = false, int SIDES = 2, typename PairFn>
+__device__ inline void trace_loop(Shared<ACC, TT, SIDES>& sh, const KCommon& k, long w,
                                   bool live, long f, int n_samples, bool split,
                                   PairFn pair, const Redo& rd = Redo{}, long item = 0) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -206,8 +207,8 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     int tt = i / SS_TROW_COUNT, r = i - tt * SS_TROW_COUNT;
     reinterpret_cast<double*>(&sh.tr[tt])[r] = k.traces[(long)r * k.ts + t_base + tt];
   }
-  for (int i = tid; i < 2 * 2 * kWin; i += kBlock) {
-    ACC* v = sh.win[i / (2 * kWin)][i % (2 * kWin)];
+  for (int i = tid; i < 2 * SIDES * kWin; i += kBlock) {
+    ACC* v = sh.win[i / (SIDES * kWin)][i % (SIDES * kWin)];
     v[0] = (ACC)0;
     v[1] = (ACC)0;
   }
@@ -247,7 +248,7 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
     }
     __syncthreads();
     if (it > 0)
-      flush_window<ACC, TT>(sh, k, buf ^ 1, t_base + it - 1, prev_base, prev_span,
+      flush_window<ACC, TT, false>(sh, k, buf ^ 1, t_base + it - 1, prev_base, prev_span,
                             n_samples, split);
     int base = min(min(sh.wmin[buf][0], sh.wmin[buf][1]),
                    min(sh.wmin[buf][2], sh.wmin[buf][3]));
@@ -266,7 +267,7 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
           double re = o.re[p], im = o.im[p];
           if (b >= 0 && b < n_samples) {
             int off = b - base;
-            int side = split ? o.right : 0;
+            int side = (SIDES == 2 && split) ? o.right : 0;
             if (off < kWin) {
               atomicAdd(&sh.win[buf][side * kWin + off][0], (ACC)re);
               atomicAdd(&sh.win[buf][side * kWin + off][1], (ACC)im);
@@ -288,7 +289,7 @@ __device__ inline void trace_loop(Shared<ACC, TT>& sh, const KCommon& k, long w,
   }
   __syncthreads();
   if (nT > 0)
-    flush_window<ACC, TT>(sh, k, (nT - 1) & 1, t_base + nT - 1, prev_base, prev_span,
+    flush_window<ACC, TT, false>(sh, k, (nT - 1) & 1, t_base + nT - 1, prev_base, prev_span,
                           n_samples, split);
   __syncthreads();
   if (tid < nT && sh.drop[tid] != 0.0) atomicAdd(&k.dropped[t_base + tid], sh.drop[tid]);
@@ -475,12 +476,18 @@ __device__ inline void load_refr_facet(const KCommon& k, long fi, int flags, Ref
 // FS (mixed tier): 0 every flag at run time; FS_SURF_PLAIN no split sides,
 // interpolated bins or per-facet gamma (compile-time), taper and roughness
 // at run time.
+// The FS_SURF_PLAIN mixed instantiations keep a one-sided bin window (18 KB
+// of LDS per workgroup) and are held to 8 waves per SIMD (60-64 VGPRs, no
+// spills): the 512 x 512 surface 0.66 -> 0.64 ms against 6 waves.
 constexpr int FS_SURF_PLAIN = 1;
 constexpr int kSurfPlainRuntimeFlags = SS_TAPER | SS_ROUGH | SS_D_PHI_AREA;
-template <int PREC, int RM, int PAT, int FS = 0>
-__global__ __launch_bounds__(kBlock) void surface_kernel(KCommon k, SurfaceConst sc) {
+constexpr int kSurfPlainWaves = 8;
+template <int PREC, int RM, int PAT, int FS = 0,
+          int WAVES = (PREC == SS_PREC_MIXED && FS == FS_SURF_PLAIN) ? kSurfPlainWaves : 1>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void surface_kernel(
+    KCommon k, SurfaceConst sc) {
   using ACC = double;  // float64 LDS windows in both tiers (see Shared)
-  __shared__ Shared<ACC, kTTSurf> sh;
+  __shared__ Shared<ACC, kTTSurf, FS == FS_SURF_PLAIN ? 1 : 2> sh;
   const long w = work_id(k);
   if (w < 0) return;
   const long f = (w % k.n_ftiles) * kBlock + threadIdx.x;
@@ -753,7 +760,8 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
 // The mixed sequential-chain instantiations are held to 3 waves per SIMD
 // (left alone the compiler settles just past 168 VGPRs and the kernel runs
 // at 2 waves per SIMD), the FS_PLAIN ones to 4 (127 VGPRs with the folded
-// flags, no spills; the 512 x 512 bed 2.86 -> 2.51 ms against 3 waves).
+// flags, no spills; the 512 x 512 bed 2.86 -> 2.51 ms against 3 waves; 5
+// waves spill 29 VGPRs to scratch and take 3.08 ms).
 template <int PREC, int NMAX, int RM, int PAT, int FS = 0,
           int WAVES = (PREC == SS_PREC_MIXED && NMAX == 0) ? (FS == FS_PLAIN ? 4 : 3) : 1>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void refracted_kernel(
