@@ -572,7 +572,8 @@ def main():
     f_bed = roofline.refracted_pair_flops(1, "tabulated", True, b.roughness[3])
     f_surf = roofline.surface_pair_flops("tabulated", True, s.roughness[3])
     achieved = f_bed * pairs_l[1] / (ms_b * 1e-3) / 1e12
-    kname = "refracted_kernel<1, 0, 1, 5>"      # mixed, sequential, area-only D_Phi, tabulated
+    # mixed, sequential, area-only D_Phi, tabulated, plain flag profile, 4 waves per SIMD
+    kname = "refracted_kernel<1, 0, 1, 5, 1, 4>"
     traffic = None
     prof = {}
     tpath = os.path.join(HERE, "profiles", "traffic.json")
@@ -589,6 +590,14 @@ def main():
             # issue counters of the same launch show how busy the vector ALUs are
             "valu_busy": prof.get("valu_busy"), "valu_utilization": prof.get("valu_utilization"),
             "valu_source": prof.get("valu_source"),
+            # the FP32 peak counts packed FP32 (two per lane per cycle); the
+            # kernels issue no packed FP32, whose own issue rate is half of it
+            "peak_nonpacked_fp32": roofline.PEAK_FP32_TFLOPS / 2,
+            "frac_nonpacked_fp32": achieved / (roofline.PEAK_FP32_TFLOPS / 2),
+            # bytes the launch cannot avoid: facets, crossed-surface table,
+            # trace records and the field, each once
+            "unique_bytes": roofline.refracted_unique_bytes(pairs_l[1] // rs.traces.n,
+                                                            rs.traces.n, wl.n_samples),
             "kernel": kname,
             "flops_per_pair": f_bed, "kernel_ms": ms_b,
             "surface_kernel": {"flops_per_pair": f_surf, "kernel_ms": ms_s,

@@ -74,3 +74,19 @@ def refracted_pair_flops(n_crossed=1, pattern="isotropic", taper=False,
 
 def incoherent_pair_flops(pattern="isotropic"):
     return INCOHERENT_BASE + ((6 + GAIN[pattern]) if pattern != "isotropic" else 0)
+
+
+# Bytes of a mixed refracted launch that no schedule can avoid: each facet
+# record once (float64 centre + float32 normal, area, edges, gamma,
+# phasors: 104 B), each crossed-interface lookup cell once (float64 centre
+# and normal: 48 B), each trace record once (13 float64) and the complex128
+# field once.
+MIXED_FACET_BYTES = 104
+TABLE_CELL_BYTES = 48
+TRACE_RECORD_BYTES = 13 * 8
+
+
+def refracted_unique_bytes(n_facets, n_traces, n_samples, table_cells=None):
+    cells = n_facets if table_cells is None else table_cells
+    return (MIXED_FACET_BYTES * n_facets + TABLE_CELL_BYTES * cells
+            + TRACE_RECORD_BYTES * n_traces + 16 * n_traces * n_samples)
