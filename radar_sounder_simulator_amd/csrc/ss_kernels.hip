@@ -129,11 +129,11 @@ struct Shared {
 };
 
 // Workspace header of the entry points that take a workspace.
-// [0..3]: work counters of a batched launch (one per job class), byte
+// [0..4]: work counters of a batched launch (one per job class), byte
 // kWsOverflow: the redo record's overflow word.
 constexpr long kWsHeader = 256;
 constexpr long kWsOverflow = 64;
-constexpr int kJobCodes = 4;  // batched launch classes, see job_code
+constexpr int kJobCodes = 5;  // batched launch classes, see job_code
 
 // Redo record of one launch: ``flags`` holds one byte per work item (1 =
 // float64 lanes pending), ``masks`` TT x 4 lane masks per flagged item.
@@ -995,12 +995,21 @@ __host__ __device__ inline bool rc_shallow(const ss_refracted_params& r) {
 }
 
 // Launch class of a job: 0 surface; refracted by solver: 1 sequential
-// chain, 2 joint solve, 3 joint solve over a flat stack.  Each class runs as
-// its own persistent launch (one pair function's registers each).
+// chain, 2 joint solve, 3 joint solve over a flat stack, 4 joint solve over
+// a stack of at most kJointTwo interfaces.  Each class runs as its own
+// persistent launch (one pair function's registers each: the two-deep
+// joint solve needs ~250 VGPRs, the four-deep ~360 VGPRs + AGPRs, so the
+// shallow stacks get 2 waves per SIMD where the deeper ones get 1).
+constexpr int kJointTwo = 2;
 __host__ __device__ inline int job_code(const ss_job& j) {
   if (j.kind == 0) return 0;
   const int r = j.refracted.refraction;
+  if (r == 1 && j.refracted.n_crossed <= kJointTwo) return 4;
   return (r >= 0 && r <= 2) ? 1 + r : 1;
+}
+// Joint stack bound of a refracted launch class (0 sequential, -1 flat).
+constexpr int class_nmax(int code) {
+  return code == 2 ? SS_MAX_JOINT : (code == 3 ? -1 : (code == 4 ? kJointTwo : 0));
 }
 
 // LDS prefix of per-job work counts: begins[i] = first work id of job i.
@@ -1082,9 +1091,19 @@ __device__ inline KCommon job_common(const ss_job& jb) {
 // run the generic pair functions: the general joint solve at 1 wave (its
 // per-lane Newton state lives in private memory), the flat stack at 2
 // (mixed) / 1 (float64), as their single launches.
-template <int PREC, int TT, int RM, int PAT, int KIND,
-          int WAVES = (KIND >= 2 ? ((KIND == 3 && PREC == SS_PREC_MIXED) ? 2 : 1)
-                                 : (PREC == SS_PREC_MIXED ? 3 : (RM == 0 ? 2 : 1)))>
+// Waves per SIMD of a batch_kernel instantiation (see above).  The plain
+// mixed ones (FS_PLAIN, feature class 0): surface 6 (78 VGPRs), sequential
+// 4 (10 VGPRs spilled; 3 measured slower), flat stack 2 (205 VGPRs; 4
+// would spill 64), the four-deep joint solve 1 (256 VGPRs + 110 AGPRs),
+// the two-deep one 2 (253 VGPRs, no spills).
+constexpr int batch_waves(int prec, int rm, int kind, int fs) {
+  return (prec == SS_PREC_MIXED && fs == FS_PLAIN)
+             ? (kind == 0 ? 6 : (kind == 1 ? 4 : (kind == 4 ? 2 : (kind == 3 ? 2 : 1))))
+             : (kind >= 2 ? ((kind == 3 && prec == SS_PREC_MIXED) ? 2 : 1)
+                          : (prec == SS_PREC_MIXED ? 3 : (rm == 0 ? 2 : 1)));
+}
+template <int PREC, int TT, int RM, int PAT, int KIND, int FS = 0,
+          int WAVES = batch_waves(PREC, RM, KIND, FS)>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void batch_kernel(BatchArgs ba,
                                                       unsigned long long* counter, Redo rd) {
   using ACC = double;  // float64 LDS windows in both tiers (see Shared)
@@ -1118,7 +1137,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
     if constexpr (KIND == 0) {
       SurfaceConst sc = make_surface_const(jb.surface, jb.pattern_table, jb.constants);
       sc.mtab = mt;
-      const bool split = (sc.flags & SS_SPLIT_SIDES) != 0;
+      if constexpr (FS == FS_PLAIN) sc.flags &= kSurfPlainRuntimeFlags;
+      const bool split = FS == FS_PLAIN ? false : (sc.flags & SS_SPLIT_SIDES) != 0;
       SurfFacet64 f64;
       load_surface_facet(k, live ? f : 0, sc.flags, f64);
       if constexpr (PREC == SS_PREC_FP64) {
@@ -1139,12 +1159,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
       // SS_MAX_JOINT; shallow stacks of at most 4 take the register-resident
       // instantiation: the branch is uniform, one job per work item),
       // flat-stack joint solve
-      constexpr int NMAX = KIND == 2 ? SS_MAX_JOINT : (KIND == 3 ? -1 : 0);
-      const bool shallow = KIND == 2 && rc_shallow(jb.refracted);
+      // (FS_PLAIN: every joint stack is shallow, the deep solve is not built)
+      constexpr int NMAX = (KIND == 2 && FS == FS_PLAIN) ? 4 : class_nmax(KIND);
+      const bool shallow = KIND == 2 && (FS == FS_PLAIN || rc_shallow(jb.refracted));
       RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
                                      jb.pattern_table, jb.constants);
       rc.mtab = mt;
-      const bool split = (rc.flags & SS_SPLIT_SIDES) != 0;
+      if constexpr (FS == FS_PLAIN) {
+        rc.flags = (rc.flags & kPlainRuntimeFlags) | SS_COHERENT;
+        if constexpr (KIND == 1) rc.n_crossed = 1;
+      }
+      const bool split = FS == FS_PLAIN ? false : (rc.flags & SS_SPLIT_SIDES) != 0;
       const long fi = live ? f : 0;
       if constexpr (PREC == SS_PREC_FP64) {
         RefrFacet fr;
@@ -1206,7 +1231,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
 // record), masked to the recorded lanes.
 template <int TT, int CODE>
 __global__ __launch_bounds__(kBlock) void batch_redo_kernel(BatchArgs ba, Redo rd) {
-  constexpr int NMAX = CODE == 2 ? SS_MAX_JOINT : (CODE == 3 ? -1 : 0);
+  constexpr int NMAX = class_nmax(CODE);
   __shared__ Shared<double, TT> sh;
   __shared__ long begins[kMaxJobs + 1];
   __shared__ unsigned long long sh_mask[kBlock / 64];
@@ -2376,8 +2401,23 @@ static int64_t batch_items(const ss_job* jobs_host, int n_jobs, int trace_tile) 
   return total;
 }
 
-// Feature class of a job mix: 0 = no roughness, isotropic antenna; 1 = at
-// most area-only D_Phi and a tabulated pattern; 2 = anything.
+// A job the plain flag profile covers: a surface job without split sides,
+// interpolated bins or per-facet gamma; a coherent refracted job without
+// split sides, diffuse term, per-facet gamma or forced redo, over one
+// crossing (sequential chain) or a stack of at most four (joint solve).
+static bool batch_plain_job(const ss_job& j) {
+  if (j.kind == 0) return !(j.surface.flags & ~kSurfPlainRuntimeFlags);
+  const int code = job_code(j);
+  if (code == 1) return plain_profile(j.refracted);
+  if (code == 2) return plain_flags(j.refracted) && rc_shallow(j.refracted);
+  return plain_flags(j.refracted);  // flat stack, two-deep joint solve
+}
+
+// Feature class of a job mix: 0 = no roughness, isotropic antenna and every
+// job plain (batch_plain_job; the mixed launches of class 0 fold those
+// flags at compile time and build the joint solve for shallow stacks
+// only); 1 = at most area-only D_Phi and a tabulated pattern; 2 =
+// anything.  Each class's kernels run the jobs of the classes below it.
 static int batch_class(const ss_job* jobs_host, int n_jobs) {
   int cls = 0;
   for (int i = 0; i < n_jobs; ++i) {
@@ -2385,7 +2425,8 @@ static int batch_class(const ss_job* jobs_host, int n_jobs) {
     const int fl = j.kind == 0 ? j.surface.flags : j.refracted.flags;
     const int pk = j.kind == 0 ? j.surface.pattern : j.refracted.pattern;
     const int rm = rough_mode(fl), pm = pattern_mode(pk);
-    const int c = (rm == 0 && pm == 0) ? 0 : ((rm <= 1 && pm != -1) ? 1 : 2);
+    int c = (rm == 0 && pm == 0) ? 0 : ((rm <= 1 && pm != -1) ? 1 : 2);
+    if (c == 0 && !batch_plain_job(j)) c = 1;
     cls = c > cls ? c : cls;
   }
   return cls;
@@ -2429,11 +2470,31 @@ static void launch_batch(bool fp64, const BatchArgs& ba, void* workspace, const 
     hipLaunchKernelGGL((batch_kernel<P, T, R, A, 0>), g, block, 0, st, ba, ctr, rd);          \
   else                                                                                        \
     hipLaunchKernelGGL((batch_kernel<P, T, R, A, 1>), g, block, 0, st, ba, ctr, rd)
+#define SS_BATCH_PLAIN(T)                                                                     \
+  if (code == 0)                                                                              \
+    hipLaunchKernelGGL((batch_kernel<SS_PREC_MIXED, T, 0, 0, 0, FS_PLAIN>), g, block, 0, st,  \
+                       ba, ctr, rd);                                                          \
+  else if (code == 1)                                                                         \
+    hipLaunchKernelGGL((batch_kernel<SS_PREC_MIXED, T, 0, 0, 1, FS_PLAIN>), g, block, 0, st,  \
+                       ba, ctr, rd);                                                          \
+  else if (code == 2)                                                                         \
+    hipLaunchKernelGGL((batch_kernel<SS_PREC_MIXED, T, 0, 0, 2, FS_PLAIN>), g, block, 0, st,  \
+                       ba, ctr, rd);                                                          \
+  else if (code == 3)                                                                         \
+    hipLaunchKernelGGL((batch_kernel<SS_PREC_MIXED, T, 0, 0, 3, FS_PLAIN>), g, block, 0, st,  \
+                       ba, ctr, rd);                                                          \
+  else                                                                                        \
+    hipLaunchKernelGGL((batch_kernel<SS_PREC_MIXED, T, 0, 0, 4, FS_PLAIN>), g, block, 0, st,  \
+                       ba, ctr, rd)
 #define SS_BATCH_C(P, T)                                                                      \
-  if (code == 2) {                                                                            \
+  if (P == SS_PREC_MIXED && cls == 0) {                                                       \
+    SS_BATCH_PLAIN(T);                                                                        \
+  } else if (code == 2) {                                                                     \
     hipLaunchKernelGGL((batch_kernel<P, T, -1, -1, 2>), g, block, 0, st, ba, ctr, rd);        \
   } else if (code == 3) {                                                                     \
     hipLaunchKernelGGL((batch_kernel<P, T, -1, -1, 3>), g, block, 0, st, ba, ctr, rd);        \
+  } else if (code == 4) {                                                                     \
+    hipLaunchKernelGGL((batch_kernel<P, T, -1, -1, 4>), g, block, 0, st, ba, ctr, rd);        \
   } else if (cls == 0) { SS_BATCH_K(P, T, 0, 0); }                                            \
   else if (cls == 1) { SS_BATCH_K(P, T, 1, 5); }                                              \
   else { SS_BATCH_K(P, T, -1, -1); }
@@ -2449,6 +2510,8 @@ static void launch_batch(bool fp64, const BatchArgs& ba, void* workspace, const 
         hipLaunchKernelGGL((batch_redo_kernel<T, 2>), rg, block, 0, st, ba, rd);              \
       else if (code == 3)                                                                     \
         hipLaunchKernelGGL((batch_redo_kernel<T, 3>), rg, block, 0, st, ba, rd);              \
+      else if (code == 4)                                                                     \
+        hipLaunchKernelGGL((batch_redo_kernel<T, 4>), rg, block, 0, st, ba, rd);              \
     }                                                                                         \
   }
     if (tt == 4) {
@@ -2460,6 +2523,7 @@ static void launch_batch(bool fp64, const BatchArgs& ba, void* workspace, const 
     }
 #undef SS_LAUNCH_BATCH
 #undef SS_BATCH_C
+#undef SS_BATCH_PLAIN
 #undef SS_BATCH_K
   }
   if (fp64 && fold)
@@ -2500,7 +2564,7 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   }
   if (workspace_bytes < ss_batch_workspace_bytes(precision, jobs_host, n_jobs, trace_tile))
     return fail(SS_EINVAL, "workspace smaller than ss_batch_workspace_bytes()");
-  long items[kJobCodes] = {0, 0, 0, 0};
+  long items[kJobCodes] = {};
   bool any_surface = false;
   for (int i = 0; i < n_jobs; ++i) {
     items[job_code(jobs_host[i])] += job_work(jobs_host[i], trace_tile);
@@ -2512,13 +2576,13 @@ int ss_batch_field(int precision, const ss_job* jobs_host, int n_jobs, void* job
   if (e != hipSuccess) return fail(SS_EHIP, std::string("job upload: ") + hipGetErrorString(e));
   const bool fp64 = precision == SS_PREC_FP64;
   const int nb = default_blocks(n_blocks);
-  unsigned grid[kJobCodes], rblocks[kJobCodes] = {0, 0, 0, 0};
+  unsigned grid[kJobCodes], rblocks[kJobCodes] = {};
   for (int k = 0; k < kJobCodes; ++k) {
     grid[k] = (unsigned)(items[k] < nb ? items[k] : nb);
     if (!fp64 && k > 0) rblocks[k] = redo_grid(items[k]);
   }
   // the redo flags and masks hold the refracted classes' work ids in turn
-  const Redo rd = fp64 ? Redo{} : redo_view(workspace, items[1] + items[2] + items[3]);
+  const Redo rd = fp64 ? Redo{} : redo_view(workspace, items[1] + items[2] + items[3] + items[4]);
   const BatchArgs ba{(const ss_job*)jobs_device, nullptr, n_jobs};
   launch_batch(fp64, ba, workspace, rd, trace_tile, grid, rblocks,
                batch_class(jobs_host, n_jobs), any_surface, st);
@@ -2589,8 +2653,8 @@ int ss_batch_step(int precision, const void* table_device, void* workspace,
     return fail(SS_EINVAL, "workspace smaller than ss_batch_step_workspace_bytes()");
   const bool fp64 = precision == SS_PREC_FP64;
   const int nb = default_blocks(n_blocks);
-  const unsigned grid[kJobCodes] = {(unsigned)nb, (unsigned)nb, (unsigned)nb, (unsigned)nb};
-  unsigned rblocks[kJobCodes] = {0, 0, 0, 0};
+  unsigned grid[kJobCodes], rblocks[kJobCodes] = {};
+  for (int k = 0; k < kJobCodes; ++k) grid[k] = (unsigned)nb;
   if (!fp64)
     for (int k = 1; k < kJobCodes; ++k) rblocks[k] = redo_grid(max_items);
   const Redo rd = fp64 ? Redo{} : redo_view(workspace, max_items);
