@@ -351,7 +351,7 @@ int64_t ss_batch_table_bytes(int max_jobs);
  * refracted jobs at this trace_tile; feature_class: 0 (no roughness,
  * isotropic, and every job plain: no split sides, interpolated bins,
  * diffuse term, per-facet gamma or forced redo, one crossing per
- * sequential job, joint stacks of at most four; the mixed launches fold
+ * sequential job, joint stacks of at most three; the mixed launches fold
  * those flags at compile time), 1 (at most area-only D_Phi, tabulated
  * pattern) or 2.  A step graph captured for a class runs every lower one. */
 int ss_batch_pack(const ss_job* jobs_host, int n_jobs, int trace_tile, void* table_host,

@@ -1094,11 +1094,14 @@ __device__ inline KCommon job_common(const ss_job& jb) {
 // Waves per SIMD of a batch_kernel instantiation (see above).  The plain
 // mixed ones (FS_PLAIN, feature class 0): surface 6 (78 VGPRs), sequential
 // 4 (10 VGPRs spilled; 3 measured slower), flat stack 2 (205 VGPRs; 4
-// would spill 64), the four-deep joint solve 1 (256 VGPRs + 110 AGPRs),
-// the two-deep one 2 (253 VGPRs, no spills).
+// would spill 64), the joint solve of up to kJointPlain = 3 interfaces 2
+// (80 VGPRs spilled, yet faster than 1 wave: the 64-scene mix at 0.25
+// scale 523 -> 512 ms; a four-deep bound needs 256 VGPRs + 110 AGPRs and
+// runs at 1, 537 ms), the two-deep one 2 (253 VGPRs, no spills).
+constexpr int kJointPlain = 3;
 constexpr int batch_waves(int prec, int rm, int kind, int fs) {
   return (prec == SS_PREC_MIXED && fs == FS_PLAIN)
-             ? (kind == 0 ? 6 : (kind == 1 ? 4 : (kind == 4 ? 2 : (kind == 3 ? 2 : 1))))
+             ? (kind == 0 ? 6 : (kind == 1 ? 4 : (kind == 3 ? 2 : 2)))
              : (kind >= 2 ? ((kind == 3 && prec == SS_PREC_MIXED) ? 2 : 1)
                           : (prec == SS_PREC_MIXED ? 3 : (rm == 0 ? 2 : 1)));
 }
@@ -1159,8 +1162,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
       // SS_MAX_JOINT; shallow stacks of at most 4 take the register-resident
       // instantiation: the branch is uniform, one job per work item),
       // flat-stack joint solve
-      // (FS_PLAIN: every joint stack is shallow, the deep solve is not built)
-      constexpr int NMAX = (KIND == 2 && FS == FS_PLAIN) ? 4 : class_nmax(KIND);
+      // (FS_PLAIN: every joint stack is at most kJointPlain deep, the deep
+      // solve is not built)
+      constexpr int NMAX = (KIND == 2 && FS == FS_PLAIN) ? kJointPlain : class_nmax(KIND);
       const bool shallow = KIND == 2 && (FS == FS_PLAIN || rc_shallow(jb.refracted));
       RefrConst rc = make_refr_const(jb.refracted, jb.iface_desc, jb.tables, jb.legs,
                                      jb.pattern_table, jb.constants);
@@ -1206,7 +1210,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
                                   [&](const TraceRec& t, PairOut& o, int) {
                                     const bool ok =
                                         (KIND == 2 && shallow)
-                                            ? refracted_pair_mixed<4, RM, PAT>(f32, t, rc, o)
+                                            ? refracted_pair_mixed<(FS == FS_PLAIN ? kJointPlain : 4),
+                                                                   RM, PAT>(f32, t, rc, o)
                                             : refracted_pair_mixed<NMAX, RM, PAT>(f32, t, rc, o,
                                                                                   flp);
                                     if ((rc.flags & SS_REDO_ALL) || !ok) {
@@ -2404,19 +2409,20 @@ static int64_t batch_items(const ss_job* jobs_host, int n_jobs, int trace_tile) 
 // A job the plain flag profile covers: a surface job without split sides,
 // interpolated bins or per-facet gamma; a coherent refracted job without
 // split sides, diffuse term, per-facet gamma or forced redo, over one
-// crossing (sequential chain) or a stack of at most four (joint solve).
+// crossing (sequential chain) or a stack of at most kJointPlain (joint
+// solve).
 static bool batch_plain_job(const ss_job& j) {
   if (j.kind == 0) return !(j.surface.flags & ~kSurfPlainRuntimeFlags);
   const int code = job_code(j);
   if (code == 1) return plain_profile(j.refracted);
-  if (code == 2) return plain_flags(j.refracted) && rc_shallow(j.refracted);
+  if (code == 2) return plain_flags(j.refracted) && j.refracted.n_crossed <= kJointPlain;
   return plain_flags(j.refracted);  // flat stack, two-deep joint solve
 }
 
 // Feature class of a job mix: 0 = no roughness, isotropic antenna and every
 // job plain (batch_plain_job; the mixed launches of class 0 fold those
-// flags at compile time and build the joint solve for shallow stacks
-// only); 1 = at most area-only D_Phi and a tabulated pattern; 2 =
+// flags at compile time and build the joint solve for stacks of at most
+// kJointPlain interfaces only); 1 = at most area-only D_Phi and a tabulated pattern; 2 =
 // anything.  Each class's kernels run the jobs of the classes below it.
 static int batch_class(const ss_job* jobs_host, int n_jobs) {
   int cls = 0;
