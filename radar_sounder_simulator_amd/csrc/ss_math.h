@@ -210,8 +210,10 @@ SS_HD void term_tab_entry(int m, float* tab) {
 // D_Phi, clamped at 0 (exactly 0 when (sigma K)^2 == 0).
 // RM (compile-time roughness mode): 1 area-only terms, 2 finite-facet
 // Gaussian terms, -1 decided by ``area_only`` at run time.  ``tab``: the
-// staged term constants (term_tab_entry) or null (computed per term).
-template <typename T, int RM = -1>
+// staged term constants (term_tab_entry) or null (computed per term);
+// TAB_ALWAYS: the caller always passes the staged table (the float32 branch
+// then carries no run-time choice between the two).
+template <typename T, int RM = -1, bool TAB_ALWAYS = false>
 SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
               bool area_only_rt, int acf, const double* cst, const float* tab = nullptr) {
   const bool area_only = RM == 1 ? true : (RM == 2 ? false : area_only_rt);
@@ -266,7 +268,8 @@ SS_HD T d_phi(T sigma, T l, T K, T A0, T B0, T Lx, T Ly, int n_terms,
       };
       using TT_ = std::true_type;
       using FF_ = std::false_type;
-      const float G = tab ? (expo ? horner(TT_{}, TT_{}) : horner(TT_{}, FF_{}))
+      const float G = (TAB_ALWAYS || tab)
+                          ? (expo ? horner(TT_{}, TT_{}) : horner(TT_{}, FF_{}))
                           : (expo ? horner(FF_{}, TT_{}) : horner(FF_{}, FF_{}));
       const float sum = (expo ? x : 1.0f) * f_exp(-x) * G;
       acc = sum * ll * ((expo ? (float)kTwoPi : (float)kPi) * Lx * Ly);

@@ -385,7 +385,7 @@ struct RefrConst {
   int joint, k_pad, n_newton, n_backtrack;  // joint solver (pads first)
   // derived
   double k_bin, t0_dt, cyc_per_m;
-  float fast_opl_tol;  // optical-path error (m) allowed to a polish-free crossing
+  float fast_opl_tol = 0.0f;  // optical-path error (m) allowed to a polish-free crossing
 };
 
 // Path sums handed from a crossing solver to the radiometry stage.
@@ -801,7 +801,7 @@ SS_HD bool radiometry_mixed(const FT& f, const TraceRec& t, const RefrConst& rc,
       are = amp * f_exp(-0.5f * sk * sk);
       if (f.A > 0.0) {
         float l1 = (float)f.l1, l2 = (float)f.l2;
-        float dp = d_phi<float, RM>((float)rc.sigma, (float)rc.lc, kk, 2.0f * kj * d1 * f_rcp(l1),
+        float dp = d_phi<float, RM, true>((float)rc.sigma, (float)rc.lc, kk, 2.0f * kj * d1 * f_rcp(l1),
                                 2.0f * kj * d2 * f_rcp(l2), l1, l2, rc.n_terms,
                                 (flags & SS_D_PHI_AREA) != 0, rc.acf, rc.cst, rc.mtab);
         float ai = base * f_sqrt(dp);

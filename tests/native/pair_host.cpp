@@ -134,7 +134,12 @@ extern "C" int host_refracted(int prec, const ss_refracted_params* prm, const do
   rc.pat.table = table; rc.cst = cst; rc.desc = desc; rc.tables = tables; rc.legs = legs;
   HostPatLookup plk;
   stage_host(rc.pat, plk);
+  // the D_Phi term constants, staged as the kernels stage them in LDS
+  std::vector<float> mtab(2 * kTermTab);
+  for (int m = 0; m < kTermTab; ++m) term_tab_entry(m, mtab.data());
+  rc.mtab = mtab.data();
   rc.k_bin = 2.0 / (prm->c * prm->dt); rc.t0_dt = prm->t0 / prm->dt; rc.cyc_per_m = prm->k0 / kPi;
+  rc.fast_opl_tol = (float)(kFastOplFrac * kBinGuard / rc.k_bin);
   rc.joint = prm->refraction == 1; rc.k_pad = prm->k_pad;
   rc.n_newton = prm->joint_newton; rc.n_backtrack = prm->joint_backtrack;
   bool split = prm->flags & SS_SPLIT_SIDES;
