@@ -118,3 +118,61 @@ def test_noalloc_rejects_calls_beyond_its_scratch():
                                         None, None, None, None, None, None, None)
     assert rc == -3
     assert b"SS_NOALLOC_SCRATCH_BYTES" in lib.ss_last_error()
+
+
+def _job(kind, flags=0, refraction=0, n_crossed=1, pattern=0):
+    """A host job record with placeholder (never dereferenced) pointers."""
+    j = _lib.Job()
+    j.kind = kind
+    j.facet_stride = j.n_facets = 1000
+    j.trace_stride = j.n_traces = 20
+    p = ctypes.c_void_p(4096)
+    j.facets = j.traces = j.field = j.dropped = j.constants = p
+    j.iface_desc = j.tables = j.legs = j.pattern_table = p
+    prm = j.surface if kind == 0 else j.refracted
+    prm.n_samples, prm.dt, prm.c, prm.flags = 256, 1e-8, 3e8, flags
+    prm.pattern, prm.pat_n = pattern, (181 if pattern else 0)
+    if kind == 1:
+        prm.refraction, prm.n_crossed = refraction, n_crossed
+        prm.joint_newton, prm.joint_backtrack = 6, 4
+    return j
+
+
+def _pack(*jobs):
+    lib = _lib.load()
+    arr = (_lib.Job * len(jobs))(*jobs)
+    buf = (ctypes.c_uint8 * lib.ss_batch_table_bytes(len(jobs)))()
+    items = (ctypes.c_int64 * 2)()
+    cls = ctypes.c_int()
+    rc = lib.ss_batch_pack(arr, len(jobs), 16, buf, len(buf), items, ctypes.byref(cls))
+    assert rc == 0, lib.ss_last_error()
+    return cls.value, (items[0], items[1])
+
+
+def test_batch_feature_class_rules():
+    """Feature class 0 (compile-time flags in the mixed launches) takes only
+    smooth, isotropic, plain jobs: coherent refracted jobs without split
+    sides, diffuse term, per-facet gamma or forced redo, one crossing per
+    sequential job and joint stacks of at most three; anything else of
+    that kind is class 1; roughness beyond the area-only D_Phi or a
+    non-tabulated pattern is class 2."""
+    coh = _lib.SS_COHERENT
+    assert _pack(_job(0)) == (0, (8, 0))          # 4 facet tiles x 2 trace tiles
+    assert _pack(_job(0, _lib.SS_TAPER))[0] == 0
+    assert _pack(_job(0, _lib.SS_SPLIT_SIDES))[0] == 1
+    assert _pack(_job(0, _lib.SS_GAMMA_FACET))[0] == 1
+    assert _pack(_job(1, coh)) == (0, (0, 8))
+    assert _pack(_job(1, coh | _lib.SS_TAPER | _lib.SS_CROSS_ROUGH))[0] == 0
+    assert _pack(_job(1, 0))[0] == 1               # incoherent refracted
+    assert _pack(_job(1, coh | _lib.SS_DIFFUSE))[0] == 1
+    assert _pack(_job(1, coh | _lib.SS_REDO_ALL))[0] == 1
+    assert _pack(_job(1, coh, refraction=0, n_crossed=2))[0] == 1
+    assert _pack(_job(1, coh, refraction=1, n_crossed=2))[0] == 0
+    assert _pack(_job(1, coh, refraction=1, n_crossed=3))[0] == 0
+    assert _pack(_job(1, coh, refraction=1, n_crossed=4))[0] == 1
+    assert _pack(_job(1, coh, refraction=2, n_crossed=8))[0] == 0
+    assert _pack(_job(0, pattern=5))[0] == 1        # tabulated pattern
+    assert _pack(_job(0, pattern=1))[0] == 2        # dipole
+    # the class of a table is its highest job's
+    assert _pack(_job(0), _job(1, coh, refraction=1, n_crossed=4))[0] == 1
+    assert _pack(_job(0), _job(1, coh), _job(0, pattern=1))[0] == 2
