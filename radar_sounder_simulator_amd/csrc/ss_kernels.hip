@@ -763,7 +763,11 @@ __device__ inline void refracted_body(Shared<ACC, kTTRefr>& sh, const KCommon& k
 // flags, no spills; the 512 x 512 bed 2.86 -> 2.51 ms against 3 waves; 5
 // waves spill 29 VGPRs to scratch and take 3.08 ms).
 template <int PREC, int NMAX, int RM, int PAT, int FS = 0,
-          int WAVES = (PREC == SS_PREC_MIXED && NMAX == 0) ? (FS == FS_PLAIN ? 4 : 3) : 1>
+          int WAVES = (PREC == SS_PREC_MIXED && NMAX == 0)
+                          ? (FS == FS_PLAIN ? 4 : 3)
+                          : ((PREC == SS_PREC_MIXED && FS == FS_PLAIN && NMAX > 0 && NMAX <= 3)
+                                 ? 2
+                                 : 1)>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void refracted_kernel(
     KCommon k, RefrConst rc, Redo rd) {
   using ACC = double;  // float64 LDS windows in both tiers (see Shared)
@@ -2259,7 +2263,28 @@ int ss_refracted_field(int precision, const ss_refracted_params* prm, const doub
       SS_REFR_PA(2);
     }
   } else if (prm->n_crossed <= 4) {
-    SS_BOTH(4, -1, -1);
+    // smooth interfaces under an isotropic or tabulated pattern, plain
+    // flags: the joint solve built for the stack's own depth bound (two
+    // and three deep at 2 waves per SIMD, as the batched classes)
+    if (!fp64 && plain_flags(*prm) && rm == 0 && (pm == 0 || pm == 5)) {
+#define SS_JPLAIN(N, A)                                                                      \
+  hipLaunchKernelGGL((refracted_kernel<SS_PREC_MIXED, N, 0, A, FS_PLAIN>), grid, block, 0,   \
+                     st, k, rc, rd)
+#define SS_JPLAIN_A(N) \
+  if (pm == 0) { SS_JPLAIN(N, 0); } else { SS_JPLAIN(N, 5); }
+      if (prm->n_crossed <= 2) {
+        SS_JPLAIN_A(2);
+      } else if (prm->n_crossed <= 3) {
+        SS_JPLAIN_A(3);
+      } else {
+        SS_JPLAIN_A(4);
+      }
+#undef SS_JPLAIN_A
+#undef SS_JPLAIN
+      SS_REDO(4, -1, -1);
+    } else {
+      SS_BOTH(4, -1, -1);
+    }
   } else if (prm->n_crossed <= 8) {
     SS_BOTH(8, -1, -1);
   } else {
