@@ -1326,8 +1326,10 @@ __global__ __launch_bounds__(kBlock) void compress_kernel(const double2* __restr
 // 1/N), transforms back with a decimation-in-time inverse (bit-reversed in,
 // natural out) and keeps the circular-convolution samples K-1 .. N-1, the
 // ones free of wrap-around.  The whole N-point float64 transform lives in
-// LDS (N <= 8192: 128 KiB); twiddles come from sincospi per butterfly, so
-// no table is needed.  This is the reference's own method
+// LDS (N <= 8192: 128 KiB); twiddles come from sincospi (one per thread
+// per stage where the span allows, below), so no table is needed.  The
+// kernels are launched with kFftThreads threads (the twiddle sharing
+// relies on it).  This is the reference's own method
 // (waveform.py:126-138 multiplies FFTs), blocked so any row length fits.
 constexpr int kFftMax = 8192;
 constexpr int kFftThreads = 512;
@@ -1337,16 +1339,25 @@ struct cd {
 };
 __device__ inline cd cmul(cd a, cd b) { return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 
+// Butterfly twiddles: the butterflies a thread takes in a stage are b =
+// tid + k kFftThreads, so for spans h <= kFftThreads (both powers of two)
+// they share j = b & (h - 1) = tid & (h - 1) and one sincospi serves the
+// whole stage; wider spans evaluate it per butterfly.  Same values either
+// way.
+
 // Forward DIF over a[0..n), n = 2^logn: natural order in, bit-reversed out.
 __device__ inline void fft_dif(cd* a, int n, int logn) {
   for (int s = 0; s < logn; ++s) {
     const int lh = logn - 1 - s, h = 1 << lh;  // butterfly span
-    for (int b = threadIdx.x; b < (n >> 1); b += blockDim.x) {
+    const bool shared_j = h <= kFftThreads;
+    double sn0 = 0.0, cs0 = 1.0;
+    if (shared_j) sincospi(-(double)(threadIdx.x & (h - 1)) / (double)h, &sn0, &cs0);
+    for (int b = threadIdx.x; b < (n >> 1); b += kFftThreads) {
       const int j = b & (h - 1);
       const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
       const cd u = a[i0], v = a[i1];
-      double sn, cs;
-      sincospi(-(double)j / (double)h, &sn, &cs);
+      double sn = sn0, cs = cs0;
+      if (!shared_j) sincospi(-(double)j / (double)h, &sn, &cs);
       a[i0] = cd{u.x + v.x, u.y + v.y};
       a[i1] = cmul(cd{u.x - v.x, u.y - v.y}, cd{cs, sn});
     }
@@ -1358,11 +1369,14 @@ __device__ inline void fft_dif(cd* a, int n, int logn) {
 __device__ inline void ifft_dit(cd* a, int n, int logn) {
   for (int lh = 0; lh < logn; ++lh) {
     const int h = 1 << lh;
-    for (int b = threadIdx.x; b < (n >> 1); b += blockDim.x) {
+    const bool shared_j = h <= kFftThreads;
+    double sn0 = 0.0, cs0 = 1.0;
+    if (shared_j) sincospi((double)(threadIdx.x & (h - 1)) / (double)h, &sn0, &cs0);
+    for (int b = threadIdx.x; b < (n >> 1); b += kFftThreads) {
       const int j = b & (h - 1);
       const int i0 = ((b >> lh) << (lh + 1)) + j, i1 = i0 + h;
-      double sn, cs;
-      sincospi((double)j / (double)h, &sn, &cs);
+      double sn = sn0, cs = cs0;
+      if (!shared_j) sincospi((double)j / (double)h, &sn, &cs);
       const cd u = a[i0], t = cmul(a[i1], cd{cs, sn});
       a[i0] = cd{u.x + t.x, u.y + t.y};
       a[i1] = cd{u.x - t.x, u.y - t.y};
